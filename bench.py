@@ -1,0 +1,189 @@
+"""Benchmark: training it/s at 1.5M Gaussians and 1600x1040 on one MI355X.
+
+Workload (one "step" = one training iteration over one view, the reference's
+Trainer::step): 1.5M trainable Gaussians drawn from a 64-component isotropic mixture,
+64 jittered cameras on a radius-3 sphere, 1600x1040 pinhole views with fx = 1300,
+ground truth rendered from a hidden 20k-Gaussian set of the same distribution and
+quantised to uint8, L1 + 0.2 (1 - SSIM), SH degree 3 active, fp32 throughout, fixed
+N (no relocation in the timed window).
+
+Prints one JSON line (the bench contract) including a roofline entry for the dominant
+kernel (per-kernel time from HIP events on the engine stream) and the CPU oracle's
+it/s on the same state as a reported baseline.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=10)
+    p.add_argument("--n", type=int, default=1_500_000)
+    p.add_argument("--width", type=int, default=1600)
+    p.add_argument("--height", type=int, default=1040)
+    p.add_argument("--fx", type=float, default=1300.0)
+    p.add_argument("--cams", type=int, default=64)
+    p.add_argument("--hidden", type=int, default=20_000)
+    p.add_argument("--cpu-steps", type=int, default=2)
+    p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--stage-steps", type=int, default=10)
+    return p.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+    import torch
+    from msplat_amd import scene
+    from msplat_amd.device import DeviceState, Engine
+
+    W, H, N = a.width, a.height, a.n
+    t_setup = time.time()
+    g = scene.synth_gaussians(1, 1, N, 3)
+    hidden = scene.synth_gaussians(1, 2, a.hidden, 3)
+    c2w = scene.synth_cameras(5, a.cams, 3.0, 0.15)
+    cams = [scene.prepare_camera(c, a.fx, a.fx, W / 2.0, H / 2.0, W, H) for c in c2w]
+    eng = Engine()
+    hst = DeviceState(hidden)
+    gts = []
+    for cam in cams:
+        eng.render(hst, cam, 3, scene.DEFAULT_BG)
+        img = eng.raster(W, H)["img"]
+        gts.append(torch.from_numpy(scene.quantize(img)).cuda())
+    del hst
+    st = DeviceState(g)
+    torch.cuda.synchronize()
+    log(f"setup {time.time() - t_setup:.1f}s")
+
+    order = np.random.default_rng(42).permutation(len(cams))
+    interval = 1  # SH degree 3 from step 3 on (all timed steps)
+    lrs = list(scene.LR_INIT)
+
+    def hp_for(step):
+        lrs[0] = scene.means_lr(step - 1) if step > 1 else scene.LR_INIT[0]
+        return scene.step_params(step, lrs, W, H, step, 3, min(step // interval, 3))
+
+    step = 0
+
+    def run(k):
+        nonlocal step
+        for _ in range(k):
+            step += 1
+            ci = order[step % len(cams)]
+            eng.train_step(st, cams[ci], gts[ci], hp_for(step), sync_inputs=False)
+
+    run(a.warmup)
+    eng.sync()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run(a.steps)
+    eng.sync()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / a.steps
+    its = a.steps / (t1 - t0)
+    stats = eng.stats()
+    log(f"{its:.2f} it/s, {ms_per_step:.3f} ms/step, I={stats['num_isect']} max_tile={stats['max_tile']}")
+
+    # per-kernel times from HIP events on the engine stream
+    import ctypes as C
+    L = eng.L
+    L.msplat_engine_reset_stage_times(eng.h)
+    L.msplat_engine_set_stage_timing(eng.h, 1)
+    run(a.stage_steps)
+    eng.sync()
+    ms = (C.c_double * 16)()
+    nl = (C.c_int64 * 16)()
+    ns = L.msplat_engine_stage_times(eng.h, ms, 16, nl)
+    L.msplat_engine_set_stage_timing(eng.h, 0)
+    stages = {L.msplat_engine_stage_name(k).decode(): (ms[k] / max(nl[k], 1)) for k in range(ns)}
+    log("stages (ms/launch): " + ", ".join(f"{k}={v:.3f}" for k, v in stages.items()))
+    radii = eng.read("radii", np.int32, (N,))
+    V = int((radii > 0).sum())
+    stats = eng.stats()
+    I = int(stats["num_isect"])
+    # algorithmic HBM bytes per launch of the per-Gaussian kernels
+    alg = {
+        "project_sh_fwd": N * (56 + 180 + 68),
+        "gaussian_update": N * 408,
+        "sh_rest_adam": V * 1080 + N * 28,
+        "isect_sort_pack": N * 24 + 8 * I * 2 + 40 * I * 2,
+    }
+    dom = max(stages, key=lambda k: stages[k])
+    roof_k = dom if dom in alg else max(alg, key=lambda k: stages.get(k, 0))
+    achieved = alg[roof_k] / (stages[roof_k] * 1e-3) / 1e9
+    roofline = {"kernel": roof_k, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "dominant_kernel": dom, "dominant_ms": round(stages[dom], 4)}
+
+    cpu = None
+    if not a.no_cpu and a.cpu_steps > 0:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle_py as O
+        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+        threads = O.set_threads(threads)
+        host = st.to_host()
+        mom = st.moments_host()
+        ost = O.HostState(host, moments=mom)
+        times = []
+        for k in range(a.cpu_steps):
+            s2 = step + 1 + k
+            ci = order[s2 % len(cams)]
+            gt = gts[ci].cpu().numpy()
+            tc = time.perf_counter()
+            O.train_step(ost, cams[ci], gt, hp_for(s2), want_ids=False)
+            times.append(time.perf_counter() - tc)
+            log(f"cpu step {k}: {times[-1]:.2f}s")
+        med = statistics.median(times)
+        cpu = {"value": round(1.0 / med, 4), "unit": "it/s", "cores": threads, "kind": "port",
+               "sample": f"{a.cpu_steps} oracle training steps (FP32, OpenMP) at N={N}, {W}x{H}, SH degree 3, "
+                         f"same state and views; median step time"}
+
+    out = {
+        "metric": "train it/s @1.5M splats 1600x1040",
+        "value": round(its, 3),
+        "unit": "it/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "single-view training step, 64-component clustered mixture",
+                   "gaussians": N, "width": W, "height": H, "fx": a.fx, "cameras": len(cams),
+                   "hidden_gt_gaussians": a.hidden, "sh_degree": 3, "loss": "0.8 L1 + 0.2 (1 - SSIM)",
+                   "intersections": I, "visible": V, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "stages_ms": {k: round(v, 4) for k, v in stages.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

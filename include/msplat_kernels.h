@@ -1,0 +1,97 @@
+/* msplat_kernels.h — kernel-level C ABI of the MI355X engine.
+ *
+ * This is the internal boundary the reference calls bindings.h (core/metal/bindings.h:20-112):
+ * the host loop hands device-resident parameter and optimiser arrays to one fused training
+ * step or one forward render. Here it is exported as plain C over device pointers so that
+ * callers that own their memory (tests, the Python bindings, other frameworks) can drive
+ * the same kernels the C API (msplat_c_api.h) drives.
+ *
+ * All pointers in MsplatDeviceState are device pointers to row-major float32 arrays of
+ * N rows: means [N,3], scales [N,3] (log), quats [N,4] (w,x,y,z), f_dc [N,3],
+ * f_rest [N,(deg+1)^2-1,3], opac [N] (logit); m[g], v[g] are the Adam moments of group g
+ * in the order means, scales, quats, f_dc, f_rest, opacity (same shapes). The work is
+ * enqueued on the engine's own HIP stream; msplat_engine_sync() waits for it.
+ */
+#ifndef MSPLAT_KERNELS_H
+#define MSPLAT_KERNELS_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "msplat_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int32_t N, degree;
+    float *means, *scales, *quats, *f_dc, *f_rest, *opac;
+    float* m[6];
+    float* v[6];
+} MsplatDeviceState;
+
+typedef struct MsplatEngine MsplatEngine;
+
+/* Creates an engine bound to the current HIP device (buffers grow on demand and are
+ * reused across steps). Replaces the reference's global MetalContext + FusedTensorCache
+ * (core/metal/msplat_metal.mm:41-183, 492-625). NULL if no HIP device is usable. */
+MsplatEngine* msplat_engine_create(void);
+void msplat_engine_destroy(MsplatEngine* e);
+
+/* One training step: msplat_train_step (reference bindings.h:71-95, msplat_metal.mm:1118-1740)
+ * followed by the SGLD update of msplat_sgld_noise_gen / msplat_sgld_noise
+ * (bindings.h:98-105) when hp->sgld_enable. gt is a device uint8 [H,W,3] image.
+ * debug != 0 additionally keeps the per-Gaussian gradients fed to Adam.
+ * Returns 0, or a negative error code (-1 bad arguments, -2 HIP error). */
+int msplat_engine_train_step(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam,
+                             const uint8_t* gt, const MsplatStepParams* hp, int debug);
+
+/* Forward render only: msplat_render (reference bindings.h:58-67). The image is left in
+ * the engine buffer "img" (device float [H,W,3]). */
+int msplat_engine_render(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam,
+                         int degrees_to_use, const float background[3]);
+
+/* Device pointer and size of an internal buffer of the last step/render: "radii",
+ * "tiles_hit", "depths", "rect", "aabb", "recA", "recB", "recC", "tile_counts",
+ * "tile_bins", "pkA", "pkB", "pkC", "img", "final_T", "final_idx", "v_rendered", "v_xy",
+ * "v_conic", "v_rgb", "v_opac", "dbg_v_mean", "dbg_v_scale", "dbg_v_quat", "dbg_v_opac",
+ * "nan_counts". Returns 0 on success, -1 for an unknown name. Replaces the reference's
+ * diagnostics msplat_last_tile_counts / msplat_dump_step_buffers (msplat_metal.mm:633-736). */
+int msplat_engine_buffer(MsplatEngine* e, const char* name, void** dptr, size_t* bytes);
+
+/* Copies min(bytes, size of the buffer) bytes of an internal buffer into host memory,
+ * synchronising with the engine stream. Returns the number of bytes copied, or -1. */
+int64_t msplat_engine_read(MsplatEngine* e, const char* name, void* host, size_t bytes);
+
+/* Waits for the engine's stream. Returns 0 or -2 on a HIP error. */
+int msplat_engine_sync(MsplatEngine* e);
+
+/* After msplat_engine_sync: out[0] = intersections I, out[1] = padded key slots,
+ * out[2] = key-buffer overflow flag, out[3] = largest tile count, out[4] = K (depth chunks),
+ * out[5] = number of tiles. */
+int msplat_engine_stats(MsplatEngine* e, int64_t out[6]);
+
+/* Loss of the last training step (sum of the per-pixel losses / (H W)), synchronising. */
+double msplat_engine_loss(MsplatEngine* e);
+
+/* The engine's HIP stream (hipStream_t as void*). */
+void* msplat_engine_stream(MsplatEngine* e);
+
+/* Time of the stream's kernels between the two most recent msplat_engine_mark() calls,
+ * in milliseconds (HIP events recorded on the engine stream). */
+void msplat_engine_mark(MsplatEngine* e);
+float msplat_engine_elapsed_ms(MsplatEngine* e);
+
+/* Per-kernel timing: when enabled, HIP events bracket each stage of each step on the
+ * engine stream; msplat_engine_stage_times fills ms[k] with the accumulated time of stage
+ * k since the last reset and returns the number of stages (names via
+ * msplat_engine_stage_name). */
+void msplat_engine_set_stage_timing(MsplatEngine* e, int enable);
+int msplat_engine_stage_times(MsplatEngine* e, double* ms, int max_stages, int64_t* launches);
+const char* msplat_engine_stage_name(int k);
+void msplat_engine_reset_stage_times(MsplatEngine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

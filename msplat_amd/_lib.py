@@ -1,0 +1,123 @@
+"""ctypes binding of libmsplat_core.so (the C ABIs in include/*.h).
+
+The library is built in-tree by msplat_amd.build; importing this module without it fails
+loudly: there is no CPU fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmsplat_core.so")
+
+
+class MsplatCamera(C.Structure):
+    _fields_ = [
+        ("viewmat", C.c_float * 16),
+        ("projmat", C.c_float * 16),
+        ("fx", C.c_float), ("fy", C.c_float), ("cx", C.c_float), ("cy", C.c_float),
+        ("cam_pos", C.c_float * 3),
+        ("width", C.c_int32), ("height", C.c_int32),
+    ]
+
+
+class MsplatStepParams(C.Structure):
+    _fields_ = [
+        ("step_size", C.c_float * 6),
+        ("bc2_sqrt", C.c_float * 6),
+        ("beta1", C.c_float), ("beta2", C.c_float), ("eps", C.c_float),
+        ("ssim_weight", C.c_float),
+        ("loss_inv_n", C.c_float),
+        ("op_reg", C.c_float), ("sc_reg", C.c_float),
+        ("reg_invis_scale", C.c_float),
+        ("background", C.c_float * 3),
+        ("noise_lr", C.c_float), ("xyz_lr", C.c_float), ("max_disp", C.c_float),
+        ("sgld_seed", C.c_uint32),
+        ("sgld_enable", C.c_int32),
+        ("degree", C.c_int32),
+        ("degrees_to_use", C.c_int32),
+    ]
+
+
+class MsplatDeviceState(C.Structure):
+    _fields_ = [
+        ("N", C.c_int32), ("degree", C.c_int32),
+        ("means", C.c_void_p), ("scales", C.c_void_p), ("quats", C.c_void_p),
+        ("f_dc", C.c_void_p), ("f_rest", C.c_void_p), ("opac", C.c_void_p),
+        ("m", C.c_void_p * 6), ("v", C.c_void_p * 6),
+    ]
+
+
+class MsplatConfig(C.Structure):
+    """Layout of the reference's MsplatConfig (core/include/msplat_c_api.h:16-30)."""
+    _fields_ = [
+        ("iterations", C.c_int), ("shDegree", C.c_int), ("shDegreeInterval", C.c_int),
+        ("ssimWeight", C.c_float), ("capMax", C.c_int), ("noiseLr", C.c_float),
+        ("opacityReg", C.c_float), ("scaleReg", C.c_float), ("cullRadius", C.c_float),
+        ("keepCrs", C.c_bool), ("downscaleFactor", C.c_float), ("bgColor", C.c_float * 3),
+    ]
+
+
+class MsplatStats(C.Structure):
+    _fields_ = [("iteration", C.c_int), ("splatCount", C.c_int), ("msPerStep", C.c_float)]
+
+
+class MsplatEvalMetrics(C.Structure):
+    _fields_ = [("psnr", C.c_float), ("ssim", C.c_float), ("l1", C.c_float), ("numTest", C.c_int),
+                ("numGaussians", C.c_int)]
+
+
+class MsplatPixelBuffer(C.Structure):
+    _fields_ = [("data", C.POINTER(C.c_float)), ("width", C.c_int), ("height", C.c_int)]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP extension first (python -m msplat_amd.build)")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, i64, f32, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double
+    fp = C.POINTER(C.c_float)
+    sig = {
+        # kernel-level ABI (msplat_kernels.h)
+        "msplat_engine_create": ([], vp),
+        "msplat_engine_destroy": ([vp], None),
+        "msplat_engine_train_step": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), vp,
+                                      C.POINTER(MsplatStepParams), C.c_int], C.c_int),
+        "msplat_engine_render": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), C.c_int, fp], C.c_int),
+        "msplat_engine_buffer": ([vp, C.c_char_p, C.POINTER(vp), C.POINTER(C.c_size_t)], C.c_int),
+        "msplat_engine_sync": ([vp], C.c_int),
+        "msplat_engine_read": ([vp, C.c_char_p, vp, C.c_size_t], i64),
+        "msplat_engine_stats": ([vp, C.POINTER(i64)], C.c_int),
+        "msplat_engine_loss": ([vp], f64),
+        "msplat_engine_stream": ([vp], vp),
+        "msplat_engine_mark": ([vp], None),
+        "msplat_engine_elapsed_ms": ([vp], f32),
+        "msplat_engine_set_stage_timing": ([vp, C.c_int], None),
+        "msplat_engine_stage_times": ([vp, C.POINTER(f64), C.c_int, C.POINTER(i64)], C.c_int),
+        "msplat_engine_stage_name": ([C.c_int], C.c_char_p),
+        "msplat_engine_reset_stage_times": ([vp], None),
+        # scene helpers (msplat_api_ext.h)
+        "msplat_camera_prepare": ([fp, f32, f32, f32, f32, C.c_int, C.c_int, C.POINTER(MsplatCamera)], None),
+        "msplat_synth_gaussians": ([C.c_int, C.c_uint64, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp], C.c_int),
+        "msplat_synth_cameras": ([C.c_uint64, C.c_int, f32, f32, fp], C.c_int),
+        "msplat_quantize_rgb": ([fp, i64, C.POINTER(C.c_uint8)], None),
+    }
+    for name, (argtypes, restype) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = argtypes
+        fn.restype = restype
+    _lib = L
+    return L
+
+
+def fptr(a):
+    """float* of a C-contiguous float32 numpy array."""
+    return a.ctypes.data_as(C.POINTER(C.c_float))

@@ -1,0 +1,405 @@
+// engine.cpp — the per-step dispatcher. Replaces the reference's Metal dispatcher
+// msplat_train_step / forward_pipeline (core/metal/msplat_metal.mm:803-1097, 1118-1740):
+// buffers are cached and grown on demand (the FusedTensorCache role, mm:492-625), the whole
+// step is enqueued on one HIP stream with no host synchronisation, and accumulators are
+// cleared by fill kernels (safe under graph capture) instead of blits.
+#include "engine.h"
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+
+namespace msplat_hip {
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        std::string msg = std::string("HIP error in ") + what + ": " + hipGetErrorString(e);
+        throw std::runtime_error(msg);
+    }
+}
+
+void DBuf::ensure(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    if (bytes <= cap) return;
+    if (p) hip_check(hipFree(p), "hipFree");
+    p = nullptr;
+    cap = 0;
+    size_t rounded = (bytes + 4095) & ~(size_t)4095;
+    hip_check(hipMalloc(&p, rounded), "hipMalloc");
+    cap = rounded;
+}
+
+void DBuf::release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    cap = 0;
+}
+
+static const char* kStageNames[ST_COUNT] = {"zero_grads", "project_sh_fwd", "isect_sort_pack", "raster_fwd",
+                                            "ssim_loss", "raster_bwd", "sh_rest_adam", "gaussian_update"};
+
+Engine::Engine() {
+    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    hip_check(hipEventCreate(&mark_[0]), "hipEventCreate");
+    hip_check(hipEventCreate(&mark_[1]), "hipEventCreate");
+    nanc_.ensure(64);
+    hip_check(hipMemset(nanc_.p, 0, 64), "hipMemset");
+    totals_.ensure(64);
+    loss_.ensure(64);
+    hip_check(hipMemset(loss_.p, 0, 64), "hipMemset");
+}
+
+Engine::~Engine() {
+    if (stream_) hipStreamSynchronize(stream_);
+    for (auto e : ev_pool_) hipEventDestroy(e);
+    hipEventDestroy(mark_[0]);
+    hipEventDestroy(mark_[1]);
+    DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
+                   &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &wctr_, &exact_,
+                   &sortoff_, &bins_, &totals_, &big_, &keys_, &pkA_, &pkB_, &pkC_, &img_, &finalT_, &finalIdx_,
+                   &vrend_, &hbuf_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
+                   &afterC_, &nanc_};
+    for (DBuf* b : all) b->release();
+    if (stream_) hipStreamDestroy(stream_);
+}
+
+// K_max rule of the depth-chunked raster path (reference msplat_metal.mm:1204-1224).
+static uint32_t k_max_rule(int64_t capacity, int num_tiles) {
+    if (num_tiles >= 400) return 1;
+    uint32_t avg = (uint32_t)(capacity / (num_tiles > 1 ? num_tiles : 1));
+    uint32_t cons = avg * 6u;
+    uint32_t K = (cons + msk::CHUNK - 1) / msk::CHUNK;
+    if (K < 2) K = 2;
+    uint32_t abs_max = (uint32_t)((capacity + msk::CHUNK - 1) / msk::CHUNK);
+    if (K > abs_max) K = abs_max;
+    return K;
+}
+
+void Engine::prepare(int N, int W, int H, int nb3, bool train) {
+    (void)nb3;
+    N_ = N;
+    W_ = W;
+    H_ = H;
+    tbx_ = (W + msk::TILE - 1) / msk::TILE;
+    tby_ = (H + msk::TILE - 1) / msk::TILE;
+    const int T = tbx_ * tby_;
+    const size_t n = (size_t)(N > 0 ? N : 1), P = (size_t)W * H;
+    capacity_ = (int64_t)N * 48;  // 48 N slots (reference msplat_metal.mm:518, 1142)
+    if (capacity_ < 64) capacity_ = 64;
+    K_ = k_max_rule(capacity_, T);
+    radii_.ensure(n * 4); tiles_.ensure(n * 4); depths_.ensure(n * 4); rect_.ensure(n * 8); aabb_.ensure(n * 8);
+    recA_.ensure(n * 16); recB_.ensure(n * 16); recC_.ensure(n * 8);
+    counts_.ensure((size_t)T * 4); wctr_.ensure((size_t)T * 4); exact_.ensure((size_t)T * 4);
+    sortoff_.ensure((size_t)(T + 1) * 4); bins_.ensure((size_t)T * 8); big_.ensure((size_t)T * 4);
+    keys_.ensure((size_t)capacity_ * 8);
+    pkA_.ensure((size_t)capacity_ * 16); pkB_.ensure((size_t)capacity_ * 16); pkC_.ensure((size_t)capacity_ * 8);
+    img_.ensure(P * 12); finalT_.ensure(P * 4); finalIdx_.ensure(P * 4);
+    if (K_ > 1) {
+        chunkT_.ensure((size_t)K_ * P * 4); chunkC_.ensure((size_t)K_ * P * 12); chunkIdx_.ensure((size_t)K_ * P * 4);
+        if (train) { prefixT_.ensure((size_t)K_ * P * 4); afterC_.ensure((size_t)K_ * P * 12); }
+    }
+    if (train) {
+        v_xy_.ensure(n * 8); v_conic_.ensure(n * 12); v_rgb_.ensure(n * 12); v_opac_.ensure(n * 4);
+        vrend_.ensure(P * 12); hbuf_.ensure(P * 60); dbuf_.ensure(P * 36);
+        const size_t nblk = (size_t)((W + 15) / 16) * ((H + 15) / 16);
+        partials_.ensure(nblk * 4);
+    }
+}
+
+void Engine::stage_event(int stage_end) {
+    if (!stage_timing_) return;
+    if (ev_used_ >= ev_pool_.size()) {
+        if (ev_pool_.size() >= 8192) {
+            flush_events();
+        } else {
+            hipEvent_t ev;
+            hip_check(hipEventCreate(&ev), "hipEventCreate");
+            ev_pool_.push_back(ev);
+            ev_stage_.push_back(-1);
+        }
+    }
+    hip_check(hipEventRecord(ev_pool_[ev_used_], stream_), "hipEventRecord");
+    ev_stage_[ev_used_] = stage_end;
+    ev_used_++;
+}
+
+void Engine::flush_events() {
+    if (ev_used_ == 0) return;
+    hip_check(hipEventSynchronize(ev_pool_[ev_used_ - 1]), "hipEventSynchronize");
+    for (size_t k = 1; k < ev_used_; k++) {
+        int st = ev_stage_[k];
+        if (st < 0) continue;
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, ev_pool_[k - 1], ev_pool_[k]), "hipEventElapsedTime");
+        stage_ms_[st] += ms;
+        stage_n_[st] += 1;
+    }
+    ev_used_ = 0;
+}
+
+void Engine::forward(const msk::Params& P, const MsplatCamera& cam, int degrees_to_use, const float bg[3]) {
+    msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
+                    aabb_.as<float2>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
+    msk::project_forward(stream_, P, cam, degrees_to_use, pj, nanc_.as<uint32_t>());
+    stage_event(ST_PROJECT);
+    msk::ListBufs L{counts_.as<uint32_t>(), wctr_.as<uint32_t>(), exact_.as<uint32_t>(), sortoff_.as<uint32_t>(),
+                    bins_.as<int2>(), keys_.as<uint64_t>(), {pkA_.as<float4>(), pkB_.as<float4>(), pkC_.as<float2>()},
+                    totals_.as<uint32_t>(), capacity_, capacity_, big_.as<uint32_t>()};
+    msk::build_lists(stream_, P.N, pj, tbx_, tby_, L, nanc_.as<uint32_t>());
+    stage_event(ST_LISTS);
+    msk::RasterBufs R{img_.as<float>(), finalT_.as<float>(), finalIdx_.as<int32_t>(), chunkT_.as<float>(),
+                      chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
+    msk::raster_forward(stream_, L, W_, H_, tbx_, tby_, bg, R);
+    stage_event(ST_RASTER_FWD);
+}
+
+static msk::Params to_params(const MsplatDeviceState& st) {
+    msk::Params P;
+    P.means = st.means; P.scales = st.scales; P.quats = st.quats; P.f_dc = st.f_dc; P.f_rest = st.f_rest;
+    P.opac = st.opac;
+    for (int g = 0; g < 6; g++) { P.m[g] = st.m[g]; P.v[g] = st.v[g]; }
+    P.N = st.N;
+    P.degree = st.degree;
+    return P;
+}
+
+static int nb3_of(int degree) { return degree <= 0 ? 0 : (degree == 1 ? 9 : degree == 2 ? 24 : 45); }
+
+int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt,
+                       const MsplatStepParams& hp, bool debug) {
+    if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0 || !gt) return -1;
+    if (cam.width > 65535 * 16 || cam.height > 65535 * 16) return -1;
+    const msk::Params P = to_params(st);
+    prepare(st.N, cam.width, cam.height, nb3_of(st.degree), true);
+    last_W = cam.width;
+    last_H = cam.height;
+    const size_t n = (size_t)(st.N > 0 ? st.N : 1);
+    stage_event(-1);
+    // accumulators of the raster backward (the reference blit-zeroes them, mm:1496-1512)
+    msk::fill_zero(stream_, v_xy_.p, n * 8);
+    msk::fill_zero(stream_, v_conic_.p, n * 12);
+    msk::fill_zero(stream_, v_rgb_.p, n * 12);
+    msk::fill_zero(stream_, v_opac_.p, n * 4);
+    stage_event(ST_ZERO);
+    forward(P, cam, hp.degrees_to_use, hp.background);
+    msk::ssim_loss(stream_, W_, H_, img_.as<float>(), gt, hp.ssim_weight, hp.loss_inv_n, hbuf_.as<float>(),
+                   dbuf_.as<float>(), vrend_.as<float>(), partials_.as<float>(), loss_.as<double>(),
+                   nanc_.as<uint32_t>());
+    stage_event(ST_SSIM);
+    msk::RasterBufs R{img_.as<float>(), finalT_.as<float>(), finalIdx_.as<int32_t>(), chunkT_.as<float>(),
+                      chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
+    msk::ListBufs L{counts_.as<uint32_t>(), wctr_.as<uint32_t>(), exact_.as<uint32_t>(), sortoff_.as<uint32_t>(),
+                    bins_.as<int2>(), keys_.as<uint64_t>(), {pkA_.as<float4>(), pkB_.as<float4>(), pkC_.as<float2>()},
+                    totals_.as<uint32_t>(), capacity_, capacity_, big_.as<uint32_t>()};
+    msk::GradBufs G{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
+    msk::raster_backward(stream_, L, R, W_, H_, tbx_, tby_, hp.background, vrend_.as<float>(), G);
+    stage_event(ST_RASTER_BWD);
+    msk::sh_rest_adam(stream_, P, cam, radii_.as<int32_t>(), v_rgb_.as<float>(), hp);
+    stage_event(ST_SH_ADAM);
+    msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
+                    aabb_.as<float2>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
+    float *dvm = nullptr, *dvs = nullptr, *dvq = nullptr, *dvo = nullptr;
+    if (debug) {
+        dbg_vm_.ensure(n * 12); dbg_vs_.ensure(n * 12); dbg_vq_.ensure(n * 16); dbg_vo_.ensure(n * 4);
+        dvm = dbg_vm_.as<float>(); dvs = dbg_vs_.as<float>(); dvq = dbg_vq_.as<float>(); dvo = dbg_vo_.as<float>();
+    }
+    debug_valid_ = debug;
+    msk::gaussian_update(stream_, P, cam, pj, G, hp, nanc_.as<uint32_t>(), dvm, dvs, dvq, dvo);
+    stage_event(ST_UPDATE);
+    hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : -2;
+}
+
+int Engine::render(const MsplatDeviceState& st, const MsplatCamera& cam, int degrees_to_use, const float bg[3]) {
+    if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0) return -1;
+    const msk::Params P = to_params(st);
+    prepare(st.N, cam.width, cam.height, nb3_of(st.degree), false);
+    last_W = cam.width;
+    last_H = cam.height;
+    stage_event(-1);
+    forward(P, cam, degrees_to_use, bg);
+    hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : -2;
+}
+
+int Engine::buffer(const std::string& name, void** p, size_t* bytes) {
+    const size_t n = (size_t)N_, P = (size_t)W_ * H_, T = (size_t)tbx_ * tby_;
+    struct Ent { const char* nm; DBuf* b; size_t sz; };
+    const Ent ents[] = {
+        {"radii", &radii_, n * 4}, {"tiles_hit", &tiles_, n * 4}, {"depths", &depths_, n * 4}, {"rect", &rect_, n * 8},
+        {"aabb", &aabb_, n * 8}, {"recA", &recA_, n * 16}, {"recB", &recB_, n * 16}, {"recC", &recC_, n * 8},
+        {"tile_counts", &counts_, T * 4}, {"tile_bins", &bins_, T * 8}, {"sort_off", &sortoff_, (T + 1) * 4},
+        {"pkA", &pkA_, (size_t)capacity_ * 16}, {"pkB", &pkB_, (size_t)capacity_ * 16},
+        {"pkC", &pkC_, (size_t)capacity_ * 8}, {"img", &img_, P * 12}, {"final_T", &finalT_, P * 4},
+        {"final_idx", &finalIdx_, P * 4}, {"v_rendered", &vrend_, P * 12}, {"v_xy", &v_xy_, n * 8},
+        {"v_conic", &v_conic_, n * 12}, {"v_rgb", &v_rgb_, n * 12}, {"v_opac", &v_opac_, n * 4},
+        {"dbg_v_mean", &dbg_vm_, n * 12}, {"dbg_v_scale", &dbg_vs_, n * 12}, {"dbg_v_quat", &dbg_vq_, n * 16},
+        {"dbg_v_opac", &dbg_vo_, n * 4}, {"nan_counts", &nanc_, 64}, {"totals", &totals_, 16},
+    };
+    for (const Ent& e : ents) {
+        if (name == e.nm) {
+            if (!e.b->p) return -1;
+            *p = e.b->p;
+            *bytes = e.sz;
+            return 0;
+        }
+    }
+    return -1;
+}
+
+int Engine::sync() { return hipStreamSynchronize(stream_) == hipSuccess ? 0 : -2; }
+
+void Engine::stats(int64_t out[6]) {
+    uint32_t t[4] = {0, 0, 0, 0};
+    if (totals_.p) hipMemcpyAsync(t, totals_.p, 16, hipMemcpyDeviceToHost, stream_);
+    hipStreamSynchronize(stream_);
+    out[0] = t[0];
+    out[1] = t[1];
+    out[2] = t[2];
+    out[3] = t[3];
+    out[4] = K_;
+    out[5] = (int64_t)tbx_ * tby_;
+}
+
+double Engine::loss() {
+    double l = 0.0;
+    hipMemcpyAsync(&l, loss_.p, 8, hipMemcpyDeviceToHost, stream_);
+    hipStreamSynchronize(stream_);
+    return (W_ > 0 && H_ > 0) ? l / ((double)W_ * H_) : 0.0;
+}
+
+void Engine::mark() {
+    hipEventRecord(mark_[mark_n_ & 1], stream_);
+    mark_n_++;
+}
+
+float Engine::elapsed_ms() {
+    if (mark_n_ < 2) return 0.f;
+    hipEvent_t a = mark_[(mark_n_ - 2) & 1], b = mark_[(mark_n_ - 1) & 1];
+    hipEventSynchronize(b);
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+void Engine::set_stage_timing(bool on) {
+    if (!on) flush_events();
+    stage_timing_ = on;
+}
+
+int Engine::stage_times(double* ms, int max_stages, int64_t* launches) {
+    flush_events();
+    int n = max_stages < ST_COUNT ? max_stages : ST_COUNT;
+    for (int k = 0; k < n; k++) {
+        ms[k] = stage_ms_[k];
+        if (launches) launches[k] = stage_n_[k];
+    }
+    return ST_COUNT;
+}
+
+void Engine::reset_stage_times() {
+    flush_events();
+    for (int k = 0; k < ST_COUNT; k++) { stage_ms_[k] = 0; stage_n_[k] = 0; }
+}
+
+}  // namespace msplat_hip
+
+// ---------------------------------------------------------------------------------------
+// extern "C" kernel-level ABI (include/msplat_kernels.h)
+using msplat_hip::Engine;
+
+struct MsplatEngine {
+    Engine* e;
+};
+
+extern "C" {
+
+MsplatEngine* msplat_engine_create(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return nullptr;
+    try {
+        MsplatEngine* h = new MsplatEngine;
+        h->e = new Engine();
+        return h;
+    } catch (const std::exception& ex) {
+        fprintf(stderr, "msplat: engine creation failed: %s\n", ex.what());
+        return nullptr;
+    }
+}
+
+void msplat_engine_destroy(MsplatEngine* e) {
+    if (!e) return;
+    delete e->e;
+    delete e;
+}
+
+int msplat_engine_train_step(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam, const uint8_t* gt,
+                             const MsplatStepParams* hp, int debug) {
+    if (!e || !st || !cam || !hp) return -1;
+    try {
+        return e->e->train_step(*st, *cam, gt, *hp, debug != 0);
+    } catch (const std::exception& ex) {
+        fprintf(stderr, "msplat: %s\n", ex.what());
+        return -2;
+    }
+}
+
+int msplat_engine_render(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam, int degrees_to_use,
+                         const float background[3]) {
+    if (!e || !st || !cam || !background) return -1;
+    try {
+        return e->e->render(*st, *cam, degrees_to_use, background);
+    } catch (const std::exception& ex) {
+        fprintf(stderr, "msplat: %s\n", ex.what());
+        return -2;
+    }
+}
+
+int msplat_engine_buffer(MsplatEngine* e, const char* name, void** dptr, size_t* bytes) {
+    if (!e || !name || !dptr || !bytes) return -1;
+    return e->e->buffer(name, dptr, bytes);
+}
+
+int64_t msplat_engine_read(MsplatEngine* e, const char* name, void* host, size_t bytes) {
+    if (!e || !name || !host) return -1;
+    void* d = nullptr;
+    size_t sz = 0;
+    if (e->e->buffer(name, &d, &sz) != 0) return -1;
+    size_t n = bytes < sz ? bytes : sz;
+    if (hipMemcpyAsync(host, d, n, hipMemcpyDeviceToHost, e->e->stream()) != hipSuccess) return -1;
+    if (hipStreamSynchronize(e->e->stream()) != hipSuccess) return -1;
+    return (int64_t)n;
+}
+
+int msplat_engine_sync(MsplatEngine* e) { return e ? e->e->sync() : -1; }
+
+int msplat_engine_stats(MsplatEngine* e, int64_t out[6]) {
+    if (!e || !out) return -1;
+    e->e->stats(out);
+    return 0;
+}
+
+double msplat_engine_loss(MsplatEngine* e) { return e ? e->e->loss() : 0.0; }
+
+void* msplat_engine_stream(MsplatEngine* e) { return e ? (void*)e->e->stream() : nullptr; }
+
+void msplat_engine_mark(MsplatEngine* e) {
+    if (e) e->e->mark();
+}
+float msplat_engine_elapsed_ms(MsplatEngine* e) { return e ? e->e->elapsed_ms() : 0.f; }
+
+void msplat_engine_set_stage_timing(MsplatEngine* e, int enable) {
+    if (e) e->e->set_stage_timing(enable != 0);
+}
+int msplat_engine_stage_times(MsplatEngine* e, double* ms, int max_stages, int64_t* launches) {
+    return e ? e->e->stage_times(ms, max_stages, launches) : 0;
+}
+const char* msplat_engine_stage_name(int k) {
+    return (k >= 0 && k < msplat_hip::ST_COUNT) ? msplat_hip::kStageNames[k] : "";
+}
+void msplat_engine_reset_stage_times(MsplatEngine* e) {
+    if (e) e->e->reset_stage_times();
+}
+
+}  // extern "C"

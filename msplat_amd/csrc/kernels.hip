@@ -1,0 +1,1439 @@
+// kernels.hip — gfx950 kernels of the training step.
+//
+// One wave = 64 lanes. Per-Gaussian kernels stream AoS parameter rows; tile kernels run one
+// 256-thread workgroup (4 waves) per 16x16 binning tile so every tile list is read from
+// HBM once per pass and broadcast from LDS; per-entry gradient sums use DPP wave
+// reductions and one global atomic per (tile, entry). Float arithmetic of every stage that
+// feeds an integer output (projection -> tile rectangles / sort keys, compositing ->
+// contributor indices) follows the reference's operation order exactly, in IEEE-754
+// single precision without contraction, so those integers match a CPU evaluation of the
+// same formulas bit for bit.
+//
+// Reference kernels are cited as metal:<line> (core/metal/msplat_metal.metal).
+#include "kernels.h"
+#include "ms_math.h"
+
+namespace msk {
+
+__constant__ float c_gauss1d[11] = {0.0010283801f, 0.0075987581f, 0.0360007721f, 0.1093606895f,
+                                    0.2130055377f, 0.2660117249f, 0.2130055377f, 0.1093606895f,
+                                    0.0360007721f, 0.0075987581f, 0.0010283801f};
+
+#define SH_C0 0.28209479177387814f
+#define SH_C1 0.4886025119029199f
+#define SH_C2_0 1.0925484305920792f
+#define SH_C2_1 (-1.0925484305920792f)
+#define SH_C2_2 0.31539156525252005f
+#define SH_C2_3 (-1.0925484305920792f)
+#define SH_C2_4 0.5462742152960396f
+#define SH_C3_0 (-0.5900435899266435f)
+#define SH_C3_1 2.890611442640554f
+#define SH_C3_2 (-0.4570457994644658f)
+#define SH_C3_3 0.3731763325901154f
+#define SH_C3_4 (-0.4570457994644658f)
+#define SH_C3_5 1.445305721320277f
+#define SH_C3_6 (-0.5900435899266435f)
+
+static inline unsigned cdiv(size_t a, unsigned b) { return (unsigned)((a + b - 1) / b); }
+
+// ------------------------------------------------------------------------------------
+// Wave64 helpers.
+__device__ __forceinline__ float dpp_row_shr1(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x111, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_row_shr2(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x112, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_row_shr4(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x114, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_row_shr8(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x118, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float dpp_bcast15(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x142, 0xA, 0xF, false));
+}
+__device__ __forceinline__ float dpp_bcast31(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x143, 0xC, 0xF, false));
+}
+// Inclusive wave sum: the total ends in lane 63.
+__device__ __forceinline__ float wave_sum_to_lane63(float v) {
+    v += dpp_row_shr1(v);
+    v += dpp_row_shr2(v);
+    v += dpp_row_shr4(v);
+    v += dpp_row_shr8(v);
+    v += dpp_bcast15(v);
+    v += dpp_bcast31(v);
+    return v;
+}
+
+// Block-wide exclusive scan of uint32 over blockDim.x (multiple of 64, <= 1024) threads.
+__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_wave, uint32_t* total) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds_wave[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for (int w = 0; w < nw; w++) { uint32_t t = lds_wave[w]; lds_wave[w] = acc; acc += t; }
+        lds_wave[nw] = acc;
+    }
+    __syncthreads();
+    uint32_t r = x - v + lds_wave[wid];
+    *total = lds_wave[nw];
+    __syncthreads();
+    return r;
+}
+
+__global__ void k_fill_zero(uint32_t* p, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0u;
+}
+
+void fill_zero(hipStream_t s, void* p, size_t bytes) {
+    if (!p || bytes == 0) return;
+    size_t n = bytes / 4;
+    unsigned blocks = (unsigned)std::min<size_t>(4096, (n + 255) / 256);
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, s, (uint32_t*)p, n);
+}
+
+// ------------------------------------------------------------------------------------
+// A5: project_and_sh_forward (metal:1719-1834). One thread per Gaussian; the block's
+// features_rest rows are staged through LDS with 16-byte loads (rows are 180 B apart).
+constexpr int PROJ_BLOCK = 128;
+
+__device__ __forceinline__ void quat_rot(const float q[4], float R[3][3]) {
+    // quat_to_rotmat, metal:134-156 (R[r][c] row-major)
+    float n2 = q[3] * q[3] + q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    float s = 1.0f / sqrtf(fmaxf(n2, 1e-8f));
+    float w = q[0] * s, x = q[1] * s, y = q[2] * s, z = q[3] * s;
+    R[0][0] = 1.f - 2.f * (y * y + z * z);
+    R[1][0] = 2.f * (x * y + w * z);
+    R[2][0] = 2.f * (x * z - w * y);
+    R[0][1] = 2.f * (x * y - w * z);
+    R[1][1] = 1.f - 2.f * (x * x + z * z);
+    R[2][1] = 2.f * (y * z + w * x);
+    R[0][2] = 2.f * (x * z + w * y);
+    R[1][2] = 2.f * (y * z - w * x);
+    R[2][2] = 1.f - 2.f * (x * x + y * y);
+}
+
+__device__ __forceinline__ void cov3d_of(const float s[3], const float q[4], float cov[6]) {
+    float R[3][3];
+    quat_rot(q, R);
+    float M[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) M[r][c] = R[r][c] * s[c];
+    cov[0] = M[0][0] * M[0][0] + M[0][1] * M[0][1] + M[0][2] * M[0][2];
+    cov[1] = M[1][0] * M[0][0] + M[1][1] * M[0][1] + M[1][2] * M[0][2];
+    cov[2] = M[2][0] * M[0][0] + M[2][1] * M[0][1] + M[2][2] * M[0][2];
+    cov[3] = M[1][0] * M[1][0] + M[1][1] * M[1][1] + M[1][2] * M[1][2];
+    cov[4] = M[2][0] * M[1][0] + M[2][1] * M[1][1] + M[2][2] * M[1][2];
+    cov[5] = M[2][0] * M[2][0] + M[2][1] * M[2][1] + M[2][2] * M[2][2];
+}
+
+__global__ __launch_bounds__(PROJ_BLOCK) void k_project(
+    int N, const float* __restrict__ means, const float* __restrict__ scales, const float* __restrict__ quats,
+    const float* __restrict__ f_dc, const float* __restrict__ f_rest, const float* __restrict__ opac, int nb3,
+    int deg_use, MsplatCamera cam, int tbx, int tby, int32_t* __restrict__ radii_o, int32_t* __restrict__ tiles_o,
+    float* __restrict__ depth_o, ushort4* __restrict__ rect_o, float2* __restrict__ aabb_o, float4* __restrict__ rA,
+    float4* __restrict__ rB, float2* __restrict__ rC, uint32_t* __restrict__ nanc) {
+    extern __shared__ float s_rest[];  // PROJ_BLOCK * nb3 floats when deg_use > 0
+    const int i0 = blockIdx.x * PROJ_BLOCK;
+    const int i = i0 + threadIdx.x;
+    if (deg_use > 0 && nb3 > 0) {
+        const int rows = min(PROJ_BLOCK, N - i0);
+        const int nf = rows * nb3;
+        const float* src = f_rest + (size_t)i0 * nb3;  // 16-byte aligned: i0 * nb3 is a multiple of 4
+        const int n4 = nf >> 2;
+        const float4* src4 = reinterpret_cast<const float4*>(src);
+        float4* dst4 = reinterpret_cast<float4*>(s_rest);
+        for (int k = threadIdx.x; k < n4; k += PROJ_BLOCK) dst4[k] = src4[k];
+        for (int k = (n4 << 2) + threadIdx.x; k < nf; k += PROJ_BLOCK) s_rest[k] = src[k];
+        __syncthreads();
+    }
+    if (i >= N) return;
+
+    int32_t rad = 0, area = 0;
+    float depth = 0.f;
+    ushort4 rect = make_ushort4(0, 0, 0, 0);
+    float2 aabb = make_float2(0.f, 0.f);
+    float4 A = make_float4(0.f, 0.f, 0.f, 0.f), B = make_float4(0.f, 0.f, 0.f, 0.f);
+    float2 C = make_float2(0.f, __builtin_bit_cast(float, (uint32_t)i));
+
+    const float p0 = means[3 * (size_t)i], p1 = means[3 * (size_t)i + 1], p2 = means[3 * (size_t)i + 2];
+    const float* vm = cam.viewmat;
+    const float pvz = vm[8] * p0 + vm[9] * p1 + vm[10] * p2 + vm[11];
+    if (pvz > 0.01f) {  // clip_near_plane (metal:159-170), clip_thresh 0.01
+        const float pvx = vm[0] * p0 + vm[1] * p1 + vm[2] * p2 + vm[3];
+        const float pvy = vm[4] * p0 + vm[5] * p1 + vm[6] * p2 + vm[7];
+        float s[3], q[4];
+#pragma unroll
+        for (int k = 0; k < 3; k++) s[k] = ms_expf_core(fminf(fmaxf(scales[3 * (size_t)i + k], -15.f), 15.f));
+        const float4 q4 = reinterpret_cast<const float4*>(quats)[i];
+        q[0] = q4.x; q[1] = q4.y; q[2] = q4.z; q[3] = q4.w;
+        float cov[6];
+        cov3d_of(s, q, cov);
+        // project_cov3d_ewa (metal:266-306)
+        const float W = (float)cam.width, H = (float)cam.height;
+        const float fx = cam.fx, fy = cam.fy;
+        const float tan_fovx = 0.5f * W / fx, tan_fovy = 0.5f * H / fy;
+        const float lim_x = 1.3f * tan_fovx, lim_y = 1.3f * tan_fovy;
+        const float px = pvz * fminf(lim_x, fmaxf(-lim_x, pvx / pvz));
+        const float py = pvz * fminf(lim_y, fmaxf(-lim_y, pvy / pvz));
+        const float rz = 1.f / pvz;
+        const float rz2 = rz * rz;
+        const float j00 = fx * rz, j11 = fy * rz;
+        const float j20 = -fx * px * rz2, j21 = -fy * py * rz2;
+        float t0[3], t1[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            t0[k] = j00 * vm[0 + k] + j20 * vm[8 + k];
+            t1[k] = j11 * vm[4 + k] + j21 * vm[8 + k];
+        }
+        const float v00 = cov[0], v01 = cov[1], v02 = cov[2], v11 = cov[3], v12 = cov[4], v22 = cov[5];
+        const float tv00 = t0[0] * v00 + t0[1] * v01 + t0[2] * v02;
+        const float tv01 = t0[0] * v01 + t0[1] * v11 + t0[2] * v12;
+        const float tv02 = t0[0] * v02 + t0[1] * v12 + t0[2] * v22;
+        const float tv10 = t1[0] * v00 + t1[1] * v01 + t1[2] * v02;
+        const float tv11 = t1[0] * v01 + t1[1] * v11 + t1[2] * v12;
+        const float tv12 = t1[0] * v02 + t1[1] * v12 + t1[2] * v22;
+        const float c0 = (tv00 * t0[0] + tv01 * t0[1] + tv02 * t0[2]) + 0.3f;
+        const float c1 = tv00 * t1[0] + tv01 * t1[1] + tv02 * t1[2];
+        const float c2 = (tv10 * t1[0] + tv11 * t1[1] + tv12 * t1[2]) + 0.3f;
+        // compute_cov2d_bounds (metal:308-332)
+        const float det = c0 * c2 - c1 * c1;
+        if (det > 1e-6f) {
+            const float inv_det = 1.f / det;
+            const float cn0 = c2 * inv_det, cn1 = -c1 * inv_det, cn2 = c0 * inv_det;
+            const float b = 0.5f * (c0 + c2);
+            const float disc = sqrtf(fmaxf(0.1f, b * b - det));
+            const float radius = ceilf(3.f * sqrtf(b + disc));
+            // project_pix (metal:335-344)
+            const float* pm = cam.projmat;
+            const float hx = pm[0] * p0 + pm[1] * p1 + pm[2] * p2 + pm[3];
+            const float hy = pm[4] * p0 + pm[5] * p1 + pm[6] * p2 + pm[7];
+            const float hw = pm[12] * p0 + pm[13] * p1 + pm[14] * p2 + pm[15];
+            const float rw = 1.f / (hw + 1e-6f);
+            const float ndx = hx * rw, ndy = hy * rw;
+            const float cxp = 0.5f * W * ndx + cam.cx - 0.5f;
+            const float cyp = 0.5f * H * ndy + cam.cy - 0.5f;
+            const float ax = ceilf(3.f * sqrtf(c0));
+            const float ay = ceilf(3.f * sqrtf(c2));
+            // get_tile_bbox (metal:57-87)
+            const float tcx = cxp / 16.f, tcy = cyp / 16.f, trx = ax / 16.f, try_ = ay / 16.f;
+            const int x0 = min(max(0, ms_f2i(tcx - trx)), tbx);
+            const int x1 = min(max(0, ms_f2i(tcx + trx + 1.f)), tbx);
+            const int y0 = min(max(0, ms_f2i(tcy - try_)), tby);
+            const int y1 = min(max(0, ms_f2i(tcy + try_ + 1.f)), tby);
+            const int a = (int)((uint32_t)(x1 - x0) * (uint32_t)(y1 - y0));
+            if (a > 0) {
+                // SH colour (metal:1805-1811; sh_coeffs_to_color metal:620-687)
+                const float d0 = p0 - cam.cam_pos[0], d1 = p1 - cam.cam_pos[1], d2 = p2 - cam.cam_pos[2];
+                const float inv = 1.f / sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+                const float x = d0 * inv, y = d1 * inv, z = d2 * inv;
+                float col[3];
+#pragma unroll
+                for (int c = 0; c < 3; c++) col[c] = SH_C0 * f_dc[3 * (size_t)i + c];
+                if (deg_use >= 1) {
+                    const float* rs = s_rest + threadIdx.x * nb3;
+                    const float xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
+#pragma unroll
+                    for (int c = 0; c < 3; c++) {
+                        col[c] += SH_C1 * (-y * rs[0 * 3 + c] + z * rs[1 * 3 + c] - x * rs[2 * 3 + c]);
+                        if (deg_use < 2) continue;
+                        col[c] += (SH_C2_0 * xy * rs[3 * 3 + c] + SH_C2_1 * yz * rs[4 * 3 + c] +
+                                   SH_C2_2 * (2.f * zz - xx - yy) * rs[5 * 3 + c] + SH_C2_3 * xz * rs[6 * 3 + c] +
+                                   SH_C2_4 * (xx - yy) * rs[7 * 3 + c]);
+                        if (deg_use < 3) continue;
+                        col[c] += (SH_C3_0 * y * (3.f * xx - yy) * rs[8 * 3 + c] + SH_C3_1 * xy * z * rs[9 * 3 + c] +
+                                   SH_C3_2 * y * (4.f * zz - xx - yy) * rs[10 * 3 + c] +
+                                   SH_C3_3 * z * (2.f * zz - 3.f * xx - 3.f * yy) * rs[11 * 3 + c] +
+                                   SH_C3_4 * x * (4.f * zz - xx - yy) * rs[12 * 3 + c] +
+                                   SH_C3_5 * z * (xx - yy) * rs[13 * 3 + c] +
+                                   SH_C3_6 * x * (xx - 3.f * yy) * rs[14 * 3 + c]);
+                    }
+                }
+                // zombie gate (metal:1818-1833)
+                const bool ok_center = ms_finite(cxp) && ms_finite(cyp);
+                const bool ok_depth = ms_finite(pvz);
+                const bool ok_aabb = ms_finite(ax) && ms_finite(ay);
+                const bool ok_conic = ms_finite(cn0) && ms_finite(cn1) && ms_finite(cn2);
+                const bool ok_rgb = ms_finite(col[0]) && ms_finite(col[1]) && ms_finite(col[2]);
+                if (ok_center && ok_depth && ok_aabb && ok_conic && ok_rgb) {
+                    rad = ms_f2i(radius);
+                    area = a;
+                    depth = pvz;
+                    rect = make_ushort4((unsigned short)x0, (unsigned short)y0, (unsigned short)x1, (unsigned short)y1);
+                    aabb = make_float2(ax, ay);
+                    const float op = 1.f / (1.f + ms_expf(-opac[i]));  // pack-time sigmoid, metal:2563
+                    if (ms_finite(op)) {
+                        A = make_float4(cxp, cyp, op, cn0);
+                        B = make_float4(cn1, cn2, col[0], col[1]);
+                        C.x = col[2];
+                    } else {
+                        atomicAdd(&nanc[13], 1u);  // pack gate: transparent entry (metal:2570-2578)
+                        atomicAdd(&nanc[14], 1u);
+                    }
+                } else {
+                    atomicAdd(&nanc[7], 1u);
+                    if (!ok_center) atomicAdd(&nanc[8], 1u);
+                    if (!ok_depth) atomicAdd(&nanc[9], 1u);
+                    if (!ok_aabb) atomicAdd(&nanc[10], 1u);
+                    if (!ok_conic) atomicAdd(&nanc[11], 1u);
+                    if (!ok_rgb) atomicAdd(&nanc[12], 1u);
+                }
+            }
+        }
+    }
+    radii_o[i] = rad;
+    tiles_o[i] = area;
+    depth_o[i] = depth;
+    rect_o[i] = rect;
+    aabb_o[i] = aabb;
+    rA[i] = A;
+    rB[i] = B;
+    rC[i] = C;
+}
+
+void project_forward(hipStream_t s, const Params& P, const MsplatCamera& cam, int degrees_to_use, ProjOut& o,
+                     uint32_t* nan_counts) {
+    if (P.N <= 0) return;
+    const int nb3 = (P.degree <= 0 ? 0 : (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15)) * 3;
+    const int tbx = (cam.width + TILE - 1) / TILE, tby = (cam.height + TILE - 1) / TILE;
+    const int du = degrees_to_use < P.degree ? degrees_to_use : P.degree;
+    size_t lds = (du > 0 && nb3 > 0) ? (size_t)PROJ_BLOCK * nb3 * sizeof(float) : 0;
+    hipLaunchKernelGGL(k_project, dim3(cdiv(P.N, PROJ_BLOCK)), dim3(PROJ_BLOCK), lds, s, P.N, P.means, P.scales,
+                       P.quats, P.f_dc, P.f_rest, P.opac, nb3, du, cam, tbx, tby, o.radii, o.tiles_hit, o.depths,
+                       o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
+}
+
+// ------------------------------------------------------------------------------------
+// A6-A9: per-tile lists. count (metal:2342-2365), prefix_sum_tiles (metal:2376-2435),
+// scatter (metal:2440-2480), hybrid_sort_pack (metal:2489-2627).
+__global__ void k_count(int N, const int32_t* __restrict__ radii, const ushort4* __restrict__ rect, int tbx,
+                        uint32_t* __restrict__ counts) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || radii[i] <= 0) return;
+    const ushort4 r = rect[i];
+    for (int y = r.y; y < r.w; y++)
+        for (int x = r.x; x < r.z; x++) atomicAdd(&counts[y * tbx + x], 1u);
+}
+
+__device__ __forceinline__ uint32_t pad_count(uint32_t c) {
+    if (c == 0u) return 0u;
+    uint32_t x = c < 32u ? 32u : c;
+    return 1u << (32 - __clz((int)(x - 1u)));
+}
+
+__global__ __launch_bounds__(1024) void k_prefix(int T, const uint32_t* __restrict__ counts,
+                                                 uint32_t* __restrict__ exact_off, uint32_t* __restrict__ sort_off,
+                                                 int2* __restrict__ bins, uint32_t* __restrict__ totals,
+                                                 uint32_t key_cap, uint32_t* __restrict__ big_list, uint32_t cap) {
+    __shared__ uint32_t lw[17];
+    __shared__ uint32_t nbig;
+    if (threadIdx.x == 0) nbig = 0;
+    uint32_t run_ex = 0, run_pd = 0, mx = 0;
+    for (int c0 = 0; c0 < T; c0 += 1024) {
+        const int t = c0 + threadIdx.x;
+        const uint32_t val = t < T ? counts[t] : 0u;
+        const uint32_t pv = pad_count(val);
+        mx = max(mx, val);
+        uint32_t tot_e, tot_p;
+        const uint32_t ex = block_exclusive_scan(val, lw, &tot_e);
+        const uint32_t pd = block_exclusive_scan(pv, lw, &tot_p);
+        if (t < T) {
+            exact_off[t] = run_ex + ex;
+            sort_off[t] = run_pd + pd;
+            bins[t] = make_int2((int)min(run_ex + ex, cap), (int)min(run_ex + ex + val, cap));  // never past the packed buffers
+            if (val > (uint32_t)SORT_SMALL_CAP) big_list[atomicAdd(&nbig, 1u)] = (uint32_t)t;
+        }
+        run_ex += tot_e;
+        run_pd += tot_p;
+    }
+    // block max of counts
+    __shared__ uint32_t smax;
+    if (threadIdx.x == 0) smax = 0;
+    __syncthreads();
+    atomicMax(&smax, mx);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sort_off[T] = run_pd;
+        totals[0] = run_ex;
+        totals[1] = run_pd;
+        totals[2] = run_pd > key_cap ? 1u : 0u;
+        totals[3] = smax;
+        totals[4] = nbig;
+    }
+}
+
+__global__ void k_scatter(int N, const int32_t* __restrict__ radii, const ushort4* __restrict__ rect,
+                          const float* __restrict__ depths, int tbx, const uint32_t* __restrict__ sort_off,
+                          uint32_t* __restrict__ wctr, uint64_t* __restrict__ keys, uint32_t key_cap) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || radii[i] <= 0) return;
+    const ushort4 r = rect[i];
+    const uint64_t key = ((uint64_t)__builtin_bit_cast(uint32_t, depths[i]) << 32) | (uint64_t)(uint32_t)i;
+    for (int y = r.y; y < r.w; y++)
+        for (int x = r.x; x < r.z; x++) {
+            const int t = y * tbx + x;
+            const uint32_t pos = atomicAdd(&wctr[t], 1u);
+            const uint32_t dst = sort_off[t] + pos;
+            if (dst < key_cap) keys[dst] = key;  // overflow dropped (metal:2474-2477)
+        }
+}
+
+__device__ __forceinline__ void pack_entry(uint64_t key, uint32_t d, int N, const Records& rec, const Records& pk) {
+    const uint32_t g = (uint32_t)(key & 0xFFFFFFFFull);
+    if (g >= (uint32_t)N) {  // out-of-range id -> transparent (metal:2554-2560)
+        pk.A[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pk.B[d] = make_float4(0.f, 0.f, 0.f, 0.f);
+        pk.C[d] = make_float2(0.f, 0.f);
+        return;
+    }
+    pk.A[d] = rec.A[g];
+    pk.B[d] = rec.B[g];
+    pk.C[d] = rec.C[g];
+}
+
+// Bitonic network over n (power of two) keys in LDS, n/2 comparators per stage. Stages whose
+// partner distance j is < 64 touch only elements owned by pairs of one wave when each
+// wave owns a contiguous run of comparators, but the generic form below keeps one barrier
+// per stage for clarity and robustness.
+template <int NT>
+__device__ __forceinline__ void bitonic_lds(uint64_t* a, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = threadIdx.x; p < (n >> 1); p += NT) {
+                const uint32_t lo = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+                const uint32_t hi = lo + j;
+                const bool asc = (lo & k) == 0;
+                const uint64_t x = a[lo], y = a[hi];
+                if ((x > y) == asc) { a[lo] = y; a[hi] = x; }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+template <int NT>
+__device__ __forceinline__ void bitonic_global(uint64_t* a, uint32_t n) {
+    for (uint32_t k = 2; k <= n; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = threadIdx.x; p < (n >> 1); p += NT) {
+                const uint32_t lo = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+                const uint32_t hi = lo + j;
+                const bool asc = (lo & k) == 0;
+                const uint64_t x = a[lo], y = a[hi];
+                if ((x > y) == asc) { a[lo] = y; a[hi] = x; }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+constexpr int SORT_SMALL_THREADS = 256;
+
+__global__ __launch_bounds__(SORT_SMALL_THREADS) void k_sort_small(
+    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ sort_off, const uint32_t* __restrict__ exact_off,
+    const uint64_t* __restrict__ keys, Records rec, Records pk, int N, uint32_t capacity, uint32_t key_cap) {
+    __shared__ uint64_t s[SORT_SMALL_CAP];
+    const int t = blockIdx.x;
+    const uint32_t count = counts[t];
+    if (count == 0 || count > (uint32_t)SORT_SMALL_CAP) return;
+    const uint32_t src = sort_off[t], padded = sort_off[t + 1] - src, dst = exact_off[t];
+    if (!(sort_off[t + 1] <= key_cap && dst + count <= capacity)) {
+        for (uint32_t i = threadIdx.x; i < count; i += SORT_SMALL_THREADS)
+            if (dst + i < capacity) pack_entry(~0ull, dst + i, 0, rec, pk);
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < padded; i += SORT_SMALL_THREADS) s[i] = i < count ? keys[src + i] : ~0ull;
+    __syncthreads();
+    bitonic_lds<SORT_SMALL_THREADS>(s, padded);
+    for (uint32_t i = threadIdx.x; i < count; i += SORT_SMALL_THREADS) pack_entry(s[i], dst + i, N, rec, pk);
+}
+
+constexpr int SORT_LARGE_THREADS = 1024;
+
+__global__ __launch_bounds__(SORT_LARGE_THREADS) void k_sort_large(
+    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ sort_off, const uint32_t* __restrict__ exact_off,
+    uint64_t* __restrict__ keys, Records rec, Records pk, int N, uint32_t capacity, uint32_t key_cap,
+    uint32_t lds_keys, const uint32_t* __restrict__ big_list, const uint32_t* __restrict__ totals) {
+    extern __shared__ uint64_t sl[];
+    const uint32_t nbig = totals[4];
+    for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {  // uniform per workgroup
+        const uint32_t t = big_list[bi];
+        const uint32_t count = counts[t];
+        const uint32_t src = sort_off[t], padded = sort_off[t + 1] - src, dst = exact_off[t];
+        if (!(sort_off[t + 1] <= key_cap && dst + count <= capacity)) {
+            for (uint32_t i = threadIdx.x; i < count; i += SORT_LARGE_THREADS)
+                if (dst + i < capacity) pack_entry(~0ull, dst + i, 0, rec, pk);
+            continue;
+        }
+        __syncthreads();
+        if (padded <= lds_keys) {
+            for (uint32_t i = threadIdx.x; i < padded; i += SORT_LARGE_THREADS) sl[i] = i < count ? keys[src + i] : ~0ull;
+            __syncthreads();
+            bitonic_lds<SORT_LARGE_THREADS>(sl, padded);
+            for (uint32_t i = threadIdx.x; i < count; i += SORT_LARGE_THREADS) pack_entry(sl[i], dst + i, N, rec, pk);
+        } else {
+            uint64_t* a = keys + src;
+            for (uint32_t i = count + threadIdx.x; i < padded; i += SORT_LARGE_THREADS) a[i] = ~0ull;
+            __syncthreads();
+            bitonic_global<SORT_LARGE_THREADS>(a, padded);
+            for (uint32_t i = threadIdx.x; i < count; i += SORT_LARGE_THREADS) pack_entry(a[i], dst + i, N, rec, pk);
+        }
+    }
+}
+
+static uint32_t g_large_lds_keys = 0;
+
+void build_lists(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L, uint32_t* nan_counts) {
+    (void)nan_counts;
+    const int T = tbx * tby;
+    fill_zero(s, L.tile_counts, (size_t)T * 4);
+    fill_zero(s, L.write_ctr, (size_t)T * 4);
+    if (N > 0) hipLaunchKernelGGL(k_count, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.rect, tbx, L.tile_counts);
+    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, T, L.tile_counts, L.exact_off, L.sort_off, L.bins,
+                       L.totals, (uint32_t)L.key_capacity, L.big_list, (uint32_t)L.capacity);
+    if (N > 0)
+        hipLaunchKernelGGL(k_scatter, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.rect, pj.depths, tbx,
+                           L.sort_off, L.write_ctr, L.keys, (uint32_t)L.key_capacity);
+    hipLaunchKernelGGL(k_sort_small, dim3(T), dim3(SORT_SMALL_THREADS), 0, s, L.tile_counts, L.sort_off, L.exact_off,
+                       L.keys, pj.rec, L.pk, N, (uint32_t)L.capacity, (uint32_t)L.key_capacity);
+    if (g_large_lds_keys == 0) {
+        int dev = 0;
+        hipDeviceProp_t prop;
+        size_t lim = 65536;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) lim = prop.sharedMemPerBlock;
+        uint32_t keys_fit = 1;
+        while ((size_t)(keys_fit * 2) * 8 <= lim && keys_fit * 2 <= 16384) keys_fit *= 2;
+        g_large_lds_keys = keys_fit;
+        (void)hipFuncSetAttribute((const void*)k_sort_large, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)(g_large_lds_keys * 8));
+    }
+    hipLaunchKernelGGL(k_sort_large, dim3(T < 512 ? T : 512), dim3(SORT_LARGE_THREADS), (size_t)g_large_lds_keys * 8, s,
+                       L.tile_counts, L.sort_off, L.exact_off, L.keys, pj.rec, L.pk, N, (uint32_t)L.capacity,
+                       (uint32_t)L.key_capacity, g_large_lds_keys, L.big_list, L.totals);
+}
+
+// ------------------------------------------------------------------------------------
+// A10/A11: rasterisation forward. One 256-thread workgroup per 16x16 tile (monolithic,
+// nd_rasterize_forward_kernel metal:499-618) or per (tile, 512-entry chunk)
+// (rasterize_forward_chunked_kernel metal:2960-3076 + merge metal:3081-3134).
+constexpr int RB = 256;
+
+__device__ __forceinline__ void composite_range(int lo, int hi, float px, float py, bool inside, const Records& pk,
+                                                float& T, float C[3], int& last) {
+    __shared__ float4 sA[RB];
+    __shared__ float4 sB[RB];
+    __shared__ float sCx[RB];
+    T = 1.f;
+    C[0] = C[1] = C[2] = 0.f;
+    last = lo - 1;
+    bool done = !inside;
+    for (int b0 = lo; b0 < hi; b0 += RB) {
+        if (__syncthreads_and(done)) break;
+        const int idx = b0 + threadIdx.x;
+        if (idx < hi) {
+            sA[threadIdx.x] = pk.A[idx];
+            sB[threadIdx.x] = pk.B[idx];
+            sCx[threadIdx.x] = pk.C[idx].x;
+        }
+        __syncthreads();
+        if (done) continue;
+        const int n = min(RB, hi - b0);
+        for (int e = 0; e < n; e++) {
+            const float4 a = sA[e];
+            const float4 b = sB[e];
+            const float dx = a.x - px, dy = a.y - py;
+            const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
+            if (!ms_finite(sigma) || sigma < 0.f || sigma >= 5.55f) continue;
+            const float alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
+            if (alpha < 1.f / 255.f) continue;
+            const float nT = T * (1.f - alpha);
+            if (nT <= 1e-4f) {
+                last = b0 + e - 1;
+                done = true;
+                break;
+            }
+            const float vis = alpha * T;
+            C[0] = fmaf(fmaxf(b.z + 0.5f, 0.f), vis, C[0]);
+            C[1] = fmaf(fmaxf(b.w + 0.5f, 0.f), vis, C[1]);
+            C[2] = fmaf(fmaxf(sCx[e] + 0.5f, 0.f), vis, C[2]);
+            T = nT;
+            last = b0 + e;
+        }
+    }
+}
+
+__global__ __launch_bounds__(RB) void k_raster_fwd(int W, int H, int tbx, const int2* __restrict__ bins, Records pk,
+                                                   float bg0, float bg1, float bg2, float* __restrict__ img,
+                                                   float* __restrict__ final_T, int32_t* __restrict__ final_idx) {
+    const int t = blockIdx.x;
+    const int tx = t % tbx, ty = t / tbx;
+    const int j = tx * TILE + (threadIdx.x & 15), i = ty * TILE + (threadIdx.x >> 4);
+    const bool inside = i < H && j < W;
+    const int2 r = bins[t];
+    float T, C[3];
+    int last;
+    composite_range(r.x, r.y, (float)j, (float)i, inside, pk, T, C, last);
+    if (inside) {
+        const size_t pid = (size_t)i * W + j;
+        final_T[pid] = T;
+        final_idx[pid] = last;
+        img[3 * pid + 0] = fminf(fmaxf(fmaf(bg0, T, C[0]), 0.f), 1.f);
+        img[3 * pid + 1] = fminf(fmaxf(fmaf(bg1, T, C[1]), 0.f), 1.f);
+        img[3 * pid + 2] = fminf(fmaxf(fmaf(bg2, T, C[2]), 0.f), 1.f);
+    }
+}
+
+__global__ __launch_bounds__(RB) void k_raster_fwd_chunk(int W, int H, int tbx, const int2* __restrict__ bins,
+                                                         Records pk, float* __restrict__ chunk_T,
+                                                         float* __restrict__ chunk_C, int32_t* __restrict__ chunk_idx) {
+    const int t = blockIdx.x, k = blockIdx.y;
+    const int tx = t % tbx, ty = t / tbx;
+    const int j = tx * TILE + (threadIdx.x & 15), i = ty * TILE + (threadIdx.x >> 4);
+    const bool inside = i < H && j < W;
+    const int2 r = bins[t];
+    const int cs = r.x + k * CHUNK, ce = min(r.x + (k + 1) * CHUNK, r.y);
+    const size_t P = (size_t)W * H;
+    const size_t o = (size_t)k * P + (size_t)i * W + j;
+    if (cs >= ce) {
+        if (inside) {
+            chunk_T[o] = 1.f;
+            chunk_C[3 * o] = 0.f; chunk_C[3 * o + 1] = 0.f; chunk_C[3 * o + 2] = 0.f;
+            chunk_idx[o] = -1;
+        }
+        return;
+    }
+    float T, C[3];
+    int last;
+    composite_range(cs, ce, (float)j, (float)i, inside, pk, T, C, last);
+    if (inside) {
+        chunk_T[o] = T;
+        chunk_C[3 * o] = C[0]; chunk_C[3 * o + 1] = C[1]; chunk_C[3 * o + 2] = C[2];
+        chunk_idx[o] = last;
+    }
+}
+
+__global__ void k_raster_merge(int W, int H, int K, const float* __restrict__ chunk_T, const float* __restrict__ chunk_C,
+                               int32_t* __restrict__ chunk_idx, float bg0, float bg1, float bg2,
+                               float* __restrict__ img, float* __restrict__ final_T, int32_t* __restrict__ final_idx) {
+    const size_t P = (size_t)W * H;
+    const size_t pid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= P) return;
+    float Tr = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
+    int last = -1, cutoff = K;
+    for (int k = 0; k < K; k++) {
+        const size_t o = (size_t)k * P + pid;
+        const int cf = chunk_idx[o];
+        if (cf < 0 && k > 0) break;
+        const float cT = chunk_T[o];
+        C0 = fmaf(chunk_C[3 * o], Tr, C0);
+        C1 = fmaf(chunk_C[3 * o + 1], Tr, C1);
+        C2 = fmaf(chunk_C[3 * o + 2], Tr, C2);
+        Tr *= cT;
+        if (cf >= 0) last = cf;
+        if (Tr <= 1e-4f) { cutoff = k + 1; break; }
+    }
+    for (int k = cutoff; k < K; k++) chunk_idx[(size_t)k * P + pid] = -1;
+    final_T[pid] = Tr;
+    final_idx[pid] = last;
+    img[3 * pid + 0] = fminf(fmaxf(fmaf(bg0, Tr, C0), 0.f), 1.f);
+    img[3 * pid + 1] = fminf(fmaxf(fmaf(bg1, Tr, C1), 0.f), 1.f);
+    img[3 * pid + 2] = fminf(fmaxf(fmaf(bg2, Tr, C2), 0.f), 1.f);
+}
+
+void raster_forward(hipStream_t s, const ListBufs& L, int W, int H, int tbx, int tby, const float bg[3], RasterBufs& R) {
+    const int T = tbx * tby;
+    if (R.K <= 1) {
+        hipLaunchKernelGGL(k_raster_fwd, dim3(T), dim3(RB), 0, s, W, H, tbx, L.bins, L.pk, bg[0], bg[1], bg[2], R.img,
+                           R.final_T, R.final_idx);
+        return;
+    }
+    hipLaunchKernelGGL(k_raster_fwd_chunk, dim3(T, R.K), dim3(RB), 0, s, W, H, tbx, L.bins, L.pk, R.chunk_T, R.chunk_C,
+                       R.chunk_idx);
+    hipLaunchKernelGGL(k_raster_merge, dim3(cdiv((size_t)W * H, 256)), dim3(256), 0, s, W, H, (int)R.K, R.chunk_T,
+                       R.chunk_C, R.chunk_idx, bg[0], bg[1], bg[2], R.img, R.final_T, R.final_idx);
+}
+
+// ------------------------------------------------------------------------------------
+// A12-A14: SSIM + L1 loss forward and backward (metal:3388-3450, 3563-3681, 3764-3850).
+// Intermediate buffers are planar ([q][H][W]) so every store of a 64-lane wave is one
+// contiguous 256-byte run.
+constexpr int SB = 16;
+
+__global__ __launch_bounds__(256) void k_ssim_h(int W, int H, const float* __restrict__ img,
+                                                const uint8_t* __restrict__ gt, float* __restrict__ hbuf) {
+    __shared__ float sg[3][SB][SB + 10];
+    __shared__ float sr[3][SB][SB + 10];
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x * SB - 5, by = blockIdx.y * SB;
+    for (int c = 0; c < 3; c++)
+        for (int k = threadIdx.x; k < SB * (SB + 10); k += 256) {
+            const int sy = k / (SB + 10), sx = k % (SB + 10);
+            const int gy = by + sy, gx = bx + sx;
+            float gv = 0.f, rv = 0.f;
+            if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
+                const size_t idx = ((size_t)gy * W + gx) * 3 + c;
+                gv = (float)gt[idx] * (1.0f / 255.0f);
+                rv = img[idx];
+            }
+            sg[c][sy][sx] = gv;
+            sr[c][sy][sx] = rv;
+        }
+    __syncthreads();
+    const int px = blockIdx.x * SB + lx, py = by + ly;
+    if (px >= W || py >= H) return;
+    const size_t P = (size_t)W * H, pid = (size_t)py * W + px;
+    for (int c = 0; c < 3; c++) {
+        float mx = 0.f, my = 0.f, sx = 0.f, sy = 0.f, cxy = 0.f;
+#pragma unroll
+        for (int d = 0; d < 11; d++) {
+            const float w = c_gauss1d[d];
+            const float gv = sg[c][ly][lx + d], rv = sr[c][ly][lx + d];
+            mx += w * gv;
+            my += w * rv;
+            sx += w * gv * gv;
+            sy += w * rv * rv;
+            cxy += w * gv * rv;
+        }
+        hbuf[(c * 5 + 0) * P + pid] = mx;
+        hbuf[(c * 5 + 1) * P + pid] = my;
+        hbuf[(c * 5 + 2) * P + pid] = sx;
+        hbuf[(c * 5 + 3) * P + pid] = sy;
+        hbuf[(c * 5 + 4) * P + pid] = cxy;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_ssim_fused(int W, int H, const float* __restrict__ img,
+                                                    const uint8_t* __restrict__ gt, const float* __restrict__ hbuf,
+                                                    float ssim_w, float* __restrict__ dbuf,
+                                                    float* __restrict__ partials, uint32_t* __restrict__ nanc) {
+    constexpr int TD = SB + 10;  // 26
+    __shared__ float shp[TD][TD][5];
+    __shared__ float sf[3][SB][TD];
+    __shared__ float s_ssim[SB][SB];
+    __shared__ float s_l1[SB][SB];
+    __shared__ float red[256];
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x * SB - 5, by = blockIdx.y * SB - 5;
+    const size_t P = (size_t)W * H;
+    s_ssim[ly][lx] = 0.f;
+    s_l1[ly][lx] = 0.f;
+    for (int c = 0; c < 3; c++) {
+        for (int k = threadIdx.x; k < TD * TD; k += 256) {
+            const int sy = k / TD, sx = k % TD;
+            const int gy = by + sy, gx = bx + sx;
+            const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+            const size_t pid = (size_t)gy * W + gx;
+#pragma unroll
+            for (int q = 0; q < 5; q++) shp[sy][sx][q] = in ? hbuf[(c * 5 + q) * P + pid] : 0.f;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < SB * TD; k += 256) {
+            const int dy = k / TD, dx = k % TD;
+            float m[5] = {0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kk = 0; kk < 11; kk++) {
+                const float w = c_gauss1d[kk];
+#pragma unroll
+                for (int q = 0; q < 5; q++) m[q] += w * shp[dy + kk][dx][q];
+            }
+            if (!(ms_finite(m[0]) && ms_finite(m[1]) && ms_finite(m[2]) && ms_finite(m[3]) && ms_finite(m[4]))) {
+                sf[0][dy][dx] = 0.f;
+                sf[1][dy][dx] = 0.f;
+                sf[2][dy][dx] = 0.f;
+                atomicAdd(&nanc[4], 1u);
+                continue;
+            }
+            const float mu_x = m[0], mu_y = m[1];
+            const float sxx = fmaxf(m[2] - mu_x * mu_x, 0.f), syy = fmaxf(m[3] - mu_y * mu_y, 0.f);
+            const float sxy = m[4] - mu_x * mu_y;
+            const float A = 2.0f * mu_x * mu_y + 0.0001f, Bv = 2.0f * sxy + 0.0009f;
+            const float Cd = mu_x * mu_x + mu_y * mu_y + 0.0001f, D = sxx + syy + 0.0009f;
+            const float iCD = 1.0f / (Cd * D);
+            const float dmu = 2.0f * Bv * (mu_x * Cd - A * mu_y) / (Cd * Cd * D);
+            const float dsyq = -A * Bv * iCD / D, dsxy = 2.0f * A * iCD;
+            sf[0][dy][dx] = dmu - 2.0f * mu_y * dsyq - mu_x * dsxy;
+            sf[1][dy][dx] = 2.0f * dsyq;
+            sf[2][dy][dx] = dsxy;
+            if (dx >= 5 && dx < 5 + SB) {
+                const int gpx = bx + dx, gpy = by + dy + 5;
+                if (gpx >= 0 && gpx < W && gpy >= 0 && gpy < H) {
+                    const size_t pid = (size_t)gpy * W + gpx;
+                    s_ssim[dy][dx - 5] += (A * Bv) / (Cd * D);
+                    s_l1[dy][dx - 5] += fabsf((float)gt[3 * pid + c] * (1.0f / 255.0f) - img[3 * pid + c]);
+                }
+            }
+        }
+        __syncthreads();
+        const int px = blockIdx.x * SB + lx, py = blockIdx.y * SB + ly;
+        if (px < W && py < H) {
+            float h1 = 0.f, h2 = 0.f, h3 = 0.f;
+#pragma unroll
+            for (int d = 0; d < 11; d++) {
+                const float w = c_gauss1d[10 - d];
+                h1 += w * sf[0][ly][lx + d];
+                h2 += w * sf[1][ly][lx + d];
+                h3 += w * sf[2][ly][lx + d];
+            }
+            const size_t pid = (size_t)py * W + px;
+            dbuf[(c * 3 + 0) * P + pid] = h1;
+            dbuf[(c * 3 + 1) * P + pid] = h2;
+            dbuf[(c * 3 + 2) * P + pid] = h3;
+        }
+        __syncthreads();
+    }
+    // pixel loss with the reference's per-thread bookkeeping (see ssim_loss_counted in the
+    // CPU restatement): in-image pixels whose sweep thread lies outside the image add only w.
+    const int px = blockIdx.x * SB + lx, py = blockIdx.y * SB + ly;
+    float pl = 0.f;
+    if (px < W && py < H) {
+        const int ii = ly * 26 + lx + 5;
+        const int tr = ii < 256 ? ii : ii - 256;
+        const bool counted = blockIdx.x * SB + (tr % 16) < W && blockIdx.y * SB + (tr / 16) < H;
+        pl = counted ? ssim_w * (1.0f - s_ssim[ly][lx] / 3.0f) + (1.0f - ssim_w) * s_l1[ly][lx] / 3.0f : ssim_w;
+    }
+    red[threadIdx.x] = pl;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_ssim_vbwd(int W, int H, const float* __restrict__ img,
+                                                   const uint8_t* __restrict__ gt, const float* __restrict__ dbuf,
+                                                   float ssim_w, float inv_n, float* __restrict__ v_rend,
+                                                   uint32_t* __restrict__ nanc) {
+    constexpr int TH = SB + 10;
+    __shared__ float sh[9][TH][SB];
+    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
+    const int bx = blockIdx.x * SB, by = blockIdx.y * SB - 5;
+    const size_t P = (size_t)W * H;
+    for (int k = threadIdx.x; k < TH * SB; k += 256) {
+        const int sy = k / SB, sx = k % SB;
+        const int gy = by + sy, gx = bx + sx;
+        const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
+        const size_t pid = (size_t)gy * W + gx;
+#pragma unroll
+        for (int q = 0; q < 9; q++) sh[q][sy][sx] = in ? dbuf[q * P + pid] : 0.f;
+    }
+    __syncthreads();
+    const int px = bx + lx, py = blockIdx.y * SB + ly;
+    if (px >= W || py >= H) return;
+    const size_t pid = (size_t)py * W + px;
+    for (int c = 0; c < 3; c++) {
+        float f1 = 0.f, f2 = 0.f, f3 = 0.f;
+#pragma unroll
+        for (int d = 0; d < 11; d++) {
+            const float w = c_gauss1d[10 - d];
+            f1 += w * sh[c * 3 + 0][ly + d][lx];
+            f2 += w * sh[c * 3 + 1][ly + d][lx];
+            f3 += w * sh[c * 3 + 2][ly + d][lx];
+        }
+        const float rv = img[3 * pid + c];
+        const float gv = (float)gt[3 * pid + c] * (1.0f / 255.0f);
+        const float vs = f1 + rv * f2 + gv * f3;
+        const float vl = (gv > rv) ? -1.0f : ((gv < rv) ? 1.0f : 0.0f);
+        float v = inv_n * (-ssim_w * vs + (1.0f - ssim_w) * vl);
+        if (!ms_finite(v)) {
+            atomicAdd(&nanc[6], 1u);
+            v = 0.f;
+        }
+        v_rend[3 * pid + c] = v;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_loss_final(int n, const float* __restrict__ partials, double* __restrict__ out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) acc += (double)partials[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+void ssim_loss(hipStream_t s, int W, int H, const float* img, const uint8_t* gt, float ssim_w, float inv_n, float* hbuf,
+               float* dbuf, float* v_rend, float* loss_partials, double* loss_out, uint32_t* nan_counts) {
+    dim3 g((W + SB - 1) / SB, (H + SB - 1) / SB);
+    hipLaunchKernelGGL(k_ssim_h, g, dim3(256), 0, s, W, H, img, gt, hbuf);
+    hipLaunchKernelGGL(k_ssim_fused, g, dim3(256), 0, s, W, H, img, gt, hbuf, ssim_w, dbuf, loss_partials, nan_counts);
+    hipLaunchKernelGGL(k_ssim_vbwd, g, dim3(256), 0, s, W, H, img, gt, dbuf, ssim_w, inv_n, v_rend, nan_counts);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, s, (int)(g.x * g.y), loss_partials, loss_out);
+}
+
+// ------------------------------------------------------------------------------------
+// A15/A16: rasterisation backward (rasterize_backward_kernel metal:931-1135; chunked
+// metal:3139-3366). Entries are walked back to front in LDS batches; each wave reduces
+// its 64 lanes' 9 gradient terms per entry with DPP, the 4 wave sums meet in LDS and one
+// thread per entry issues the global atomics.
+constexpr int BB = 128;  // backward batch
+
+__device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bool inside, float px, float py,
+                                               float T, float buf0, float buf1, float buf2, float T_final,
+                                               const float bg[3], float vo0, float vo1, float vo2,
+                                               const Records& pk, GradBufs& G, int tile_max) {
+    __shared__ float4 sA[BB];
+    __shared__ float4 sB[BB];
+    __shared__ float2 sC[BB];
+    __shared__ float red[4][BB][9];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const float tfb0 = T_final * bg[0], tfb1 = T_final * bg[1], tfb2 = T_final * bg[2];
+    const int top = min(hi, tile_max + 1);  // entries past every pixel's last contributor are skipped
+    for (int bhi = top; bhi > lo; bhi -= BB) {
+        const int blo = max(lo, bhi - BB);
+        const int n = bhi - blo;
+        __syncthreads();
+        if ((int)threadIdx.x < n) {
+            const int idx = bhi - 1 - threadIdx.x;  // slot 0 = furthest back
+            sA[threadIdx.x] = pk.A[idx];
+            sB[threadIdx.x] = pk.B[idx];
+            sC[threadIdx.x] = pk.C[idx];
+        }
+        __syncthreads();
+        for (int tt = 0; tt < n; tt++) {
+            const int e = bhi - 1 - tt;
+            bool valid = inside && e <= bin_final;
+            float alpha = 0.f, dx = 0.f, dy = 0.f;
+            const float4 a = sA[tt];
+            const float4 b = sB[tt];
+            if (valid) {
+                dx = a.x - px;
+                dy = a.y - py;
+                const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
+                if (sigma < 0.f || sigma >= 5.55f) {
+                    valid = false;
+                } else {
+                    alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
+                    if (alpha < 1.f / 255.f) valid = false;
+                }
+            }
+            const bool contrib = valid && alpha < 0.999f;
+            if (__ballot(contrib) == 0ull) {
+                if (lane == 63) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) red[wid][tt][k] = 0.f;
+                }
+                continue;
+            }
+            float g[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            if (contrib) {
+                const float ra = 1.f / (1.f - alpha);
+                T *= ra;
+                const float fac = alpha * T;
+                const float cz = sC[tt].x;
+                const float r0 = fmaxf(b.z + 0.5f, 0.f), r1 = fmaxf(b.w + 0.5f, 0.f), r2 = fmaxf(cz + 0.5f, 0.f);
+                const float q0 = fmaf(r0, T, fmaf(-buf0, ra, -ra * tfb0));
+                const float q1 = fmaf(r1, T, fmaf(-buf1, ra, -ra * tfb1));
+                const float q2 = fmaf(r2, T, fmaf(-buf2, ra, -ra * tfb2));
+                const float v_alpha = q0 * vo0 + q1 * vo1 + q2 * vo2;
+                buf0 = fmaf(r0, fac, buf0);
+                buf1 = fmaf(r1, fac, buf1);
+                buf2 = fmaf(r2, fac, buf2);
+                const float v_sigma = -alpha * v_alpha;
+                g[0] = fac * vo0;
+                g[1] = fac * vo1;
+                g[2] = fac * vo2;
+                const float hs = 0.5f * v_sigma;
+                g[3] = hs * (dx * dx);
+                g[4] = hs * (dx * dy);
+                g[5] = hs * (dy * dy);
+                g[6] = v_sigma * fmaf(a.w, dx, b.x * dy);
+                g[7] = v_sigma * fmaf(b.x, dx, b.y * dy);
+                g[8] = -v_sigma * (1.f - a.z);
+            }
+#pragma unroll
+            for (int k = 0; k < 9; k++) g[k] = wave_sum_to_lane63(g[k]);
+            if (lane == 63) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) red[wid][tt][k] = g[k];
+            }
+        }
+        __syncthreads();
+        if ((int)threadIdx.x < n) {
+            const int tt = threadIdx.x;
+            float s[9];
+#pragma unroll
+            for (int k = 0; k < 9; k++) s[k] = red[0][tt][k] + red[1][tt][k] + red[2][tt][k] + red[3][tt][k];
+            bool any = false;
+#pragma unroll
+            for (int k = 0; k < 9; k++) any |= (s[k] != 0.f);
+            if (any) {
+                const float4 b = sB[tt];
+                const float2 c = sC[tt];
+                const uint32_t gid = __builtin_bit_cast(uint32_t, c.y);
+                if (b.z + 0.5f >= 0.f) atomicAdd(&G.v_rgb[3 * (size_t)gid + 0], s[0]);
+                if (b.w + 0.5f >= 0.f) atomicAdd(&G.v_rgb[3 * (size_t)gid + 1], s[1]);
+                if (c.x + 0.5f >= 0.f) atomicAdd(&G.v_rgb[3 * (size_t)gid + 2], s[2]);
+                atomicAdd(&G.v_conic[3 * (size_t)gid + 0], s[3]);
+                atomicAdd(&G.v_conic[3 * (size_t)gid + 1], s[4]);
+                atomicAdd(&G.v_conic[3 * (size_t)gid + 2], s[5]);
+                atomicAdd(&G.v_xy[gid].x, s[6]);
+                atomicAdd(&G.v_xy[gid].y, s[7]);
+                atomicAdd(&G.v_opac[gid], s[8]);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ int block_max_int(int v) {
+    __shared__ int sm;
+    if (threadIdx.x == 0) sm = INT_MIN;
+    __syncthreads();
+    atomicMax(&sm, v);
+    __syncthreads();
+    const int r = sm;
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(RB) void k_raster_bwd(int W, int H, int tbx, const int2* __restrict__ bins, Records pk,
+                                                   float bg0, float bg1, float bg2, const float* __restrict__ final_T,
+                                                   const int32_t* __restrict__ final_idx,
+                                                   const float* __restrict__ v_rend, GradBufs G) {
+    const int t = blockIdx.x;
+    const int tx = t % tbx, ty = t / tbx;
+    const int j = tx * TILE + (threadIdx.x & 15), i = ty * TILE + (threadIdx.x >> 4);
+    const bool inside = i < H && j < W;
+    const int2 r = bins[t];
+    if (r.y <= r.x) return;
+    const size_t pid = inside ? (size_t)i * W + j : 0;
+    const float Tf = inside ? final_T[pid] : 1.f;
+    const int bf = inside ? final_idx[pid] : -1;
+    const float vo0 = inside ? v_rend[3 * pid] : 0.f, vo1 = inside ? v_rend[3 * pid + 1] : 0.f,
+                vo2 = inside ? v_rend[3 * pid + 2] : 0.f;
+    const int tmax = block_max_int(inside ? bf : INT_MIN);
+    const float bg[3] = {bg0, bg1, bg2};
+    backward_range(r.x, r.y, bf, inside, (float)j, (float)i, Tf, 0.f, 0.f, 0.f, Tf, bg, vo0, vo1, vo2, pk, G, tmax);
+}
+
+__global__ void k_chunk_prefix_suffix(int W, int H, int K, const float* __restrict__ chunk_T,
+                                      const float* __restrict__ chunk_C, float* __restrict__ prefix_T,
+                                      float* __restrict__ after_C) {
+    const size_t P = (size_t)W * H;
+    const size_t pid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (pid >= P) return;
+    float pT = 1.f;
+    for (int k = 0; k < K; k++) {
+        const size_t o = (size_t)k * P + pid;
+        prefix_T[o] = pT;
+        pT *= chunk_T[o];
+    }
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int k = K - 1; k >= 0; k--) {
+        const size_t o = (size_t)k * P + pid;
+        after_C[3 * o] = a0; after_C[3 * o + 1] = a1; after_C[3 * o + 2] = a2;
+        const float pk = prefix_T[o];
+        a0 += pk * chunk_C[3 * o];
+        a1 += pk * chunk_C[3 * o + 1];
+        a2 += pk * chunk_C[3 * o + 2];
+    }
+}
+
+__global__ __launch_bounds__(RB) void k_raster_bwd_chunk(int W, int H, int tbx, const int2* __restrict__ bins,
+                                                         Records pk, float bg0, float bg1, float bg2,
+                                                         const float* __restrict__ final_T,
+                                                         const int32_t* __restrict__ chunk_idx,
+                                                         const float* __restrict__ prefix_T,
+                                                         const float* __restrict__ chunk_T,
+                                                         const float* __restrict__ after_C,
+                                                         const float* __restrict__ v_rend, GradBufs G) {
+    const int t = blockIdx.x, k = blockIdx.y;
+    const int tx = t % tbx, ty = t / tbx;
+    const int j = tx * TILE + (threadIdx.x & 15), i = ty * TILE + (threadIdx.x >> 4);
+    const bool inside = i < H && j < W;
+    const int2 r = bins[t];
+    const int cs = r.x + k * CHUNK, ce = min(r.x + (k + 1) * CHUNK, r.y);
+    if (cs >= ce) return;
+    const size_t P = (size_t)W * H;
+    const size_t pid = inside ? (size_t)i * W + j : 0;
+    const size_t o = (size_t)k * P + pid;
+    const int bf = inside ? chunk_idx[o] : -1;
+    const bool act = inside && bf >= cs;
+    const int tmax = block_max_int(act ? bf : INT_MIN);
+    if (tmax < cs) return;  // uniform: no pixel of this tile has a contributor in this chunk
+    const float Ts = act ? prefix_T[o] * chunk_T[o] : 1.f;
+    const float b0 = act ? after_C[3 * o] : 0.f, b1 = act ? after_C[3 * o + 1] : 0.f, b2 = act ? after_C[3 * o + 2] : 0.f;
+    const float Tf = inside ? final_T[pid] : 1.f;
+    const float vo0 = inside ? v_rend[3 * pid] : 0.f, vo1 = inside ? v_rend[3 * pid + 1] : 0.f,
+                vo2 = inside ? v_rend[3 * pid + 2] : 0.f;
+    const float bg[3] = {bg0, bg1, bg2};
+    backward_range(cs, ce, act ? bf : -1, act, (float)j, (float)i, Ts, b0, b1, b2, Tf, bg, vo0, vo1, vo2, pk, G, tmax);
+}
+
+void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, int W, int H, int tbx, int tby,
+                     const float bg[3], const float* v_rend, GradBufs& G) {
+    const int T = tbx * tby;
+    if (R.K <= 1) {
+        hipLaunchKernelGGL(k_raster_bwd, dim3(T), dim3(RB), 0, s, W, H, tbx, L.bins, L.pk, bg[0], bg[1], bg[2], R.final_T,
+                           R.final_idx, v_rend, G);
+        return;
+    }
+    hipLaunchKernelGGL(k_chunk_prefix_suffix, dim3(cdiv((size_t)W * H, 256)), dim3(256), 0, s, W, H, (int)R.K,
+                       R.chunk_T, R.chunk_C, R.prefix_T, R.after_C);
+    hipLaunchKernelGGL(k_raster_bwd_chunk, dim3(T, R.K), dim3(RB), 0, s, W, H, tbx, L.bins, L.pk, bg[0], bg[1], bg[2],
+                       R.final_T, R.chunk_idx, R.prefix_T, R.chunk_T, R.after_C, v_rend, G);
+}
+
+// ------------------------------------------------------------------------------------
+// A17 (SH part): Adam on features_rest, one thread per coefficient. The reference fuses
+// this into the projection backward (metal:1987-2041); here each coefficient's Adam
+// update is an independent coalesced element, the basis value being recomputed from the
+// Gaussian's (pre-update) mean. Visible Gaussians and bases <= degrees_to_use only.
+__device__ __forceinline__ void adam_el(float& p, float& m, float& v, float g, float step, float b1, float b2,
+                                        float bc2s, float eps) {
+    const float mn = fmaf(b1, m, (1.0f - b1) * g);
+    const float vn = fmaf(b2, v, (1.0f - b2) * g * g);
+    p -= step * mn / (sqrtf(vn) / bc2s + eps);
+    m = mn;
+    v = vn;
+}
+
+__device__ __forceinline__ float sh_basis_one(int b, float x, float y, float z) {
+    const float xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
+    switch (b) {
+        case 0: return -SH_C1 * y;
+        case 1: return SH_C1 * z;
+        case 2: return -SH_C1 * x;
+        case 3: return SH_C2_0 * xy;
+        case 4: return SH_C2_1 * yz;
+        case 5: return SH_C2_2 * (2.f * zz - xx - yy);
+        case 6: return SH_C2_3 * xz;
+        case 7: return SH_C2_4 * (xx - yy);
+        case 8: return SH_C3_0 * y * (3.f * xx - yy);
+        case 9: return SH_C3_1 * xy * z;
+        case 10: return SH_C3_2 * y * (4.f * zz - xx - yy);
+        case 11: return SH_C3_3 * z * (2.f * zz - 3.f * xx - 3.f * yy);
+        case 12: return SH_C3_4 * x * (4.f * zz - xx - yy);
+        case 13: return SH_C3_5 * z * (xx - yy);
+        default: return SH_C3_6 * x * (xx - 3.f * yy);
+    }
+}
+
+template <int NB3>
+__global__ __launch_bounds__(256) void k_sh_rest_adam(int N, int nuse3, const float* __restrict__ means, float cpx,
+                                                      float cpy, float cpz, const int32_t* __restrict__ radii,
+                                                      const float* __restrict__ v_rgb, float* __restrict__ f_rest,
+                                                      float* __restrict__ m4, float* __restrict__ v4, float step,
+                                                      float bc2s, float b1, float b2, float eps) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)N * NB3) return;
+    const int i = (int)(e / NB3);
+    const int r = (int)(e - (size_t)i * NB3);
+    if (r >= nuse3 || radii[i] <= 0) return;
+    const float vc0 = v_rgb[3 * (size_t)i], vc1 = v_rgb[3 * (size_t)i + 1], vc2 = v_rgb[3 * (size_t)i + 2];
+    if (!(ms_finite(vc0) && ms_finite(vc1) && ms_finite(vc2))) return;  // SH Adam gate (metal:1975-1978)
+    const float d0 = means[3 * (size_t)i] - cpx, d1 = means[3 * (size_t)i + 1] - cpy, d2 = means[3 * (size_t)i + 2] - cpz;
+    const float inv = 1.f / sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+    const int b = r / 3, c = r - 3 * (r / 3);
+    const float vc = c == 0 ? vc0 : (c == 1 ? vc1 : vc2);
+    const float g = sh_basis_one(b, d0 * inv, d1 * inv, d2 * inv) * vc;
+    float p = f_rest[e], m = m4[e], v = v4[e];
+    adam_el(p, m, v, g, step, b1, b2, bc2s, eps);
+    f_rest[e] = p;
+    m4[e] = m;
+    v4[e] = v;
+}
+
+void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const int32_t* radii, const float* v_rgb,
+                  const MsplatStepParams& hp) {
+    const int du = hp.degrees_to_use < P.degree ? hp.degrees_to_use : P.degree;
+    if (du < 1 || P.N <= 0) return;
+    const int nuse3 = (du >= 3 ? 15 : du == 2 ? 8 : 3) * 3;
+    const int nb3 = (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15) * 3;
+    const size_t total = (size_t)P.N * nb3;
+#define LAUNCH_SH(NB)                                                                                              \
+    hipLaunchKernelGGL(k_sh_rest_adam<NB>, dim3(cdiv(total, 256)), dim3(256), 0, s, P.N, nuse3, P.means, cam.cam_pos[0], \
+                       cam.cam_pos[1], cam.cam_pos[2], radii, v_rgb, P.f_rest, P.m[4], P.v[4], hp.step_size[4],     \
+                       hp.bc2_sqrt[4], hp.beta1, hp.beta2, hp.eps)
+    if (nb3 == 9) LAUNCH_SH(9);
+    else if (nb3 == 24) LAUNCH_SH(24);
+    else LAUNCH_SH(45);
+#undef LAUNCH_SH
+}
+
+// ------------------------------------------------------------------------------------
+// A17 (projection part) + A18 + A19 + A20/A21, one thread per Gaussian: projection VJP
+// (metal:1860-1958), DC Adam (metal:1980-1985), regulariser gradients (metal:3987-4026),
+// Adam on means / log-scales / quaternions / opacity (metal:1680-1713) and the SGLD
+// perturbation (metal:3871-3967) of the updated parameters.
+struct M3 {
+    float m[3][3];
+};
+__device__ __forceinline__ M3 mm(const M3& a, const M3& b) {
+    M3 o;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) o.m[r][c] = a.m[r][0] * b.m[0][c] + a.m[r][1] * b.m[1][c] + a.m[r][2] * b.m[2][c];
+    return o;
+}
+__device__ __forceinline__ M3 tr(const M3& a) {
+    M3 o;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) o.m[r][c] = a.m[c][r];
+    return o;
+}
+
+__device__ void proj_vjp(const float p[3], const float ls[3], const float q[4], const MsplatCamera& cam,
+                         const float conic[3], float vxy0, float vxy1, const float vcn[3], float v_mean[3],
+                         float v_scale[3], float v_quat[4]) {
+    const float* pm = cam.projmat;
+    const float* vm = cam.viewmat;
+    const float W = (float)cam.width, H = (float)cam.height;
+    // project_pix_vjp (metal:1252-1270), perspective divide held fixed
+    const float hw = pm[12] * p[0] + pm[13] * p[1] + pm[14] * p[2] + pm[15];
+    const float rw = 1.f / (hw + 1e-6f);
+    const float vp0 = 0.5f * W * vxy0 * rw, vp1 = 0.5f * H * vxy1 * rw;
+    v_mean[0] = pm[0] * vp0 + pm[4] * vp1;
+    v_mean[1] = pm[1] * vp0 + pm[5] * vp1;
+    v_mean[2] = pm[2] * vp0 + pm[6] * vp1;
+    // cov2d_to_conic_vjp (metal:1289-1300)
+    const float X00 = conic[0], X01 = conic[1], X11 = conic[2];
+    const float G00 = vcn[0], G01 = vcn[1], G11 = vcn[2];
+    const float P00 = -(X00 * G00 + X01 * G01), P01 = -(X00 * G01 + X01 * G11);
+    const float P10 = -(X01 * G00 + X11 * G01), P11 = -(X01 * G01 + X11 * G11);
+    const float Q00 = P00 * X00 + P01 * X01, Q01 = P00 * X01 + P01 * X11;
+    const float Q10 = P10 * X00 + P11 * X01, Q11 = P10 * X01 + P11 * X11;
+    const float vc2_0 = Q00, vc2_1 = Q01 + Q10, vc2_2 = Q11;
+    // cov3d from scale / rotation
+    float s[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) s[k] = ms_expf_core(fminf(fmaxf(ls[k], -15.f), 15.f));
+    float R[3][3];
+    quat_rot(q, R);
+    M3 Mm;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) Mm.m[r][c] = R[r][c] * s[c];
+    M3 V = mm(Mm, tr(Mm));
+    // project_cov3d_ewa_vjp (metal:1375-1443)
+    const float fx = cam.fx, fy = cam.fy;
+    const float tan_fovx = 0.5f * W / fx, tan_fovy = 0.5f * H / fy;
+    const float pvx0 = vm[0] * p[0] + vm[1] * p[1] + vm[2] * p[2] + vm[3];
+    const float pvy0 = vm[4] * p[0] + vm[5] * p[1] + vm[6] * p[2] + vm[7];
+    const float pvz = vm[8] * p[0] + vm[9] * p[1] + vm[10] * p[2] + vm[11];
+    const float lim_x = 1.3f * tan_fovx, lim_y = 1.3f * tan_fovy;
+    const float pvx = pvz * fminf(lim_x, fmaxf(-lim_x, pvx0 / pvz));
+    const float pvy = pvz * fminf(lim_y, fmaxf(-lim_y, pvy0 / pvz));
+    const float rz = 1.f / pvz, rz2 = rz * rz;
+    M3 Wm, J, vc;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) Wm.m[r][c] = vm[r * 4 + c];
+    J.m[0][0] = fx * rz; J.m[0][1] = 0.f; J.m[0][2] = -fx * pvx * rz2;
+    J.m[1][0] = 0.f; J.m[1][1] = fy * rz; J.m[1][2] = -fy * pvy * rz2;
+    J.m[2][0] = 0.f; J.m[2][1] = 0.f; J.m[2][2] = 0.f;
+    vc.m[0][0] = vc2_0; vc.m[0][1] = 0.5f * vc2_1; vc.m[0][2] = 0.f;
+    vc.m[1][0] = 0.5f * vc2_1; vc.m[1][1] = vc2_2; vc.m[1][2] = 0.f;
+    vc.m[2][0] = 0.f; vc.m[2][1] = 0.f; vc.m[2][2] = 0.f;
+    const M3 Tm = mm(J, Wm);
+    const M3 vV = mm(mm(tr(Tm), vc), Tm);
+    const M3 vT1 = mm(mm(vc, Tm), V);   // V symmetric: V^T == V
+    const M3 vT2 = mm(mm(tr(vc), Tm), V);
+    M3 vT;
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) vT.m[r][c] = vT1.m[r][c] + vT2.m[r][c];
+    const float vcov3[6] = {vV.m[0][0], vV.m[1][0] + vV.m[0][1], vV.m[2][0] + vV.m[0][2],
+                            vV.m[1][1], vV.m[2][1] + vV.m[1][2], vV.m[2][2]};
+    const M3 vJ = mm(vT, tr(Wm));
+    const float fx_rz2 = fx * rz2, fy_rz2 = fy * rz2, rz3 = rz2 * rz;
+    const float vt0 = -fx_rz2 * vJ.m[0][2], vt1 = -fy_rz2 * vJ.m[1][2];
+    const float vt2 = -fx_rz2 * vJ.m[0][0] + 2.f * fx * pvx * rz3 * vJ.m[0][2] - fy_rz2 * vJ.m[1][1] +
+                      2.f * fy * pvy * rz3 * vJ.m[1][2];
+#pragma unroll
+    for (int k = 0; k < 3; k++) v_mean[k] += vt0 * Wm.m[0][k] + vt1 * Wm.m[1][k] + vt2 * Wm.m[2][k];
+    // scale_rot_to_cov3d_vjp (metal:1536-1569) + quat_to_rotmat_vjp (metal:1445-1490)
+    M3 vVs;
+    vVs.m[0][0] = 2.f * vcov3[0]; vVs.m[0][1] = 2.f * (0.5f * vcov3[1]); vVs.m[0][2] = 2.f * (0.5f * vcov3[2]);
+    vVs.m[1][0] = 2.f * (0.5f * vcov3[1]); vVs.m[1][1] = 2.f * vcov3[3]; vVs.m[1][2] = 2.f * (0.5f * vcov3[4]);
+    vVs.m[2][0] = 2.f * (0.5f * vcov3[2]); vVs.m[2][1] = 2.f * (0.5f * vcov3[4]); vVs.m[2][2] = 2.f * vcov3[5];
+    const M3 vM = mm(vVs, Mm);
+#pragma unroll
+    for (int k = 0; k < 3; k++) v_scale[k] = R[0][k] * vM.m[0][k] + R[1][k] * vM.m[1][k] + R[2][k] * vM.m[2][k];
+    float g[3][3];  // g[c][r] = column-major v_R
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < 3; c++) g[c][r] = vM.m[r][c] * s[c];
+    const float n2 = q[3] * q[3] + q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    const float sn = 1.f / sqrtf(fmaxf(n2, 1e-8f));
+    const float w = q[0] * sn, x = q[1] * sn, y = q[2] * sn, z = q[3] * sn;
+    v_quat[0] = 2.f * (x * (g[1][2] - g[2][1]) + y * (g[2][0] - g[0][2]) + z * (g[0][1] - g[1][0]));
+    v_quat[1] = 2.f * (-2.f * x * (g[1][1] + g[2][2]) + y * (g[0][1] + g[1][0]) + z * (g[0][2] + g[2][0]) +
+                       w * (g[1][2] - g[2][1]));
+    v_quat[2] = 2.f * (x * (g[0][1] + g[1][0]) - 2.f * y * (g[0][0] + g[2][2]) + z * (g[1][2] + g[2][1]) +
+                       w * (g[2][0] - g[0][2]));
+    v_quat[3] = 2.f * (x * (g[0][2] + g[2][0]) + y * (g[1][2] + g[2][1]) - 2.f * z * (g[0][0] + g[1][1]) +
+                       w * (g[0][1] - g[1][0]));
+#pragma unroll
+    for (int k = 0; k < 3; k++) v_scale[k] *= s[k];
+}
+
+__device__ __forceinline__ uint32_t pcg_hash32(uint32_t seed, uint32_t idx) {
+    uint32_t x = seed * 747796405u + idx * 2891336453u;
+    x = ((x >> ((x >> 28) + 4)) ^ x) * 277803737u;
+    return (x >> 22) ^ x;
+}
+__device__ __forceinline__ float pcg_unit(uint32_t h) { return ((float)(h >> 9) + 0.5f) * (1.0f / 8388608.0f); }
+
+__global__ __launch_bounds__(256) void k_gaussian_update(
+    int N, float* __restrict__ means, float* __restrict__ scales, float* __restrict__ quats, float* __restrict__ f_dc,
+    float* __restrict__ opac, float* __restrict__ m0, float* __restrict__ v0, float* __restrict__ m1,
+    float* __restrict__ v1, float* __restrict__ m2, float* __restrict__ v2, float* __restrict__ m3,
+    float* __restrict__ v3, float* __restrict__ m5, float* __restrict__ v5, MsplatCamera cam,
+    const int32_t* __restrict__ radii, const float4* __restrict__ rA, const float4* __restrict__ rB, GradBufs G,
+    MsplatStepParams hp, uint32_t* __restrict__ nanc, float* __restrict__ dbg_vm, float* __restrict__ dbg_vs,
+    float* __restrict__ dbg_vq, float* __restrict__ dbg_vo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float p[3] = {means[3 * (size_t)i], means[3 * (size_t)i + 1], means[3 * (size_t)i + 2]};
+    float ls[3] = {scales[3 * (size_t)i], scales[3 * (size_t)i + 1], scales[3 * (size_t)i + 2]};
+    const float4 q4 = reinterpret_cast<const float4*>(quats)[i];
+    float q[4] = {q4.x, q4.y, q4.z, q4.w};
+    float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
+    const float b1 = hp.beta1, b2 = hp.beta2, eps = hp.eps;
+    if (radii[i] > 0) {
+        const float4 a = rA[i], b = rB[i];
+        const float conic[3] = {a.w, b.x, b.y};
+        const float2 vxy = G.v_xy[i];
+        const float vcn[3] = {G.v_conic[3 * (size_t)i], G.v_conic[3 * (size_t)i + 1], G.v_conic[3 * (size_t)i + 2]};
+        proj_vjp(p, ls, q, cam, conic, vxy.x, vxy.y, vcn, vm, vs, vq);
+        const float vc[3] = {G.v_rgb[3 * (size_t)i], G.v_rgb[3 * (size_t)i + 1], G.v_rgb[3 * (size_t)i + 2]};
+        if (ms_finite(vc[0]) && ms_finite(vc[1]) && ms_finite(vc[2])) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const size_t k = 3 * (size_t)i + c;
+                float pp = f_dc[k], mm_ = m3[k], vv = v3[k];
+                adam_el(pp, mm_, vv, SH_C0 * vc[c], hp.step_size[3], b1, b2, hp.bc2_sqrt[3], eps);
+                f_dc[k] = pp; m3[k] = mm_; v3[k] = vv;
+            }
+        } else {
+            atomicAdd(&nanc[3], 1u);
+        }
+    }
+    float vo = G.v_opac[i];
+    if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
+        const float vdc = v3[3 * (size_t)i] + v3[3 * (size_t)i + 1] + v3[3 * (size_t)i + 2];
+        const float duty = (vdc == 0.0f) ? 1.0f : hp.reg_invis_scale;
+        const float invN = duty / (float)N;
+        const float sig = 1.0f / (1.0f + ms_expf(-opac[i]));
+        vo += hp.op_reg * sig * (1.0f - sig) * invN;
+        const float inv3N = invN * (1.0f / 3.0f);
+#pragma unroll
+        for (int k = 0; k < 3; k++) vs[k] += hp.sc_reg * ms_expf_core(fminf(fmaxf(ls[k], -15.f), 15.f)) * inv3N;
+    }
+    if (dbg_vm) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) { dbg_vm[3 * (size_t)i + k] = vm[k]; dbg_vs[3 * (size_t)i + k] = vs[k]; }
+#pragma unroll
+        for (int k = 0; k < 4; k++) dbg_vq[4 * (size_t)i + k] = vq[k];
+        dbg_vo[i] = vo;
+    }
+    // fused_adam (metal:1680-1713) with the non-finite gradient gate, groups 0, 1, 2, 5
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const size_t e = 3 * (size_t)i + k;
+        if (ms_finite(vm[k])) {
+            float mm_ = m0[e], vv = v0[e];
+            adam_el(p[k], mm_, vv, vm[k], hp.step_size[0], b1, b2, hp.bc2_sqrt[0], eps);
+            m0[e] = mm_; v0[e] = vv;
+        } else atomicAdd(&nanc[0], 1u);
+        if (ms_finite(vs[k])) {
+            float mm_ = m1[e], vv = v1[e];
+            adam_el(ls[k], mm_, vv, vs[k], hp.step_size[1], b1, b2, hp.bc2_sqrt[1], eps);
+            m1[e] = mm_; v1[e] = vv;
+        } else atomicAdd(&nanc[1], 1u);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const size_t e = 4 * (size_t)i + k;
+        if (ms_finite(vq[k])) {
+            float mm_ = m2[e], vv = v2[e];
+            adam_el(q[k], mm_, vv, vq[k], hp.step_size[2], b1, b2, hp.bc2_sqrt[2], eps);
+            m2[e] = mm_; v2[e] = vv;
+        } else atomicAdd(&nanc[2], 1u);
+    }
+    float o = opac[i];
+    if (ms_finite(vo)) {
+        float mm_ = m5[i], vv = v5[i];
+        adam_el(o, mm_, vv, vo, hp.step_size[5], b1, b2, hp.bc2_sqrt[5], eps);
+        m5[i] = mm_; v5[i] = vv;
+    } else atomicAdd(&nanc[5], 1u);
+    // SGLD (sgld_noise_gen metal:3871-3890, sgld_noise metal:3892-3967) on the updated state
+    if (hp.sgld_enable) {
+        const uint32_t base = (uint32_t)i * 4u;
+        const float u0 = pcg_unit(pcg_hash32(hp.sgld_seed, base)), u1 = pcg_unit(pcg_hash32(hp.sgld_seed, base + 1));
+        const float u2 = pcg_unit(pcg_hash32(hp.sgld_seed, base + 2)), u3 = pcg_unit(pcg_hash32(hp.sgld_seed, base + 3));
+        const float r01 = sqrtf(-2.0f * ms_logf(u0)), r23 = sqrtf(-2.0f * ms_logf(u2));
+        float s01, c01, s23, c23;
+        ms_sincosf(6.283185307179586f * u1, &s01, &c01);
+        ms_sincosf(6.283185307179586f * u3, &s23, &c23);
+        const float nz0 = r01 * c01, nz1 = r01 * s01, nz2 = r23 * c23;
+        const float op_sig = 1.0f / (1.0f + ms_expf(-o));
+        const float weight = 1.0f / (1.0f + ms_expf(-100.0f * ((1.0f - op_sig) - 0.995f)));
+        const float ns = hp.noise_lr * hp.xyz_lr * weight;
+        const float n0 = nz0 * ns, n1 = nz1 * ns, n2 = nz2 * ns;
+        float qw = q[0], qx = q[1], qy = q[2], qz = q[3];
+        const float ql = sqrtf(fmaxf(qw * qw + qx * qx + qy * qy + qz * qz, 1e-8f));
+        qw /= ql; qx /= ql; qy /= ql; qz /= ql;
+        const float sx = ms_expf_core(fminf(fmaxf(ls[0], -15.f), 15.f));
+        const float sy = ms_expf_core(fminf(fmaxf(ls[1], -15.f), 15.f));
+        const float sz = ms_expf_core(fminf(fmaxf(ls[2], -15.f), 15.f));
+        float rt0 = (1 - 2 * (qy * qy + qz * qz)) * n0 + 2 * (qx * qy + qw * qz) * n1 + 2 * (qx * qz - qw * qy) * n2;
+        float rt1 = 2 * (qx * qy - qw * qz) * n0 + (1 - 2 * (qx * qx + qz * qz)) * n1 + 2 * (qy * qz + qw * qx) * n2;
+        float rt2 = 2 * (qx * qz + qw * qy) * n0 + 2 * (qy * qz - qw * qx) * n1 + (1 - 2 * (qx * qx + qy * qy)) * n2;
+        rt0 *= sx * sx; rt1 *= sy * sy; rt2 *= sz * sz;
+        float d0 = (1 - 2 * (qy * qy + qz * qz)) * rt0 + 2 * (qx * qy - qw * qz) * rt1 + 2 * (qx * qz + qw * qy) * rt2;
+        float d1 = 2 * (qx * qy + qw * qz) * rt0 + (1 - 2 * (qx * qx + qz * qz)) * rt1 + 2 * (qy * qz - qw * qx) * rt2;
+        float d2 = 2 * (qx * qz - qw * qy) * rt0 + 2 * (qy * qz + qw * qx) * rt1 + (1 - 2 * (qx * qx + qy * qy)) * rt2;
+        if (ms_finite(d0) && ms_finite(d1) && ms_finite(d2)) {
+            const float len2 = d0 * d0 + d1 * d1 + d2 * d2;
+            if (ms_finite(len2)) {
+                if (len2 > hp.max_disp * hp.max_disp) {
+                    const float inv = hp.max_disp / sqrtf(len2);
+                    d0 *= inv; d1 *= inv; d2 *= inv;
+                }
+                p[0] += d0; p[1] += d1; p[2] += d2;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) { means[3 * (size_t)i + k] = p[k]; scales[3 * (size_t)i + k] = ls[k]; }
+    reinterpret_cast<float4*>(quats)[i] = make_float4(q[0], q[1], q[2], q[3]);
+    opac[i] = o;
+}
+
+void gaussian_update(hipStream_t s, const Params& P, const MsplatCamera& cam, const ProjOut& pj, const GradBufs& G,
+                     const MsplatStepParams& hp, uint32_t* nan_counts, float* dbg_v_mean, float* dbg_v_scale,
+                     float* dbg_v_quat, float* dbg_v_opac) {
+    if (P.N <= 0) return;
+    hipLaunchKernelGGL(k_gaussian_update, dim3(cdiv(P.N, 256)), dim3(256), 0, s, P.N, P.means, P.scales, P.quats, P.f_dc,
+                       P.opac, P.m[0], P.v[0], P.m[1], P.v[1], P.m[2], P.v[2], P.m[3], P.v[3], P.m[5], P.v[5], cam,
+                       pj.radii, pj.rec.A, pj.rec.B, G, hp, nan_counts, dbg_v_mean, dbg_v_scale, dbg_v_quat, dbg_v_opac);
+}
+
+}  // namespace msk

@@ -1,0 +1,160 @@
+// oracle_capi.cpp — extern "C" entry points of the CPU restatement (TEST INFRASTRUCTURE).
+// Loaded with ctypes by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+// only; the product never links it.
+#include <cstdio>
+#include "oracle.hpp"
+#include "oracle_mcmc.hpp"
+
+using namespace oracle;
+
+#define ORACLE_STRUCTS(SUF, FT)                                                                  \
+    typedef struct {                                                                             \
+        int32_t N, degree;                                                                       \
+        FT *means, *scales, *quats, *f_dc, *f_rest, *opac;                                       \
+        FT* m[6];                                                                                \
+        FT* v[6];                                                                                \
+    } OracleState##SUF;                                                                          \
+    typedef struct {                                                                             \
+        FT *xys, *depths, *conics, *colors, *aabb;                                               \
+        int32_t *radii, *tiles_hit;                                                              \
+        int32_t *tile_counts, *tile_bins, *ids;                                                  \
+        int64_t ids_cap, num_isect, num_padded;                                                  \
+        int32_t overflow;                                                                        \
+        uint32_t k_max;                                                                          \
+        FT *img, *final_T;                                                                       \
+        int32_t* final_idx;                                                                      \
+        FT* v_rendered;                                                                          \
+        double loss_sum;                                                                         \
+        FT *v_xy, *v_conic, *v_rgb, *v_opac, *v_mean, *v_scale, *v_quat;                         \
+        uint32_t nan_adam[6];                                                                    \
+    } OracleOut##SUF;
+
+extern "C" {
+ORACLE_STRUCTS(F32, float)
+ORACLE_STRUCTS(F64, double)
+}
+
+template <class T, class SS> static State<T> make_state(SS* s) {
+    State<T> S;
+    S.N = s->N;
+    S.degree = s->degree;
+    S.means = s->means; S.scales = s->scales; S.quats = s->quats;
+    S.f_dc = s->f_dc; S.f_rest = s->f_rest; S.opac = s->opac;
+    for (int g = 0; g < 6; g++) { S.m[g] = s->m[g]; S.v[g] = s->v[g]; }
+    return S;
+}
+
+template <class T> static void cp(T* dst, const std::vector<T>& src) {
+    if (dst && !src.empty()) std::memcpy(dst, src.data(), src.size() * sizeof(T));
+}
+
+template <class T, class OO> static void export_out(const StepOut<T>& O, OO* o, bool train) {
+    cp(o->xys, O.proj.xys); cp(o->depths, O.proj.depths); cp(o->conics, O.proj.conics);
+    cp(o->colors, O.proj.colors); cp(o->aabb, O.proj.aabb);
+    cp(o->radii, O.proj.radii); cp(o->tiles_hit, O.proj.tiles_hit);
+    if (o->tile_counts) for (size_t t = 0; t < O.lists.counts.size(); t++) o->tile_counts[t] = (int32_t)O.lists.counts[t];
+    cp(o->tile_bins, O.lists.bins);
+    o->num_isect = O.lists.num_isect;
+    o->num_padded = O.lists.num_padded;
+    o->overflow = O.lists.overflow ? 1 : 0;
+    if (o->ids && (int64_t)O.lists.ids.size() <= o->ids_cap) cp(o->ids, O.lists.ids);
+    o->k_max = O.raster.K;
+    cp(o->img, O.raster.img); cp(o->final_T, O.raster.final_T); cp(o->final_idx, O.raster.final_idx);
+    if (!train) return;
+    cp(o->v_rendered, O.v_rend);
+    o->loss_sum = O.loss_sum;
+    cp(o->v_xy, O.rg.v_xy); cp(o->v_conic, O.rg.v_conic); cp(o->v_rgb, O.rg.v_rgb);
+    cp(o->v_opac, O.v_opac_total); cp(o->v_mean, O.v_mean); cp(o->v_scale, O.v_scale); cp(o->v_quat, O.v_quat);
+    for (int k = 0; k < 6; k++) o->nan_adam[k] = O.nan_adam[k];
+}
+
+extern "C" {
+
+int oracle_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+int oracle_train_step_f32(OracleStateF32* s, const MsplatCamera* cam, const uint8_t* gt, const MsplatStepParams* hp,
+                          OracleOutF32* o) {
+    State<float> S = make_state<float>(s);
+    StepOut<float> O;
+    train_step<float>(S, *cam, gt, *hp, O);
+    export_out(O, o, true);
+    return O.lists.overflow ? 1 : 0;
+}
+
+int oracle_train_step_f64(OracleStateF64* s, const MsplatCamera* cam, const uint8_t* gt, const MsplatStepParams* hp,
+                          OracleOutF64* o) {
+    State<double> S = make_state<double>(s);
+    StepOut<double> O;
+    train_step<double>(S, *cam, gt, *hp, O);
+    export_out(O, o, true);
+    return O.lists.overflow ? 1 : 0;
+}
+
+int oracle_render_f32(OracleStateF32* s, const MsplatCamera* cam, int degrees_to_use, const float* bg, OracleOutF32* o) {
+    State<float> S = make_state<float>(s);
+    StepOut<float> O;
+    render<float>(S, *cam, degrees_to_use, bg, O);
+    export_out(O, o, false);
+    return O.lists.overflow ? 1 : 0;
+}
+
+int oracle_render_f64(OracleStateF64* s, const MsplatCamera* cam, int degrees_to_use, const float* bg, OracleOutF64* o) {
+    State<double> S = make_state<double>(s);
+    StepOut<double> O;
+    render<double>(S, *cam, degrees_to_use, bg, O);
+    export_out(O, o, false);
+    return O.lists.overflow ? 1 : 0;
+}
+
+// SSIM + L1 loss on a given image (used to pin the loss against an independent SSIM).
+double oracle_ssim_loss_f64(int W, int H, const double* rend, const uint8_t* gt, double w, double inv_n, double* v_rend) {
+    std::vector<double> v;
+    double l = ssim_loss<double>(W, H, rend, gt, w, inv_n, v);
+    if (v_rend) std::memcpy(v_rend, v.data(), v.size() * sizeof(double));
+    return l;
+}
+double oracle_ssim_loss_f32(int W, int H, const float* rend, const uint8_t* gt, float w, float inv_n, float* v_rend) {
+    std::vector<float> v;
+    double l = ssim_loss<float>(W, H, rend, gt, w, inv_n, v);
+    if (v_rend) std::memcpy(v_rend, v.data(), v.size() * sizeof(float));
+    return l;
+}
+
+void oracle_sgld_noise_f32(int N, uint32_t seed, float* out) {
+    for (int i = 0; i < N; i++) sgld_noise3<float>(seed, (uint32_t)i, out + 3 * (size_t)i);
+}
+uint32_t oracle_pcg_hash32(uint32_t seed, uint32_t idx) { return pcg_hash32(seed, idx); }
+
+float oracle_ms_expf(float x) { return ms_expf(x); }
+float oracle_ms_logf(float x) { return ms_logf(x); }
+void oracle_ms_sincosf(float a, float* s, float* c) { ms_sincosf(a, s, c); }
+
+uint32_t oracle_k_max(int64_t capacity, int num_tiles) { return k_max_rule(capacity, num_tiles); }
+
+// Eq. 9 (model:311-332) in the relocation's arithmetic (see oracle_mcmc.hpp).
+void oracle_eq9(float o_old, const float* s_old, int n, float* o_new, float* s_new) {
+    relocation_eq9(o_old, s_old, n, o_new, s_new);
+}
+
+int oracle_mcmc_step(OracleStateF32* s, int32_t cap_max, float cull_radius, uint32_t reloc_count_in, int32_t* out_info) {
+    State<float> S = make_state<float>(s);
+    McmcInfo info;
+    mcmc_relocate_and_grow(S, cap_max, cull_radius, reloc_count_in, info);
+    out_info[0] = info.num_active;
+    out_info[1] = info.dead;
+    out_info[2] = info.grown;
+    out_info[3] = info.nonfinite;
+    out_info[4] = info.culled;
+    out_info[5] = (int32_t)info.reloc_count_out;
+    return 0;
+}
+
+}  // extern "C"

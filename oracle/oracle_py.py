@@ -1,0 +1,203 @@
+"""ctypes binding of the CPU restatement (TEST INFRASTRUCTURE).
+
+Used only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg, as the
+checker; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-s"], check=True)
+    return LIB_PATH
+
+
+def _state_struct(ft):
+    P = C.POINTER(ft)
+
+    class S(C.Structure):
+        _fields_ = [("N", C.c_int32), ("degree", C.c_int32), ("means", P), ("scales", P), ("quats", P),
+                    ("f_dc", P), ("f_rest", P), ("opac", P), ("m", P * 6), ("v", P * 6)]
+    return S
+
+
+def _out_struct(ft):
+    P = C.POINTER(ft)
+    I = C.POINTER(C.c_int32)
+
+    class O(C.Structure):
+        _fields_ = [("xys", P), ("depths", P), ("conics", P), ("colors", P), ("aabb", P),
+                    ("radii", I), ("tiles_hit", I), ("tile_counts", I), ("tile_bins", I), ("ids", I),
+                    ("ids_cap", C.c_int64), ("num_isect", C.c_int64), ("num_padded", C.c_int64),
+                    ("overflow", C.c_int32), ("k_max", C.c_uint32),
+                    ("img", P), ("final_T", P), ("final_idx", I), ("v_rendered", P), ("loss_sum", C.c_double),
+                    ("v_xy", P), ("v_conic", P), ("v_rgb", P), ("v_opac", P), ("v_mean", P), ("v_scale", P),
+                    ("v_quat", P), ("nan_adam", C.c_uint32 * 6)]
+    return O
+
+
+StateF32, StateF64 = _state_struct(C.c_float), _state_struct(C.c_double)
+OutF32, OutF64 = _out_struct(C.c_float), _out_struct(C.c_double)
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        from msplat_amd._lib import MsplatCamera, MsplatStepParams  # struct layouts only
+        L = C.CDLL(LIB_PATH)
+        L.oracle_train_step_f32.argtypes = [C.POINTER(StateF32), C.POINTER(MsplatCamera), C.c_void_p,
+                                            C.POINTER(MsplatStepParams), C.POINTER(OutF32)]
+        L.oracle_train_step_f64.argtypes = [C.POINTER(StateF64), C.POINTER(MsplatCamera), C.c_void_p,
+                                            C.POINTER(MsplatStepParams), C.POINTER(OutF64)]
+        L.oracle_render_f32.argtypes = [C.POINTER(StateF32), C.POINTER(MsplatCamera), C.c_int,
+                                        C.POINTER(C.c_float), C.POINTER(OutF32)]
+        L.oracle_render_f64.argtypes = [C.POINTER(StateF64), C.POINTER(MsplatCamera), C.c_int,
+                                        C.POINTER(C.c_float), C.POINTER(OutF64)]
+        L.oracle_ssim_loss_f64.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_double, C.c_double,
+                                           C.c_void_p]
+        L.oracle_ssim_loss_f64.restype = C.c_double
+        L.oracle_ssim_loss_f32.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
+                                           C.c_void_p]
+        L.oracle_ssim_loss_f32.restype = C.c_double
+        L.oracle_sgld_noise_f32.argtypes = [C.c_int, C.c_uint32, C.c_void_p]
+        L.oracle_pcg_hash32.argtypes = [C.c_uint32, C.c_uint32]
+        L.oracle_pcg_hash32.restype = C.c_uint32
+        L.oracle_ms_expf.argtypes = [C.c_float]
+        L.oracle_ms_expf.restype = C.c_float
+        L.oracle_ms_logf.argtypes = [C.c_float]
+        L.oracle_ms_logf.restype = C.c_float
+        L.oracle_ms_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        L.oracle_k_max.argtypes = [C.c_int64, C.c_int]
+        L.oracle_k_max.restype = C.c_uint32
+        L.oracle_eq9.argtypes = [C.c_float, C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_void_p]
+        L.oracle_mcmc_step.argtypes = [C.POINTER(StateF32), C.c_int32, C.c_float, C.c_uint32, C.c_void_p]
+        L.oracle_set_threads.argtypes = [C.c_int]
+        L.oracle_set_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+GROUP_SHAPES = ("means", "scales", "quats", "f_dc", "f_rest", "opac")
+
+
+class HostState:
+    """Host copy of the trainable state (params + Adam moments), numpy arrays of a given dtype."""
+
+    def __init__(self, g: dict, dtype=np.float32, moments=None, capacity=None):
+        self.dtype = dtype
+        self.degree = int(g["degree"])
+        n = g["means"].shape[0]
+        self.N = n
+        cap = capacity or n
+        self.cap = cap
+
+        def alloc(a):
+            shape = (cap,) + a.shape[1:]
+            out = np.zeros(shape, dtype)
+            out[:n] = a
+            return out
+        self.p = {k: alloc(np.asarray(g[k])) for k in GROUP_SHAPES}
+        if moments is None:
+            self.m = {k: np.zeros_like(self.p[k]) for k in GROUP_SHAPES}
+            self.v = {k: np.zeros_like(self.p[k]) for k in GROUP_SHAPES}
+        else:
+            self.m = {k: alloc(np.asarray(moments[0][k])) for k in GROUP_SHAPES}
+            self.v = {k: alloc(np.asarray(moments[1][k])) for k in GROUP_SHAPES}
+
+    def struct(self):
+        S = StateF32 if self.dtype == np.float32 else StateF64
+        ft = C.c_float if self.dtype == np.float32 else C.c_double
+        s = S()
+        s.N, s.degree = self.N, self.degree
+
+        def ptr(a):
+            return a.ctypes.data_as(C.POINTER(ft))
+        s.means, s.scales, s.quats = ptr(self.p["means"]), ptr(self.p["scales"]), ptr(self.p["quats"])
+        s.f_dc, s.f_rest, s.opac = ptr(self.p["f_dc"]), ptr(self.p["f_rest"]), ptr(self.p["opac"])
+        for gi, k in enumerate(GROUP_SHAPES):
+            s.m[gi] = ptr(self.m[k])
+            s.v[gi] = ptr(self.v[k])
+        return s
+
+    def view(self):
+        return {k: self.p[k][: self.N] for k in GROUP_SHAPES}
+
+
+def _run(fn_name, st: HostState, cam, gt, hp, want_ids=True, render_args=None):
+    L = lib()
+    f64 = st.dtype == np.float64
+    O = OutF64 if f64 else OutF32
+    ft = C.c_float if not f64 else C.c_double
+    N = st.N
+    W, H = cam.width, cam.height
+    T = ((W + 15) // 16) * ((H + 15) // 16)
+    out = {
+        "xys": np.zeros((N, 2), st.dtype), "depths": np.zeros(N, st.dtype), "conics": np.zeros((N, 3), st.dtype),
+        "colors": np.zeros((N, 3), st.dtype), "aabb": np.zeros((N, 2), st.dtype),
+        "radii": np.zeros(N, np.int32), "tiles_hit": np.zeros(N, np.int32),
+        "tile_counts": np.zeros(T, np.int32), "tile_bins": np.zeros((T, 2), np.int32),
+        "img": np.zeros((H, W, 3), st.dtype), "final_T": np.zeros((H, W), st.dtype),
+        "final_idx": np.zeros((H, W), np.int32),
+    }
+    train = render_args is None
+    if train:
+        out.update({"v_rendered": np.zeros((H, W, 3), st.dtype), "v_xy": np.zeros((N, 2), st.dtype),
+                    "v_conic": np.zeros((N, 3), st.dtype), "v_rgb": np.zeros((N, 3), st.dtype),
+                    "v_opac": np.zeros(N, st.dtype), "v_mean": np.zeros((N, 3), st.dtype),
+                    "v_scale": np.zeros((N, 3), st.dtype), "v_quat": np.zeros((N, 4), st.dtype)})
+    ids_cap = int(N) * 48 if want_ids else 0
+    ids = np.zeros(max(ids_cap, 1), np.int32)
+    o = O()
+    for k, a in out.items():
+        if a.dtype == np.int32:
+            setattr(o, k, a.ctypes.data_as(C.POINTER(C.c_int32)))
+        else:
+            setattr(o, k, a.ctypes.data_as(C.POINTER(ft)))
+    o.ids = ids.ctypes.data_as(C.POINTER(C.c_int32)) if want_ids else None
+    o.ids_cap = ids_cap
+    s = st.struct()
+    if train:
+        getattr(L, fn_name)(s, cam, gt.ctypes.data, hp, o)
+    else:
+        deg_use, bg = render_args
+        bgarr = (C.c_float * 3)(*bg)
+        getattr(L, fn_name)(s, cam, deg_use, bgarr, o)
+    out["num_isect"] = o.num_isect
+    out["num_padded"] = o.num_padded
+    out["overflow"] = o.overflow
+    out["k_max"] = o.k_max
+    if want_ids:
+        out["ids"] = ids[: o.num_isect].copy()
+    if train:
+        out["loss_sum"] = o.loss_sum
+        out["loss"] = o.loss_sum / (W * H)
+        out["nan_adam"] = list(o.nan_adam)
+    return out
+
+
+def train_step(st: HostState, cam, gt: np.ndarray, hp, want_ids=True):
+    """One training step on the host state (updated in place); returns the intermediates."""
+    gt = np.ascontiguousarray(gt, np.uint8)
+    return _run("oracle_train_step_f64" if st.dtype == np.float64 else "oracle_train_step_f32", st, cam, gt, hp,
+                want_ids)
+
+
+def render(st: HostState, cam, degrees_to_use: int, bg, want_ids=True):
+    return _run("oracle_render_f64" if st.dtype == np.float64 else "oracle_render_f32", st, cam, None, None,
+                want_ids, render_args=(degrees_to_use, bg))
+
+
+def set_threads(n: int) -> int:
+    return lib().oracle_set_threads(n)
