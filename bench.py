@@ -56,9 +56,8 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "described in DESIGN.md, not implemented)"}))
         sys.exit(2)
-    import torch
     from msplat_amd import scene
-    from msplat_amd.device import DeviceState, Engine
+    from msplat_amd.device import DeviceArray, DeviceState, Engine
 
     W, H, N = a.width, a.height, a.n
     t_setup = time.time()
@@ -72,10 +71,10 @@ def main():
     for cam in cams:
         eng.render(hst, cam, 3, scene.DEFAULT_BG)
         img = eng.raster(W, H)["img"]
-        gts.append(torch.from_numpy(scene.quantize(img)).cuda())
+        q = scene.quantize(img)
+        gts.append(DeviceArray(q.shape, np.uint8, q))
     del hst
     st = DeviceState(g)
-    torch.cuda.synchronize()
     log(f"setup {time.time() - t_setup:.1f}s")
 
     order = np.random.default_rng(42).permutation(len(cams))
@@ -93,11 +92,10 @@ def main():
         for _ in range(k):
             step += 1
             ci = order[step % len(cams)]
-            eng.train_step(st, cams[ci], gts[ci], hp_for(step), sync_inputs=False)
+            eng.train_step(st, cams[ci], gts[ci], hp_for(step))
 
     run(a.warmup)
     eng.sync()
-    torch.cuda.synchronize()
     t0 = time.perf_counter()
     run(a.steps)
     eng.sync()
@@ -151,7 +149,7 @@ def main():
         for k in range(a.cpu_steps):
             s2 = step + 1 + k
             ci = order[s2 % len(cams)]
-            gt = gts[ci].cpu().numpy()
+            gt = gts[ci].download()
             tc = time.perf_counter()
             O.train_step(ost, cams[ci], gt, hp_for(s2), want_ids=False)
             times.append(time.perf_counter() - tc)

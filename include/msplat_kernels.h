@@ -32,6 +32,15 @@ typedef struct {
 
 typedef struct MsplatEngine MsplatEngine;
 
+/* Device memory helpers (hipMalloc / hipFree / synchronous copies on the current device),
+ * so that callers need no second GPU runtime to own the arrays they pass in. */
+int msplat_dev_count(void);
+void* msplat_dev_alloc(size_t bytes);
+void msplat_dev_free(void* p);
+int msplat_dev_upload(void* dst, const void* src, size_t bytes);
+int msplat_dev_download(void* dst, const void* src, size_t bytes);
+int msplat_dev_memset(void* dst, int value, size_t bytes);
+
 /* Creates an engine bound to the current HIP device (buffers grow on demand and are
  * reused across steps). Replaces the reference's global MetalContext + FusedTensorCache
  * (core/metal/msplat_metal.mm:41-183, 492-625). NULL if no HIP device is usable. */

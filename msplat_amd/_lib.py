@@ -87,6 +87,12 @@ def lib() -> C.CDLL:
     fp = C.POINTER(C.c_float)
     sig = {
         # kernel-level ABI (msplat_kernels.h)
+        "msplat_dev_count": ([], C.c_int),
+        "msplat_dev_alloc": ([C.c_size_t], vp),
+        "msplat_dev_free": ([vp], None),
+        "msplat_dev_upload": ([vp, vp, C.c_size_t], C.c_int),
+        "msplat_dev_download": ([vp, vp, C.c_size_t], C.c_int),
+        "msplat_dev_memset": ([vp, C.c_int, C.c_size_t], C.c_int),
         "msplat_engine_create": ([], vp),
         "msplat_engine_destroy": ([vp], None),
         "msplat_engine_train_step": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), vp,

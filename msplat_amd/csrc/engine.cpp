@@ -315,6 +315,28 @@ struct MsplatEngine {
 
 extern "C" {
 
+int msplat_dev_count(void) {
+    int n = 0;
+    return hipGetDeviceCount(&n) == hipSuccess ? n : 0;
+}
+void* msplat_dev_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 256) != hipSuccess) return nullptr;
+    return p;
+}
+void msplat_dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+int msplat_dev_upload(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? 0 : -2;
+}
+int msplat_dev_download(void* dst, const void* src, size_t bytes) {
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? 0 : -2;
+}
+int msplat_dev_memset(void* dst, int value, size_t bytes) {
+    return hipMemset(dst, value, bytes) == hipSuccess ? 0 : -2;
+}
+
 MsplatEngine* msplat_engine_create(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return nullptr;

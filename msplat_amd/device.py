@@ -1,8 +1,8 @@
 """Device-resident training state and the kernel-level engine, driven from Python.
 
-PyTorch is used only to own device memory (tensors on cuda:0 under ROCm); every kernel
-that runs is one of the library's hand-written HIP kernels, called through the C ABI of
-include/msplat_kernels.h.
+Device memory is allocated through the library's own HIP runtime (msplat_dev_* in
+include/msplat_kernels.h); numpy arrays are the host side. Every kernel that runs is one
+of the library's hand-written HIP kernels.
 """
 from __future__ import annotations
 
@@ -15,35 +15,79 @@ from . import _lib
 GROUPS = ("means", "scales", "quats", "f_dc", "f_rest", "opac")
 
 
-def _torch():
-    import torch
-    if not torch.cuda.is_available():
+class DeviceArray:
+    """A float32/uint8/... array in device memory (owned, freed on close)."""
+
+    def __init__(self, shape, dtype=np.float32, host: np.ndarray | None = None):
+        self.L = _lib.lib()
+        self.shape = tuple(int(s) for s in shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape, dtype=np.int64)) * self.dtype.itemsize
+        self.ptr = self.L.msplat_dev_alloc(max(self.nbytes, 256))
+        if not self.ptr:
+            raise MemoryError(f"device allocation of {self.nbytes} bytes failed")
+        if host is not None:
+            self.upload(host)
+        else:
+            self.zero()
+
+    def upload(self, host: np.ndarray, offset_rows: int = 0):
+        h = np.ascontiguousarray(host, self.dtype)
+        row = self.nbytes // max(self.shape[0], 1) if self.shape else self.nbytes
+        if h.nbytes + offset_rows * row > self.nbytes:
+            raise ValueError("upload larger than the device array")
+        if h.nbytes and self.L.msplat_dev_upload(C.c_void_p(self.ptr + offset_rows * row), h.ctypes.data, h.nbytes):
+            raise RuntimeError("device upload failed")
+
+    def download(self, rows: int | None = None) -> np.ndarray:
+        shape = self.shape if rows is None else (rows,) + self.shape[1:]
+        out = np.empty(shape, self.dtype)
+        if out.nbytes and self.L.msplat_dev_download(out.ctypes.data, self.ptr, out.nbytes):
+            raise RuntimeError("device download failed")
+        return out
+
+    def zero(self):
+        if self.L.msplat_dev_memset(self.ptr, 0, max(self.nbytes, 256)):
+            raise RuntimeError("device memset failed")
+
+    def close(self):
+        if self.ptr:
+            self.L.msplat_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def require_device():
+    if _lib.lib().msplat_dev_count() <= 0:
         raise RuntimeError("msplat_amd: no HIP device is available")
-    return torch
 
 
 class DeviceState:
     """Parameters + Adam moments on the GPU (cap-sized rows, first N active)."""
 
     def __init__(self, g: dict, capacity: int | None = None, moments=None):
-        torch = _torch()
+        require_device()
         n = g["means"].shape[0]
         self.N = n
         self.degree = int(g["degree"])
         cap = capacity or n
         self.cap = cap
-        dev = torch.device("cuda:0")
 
         def up(a):
             a = np.ascontiguousarray(a, np.float32)
-            t = torch.zeros((cap,) + a.shape[1:], dtype=torch.float32, device=dev)
+            d = DeviceArray((cap,) + a.shape[1:], np.float32)
             if n:
-                t[:n] = torch.from_numpy(a).to(dev)
-            return t
+                d.upload(a)
+            return d
         self.p = {k: up(g[k]) for k in GROUPS}
         if moments is None:
-            self.m = {k: torch.zeros_like(self.p[k]) for k in GROUPS}
-            self.v = {k: torch.zeros_like(self.p[k]) for k in GROUPS}
+            self.m = {k: DeviceArray(self.p[k].shape) for k in GROUPS}
+            self.v = {k: DeviceArray(self.p[k].shape) for k in GROUPS}
         else:
             self.m = {k: up(moments[0][k]) for k in GROUPS}
             self.v = {k: up(moments[1][k]) for k in GROUPS}
@@ -51,30 +95,28 @@ class DeviceState:
     def struct(self) -> _lib.MsplatDeviceState:
         s = _lib.MsplatDeviceState()
         s.N, s.degree = self.N, self.degree
-        s.means, s.scales, s.quats = self.p["means"].data_ptr(), self.p["scales"].data_ptr(), self.p["quats"].data_ptr()
-        s.f_dc, s.f_rest, s.opac = self.p["f_dc"].data_ptr(), self.p["f_rest"].data_ptr(), self.p["opac"].data_ptr()
+        s.means, s.scales, s.quats = self.p["means"].ptr, self.p["scales"].ptr, self.p["quats"].ptr
+        s.f_dc, s.f_rest, s.opac = self.p["f_dc"].ptr, self.p["f_rest"].ptr, self.p["opac"].ptr
         for gi, k in enumerate(GROUPS):
-            s.m[gi] = self.m[k].data_ptr()
-            s.v[gi] = self.v[k].data_ptr()
+            s.m[gi] = self.m[k].ptr
+            s.v[gi] = self.v[k].ptr
         return s
 
     def to_host(self) -> dict:
-        _torch().cuda.synchronize()
-        out = {k: self.p[k][: self.N].cpu().numpy() for k in GROUPS}
+        out = {k: self.p[k].download(self.N) for k in GROUPS}
         out["degree"] = self.degree
         return out
 
     def moments_host(self):
-        _torch().cuda.synchronize()
-        return ({k: self.m[k][: self.N].cpu().numpy() for k in GROUPS},
-                {k: self.v[k][: self.N].cpu().numpy() for k in GROUPS})
+        return ({k: self.m[k].download(self.N) for k in GROUPS},
+                {k: self.v[k].download(self.N) for k in GROUPS})
 
 
 class Engine:
     """Owner of one MsplatEngine (kernel-level ABI)."""
 
     def __init__(self):
-        _torch()
+        require_device()
         self.L = _lib.lib()
         self.h = self.L.msplat_engine_create()
         if not self.h:
@@ -91,19 +133,14 @@ class Engine:
         except Exception:
             pass
 
-    def train_step(self, st: DeviceState, cam, gt, hp, debug=False, sync_inputs=True):
-        if sync_inputs:
-            _torch().cuda.synchronize()  # inputs written on torch's stream
-        s = st.struct()
-        rc = self.L.msplat_engine_train_step(self.h, s, cam, gt.data_ptr(), hp, 1 if debug else 0)
+    def train_step(self, st: DeviceState, cam, gt: DeviceArray, hp, debug=False):
+        rc = self.L.msplat_engine_train_step(self.h, st.struct(), cam, gt.ptr, hp, 1 if debug else 0)
         if rc != 0:
             raise RuntimeError(f"msplat_engine_train_step failed ({rc})")
 
     def render(self, st: DeviceState, cam, degrees_to_use, bg):
-        _torch().cuda.synchronize()
-        s = st.struct()
         bgarr = (C.c_float * 3)(*bg)
-        rc = self.L.msplat_engine_render(self.h, s, cam, degrees_to_use, bgarr)
+        rc = self.L.msplat_engine_render(self.h, st.struct(), cam, degrees_to_use, bgarr)
         if rc != 0:
             raise RuntimeError(f"msplat_engine_render failed ({rc})")
 

@@ -65,7 +65,7 @@ def test_render_parity(W, H):
 @pytest.mark.parametrize("W,H", [(160, 144), (336, 336)], ids=["chunked_K", "monolithic"])
 def test_train_step_parity(W, H):
     DeviceState, Engine = _gpu()
-    import torch
+    from msplat_amd.device import DeviceArray
     g, hidden, cams = small_scene(W=W, H=H)
     N = g["means"].shape[0]
     eng = Engine()
@@ -74,7 +74,7 @@ def test_train_step_parity(W, H):
         cam = cams[ci]
         geng, img = _render_gt(Engine, DeviceState, hidden, cam)
         gt = scene.quantize(img)
-        gt_dev = torch.from_numpy(gt).cuda()
+        gt_dev = DeviceArray(gt.shape, np.uint8, gt)
         hp = scene.step_params(step, scene.LR_INIT, W, H, step, 3, 3)
         host = dst.to_host()
         mom = dst.moments_host()
