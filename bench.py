@@ -127,11 +127,11 @@ def main():
         "project_sh_fwd": N * (56 + 180 + 68),
         "gaussian_update": N * 408,
         "sh_rest_adam": V * 1080 + N * 28,
-        "isect_sort_pack": N * 24 + 8 * I * 2 + 40 * I * 2,
+
     }
     dom = max(stages, key=lambda k: stages[k])
     roof_k = dom if dom in alg else max(alg, key=lambda k: stages.get(k, 0))
-    achieved = alg[roof_k] / (stages[roof_k] * 1e-3) / 1e9
+    achieved = alg[roof_k] / (stages[roof_k] * 1e-3) / 1e9 if stages.get(roof_k, 0) > 0 else 0.0
     roofline = {"kernel": roof_k, "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "dominant_kernel": dom, "dominant_ms": round(stages[dom], 4)}

@@ -36,8 +36,9 @@ void DBuf::release() {
     cap = 0;
 }
 
-static const char* kStageNames[ST_COUNT] = {"zero_grads", "project_sh_fwd", "isect_sort_pack", "raster_fwd",
-                                            "ssim_loss", "raster_bwd", "sh_rest_adam", "gaussian_update"};
+static const char* kStageNames[ST_COUNT] = {"zero_grads",  "project_sh_fwd", "depth_rank", "tile_count_scan",
+                                            "tile_scatter", "tile_sort_pack", "raster_fwd", "ssim_loss",
+                                            "raster_bwd",  "sh_rest_adam",   "gaussian_update"};
 
 Engine::Engine() {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
@@ -56,8 +57,8 @@ Engine::~Engine() {
     hipEventDestroy(mark_[0]);
     hipEventDestroy(mark_[1]);
     DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
-                   &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &wctr_, &exact_,
-                   &sortoff_, &bins_, &totals_, &big_, &keys_, &pkA_, &pkB_, &pkC_, &img_, &finalT_, &finalIdx_,
+                   &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
+                   &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &keys_, &ids_, &hist_, &img_, &finalT_, &finalIdx_,
                    &vrend_, &hbuf_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
                    &afterC_, &nanc_};
     for (DBuf* b : all) b->release();
@@ -89,11 +90,13 @@ void Engine::prepare(int N, int W, int H, int nb3, bool train) {
     if (capacity_ < 64) capacity_ = 64;
     K_ = k_max_rule(capacity_, T);
     radii_.ensure(n * 4); tiles_.ensure(n * 4); depths_.ensure(n * 4); rect_.ensure(n * 8); aabb_.ensure(n * 8);
-    recA_.ensure(n * 16); recB_.ensure(n * 16); recC_.ensure(n * 8);
-    counts_.ensure((size_t)T * 4); wctr_.ensure((size_t)T * 4); exact_.ensure((size_t)T * 4);
-    sortoff_.ensure((size_t)(T + 1) * 4); bins_.ensure((size_t)T * 8); big_.ensure((size_t)T * 4);
-    keys_.ensure((size_t)capacity_ * 8);
-    pkA_.ensure((size_t)capacity_ * 16); pkB_.ensure((size_t)capacity_ * 16); pkC_.ensure((size_t)capacity_ * 8);
+    recA_.ensure((n + 1) * 16); recB_.ensure((n + 1) * 16); recC_.ensure((n + 1) * 8);  // row N: zero record
+    counts_.ensure((size_t)T * 4); exact_.ensure((size_t)T * 4); bins_.ensure((size_t)T * 8);
+    keys_.ensure((size_t)capacity_ * 4);
+    dkeys_.ensure(n * 4); dkeys_out_.ensure(n * 4); dvals_.ensure(n * 4); sorted_ids_.ensure(n * 4);
+    temp_.ensure(msk::lists_temp_bytes(N));
+    ids_.ensure((size_t)capacity_ * 4);
+    hist_.ensure(msk::lists_hist_bytes(N, T));
     img_.ensure(P * 12); finalT_.ensure(P * 4); finalIdx_.ensure(P * 4);
     if (K_ > 1) {
         chunkT_.ensure((size_t)K_ * P * 4); chunkC_.ensure((size_t)K_ * P * 12); chunkIdx_.ensure((size_t)K_ * P * 4);
@@ -142,16 +145,46 @@ void Engine::forward(const msk::Params& P, const MsplatCamera& cam, int degrees_
     msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
                     aabb_.as<float2>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
     msk::project_forward(stream_, P, cam, degrees_to_use, pj, nanc_.as<uint32_t>());
+    // the transparent record N
+    msk::fill_zero(stream_, recA_.as<float4>() + P.N, 16);
+    msk::fill_zero(stream_, recB_.as<float4>() + P.N, 16);
+    msk::fill_zero(stream_, recC_.as<float2>() + P.N, 8);
     stage_event(ST_PROJECT);
-    msk::ListBufs L{counts_.as<uint32_t>(), wctr_.as<uint32_t>(), exact_.as<uint32_t>(), sortoff_.as<uint32_t>(),
-                    bins_.as<int2>(), keys_.as<uint64_t>(), {pkA_.as<float4>(), pkB_.as<float4>(), pkC_.as<float2>()},
-                    totals_.as<uint32_t>(), capacity_, capacity_, big_.as<uint32_t>()};
-    msk::build_lists(stream_, P.N, pj, tbx_, tby_, L, nanc_.as<uint32_t>());
-    stage_event(ST_LISTS);
+    lists_ = list_bufs();
+    msk::ListBufs& L = lists_;
+    msk::lists_rank(stream_, P.N, pj, L);
+    stage_event(ST_RANK);
+    msk::lists_count(stream_, P.N, pj, tbx_, tby_, L);
+    stage_event(ST_COUNTS);
+    msk::lists_scatter(stream_, P.N, pj, tbx_, L);
+    stage_event(ST_SCATTER);
+    msk::lists_sort_pack(stream_, P.N, pj, tbx_, tby_, L);
+    stage_event(ST_SORT);
     msk::RasterBufs R{img_.as<float>(), finalT_.as<float>(), finalIdx_.as<int32_t>(), chunkT_.as<float>(),
                       chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
     msk::raster_forward(stream_, L, W_, H_, tbx_, tby_, bg, R);
     stage_event(ST_RASTER_FWD);
+}
+
+msk::ListBufs Engine::list_bufs() {
+    msk::ListBufs L;
+    L.tile_counts = counts_.as<uint32_t>();
+    L.exact_off = exact_.as<uint32_t>();
+    L.bins = bins_.as<int2>();
+    L.keys = keys_.as<uint32_t>();
+    L.ids = ids_.as<uint32_t>();
+    L.view = {ids_.as<uint32_t>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
+    L.hist = hist_.as<uint32_t>();
+    L.totals = totals_.as<uint32_t>();
+    L.capacity = capacity_;
+    L.num_tiles = tbx_ * tby_;
+    L.dkeys = dkeys_.as<uint32_t>();
+    L.dkeys_out = dkeys_out_.as<uint32_t>();
+    L.dvals = dvals_.as<uint32_t>();
+    L.sorted_ids = sorted_ids_.as<uint32_t>();
+    L.temp = temp_.p;
+    L.temp_bytes = temp_.cap;
+    return L;
 }
 
 static msk::Params to_params(const MsplatDeviceState& st) {
@@ -170,6 +203,7 @@ int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, con
                        const MsplatStepParams& hp, bool debug) {
     if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0 || !gt) return -1;
     if (cam.width > 65535 * 16 || cam.height > 65535 * 16) return -1;
+    if (!msk::lists_supported(st.N, ((cam.width + 15) / 16) * ((cam.height + 15) / 16))) return -1;
     const msk::Params P = to_params(st);
     prepare(st.N, cam.width, cam.height, nb3_of(st.degree), true);
     last_W = cam.width;
@@ -189,9 +223,7 @@ int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, con
     stage_event(ST_SSIM);
     msk::RasterBufs R{img_.as<float>(), finalT_.as<float>(), finalIdx_.as<int32_t>(), chunkT_.as<float>(),
                       chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
-    msk::ListBufs L{counts_.as<uint32_t>(), wctr_.as<uint32_t>(), exact_.as<uint32_t>(), sortoff_.as<uint32_t>(),
-                    bins_.as<int2>(), keys_.as<uint64_t>(), {pkA_.as<float4>(), pkB_.as<float4>(), pkC_.as<float2>()},
-                    totals_.as<uint32_t>(), capacity_, capacity_, big_.as<uint32_t>()};
+    msk::ListBufs& L = lists_;
     msk::GradBufs G{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
     msk::raster_backward(stream_, L, R, W_, H_, tbx_, tby_, hp.background, vrend_.as<float>(), G);
     stage_event(ST_RASTER_BWD);
@@ -213,6 +245,7 @@ int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, con
 
 int Engine::render(const MsplatDeviceState& st, const MsplatCamera& cam, int degrees_to_use, const float bg[3]) {
     if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0) return -1;
+    if (!msk::lists_supported(st.N, ((cam.width + 15) / 16) * ((cam.height + 15) / 16))) return -1;
     const msk::Params P = to_params(st);
     prepare(st.N, cam.width, cam.height, nb3_of(st.degree), false);
     last_W = cam.width;
@@ -229,13 +262,12 @@ int Engine::buffer(const std::string& name, void** p, size_t* bytes) {
     const Ent ents[] = {
         {"radii", &radii_, n * 4}, {"tiles_hit", &tiles_, n * 4}, {"depths", &depths_, n * 4}, {"rect", &rect_, n * 8},
         {"aabb", &aabb_, n * 8}, {"recA", &recA_, n * 16}, {"recB", &recB_, n * 16}, {"recC", &recC_, n * 8},
-        {"tile_counts", &counts_, T * 4}, {"tile_bins", &bins_, T * 8}, {"sort_off", &sortoff_, (T + 1) * 4},
-        {"pkA", &pkA_, (size_t)capacity_ * 16}, {"pkB", &pkB_, (size_t)capacity_ * 16},
-        {"pkC", &pkC_, (size_t)capacity_ * 8}, {"img", &img_, P * 12}, {"final_T", &finalT_, P * 4},
+        {"tile_counts", &counts_, T * 4}, {"tile_bins", &bins_, T * 8},
+        {"ids", &ids_, (size_t)capacity_ * 4}, {"img", &img_, P * 12}, {"final_T", &finalT_, P * 4},
         {"final_idx", &finalIdx_, P * 4}, {"v_rendered", &vrend_, P * 12}, {"v_xy", &v_xy_, n * 8},
         {"v_conic", &v_conic_, n * 12}, {"v_rgb", &v_rgb_, n * 12}, {"v_opac", &v_opac_, n * 4},
         {"dbg_v_mean", &dbg_vm_, n * 12}, {"dbg_v_scale", &dbg_vs_, n * 12}, {"dbg_v_quat", &dbg_vq_, n * 16},
-        {"dbg_v_opac", &dbg_vo_, n * 4}, {"nan_counts", &nanc_, 64}, {"totals", &totals_, 16},
+        {"dbg_v_opac", &dbg_vo_, n * 4}, {"nan_counts", &nanc_, 64}, {"totals", &totals_, 32}, {"sorted_ids", &sorted_ids_, n * 4},
     };
     for (const Ent& e : ents) {
         if (name == e.nm) {
@@ -251,8 +283,8 @@ int Engine::buffer(const std::string& name, void** p, size_t* bytes) {
 int Engine::sync() { return hipStreamSynchronize(stream_) == hipSuccess ? 0 : -2; }
 
 void Engine::stats(int64_t out[6]) {
-    uint32_t t[4] = {0, 0, 0, 0};
-    if (totals_.p) hipMemcpyAsync(t, totals_.p, 16, hipMemcpyDeviceToHost, stream_);
+    uint32_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (totals_.p) (void)hipMemcpyAsync(t, totals_.p, 32, hipMemcpyDeviceToHost, stream_);
     hipStreamSynchronize(stream_);
     out[0] = t[0];
     out[1] = t[1];

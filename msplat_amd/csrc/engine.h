@@ -19,7 +19,10 @@ struct DBuf {
     template <class T> T* as() const { return reinterpret_cast<T*>(p); }
 };
 
-enum Stage { ST_ZERO = 0, ST_PROJECT, ST_LISTS, ST_RASTER_FWD, ST_SSIM, ST_RASTER_BWD, ST_SH_ADAM, ST_UPDATE, ST_COUNT };
+enum Stage {
+    ST_ZERO = 0, ST_PROJECT, ST_RANK, ST_COUNTS, ST_SCATTER, ST_SORT, ST_RASTER_FWD, ST_SSIM, ST_RASTER_BWD,
+    ST_SH_ADAM, ST_UPDATE, ST_COUNT
+};
 
 class Engine {
 public:
@@ -44,6 +47,7 @@ public:
 private:
     void prepare(int N, int W, int H, int nb3, bool train);
     void forward(const msk::Params& P, const MsplatCamera& cam, int degrees_to_use, const float bg[3]);
+    msk::ListBufs list_bufs();
     void stage_event(int stage_end);
     void flush_events();
 
@@ -56,14 +60,16 @@ private:
     DBuf v_xy_, v_conic_, v_rgb_, v_opac_;
     DBuf dbg_vm_, dbg_vs_, dbg_vq_, dbg_vo_;
     // per-tile
-    DBuf counts_, wctr_, exact_, sortoff_, bins_, totals_, big_;
+    DBuf counts_, exact_, bins_, totals_;
+    DBuf dkeys_, dkeys_out_, dvals_, sorted_ids_, temp_;
     // per-intersection
-    DBuf keys_, pkA_, pkB_, pkC_;
+    DBuf keys_, ids_, hist_;
     // per-pixel
     DBuf img_, finalT_, finalIdx_, vrend_, hbuf_, dbuf_, partials_, loss_;
     DBuf chunkT_, chunkC_, chunkIdx_, prefixT_, afterC_;
     DBuf nanc_;
     bool debug_valid_ = false;
+    msk::ListBufs lists_{};
     // timing
     hipEvent_t mark_[2] = {nullptr, nullptr};
     int mark_n_ = 0;

@@ -21,6 +21,13 @@ struct Records {
     float2* C;
 };
 
+// Tile-list entries as rasterised: Gaussian ids into per-Gaussian records (row N is an
+// all-zero, fully transparent record).
+struct EntryView {
+    const uint32_t* ids;
+    Records rec;
+};
+
 struct ProjOut {
     int32_t* radii;
     int32_t* tiles_hit;
@@ -32,15 +39,19 @@ struct ProjOut {
 
 struct ListBufs {
     uint32_t* tile_counts;     // [T]
-    uint32_t* write_ctr;       // [T]
     uint32_t* exact_off;       // [T]
-    uint32_t* sort_off;        // [T+1]
-    int2* bins;                // [T]
-    uint64_t* keys;            // [key_capacity]
-    Records pk;                // [capacity]
-    uint32_t* totals;          // [8]: exact total, padded total, overflow flag, max count, #large tiles
-    int64_t capacity, key_capacity;
-    uint32_t* big_list;        // [T] tiles with more than SORT_SMALL_CAP entries
+    int2* bins;                // [T] (start, end) of each tile's list
+    uint32_t* keys;            // [capacity] depth ranks, bucketed per tile
+    uint32_t* ids;             // [capacity] sorted per-tile Gaussian ids (the packed lists)
+    EntryView view;            // ids + the projection's records
+    uint32_t* hist;            // [chunks x T] per-chunk tile histograms -> bucket offsets
+    uint32_t* totals;          // [8]: total, padded total (reference layout), overflow flag, max count
+    int64_t capacity;
+    int num_tiles;
+    // global depth order
+    uint32_t *dkeys, *dkeys_out, *dvals, *sorted_ids;  // [N]
+    void* temp;                // radix-sort scratch
+    size_t temp_bytes;
 };
 
 struct RasterBufs {
@@ -68,7 +79,13 @@ struct Params {
 
 void project_forward(hipStream_t s, const Params& P, const MsplatCamera& cam, int degrees_to_use, ProjOut& o,
                      uint32_t* nan_counts);
-void build_lists(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L, uint32_t* nan_counts);
+size_t lists_temp_bytes(int N);
+size_t lists_hist_bytes(int N, int T);
+bool lists_supported(int N, int T);
+void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L);
+void lists_count(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L);
+void lists_scatter(hipStream_t s, int N, const ProjOut& pj, int tbx, ListBufs& L);
+void lists_sort_pack(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L);
 void raster_forward(hipStream_t s, const ListBufs& L, int W, int H, int tbx, int tby, const float bg[3], RasterBufs& R);
 void ssim_loss(hipStream_t s, int W, int H, const float* img, const uint8_t* gt, float ssim_w, float inv_n,
                float* hbuf /*[H W 15]*/, float* dbuf /*[H W 9]*/, float* v_rend, float* loss_partials,

@@ -10,6 +10,8 @@
 // same formulas bit for bit.
 //
 // Reference kernels are cited as metal:<line> (core/metal/msplat_metal.metal).
+#include <hipcub/hipcub.hpp>
+
 #include "kernels.h"
 #include "ms_math.h"
 
@@ -318,30 +320,89 @@ void project_forward(hipStream_t s, const Params& P, const MsplatCamera& cam, in
 }
 
 // ------------------------------------------------------------------------------------
-// A6-A9: per-tile lists. count (metal:2342-2365), prefix_sum_tiles (metal:2376-2435),
-// scatter (metal:2440-2480), hybrid_sort_pack (metal:2489-2627).
-__global__ void k_count(int N, const int32_t* __restrict__ radii, const ushort4* __restrict__ rect, int tbx,
-                        uint32_t* __restrict__ counts) {
+// A6-A9: per-tile lists sorted by (depth, id). The reference counts per-tile
+// intersections (metal:2342-2365), scans them (metal:2376-2435), scatters 64-bit
+// (depth bits << 32 | id) keys into padded buckets (metal:2440-2480) and bitonic-sorts each
+// bucket with inline packing (metal:2489-2627). The same lists are built here as follows.
+//  1. One stable radix sort of the N Gaussians by depth bits gives the global (depth, id)
+//     order; r is a Gaussian's rank in it.
+//  2. The ranks are cut into G-wide chunks, one workgroup each. A workgroup counts its
+//     chunk's tile hits in an LDS histogram (no global atomics); a column scan over the
+//     chunks gives every (chunk, tile) pair its offset inside the tile's bucket.
+//  3. The scatter writes each hit's rank at that offset + an LDS cursor. A tile's bucket is
+//     thus a run of chunk segments in chunk order: all ranks of segment b are below those
+//     of segment b + 1, only the order inside a segment is arbitrary.
+//  4. Each tile's bucket is cut into windows of about 64 entries whose boundaries fall on
+//     segment starts; sorting every window by rank (a bitonic network in registers with
+//     wave shuffles, LDS for the rare long window) sorts the bucket. Packing writes
+//     sorted_ids[rank], the Gaussian id.
+// The resulting per-tile id lists are the reference's sorted lists exactly.
+__global__ void k_depth_keys(int N, const int32_t* __restrict__ radii, const float* __restrict__ depths,
+                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || radii[i] <= 0) return;
-    const ushort4 r = rect[i];
-    for (int y = r.y; y < r.w; y++)
-        for (int x = r.x; x < r.z; x++) atomicAdd(&counts[y * tbx + x], 1u);
+    if (i >= N) return;
+    keys[i] = radii[i] > 0 ? __builtin_bit_cast(uint32_t, depths[i]) : 0xFFFFFFFFu;
+    vals[i] = (uint32_t)i;
 }
 
-__device__ __forceinline__ uint32_t pad_count(uint32_t c) {
+__device__ __forceinline__ uint32_t pad_count(uint32_t c) {  // metal:2248-2256
     if (c == 0u) return 0u;
     uint32_t x = c < 32u ? 32u : c;
     return 1u << (32 - __clz((int)(x - 1u)));
 }
 
+constexpr int HIST_THREADS = 512;
+constexpr int MAX_CHUNK = 1984;  // Gaussians per chunk: every window fits 2048 keys
+
+__global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const uint32_t* __restrict__ sorted_ids,
+                                                            const int32_t* __restrict__ radii,
+                                                            const ushort4* __restrict__ rect, int tbx, int T,
+                                                            uint32_t* __restrict__ mat) {
+    extern __shared__ uint32_t hist[];
+    for (int t = threadIdx.x; t < T; t += HIST_THREADS) hist[t] = 0u;
+    __syncthreads();
+    const int r0 = blockIdx.x * G, r1 = min(N, r0 + G);
+    for (int r = r0 + threadIdx.x; r < r1; r += HIST_THREADS) {
+        const uint32_t g = sorted_ids[r];
+        if (radii[g] <= 0) continue;
+        const ushort4 q = rect[g];
+        for (int y = q.y; y < q.w; y++)
+            for (int x = q.x; x < q.z; x++) atomicAdd(&hist[y * tbx + x], 1u);
+    }
+    __syncthreads();
+    uint32_t* row = mat + (size_t)blockIdx.x * T;
+    for (int t = threadIdx.x; t < T; t += HIST_THREADS) row[t] = hist[t];
+}
+
+// Exclusive scan of each tile's column over the chunks; the column total is the tile count.
+__global__ void k_col_scan(int T, int nblk, uint32_t* __restrict__ mat, uint32_t* __restrict__ counts) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    uint32_t run = 0;
+    int b = 0;
+    for (; b + 8 <= nblk; b += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = mat[(size_t)(b + u) * T + t];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            mat[(size_t)(b + u) * T + t] = run;
+            run += v[u];
+        }
+    }
+    for (; b < nblk; b++) {
+        const size_t o = (size_t)b * T + t;
+        const uint32_t v = mat[o];
+        mat[o] = run;
+        run += v;
+    }
+    counts[t] = run;
+}
+
 __global__ __launch_bounds__(1024) void k_prefix(int T, const uint32_t* __restrict__ counts,
-                                                 uint32_t* __restrict__ exact_off, uint32_t* __restrict__ sort_off,
-                                                 int2* __restrict__ bins, uint32_t* __restrict__ totals,
-                                                 uint32_t key_cap, uint32_t* __restrict__ big_list, uint32_t cap) {
+                                                 uint32_t* __restrict__ exact_off, int2* __restrict__ bins,
+                                                 uint32_t* __restrict__ totals, uint32_t cap) {
     __shared__ uint32_t lw[17];
-    __shared__ uint32_t nbig;
-    if (threadIdx.x == 0) nbig = 0;
     uint32_t run_ex = 0, run_pd = 0, mx = 0;
     for (int c0 = 0; c0 < T; c0 += 1024) {
         const int t = c0 + threadIdx.x;
@@ -353,177 +414,243 @@ __global__ __launch_bounds__(1024) void k_prefix(int T, const uint32_t* __restri
         const uint32_t pd = block_exclusive_scan(pv, lw, &tot_p);
         if (t < T) {
             exact_off[t] = run_ex + ex;
-            sort_off[t] = run_pd + pd;
-            bins[t] = make_int2((int)min(run_ex + ex, cap), (int)min(run_ex + ex + val, cap));  // never past the packed buffers
-            if (val > (uint32_t)SORT_SMALL_CAP) big_list[atomicAdd(&nbig, 1u)] = (uint32_t)t;
+            // never past the list buffers (the reference scatter drops overflow, metal:2474-2477)
+            bins[t] = make_int2((int)min(run_ex + ex, cap), (int)min(run_ex + ex + val, cap));
         }
         run_ex += tot_e;
         run_pd += tot_p;
     }
-    // block max of counts
     __shared__ uint32_t smax;
     if (threadIdx.x == 0) smax = 0;
     __syncthreads();
     atomicMax(&smax, mx);
     __syncthreads();
     if (threadIdx.x == 0) {
-        sort_off[T] = run_pd;
         totals[0] = run_ex;
-        totals[1] = run_pd;
-        totals[2] = run_pd > key_cap ? 1u : 0u;
+        totals[1] = run_pd;   // what the reference's padded buckets would occupy
+        totals[2] = run_ex > cap ? 1u : 0u;
         totals[3] = smax;
-        totals[4] = nbig;
     }
 }
 
-__global__ void k_scatter(int N, const int32_t* __restrict__ radii, const ushort4* __restrict__ rect,
-                          const float* __restrict__ depths, int tbx, const uint32_t* __restrict__ sort_off,
-                          uint32_t* __restrict__ wctr, uint64_t* __restrict__ keys, uint32_t key_cap) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N || radii[i] <= 0) return;
-    const ushort4 r = rect[i];
-    const uint64_t key = ((uint64_t)__builtin_bit_cast(uint32_t, depths[i]) << 32) | (uint64_t)(uint32_t)i;
-    for (int y = r.y; y < r.w; y++)
-        for (int x = r.x; x < r.z; x++) {
-            const int t = y * tbx + x;
-            const uint32_t pos = atomicAdd(&wctr[t], 1u);
-            const uint32_t dst = sort_off[t] + pos;
-            if (dst < key_cap) keys[dst] = key;  // overflow dropped (metal:2474-2477)
+__global__ __launch_bounds__(HIST_THREADS) void k_tile_scatter(int N, int G, const uint32_t* __restrict__ sorted_ids,
+                                                               const int32_t* __restrict__ radii,
+                                                               const ushort4* __restrict__ rect, int tbx, int T,
+                                                               const uint32_t* __restrict__ mat,
+                                                               const uint32_t* __restrict__ exact_off,
+                                                               uint32_t* __restrict__ keys, uint32_t cap) {
+    extern __shared__ uint32_t cur[];
+    const uint32_t* row = mat + (size_t)blockIdx.x * T;
+    for (int t = threadIdx.x; t < T; t += HIST_THREADS) cur[t] = exact_off[t] + row[t];
+    __syncthreads();
+    const int r0 = blockIdx.x * G, r1 = min(N, r0 + G);
+    for (int r = r0 + threadIdx.x; r < r1; r += HIST_THREADS) {
+        const uint32_t g = sorted_ids[r];
+        if (radii[g] <= 0) continue;
+        const ushort4 q = rect[g];
+        for (int y = q.y; y < q.w; y++)
+            for (int x = q.x; x < q.z; x++) {
+                const uint32_t dst = atomicAdd(&cur[y * tbx + x], 1u);
+                if (dst < cap) keys[dst] = (uint32_t)r;
+            }
+    }
+}
+
+// One compare-exchange stage with partner distance j < 128 on the two consecutive
+// elements e0, e0 + 1 a lane holds (e0 = 2 lane).
+__device__ __forceinline__ void reg_stage(uint32_t& v0, uint32_t& v1, uint32_t e0, uint32_t k, uint32_t j, int lane) {
+    const bool asc = (e0 & k) == 0;
+    if (j == 1) {
+        const uint32_t lo = min(v0, v1), hi = max(v0, v1);
+        v0 = asc ? lo : hi;
+        v1 = asc ? hi : lo;
+    } else {
+        const int m = (int)(j >> 1);
+        const uint32_t p0 = __shfl_xor(v0, m, 64), p1 = __shfl_xor(v1, m, 64);
+        const bool keep_min = (((lane & m) == 0) == asc);
+        v0 = keep_min ? min(v0, p0) : max(v0, p0);
+        v1 = keep_min ? min(v1, p1) : max(v1, p1);
+    }
+}
+
+// Bitonic sort of 64 keys, one per lane, ascending by lane.
+__device__ __forceinline__ uint32_t wave_sort64(uint32_t v, int lane) {
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1)
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint32_t p = __shfl_xor(v, j, 64);
+            const bool keep_min = (((lane & j) == 0) == ((lane & k) == 0));
+            v = keep_min ? min(v, p) : max(v, p);
         }
+    return v;
 }
 
-__device__ __forceinline__ void pack_entry(uint64_t key, uint32_t d, int N, const Records& rec, const Records& pk) {
-    const uint32_t g = (uint32_t)(key & 0xFFFFFFFFull);
-    if (g >= (uint32_t)N) {  // out-of-range id -> transparent (metal:2554-2560)
-        pk.A[d] = make_float4(0.f, 0.f, 0.f, 0.f);
-        pk.B[d] = make_float4(0.f, 0.f, 0.f, 0.f);
-        pk.C[d] = make_float2(0.f, 0.f);
-        return;
-    }
-    pk.A[d] = rec.A[g];
-    pk.B[d] = rec.B[g];
-    pk.C[d] = rec.C[g];
-}
-
-// Bitonic network over n (power of two) keys in LDS, n/2 comparators per stage. Stages whose
-// partner distance j is < 64 touch only elements owned by pairs of one wave when each
-// wave owns a contiguous run of comparators, but the generic form below keeps one barrier
-// per stage for clarity and robustness.
-template <int NT>
-__device__ __forceinline__ void bitonic_lds(uint64_t* a, uint32_t n) {
-    for (uint32_t k = 2; k <= n; k <<= 1) {
+// Bitonic sort of p (power of two, 256..2048) keys in a wave-private LDS region, by one
+// wave: LDS accesses of one wave are performed in issue order, so a wave-level fence
+// between stages is enough.
+__device__ void wave_sort_lds(uint32_t* a, uint32_t p, int lane) {
+    for (uint32_t k = 2; k <= p; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t p = threadIdx.x; p < (n >> 1); p += NT) {
-                const uint32_t lo = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+            for (uint32_t q = lane; q < (p >> 1); q += 64) {
+                const uint32_t lo = ((q & ~(j - 1)) << 1) | (q & (j - 1));
                 const uint32_t hi = lo + j;
                 const bool asc = (lo & k) == 0;
-                const uint64_t x = a[lo], y = a[hi];
+                const uint32_t x = a[lo], y = a[hi];
                 if ((x > y) == asc) { a[lo] = y; a[hi] = x; }
             }
-            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         }
-    }
 }
 
-template <int NT>
-__device__ __forceinline__ void bitonic_global(uint64_t* a, uint32_t n) {
-    for (uint32_t k = 2; k <= n; k <<= 1) {
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t p = threadIdx.x; p < (n >> 1); p += NT) {
-                const uint32_t lo = ((p & ~(j - 1)) << 1) | (p & (j - 1));
-                const uint32_t hi = lo + j;
-                const bool asc = (lo & k) == 0;
-                const uint64_t x = a[lo], y = a[hi];
-                if ((x > y) == asc) { a[lo] = y; a[hi] = x; }
-            }
-            __syncthreads();
-        }
-    }
-}
+constexpr int SORT_THREADS = 256;
+constexpr int WIN = 64;
+constexpr int WAVE_LDS_KEYS = 2048;
 
-constexpr int SORT_SMALL_THREADS = 256;
-
-__global__ __launch_bounds__(SORT_SMALL_THREADS) void k_sort_small(
-    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ sort_off, const uint32_t* __restrict__ exact_off,
-    const uint64_t* __restrict__ keys, Records rec, Records pk, int N, uint32_t capacity, uint32_t key_cap) {
-    __shared__ uint64_t s[SORT_SMALL_CAP];
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(int T, int nblk, const uint32_t* __restrict__ counts,
+                                                            const uint32_t* __restrict__ exact_off,
+                                                            const uint32_t* __restrict__ mat,
+                                                            const uint32_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ sorted_ids,
+                                                            uint32_t* __restrict__ ids, int N, uint32_t cap) {
+    extern __shared__ uint32_t sm[];
+    uint32_t* off = sm;                          // [nblk + 1] segment starts of this tile
+    uint32_t* wl = sm + ((nblk + 4) & ~3);       // [4][WAVE_LDS_KEYS]
     const int t = blockIdx.x;
     const uint32_t count = counts[t];
-    if (count == 0 || count > (uint32_t)SORT_SMALL_CAP) return;
-    const uint32_t src = sort_off[t], padded = sort_off[t + 1] - src, dst = exact_off[t];
-    if (!(sort_off[t + 1] <= key_cap && dst + count <= capacity)) {
-        for (uint32_t i = threadIdx.x; i < count; i += SORT_SMALL_THREADS)
-            if (dst + i < capacity) pack_entry(~0ull, dst + i, 0, rec, pk);
+    if (count == 0) return;
+    const uint32_t base = exact_off[t];
+    if (base + count > cap) {
+        for (uint32_t i = threadIdx.x; i < count; i += SORT_THREADS)
+            if (base + i < cap) ids[base + i] = (uint32_t)N;
         return;
     }
-    for (uint32_t i = threadIdx.x; i < padded; i += SORT_SMALL_THREADS) s[i] = i < count ? keys[src + i] : ~0ull;
+    for (int b = threadIdx.x; b < nblk; b += SORT_THREADS) off[b] = mat[(size_t)b * T + t];
+    if (threadIdx.x == 0) off[nblk] = count;
     __syncthreads();
-    bitonic_lds<SORT_SMALL_THREADS>(s, padded);
-    for (uint32_t i = threadIdx.x; i < count; i += SORT_SMALL_THREADS) pack_entry(s[i], dst + i, N, rec, pk);
-}
-
-constexpr int SORT_LARGE_THREADS = 1024;
-
-__global__ __launch_bounds__(SORT_LARGE_THREADS) void k_sort_large(
-    const uint32_t* __restrict__ counts, const uint32_t* __restrict__ sort_off, const uint32_t* __restrict__ exact_off,
-    uint64_t* __restrict__ keys, Records rec, Records pk, int N, uint32_t capacity, uint32_t key_cap,
-    uint32_t lds_keys, const uint32_t* __restrict__ big_list, const uint32_t* __restrict__ totals) {
-    extern __shared__ uint64_t sl[];
-    const uint32_t nbig = totals[4];
-    for (uint32_t bi = blockIdx.x; bi < nbig; bi += gridDim.x) {  // uniform per workgroup
-        const uint32_t t = big_list[bi];
-        const uint32_t count = counts[t];
-        const uint32_t src = sort_off[t], padded = sort_off[t + 1] - src, dst = exact_off[t];
-        if (!(sort_off[t + 1] <= key_cap && dst + count <= capacity)) {
-            for (uint32_t i = threadIdx.x; i < count; i += SORT_LARGE_THREADS)
-                if (dst + i < capacity) pack_entry(~0ull, dst + i, 0, rec, pk);
-            continue;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t* wlds = wl + wid * WAVE_LDS_KEYS;
+    const uint32_t nwin = (count + WIN - 1) / WIN;
+    // window w = entries [first segment start >= 64 w, first segment start >= 64 (w + 1))
+    auto seg_start_at_or_after = [&](uint32_t x) -> uint32_t {
+        int lo = 0, hi = nblk;  // off[nblk] = count >= x for x <= count
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (off[mid] >= x) hi = mid; else lo = mid + 1;
         }
-        __syncthreads();
-        if (padded <= lds_keys) {
-            for (uint32_t i = threadIdx.x; i < padded; i += SORT_LARGE_THREADS) sl[i] = i < count ? keys[src + i] : ~0ull;
-            __syncthreads();
-            bitonic_lds<SORT_LARGE_THREADS>(sl, padded);
-            for (uint32_t i = threadIdx.x; i < count; i += SORT_LARGE_THREADS) pack_entry(sl[i], dst + i, N, rec, pk);
+        return off[lo];
+    };
+    for (uint32_t w = wid; w < nwin; w += SORT_THREADS / 64) {
+        const uint32_t lo = seg_start_at_or_after(w * WIN);
+        const uint32_t hi = w + 1 < nwin ? seg_start_at_or_after((w + 1) * WIN) : count;
+        if (hi <= lo) continue;
+        const uint32_t n = hi - lo;
+        const uint32_t* ks = keys + base + lo;
+        uint32_t* out = ids + base + lo;
+        if (n <= 64) {
+            uint32_t v = (uint32_t)lane < n ? ks[lane] : 0xFFFFFFFFu;
+            v = wave_sort64(v, lane);
+            if ((uint32_t)lane < n) out[lane] = v < (uint32_t)N ? sorted_ids[v] : (uint32_t)N;
+        } else if (n <= 128) {
+            const uint32_t e0 = 2 * lane;
+            uint32_t v0 = e0 < n ? ks[e0] : 0xFFFFFFFFu, v1 = e0 + 1 < n ? ks[e0 + 1] : 0xFFFFFFFFu;
+            for (uint32_t k = 2; k <= 128; k <<= 1)
+                for (uint32_t j = k >> 1; j > 0; j >>= 1) reg_stage(v0, v1, e0, k, j, lane);
+            if (e0 < n) out[e0] = v0 < (uint32_t)N ? sorted_ids[v0] : (uint32_t)N;
+            if (e0 + 1 < n) out[e0 + 1] = v1 < (uint32_t)N ? sorted_ids[v1] : (uint32_t)N;
         } else {
-            uint64_t* a = keys + src;
-            for (uint32_t i = count + threadIdx.x; i < padded; i += SORT_LARGE_THREADS) a[i] = ~0ull;
-            __syncthreads();
-            bitonic_global<SORT_LARGE_THREADS>(a, padded);
-            for (uint32_t i = threadIdx.x; i < count; i += SORT_LARGE_THREADS) pack_entry(a[i], dst + i, N, rec, pk);
+            uint32_t p = 256;
+            while (p < n) p <<= 1;  // n <= MAX_CHUNK + WIN - 1 < WAVE_LDS_KEYS
+            for (uint32_t i = lane; i < p; i += 64) wlds[i] = i < n ? ks[i] : 0xFFFFFFFFu;
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            wave_sort_lds(wlds, p, lane);
+            for (uint32_t i = lane; i < n; i += 64) {
+                const uint32_t v = wlds[i];
+                out[i] = v < (uint32_t)N ? sorted_ids[v] : (uint32_t)N;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
         }
     }
 }
 
-static uint32_t g_large_lds_keys = 0;
+static int hist_blocks(int N) {
+    int nb = (N + MAX_CHUNK - 1) / MAX_CHUNK;
+    return nb < 1 ? 1 : nb;
+}
 
-void build_lists(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L, uint32_t* nan_counts) {
-    (void)nan_counts;
-    const int T = tbx * tby;
-    fill_zero(s, L.tile_counts, (size_t)T * 4);
-    fill_zero(s, L.write_ctr, (size_t)T * 4);
-    if (N > 0) hipLaunchKernelGGL(k_count, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.rect, tbx, L.tile_counts);
-    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, T, L.tile_counts, L.exact_off, L.sort_off, L.bins,
-                       L.totals, (uint32_t)L.key_capacity, L.big_list, (uint32_t)L.capacity);
-    if (N > 0)
-        hipLaunchKernelGGL(k_scatter, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.rect, pj.depths, tbx,
-                           L.sort_off, L.write_ctr, L.keys, (uint32_t)L.key_capacity);
-    hipLaunchKernelGGL(k_sort_small, dim3(T), dim3(SORT_SMALL_THREADS), 0, s, L.tile_counts, L.sort_off, L.exact_off,
-                       L.keys, pj.rec, L.pk, N, (uint32_t)L.capacity, (uint32_t)L.key_capacity);
-    if (g_large_lds_keys == 0) {
+size_t lists_temp_bytes(int N) {
+    size_t bytes = 0;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
+                                             (uint32_t*)nullptr, (uint32_t*)nullptr, N > 0 ? N : 1, 0, 32);
+    return bytes;
+}
+
+size_t lists_hist_bytes(int N, int T) { return (size_t)hist_blocks(N) * (size_t)T * 4; }
+
+static size_t g_lds_limit = 0;
+static size_t lds_limit() {
+    if (g_lds_limit == 0) {
         int dev = 0;
         hipDeviceProp_t prop;
-        size_t lim = 65536;
-        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess) lim = prop.sharedMemPerBlock;
-        uint32_t keys_fit = 1;
-        while ((size_t)(keys_fit * 2) * 8 <= lim && keys_fit * 2 <= 16384) keys_fit *= 2;
-        g_large_lds_keys = keys_fit;
-        (void)hipFuncSetAttribute((const void*)k_sort_large, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  (int)(g_large_lds_keys * 8));
+        g_lds_limit = 65536;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            g_lds_limit = prop.sharedMemPerBlock;
     }
-    hipLaunchKernelGGL(k_sort_large, dim3(T < 512 ? T : 512), dim3(SORT_LARGE_THREADS), (size_t)g_large_lds_keys * 8, s,
-                       L.tile_counts, L.sort_off, L.exact_off, L.keys, pj.rec, L.pk, N, (uint32_t)L.capacity,
-                       (uint32_t)L.key_capacity, g_large_lds_keys, L.big_list, L.totals);
+    return g_lds_limit;
+}
+
+bool lists_supported(int N, int T) {
+    const size_t sort_lds = ((size_t)(hist_blocks(N) + 4) & ~(size_t)3) * 4 + 4 * WAVE_LDS_KEYS * 4;
+    return (size_t)T * 4 <= lds_limit() && sort_lds <= lds_limit();
+}
+
+void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L) {
+    if (N <= 0) return;
+    hipLaunchKernelGGL(k_depth_keys, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.depths, L.dkeys, L.dvals);
+    size_t bytes = L.temp_bytes;
+    (void)hipcub::DeviceRadixSort::SortPairs(L.temp, bytes, L.dkeys, L.dkeys_out, L.dvals, L.sorted_ids, N, 0, 32, s);
+}
+
+void lists_count(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L) {
+    const int T = tbx * tby;
+    if (N > 0) {
+        const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
+        const size_t lds = (size_t)T * 4;
+        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_hist, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, pj.radii, pj.rect,
+                           tbx, T, L.hist);
+        hipLaunchKernelGGL(k_col_scan, dim3(cdiv(T, 64)), dim3(64), 0, s, T, nblk, L.hist, L.tile_counts);
+    } else {
+        fill_zero(s, L.tile_counts, (size_t)T * 4);
+    }
+    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, T, L.tile_counts, L.exact_off, L.bins, L.totals,
+                       (uint32_t)L.capacity);
+}
+
+void lists_scatter(hipStream_t s, int N, const ProjOut& pj, int tbx, ListBufs& L) {
+    if (N <= 0) return;
+    const int T = L.num_tiles;
+    const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
+    const size_t lds = (size_t)T * 4;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_tile_scatter, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, pj.radii, pj.rect, tbx,
+                       T, L.hist, L.exact_off, L.keys, (uint32_t)L.capacity);
+}
+
+void lists_sort_pack(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L) {
+    (void)pj;
+    if (N <= 0) return;
+    const int T = tbx * tby;
+    const int nblk = hist_blocks(N);
+    const size_t lds = ((size_t)(nblk + 4) & ~(size_t)3) * 4 + 4 * WAVE_LDS_KEYS * 4;
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_sort_pack, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_sort_pack, dim3(T), dim3(SORT_THREADS), lds, s, T, nblk, L.tile_counts, L.exact_off, L.hist,
+                       L.keys, L.sorted_ids, L.ids, N, (uint32_t)L.capacity);
 }
 
 // ------------------------------------------------------------------------------------
@@ -532,7 +659,12 @@ void build_lists(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, List
 // (rasterize_forward_chunked_kernel metal:2960-3076 + merge metal:3081-3134).
 constexpr int RB = 256;
 
-__device__ __forceinline__ void composite_range(int lo, int hi, float px, float py, bool inside, const Records& pk,
+// Pixel of a thread inside its 16x16 tile: each 64-lane wave covers one 8x8 quadrant, so
+// the lanes of a wave see nearby pixels (coherent early-outs, fewer divergent passes).
+__device__ __forceinline__ int tile_lx() { return (int)((threadIdx.x & 7) | ((threadIdx.x >> 3) & 8)); }
+__device__ __forceinline__ int tile_ly() { return (int)(((threadIdx.x >> 3) & 7) | ((threadIdx.x >> 4) & 8)); }
+
+__device__ __forceinline__ void composite_range(int lo, int hi, float px, float py, bool inside, const EntryView& ev,
                                                 float& T, float C[3], int& last) {
     __shared__ float4 sA[RB];
     __shared__ float4 sB[RB];
@@ -545,9 +677,10 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
         if (__syncthreads_and(done)) break;
         const int idx = b0 + threadIdx.x;
         if (idx < hi) {
-            sA[threadIdx.x] = pk.A[idx];
-            sB[threadIdx.x] = pk.B[idx];
-            sCx[threadIdx.x] = pk.C[idx].x;
+            const uint32_t g = ev.ids[idx];
+            sA[threadIdx.x] = ev.rec.A[g];
+            sB[threadIdx.x] = ev.rec.B[g];
+            sCx[threadIdx.x] = ev.rec.C[g].x;
         }
         __syncthreads();
         if (done) continue;
@@ -576,12 +709,12 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
     }
 }
 
-__global__ __launch_bounds__(RB) void k_raster_fwd(int W, int H, int tbx, const int2* __restrict__ bins, Records pk,
+__global__ __launch_bounds__(RB) void k_raster_fwd(int W, int H, int tbx, const int2* __restrict__ bins, EntryView pk,
                                                    float bg0, float bg1, float bg2, float* __restrict__ img,
                                                    float* __restrict__ final_T, int32_t* __restrict__ final_idx) {
     const int t = blockIdx.x;
     const int tx = t % tbx, ty = t / tbx;
-    const int j = tx * TILE + (threadIdx.x & 15), i = ty * TILE + (threadIdx.x >> 4);
+    const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
     const bool inside = i < H && j < W;
     const int2 r = bins[t];
     float T, C[3];
@@ -598,11 +731,11 @@ __global__ __launch_bounds__(RB) void k_raster_fwd(int W, int H, int tbx, const 
 }
 
 __global__ __launch_bounds__(RB) void k_raster_fwd_chunk(int W, int H, int tbx, const int2* __restrict__ bins,
-                                                         Records pk, float* __restrict__ chunk_T,
+                                                         EntryView pk, float* __restrict__ chunk_T,
                                                          float* __restrict__ chunk_C, int32_t* __restrict__ chunk_idx) {
     const int t = blockIdx.x, k = blockIdx.y;
     const int tx = t % tbx, ty = t / tbx;
-    const int j = tx * TILE + (threadIdx.x & 15), i = ty * TILE + (threadIdx.x >> 4);
+    const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
     const bool inside = i < H && j < W;
     const int2 r = bins[t];
     const int cs = r.x + k * CHUNK, ce = min(r.x + (k + 1) * CHUNK, r.y);
@@ -657,11 +790,11 @@ __global__ void k_raster_merge(int W, int H, int K, const float* __restrict__ ch
 void raster_forward(hipStream_t s, const ListBufs& L, int W, int H, int tbx, int tby, const float bg[3], RasterBufs& R) {
     const int T = tbx * tby;
     if (R.K <= 1) {
-        hipLaunchKernelGGL(k_raster_fwd, dim3(T), dim3(RB), 0, s, W, H, tbx, L.bins, L.pk, bg[0], bg[1], bg[2], R.img,
+        hipLaunchKernelGGL(k_raster_fwd, dim3(T), dim3(RB), 0, s, W, H, tbx, L.bins, L.view, bg[0], bg[1], bg[2], R.img,
                            R.final_T, R.final_idx);
         return;
     }
-    hipLaunchKernelGGL(k_raster_fwd_chunk, dim3(T, R.K), dim3(RB), 0, s, W, H, tbx, L.bins, L.pk, R.chunk_T, R.chunk_C,
+    hipLaunchKernelGGL(k_raster_fwd_chunk, dim3(T, R.K), dim3(RB), 0, s, W, H, tbx, L.bins, L.view, R.chunk_T, R.chunk_C,
                        R.chunk_idx);
     hipLaunchKernelGGL(k_raster_merge, dim3(cdiv((size_t)W * H, 256)), dim3(256), 0, s, W, H, (int)R.K, R.chunk_T,
                        R.chunk_C, R.chunk_idx, bg[0], bg[1], bg[2], R.img, R.final_T, R.final_idx);
@@ -889,7 +1022,7 @@ constexpr int BB = 128;  // backward batch
 __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bool inside, float px, float py,
                                                float T, float buf0, float buf1, float buf2, float T_final,
                                                const float bg[3], float vo0, float vo1, float vo2,
-                                               const Records& pk, GradBufs& G, int tile_max) {
+                                               const EntryView& ev, GradBufs& G, int tile_max) {
     __shared__ float4 sA[BB];
     __shared__ float4 sB[BB];
     __shared__ float2 sC[BB];
@@ -903,9 +1036,10 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
         __syncthreads();
         if ((int)threadIdx.x < n) {
             const int idx = bhi - 1 - threadIdx.x;  // slot 0 = furthest back
-            sA[threadIdx.x] = pk.A[idx];
-            sB[threadIdx.x] = pk.B[idx];
-            sC[threadIdx.x] = pk.C[idx];
+            const uint32_t g = ev.ids[idx];
+            sA[threadIdx.x] = ev.rec.A[g];
+            sB[threadIdx.x] = ev.rec.B[g];
+            sC[threadIdx.x] = make_float2(ev.rec.C[g].x, __builtin_bit_cast(float, g));
         }
         __syncthreads();
         for (int tt = 0; tt < n; tt++) {
@@ -1004,13 +1138,13 @@ __device__ __forceinline__ int block_max_int(int v) {
     return r;
 }
 
-__global__ __launch_bounds__(RB) void k_raster_bwd(int W, int H, int tbx, const int2* __restrict__ bins, Records pk,
+__global__ __launch_bounds__(RB) void k_raster_bwd(int W, int H, int tbx, const int2* __restrict__ bins, EntryView pk,
                                                    float bg0, float bg1, float bg2, const float* __restrict__ final_T,
                                                    const int32_t* __restrict__ final_idx,
                                                    const float* __restrict__ v_rend, GradBufs G) {
     const int t = blockIdx.x;
     const int tx = t % tbx, ty = t / tbx;
-    const int j = tx * TILE + (threadIdx.x & 15), i = ty * TILE + (threadIdx.x >> 4);
+    const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
     const bool inside = i < H && j < W;
     const int2 r = bins[t];
     if (r.y <= r.x) return;
@@ -1048,7 +1182,7 @@ __global__ void k_chunk_prefix_suffix(int W, int H, int K, const float* __restri
 }
 
 __global__ __launch_bounds__(RB) void k_raster_bwd_chunk(int W, int H, int tbx, const int2* __restrict__ bins,
-                                                         Records pk, float bg0, float bg1, float bg2,
+                                                         EntryView pk, float bg0, float bg1, float bg2,
                                                          const float* __restrict__ final_T,
                                                          const int32_t* __restrict__ chunk_idx,
                                                          const float* __restrict__ prefix_T,
@@ -1057,7 +1191,7 @@ __global__ __launch_bounds__(RB) void k_raster_bwd_chunk(int W, int H, int tbx, 
                                                          const float* __restrict__ v_rend, GradBufs G) {
     const int t = blockIdx.x, k = blockIdx.y;
     const int tx = t % tbx, ty = t / tbx;
-    const int j = tx * TILE + (threadIdx.x & 15), i = ty * TILE + (threadIdx.x >> 4);
+    const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
     const bool inside = i < H && j < W;
     const int2 r = bins[t];
     const int cs = r.x + k * CHUNK, ce = min(r.x + (k + 1) * CHUNK, r.y);
@@ -1082,13 +1216,13 @@ void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, int 
                      const float bg[3], const float* v_rend, GradBufs& G) {
     const int T = tbx * tby;
     if (R.K <= 1) {
-        hipLaunchKernelGGL(k_raster_bwd, dim3(T), dim3(RB), 0, s, W, H, tbx, L.bins, L.pk, bg[0], bg[1], bg[2], R.final_T,
+        hipLaunchKernelGGL(k_raster_bwd, dim3(T), dim3(RB), 0, s, W, H, tbx, L.bins, L.view, bg[0], bg[1], bg[2], R.final_T,
                            R.final_idx, v_rend, G);
         return;
     }
     hipLaunchKernelGGL(k_chunk_prefix_suffix, dim3(cdiv((size_t)W * H, 256)), dim3(256), 0, s, W, H, (int)R.K,
                        R.chunk_T, R.chunk_C, R.prefix_T, R.after_C);
-    hipLaunchKernelGGL(k_raster_bwd_chunk, dim3(T, R.K), dim3(RB), 0, s, W, H, tbx, L.bins, L.pk, bg[0], bg[1], bg[2],
+    hipLaunchKernelGGL(k_raster_bwd_chunk, dim3(T, R.K), dim3(RB), 0, s, W, H, tbx, L.bins, L.view, bg[0], bg[1], bg[2],
                        R.final_T, R.chunk_idx, R.prefix_T, R.chunk_T, R.after_C, v_rend, G);
 }
 
@@ -1127,29 +1261,96 @@ __device__ __forceinline__ float sh_basis_one(int b, float x, float y, float z) 
     }
 }
 
+// SH basis values of degrees 1..3 in the order of the fused SH backward (metal:1993-2033).
+__device__ __forceinline__ void sh_basis_all(float x, float y, float z, float* b) {
+    const float xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
+    b[0] = -SH_C1 * y;
+    b[1] = SH_C1 * z;
+    b[2] = -SH_C1 * x;
+    b[3] = SH_C2_0 * xy;
+    b[4] = SH_C2_1 * yz;
+    b[5] = SH_C2_2 * (2.f * zz - xx - yy);
+    b[6] = SH_C2_3 * xz;
+    b[7] = SH_C2_4 * (xx - yy);
+    b[8] = SH_C3_0 * y * (3.f * xx - yy);
+    b[9] = SH_C3_1 * xy * z;
+    b[10] = SH_C3_2 * y * (4.f * zz - xx - yy);
+    b[11] = SH_C3_3 * z * (2.f * zz - 3.f * xx - 3.f * yy);
+    b[12] = SH_C3_4 * x * (4.f * zz - xx - yy);
+    b[13] = SH_C3_5 * z * (xx - yy);
+    b[14] = SH_C3_6 * x * (xx - 3.f * yy);
+}
+
+// Each workgroup updates 1024 consecutive coefficients: the Gaussians they belong to get
+// their basis values and colour gradient staged in LDS once, then every thread updates 4
+// coefficients with 16-byte loads and stores of parameter, exp_avg and exp_avg_sq.
+constexpr int SH_EPB = 1024;
+
 template <int NB3>
 __global__ __launch_bounds__(256) void k_sh_rest_adam(int N, int nuse3, const float* __restrict__ means, float cpx,
                                                       float cpy, float cpz, const int32_t* __restrict__ radii,
                                                       const float* __restrict__ v_rgb, float* __restrict__ f_rest,
                                                       float* __restrict__ m4, float* __restrict__ v4, float step,
                                                       float bc2s, float b1, float b2, float eps) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)N * NB3) return;
-    const int i = (int)(e / NB3);
-    const int r = (int)(e - (size_t)i * NB3);
-    if (r >= nuse3 || radii[i] <= 0) return;
-    const float vc0 = v_rgb[3 * (size_t)i], vc1 = v_rgb[3 * (size_t)i + 1], vc2 = v_rgb[3 * (size_t)i + 2];
-    if (!(ms_finite(vc0) && ms_finite(vc1) && ms_finite(vc2))) return;  // SH Adam gate (metal:1975-1978)
-    const float d0 = means[3 * (size_t)i] - cpx, d1 = means[3 * (size_t)i + 1] - cpy, d2 = means[3 * (size_t)i + 2] - cpz;
-    const float inv = 1.f / sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
-    const int b = r / 3, c = r - 3 * (r / 3);
-    const float vc = c == 0 ? vc0 : (c == 1 ? vc1 : vc2);
-    const float g = sh_basis_one(b, d0 * inv, d1 * inv, d2 * inv) * vc;
-    float p = f_rest[e], m = m4[e], v = v4[e];
-    adam_el(p, m, v, g, step, b1, b2, bc2s, eps);
-    f_rest[e] = p;
-    m4[e] = m;
-    v4[e] = v;
+    constexpr int MAXG = SH_EPB / NB3 + 2;
+    __shared__ float s_b[MAXG][16];  // 15 basis values + visibility flag
+    __shared__ float s_vc[MAXG][3];
+    const size_t total = (size_t)N * NB3;
+    const size_t e0 = (size_t)blockIdx.x * SH_EPB;
+    const int g0 = (int)(e0 / NB3);
+    const size_t e_last = min(total, e0 + SH_EPB) - 1;
+    const int ng = (int)(e_last / NB3) - g0 + 1;
+    for (int k = threadIdx.x; k < ng; k += 256) {
+        const int g = g0 + k;
+        const float vc0 = v_rgb[3 * (size_t)g], vc1 = v_rgb[3 * (size_t)g + 1], vc2 = v_rgb[3 * (size_t)g + 2];
+        // visible Gaussians with a finite colour gradient only (metal:1894, 1975-1978)
+        const bool ok = radii[g] > 0 && ms_finite(vc0) && ms_finite(vc1) && ms_finite(vc2);
+        s_b[k][15] = ok ? 1.f : 0.f;
+        s_vc[k][0] = vc0; s_vc[k][1] = vc1; s_vc[k][2] = vc2;
+        if (ok) {
+            const float d0 = means[3 * (size_t)g] - cpx, d1 = means[3 * (size_t)g + 1] - cpy, d2 = means[3 * (size_t)g + 2] - cpz;
+            const float inv = 1.f / sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+            sh_basis_all(d0 * inv, d1 * inv, d2 * inv, s_b[k]);
+        }
+    }
+    __syncthreads();
+    const size_t e = e0 + 4 * (size_t)threadIdx.x;
+    if (e >= total) return;
+    float p[4], m[4], v[4];
+    const bool vec = e + 4 <= total;
+    if (vec) {
+        const float4 a = reinterpret_cast<const float4*>(f_rest)[e >> 2];
+        const float4 bm = reinterpret_cast<const float4*>(m4)[e >> 2];
+        const float4 bv = reinterpret_cast<const float4*>(v4)[e >> 2];
+        p[0] = a.x; p[1] = a.y; p[2] = a.z; p[3] = a.w;
+        m[0] = bm.x; m[1] = bm.y; m[2] = bm.z; m[3] = bm.w;
+        v[0] = bv.x; v[1] = bv.y; v[2] = bv.z; v[3] = bv.w;
+    } else {
+        for (int q = 0; q < 4; q++) {
+            const bool in = e + q < total;
+            p[q] = in ? f_rest[e + q] : 0.f; m[q] = in ? m4[e + q] : 0.f; v[q] = in ? v4[e + q] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const size_t el = e + q;
+        if (el >= total) break;
+        const int g = (int)(el / NB3);
+        const int r = (int)(el - (size_t)g * NB3);
+        const int k = g - g0;
+        if (s_b[k][15] == 0.f || r >= nuse3) continue;
+        const int b = r / 3, c = r - 3 * b;
+        const float g_el = s_b[k][b] * s_vc[k][c];
+        adam_el(p[q], m[q], v[q], g_el, step, b1, b2, bc2s, eps);
+    }
+    if (vec) {
+        reinterpret_cast<float4*>(f_rest)[e >> 2] = make_float4(p[0], p[1], p[2], p[3]);
+        reinterpret_cast<float4*>(m4)[e >> 2] = make_float4(m[0], m[1], m[2], m[3]);
+        reinterpret_cast<float4*>(v4)[e >> 2] = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        for (int q = 0; q < 4; q++)
+            if (e + q < total) { f_rest[e + q] = p[q]; m4[e + q] = m[q]; v4[e + q] = v[q]; }
+    }
 }
 
 void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const int32_t* radii, const float* v_rgb,
@@ -1160,7 +1361,7 @@ void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const
     const int nb3 = (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15) * 3;
     const size_t total = (size_t)P.N * nb3;
 #define LAUNCH_SH(NB)                                                                                              \
-    hipLaunchKernelGGL(k_sh_rest_adam<NB>, dim3(cdiv(total, 256)), dim3(256), 0, s, P.N, nuse3, P.means, cam.cam_pos[0], \
+    hipLaunchKernelGGL(k_sh_rest_adam<NB>, dim3(cdiv(total, SH_EPB)), dim3(256), 0, s, P.N, nuse3, P.means, cam.cam_pos[0], \
                        cam.cam_pos[1], cam.cam_pos[2], radii, v_rgb, P.f_rest, P.m[4], P.v[4], hp.step_size[4],     \
                        hp.bc2_sqrt[4], hp.beta1, hp.beta2, hp.eps)
     if (nb3 == 9) LAUNCH_SH(9);

@@ -180,11 +180,10 @@ class Engine:
         }
 
     def lists(self, num_tiles: int, num_isect: int) -> dict:
-        pkC = self.read("pkC", np.float32, (num_isect, 2))
         return {
             "tile_counts": self.read("tile_counts", np.int32, (num_tiles,)),
             "tile_bins": self.read("tile_bins", np.int32, (num_tiles, 2)),
-            "ids": pkC[:, 1].view(np.int32).copy(),
+            "ids": self.read("ids", np.int32, (num_isect,)),
         }
 
     def raster(self, W: int, H: int) -> dict:

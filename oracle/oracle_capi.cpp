@@ -158,3 +158,83 @@ int oracle_mcmc_step(OracleStateF32* s, int32_t cap_max, float cull_radius, uint
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Entry points for the pins in tests/test_oracle.py (FP64 unless noted).
+extern "C" {
+
+// Rasterise n 2-D Gaussians given directly in screen space (xy, conic, raw rgb, opacity
+// logit, depth) through the same list build, compositing and (optionally) backward as a
+// training step. K = 0 selects the reference's K rule.
+int oracle_raster2d_f64(int n, const double* xy, const double* conic, const double* rgb, const double* opac_logit,
+                        const double* depth, int W, int H, const float* bg, int K, double* img, double* final_T,
+                        int32_t* final_idx, const double* v_out, double* v_xy, double* v_conic, double* v_rgb,
+                        double* v_opac) {
+    using T = double;
+    Proj<T> pj;
+    pj.resize(n);
+    const int tbx = (W + 15) / 16, tby = (H + 15) / 16;
+    for (int i = 0; i < n; i++) {
+        pj.xys[2 * i] = xy[2 * i];
+        pj.xys[2 * i + 1] = xy[2 * i + 1];
+        for (int c = 0; c < 3; c++) { pj.conics[3 * i + c] = conic[3 * i + c]; pj.colors[3 * i + c] = rgb[3 * i + c]; }
+        pj.depths[i] = depth[i];
+        pj.opac_sig[i] = sigmoid<T>(opac_logit[i]);
+        // covariance extents from the conic's inverse (det of the conic = 1/det cov)
+        T det = conic[3 * i] * conic[3 * i + 2] - conic[3 * i + 1] * conic[3 * i + 1];
+        T sxx = conic[3 * i + 2] / det, syy = conic[3 * i] / det;
+        pj.aabb[2 * i] = std::ceil(3.0 * std::sqrt(sxx));
+        pj.aabb[2 * i + 1] = std::ceil(3.0 * std::sqrt(syy));
+        int x0, y0, x1, y1;
+        tile_bbox<T>(pj.xys[2 * i], pj.xys[2 * i + 1], pj.aabb[2 * i], pj.aabb[2 * i + 1], tbx, tby, x0, y0, x1, y1);
+        int area = (x1 - x0) * (y1 - y0);
+        pj.radii[i] = area > 0 ? 1 : 0;
+        pj.tiles_hit[i] = area > 0 ? area : 0;
+    }
+    Lists<T> L;
+    build_lists<T>(n, pj, W, H, L);
+    uint32_t KK = K > 0 ? (uint32_t)K : k_max_rule(L.capacity, L.num_tiles);
+    Raster<T> R;
+    raster_forward<T>(L, W, H, bg, KK, R);
+    std::memcpy(img, R.img.data(), R.img.size() * sizeof(T));
+    std::memcpy(final_T, R.final_T.data(), R.final_T.size() * sizeof(T));
+    std::memcpy(final_idx, R.final_idx.data(), R.final_idx.size() * sizeof(int32_t));
+    if (v_out) {
+        RasterGrads<T> G;
+        raster_backward<T>(L, R, bg, v_out, n, G);
+        std::memcpy(v_xy, G.v_xy.data(), G.v_xy.size() * sizeof(T));
+        std::memcpy(v_conic, G.v_conic.data(), G.v_conic.size() * sizeof(T));
+        std::memcpy(v_rgb, G.v_rgb.data(), G.v_rgb.size() * sizeof(T));
+        std::memcpy(v_opac, G.v_opac.data(), G.v_opac.size() * sizeof(T));
+    }
+    return (int)KK;
+}
+
+// Projection of n Gaussians (FP64): pixel centres, conics, depths; and the VJP of the
+// projection for given v_xy / v_conic.
+void oracle_project_f64(int n, const double* means, const double* scales, const double* quats, const MsplatCamera* cam,
+                        double* xys, double* conics, double* depths, int32_t* radii) {
+    using T = double;
+    std::vector<T> dc(3 * (size_t)n, 0.0), rest(1, 0.0), op(n, 0.0);
+    Proj<T> pj;
+    project_forward<T>(n, means, scales, quats, dc.data(), rest.data(), op.data(), 0, 0, *cam, pj);
+    std::memcpy(xys, pj.xys.data(), pj.xys.size() * sizeof(T));
+    std::memcpy(conics, pj.conics.data(), pj.conics.size() * sizeof(T));
+    std::memcpy(depths, pj.depths.data(), pj.depths.size() * sizeof(T));
+    std::memcpy(radii, pj.radii.data(), pj.radii.size() * sizeof(int32_t));
+}
+
+void oracle_project_vjp_f64(int n, const double* means, const double* scales, const double* quats,
+                            const MsplatCamera* cam, const double* conics, const double* v_xy, const double* v_conic,
+                            double* v_mean, double* v_scale, double* v_quat) {
+    for (int i = 0; i < n; i++)
+        proj_backward_one<double>(means + 3 * i, scales + 3 * i, quats + 4 * i, *cam, conics + 3 * i, v_xy + 2 * i,
+                                  v_conic + 3 * i, v_mean + 3 * i, v_scale + 3 * i, v_quat + 4 * i);
+}
+
+// SH colour (raw, before +0.5) for a unit view direction.
+void oracle_sh_color_f64(int degree, const double* dir, const double* dc, const double* rest, double* out) {
+    sh_to_color<double>(degree, dir, dc, rest, out);
+}
+
+}  // extern "C"

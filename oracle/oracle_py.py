@@ -85,6 +85,13 @@ def lib():
         L.oracle_mcmc_step.argtypes = [C.POINTER(StateF32), C.c_int32, C.c_float, C.c_uint32, C.c_void_p]
         L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_set_threads.restype = C.c_int
+        vp = C.c_void_p
+        L.oracle_raster2d_f64.argtypes = [C.c_int, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int,
+                                          vp, vp, vp, vp, vp, vp, vp, vp]
+        L.oracle_raster2d_f64.restype = C.c_int
+        L.oracle_project_f64.argtypes = [C.c_int, vp, vp, vp, C.POINTER(MsplatCamera), vp, vp, vp, vp]
+        L.oracle_project_vjp_f64.argtypes = [C.c_int, vp, vp, vp, C.POINTER(MsplatCamera), vp, vp, vp, vp, vp, vp]
+        L.oracle_sh_color_f64.argtypes = [C.c_int, vp, vp, vp, vp]
         _lib = L
     return _lib
 
