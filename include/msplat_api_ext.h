@@ -9,6 +9,9 @@
 #include <stddef.h>
 #include <stdint.h>
 #include "msplat_types.h"
+#include "msplat_c_api.h"
+
+typedef struct MsplatEngine MsplatEngine;
 
 #ifdef __cplusplus
 extern "C" {
@@ -28,6 +31,52 @@ int msplat_synth_cameras(uint64_t seed, int count, float radius, float jitter, f
 
 /* Float RGB -> uint8 as the reference quantises ground truth (core/src/input_data.cpp:66-69). */
 void msplat_quantize_rgb(const float* src, int64_t n, uint8_t* dst);
+
+/* Camera matrices from a vertical field of view (radians) and the render size, as the
+ * reference's viewer builds them (core/src/msplat_api.mm:486-528). */
+void msplat_camera_from_fov(const float c2w[16], float fov_y, int width, int height, MsplatCamera* out);
+
+/* ---- Datasets built from memory -------------------------------------------------------
+ * msplat_dataset_create also accepts "synthetic:<preset>[:key=value,...]":
+ *   presets  c0  (50k uniform-cube Gaussians, 16 cameras 256x256, f = 256)
+ *            c1  (1.5M 64-component mixture, 64 jittered cameras 1600x1040, f = 1300)
+ *   keys     n, hidden, cams, w, h, f, kind (0 cube / 1 mixture), seed, hidden_seed,
+ *            cam_seed, radius, jitter, degree
+ * Ground truth is rendered on the GPU from the hidden set at SH degree 3 over the default
+ * background and quantised to uint8; the trainable set is the initial model. Cameras sit
+ * on a sphere of the given radius (no normalisation; see input_data.cpp:115-169). */
+MsplatDataset msplat_dataset_create_empty(void);
+/* Adds a camera with host uint8 RGB ground truth [height, width, 3]; test = held out.
+ * Returns its index within the train or test list, or -1. */
+int msplat_dataset_add_camera(MsplatDataset ds, const float c2w[16], float fx, float fy, float cx, float cy, int width,
+                              int height, const uint8_t* rgb, bool test);
+/* Initial Gaussians (row-major host arrays; f_rest [n, (degree+1)^2-1, 3]). Returns 0 / -1. */
+int msplat_dataset_set_init_gaussians(MsplatDataset ds, int n, int degree, const float* means, const float* log_scales,
+                                      const float* quats, const float* opac_logits, const float* f_dc,
+                                      const float* f_rest);
+
+/* ---- Trainer introspection (tests, benchmarks) --------------------------------------- */
+/* The scalars Model::fullIteration would hand to the kernels for (camera, step, adam step). */
+int msplat_trainer_step_params(MsplatTrainer t, int cameraIndex, int step, int adamStep, MsplatStepParams* out);
+/* Train-camera index used by the most recent step (-1 before the first). */
+int msplat_trainer_last_camera(MsplatTrainer t);
+/* Mean loss of the most recent step (waits for the device). */
+double msplat_trainer_last_loss(MsplatTrainer t);
+/* Copies `count` floats of the active rows of group g (0 means, 1 scales, 2 quats, 3 f_dc,
+ * 4 f_rest, 5 opacity); which = 0 parameters, 1 Adam exp_avg, 2 exp_avg_sq. Returns the
+ * number of floats available, or -1. */
+int64_t msplat_trainer_read_group(MsplatTrainer t, int group, int which, float* dst, int64_t count);
+/* Runs Model::mcmcAfterTrain(step) now (its step gate applies). info = {num_active, dead,
+ * grown, nonfinite, culled, relocation counter}. Returns 0 / -1. */
+int msplat_trainer_run_mcmc(MsplatTrainer t, int step, int32_t info[6]);
+/* Adam step counter and current learning rates (lr[6]). */
+int msplat_trainer_optim_state(MsplatTrainer t, int* adamSteps, float lr[6]);
+/* The trainer's engine (borrowed: msplat_engine_destroy releases only the handle). */
+MsplatEngine* msplat_trainer_engine(MsplatTrainer t);
+
+/* Eq. 9 on the GPU for (opacity, ensemble size) pairs with unit scale: o_new[i] and the
+ * scale coefficient s_new[i] (known-answer tests). Host arrays. Returns 0 / -1. */
+int msplat_eq9_device(int n, const float* o_old, const int* counts, float* o_new, float* s_new);
 
 #ifdef __cplusplus
 }

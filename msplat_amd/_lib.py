@@ -115,6 +115,56 @@ def lib() -> C.CDLL:
         "msplat_synth_gaussians": ([C.c_int, C.c_uint64, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp], C.c_int),
         "msplat_synth_cameras": ([C.c_uint64, C.c_int, f32, f32, fp], C.c_int),
         "msplat_quantize_rgb": ([fp, i64, C.POINTER(C.c_uint8)], None),
+        "msplat_camera_from_fov": ([fp, f32, C.c_int, C.c_int, C.POINTER(MsplatCamera)], None),
+        # the msplat C API (msplat_c_api.h)
+        "msplat_dataset_create": ([C.c_char_p, f32, C.c_bool, C.c_int], vp),
+        "msplat_dataset_destroy": ([vp], None),
+        "msplat_dataset_num_train": ([vp], C.c_int),
+        "msplat_dataset_num_test": ([vp], C.c_int),
+        "msplat_dataset_camera_pose": ([vp, C.c_int, fp], None),
+        "msplat_trainer_create": ([vp, MsplatConfig], vp),
+        "msplat_trainer_destroy": ([vp], None),
+        "msplat_trainer_step": ([vp], MsplatStats),
+        "msplat_trainer_train": ([vp], None),
+        "msplat_trainer_evaluate": ([vp], MsplatEvalMetrics),
+        "msplat_trainer_render": ([vp, C.c_int, C.c_bool], MsplatPixelBuffer),
+        "msplat_trainer_render_pose": ([vp, fp, C.c_int], MsplatPixelBuffer),
+        "msplat_trainer_render_pose_to_buffer": ([vp, fp, C.c_int, C.POINTER(C.c_uint8), C.POINTER(C.c_int),
+                                                  C.POINTER(C.c_int)], None),
+        "msplat_trainer_export_ply": ([vp, C.c_char_p], None),
+        "msplat_trainer_export_splat": ([vp, C.c_char_p], None),
+        "msplat_trainer_save_checkpoint": ([vp, C.c_char_p], None),
+        "msplat_trainer_load_checkpoint": ([vp, C.c_char_p], C.c_int),
+        "msplat_trainer_splat_count": ([vp], C.c_int),
+        "msplat_trainer_iteration": ([vp], C.c_int),
+        "msplat_set_metallib_path": ([C.c_char_p], None),
+        "msplat_sync": ([], None),
+        "msplat_cleanup": ([], None),
+        "msplat_viewer_create": ([C.c_char_p, C.c_int, C.c_int, f32], vp),
+        "msplat_viewer_resize": ([vp, C.c_int, C.c_int], None),
+        "msplat_viewer_set_fov": ([vp, f32], None),
+        "msplat_viewer_set_background": ([vp, f32, f32, f32], None),
+        "msplat_viewer_render": ([vp, fp, C.POINTER(C.c_uint8)], None),
+        "msplat_viewer_splat_count": ([vp], C.c_int),
+        "msplat_viewer_get_size": ([vp, C.POINTER(C.c_int), C.POINTER(C.c_int)], None),
+        "msplat_viewer_render_buffer": ([vp, fp, C.POINTER(C.c_int), C.POINTER(C.c_int)], vp),
+        "msplat_viewer_bounds": ([vp, fp, fp], None),
+        "msplat_viewer_density_center": ([vp, fp], None),
+        "msplat_viewer_command_queue": ([vp], vp),
+        "msplat_viewer_destroy": ([vp], None),
+        # C API extensions (msplat_api_ext.h)
+        "msplat_dataset_create_empty": ([], vp),
+        "msplat_dataset_add_camera": ([vp, fp, f32, f32, f32, f32, C.c_int, C.c_int, C.POINTER(C.c_uint8),
+                                       C.c_bool], C.c_int),
+        "msplat_dataset_set_init_gaussians": ([vp, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp], C.c_int),
+        "msplat_trainer_step_params": ([vp, C.c_int, C.c_int, C.c_int, C.POINTER(MsplatStepParams)], C.c_int),
+        "msplat_trainer_last_camera": ([vp], C.c_int),
+        "msplat_trainer_last_loss": ([vp], f64),
+        "msplat_trainer_read_group": ([vp, C.c_int, C.c_int, fp, i64], i64),
+        "msplat_trainer_run_mcmc": ([vp, C.c_int, C.POINTER(i32)], C.c_int),
+        "msplat_trainer_optim_state": ([vp, C.POINTER(C.c_int), fp], C.c_int),
+        "msplat_trainer_engine": ([vp], vp),
+        "msplat_eq9_device": ([C.c_int, fp, C.POINTER(C.c_int), fp, fp], C.c_int),
     }
     for name, (argtypes, restype) in sig.items():
         fn = getattr(L, name)
@@ -122,6 +172,16 @@ def lib() -> C.CDLL:
         fn.restype = restype
     _lib = L
     return L
+
+
+def default_config() -> MsplatConfig:
+    """msplat_default_config() (a static inline in the header, restated for ctypes)."""
+    c = MsplatConfig()
+    c.iterations, c.shDegree, c.shDegreeInterval, c.ssimWeight = 30000, 3, 1000, 0.2
+    c.capMax, c.noiseLr, c.opacityReg, c.scaleReg, c.cullRadius = 1000000, 5e5, 0.01, 0.01, 3.0
+    c.keepCrs, c.downscaleFactor = False, 1.0
+    c.bgColor[0], c.bgColor[1], c.bgColor[2] = 0.6130, 0.0101, 0.3984
+    return c
 
 
 def fptr(a):

@@ -21,6 +21,7 @@ LIB = os.path.join(HERE, "libmsplat_core.so")
 SOURCES = [
     "kernels.hip",
     "mcmc.hip",
+    "eval.hip",
     "engine.cpp",
     "model.cpp",
     "api.cpp",
@@ -58,7 +59,7 @@ def _compile(src: str) -> str:
 
 def build(verbose: bool = True) -> str:
     os.makedirs(OBJDIR, exist_ok=True)
-    srcs = [s for s in SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    srcs = list(SOURCES)
     with ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         objs = list(ex.map(_compile, srcs))
     tmp = LIB + ".tmp"

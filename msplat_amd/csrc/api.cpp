@@ -1,0 +1,725 @@
+// api.cpp — the msplat C API (include/msplat_c_api.h) over the device-resident Model.
+// Reference: core/src/msplat_api.mm (Dataset 20-56, Trainer 58-290, C wrappers 292-397,
+// viewer 399-720). Every entry point catches C++ exceptions and reports failure through its
+// return value, so nothing unwinds across the C boundary.
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/msplat_api_ext.h"
+#include "../../include/msplat_c_api.h"
+#include "eval.h"
+#include "model.h"
+
+namespace msplat_hip {
+
+// ---------------------------------------------------------------------------------------
+// Dataset
+
+struct DatasetImpl {
+    std::vector<TrainCamera> train, test;
+    std::vector<std::unique_ptr<DBuf>> gt;  // owns every camera's device ground truth
+    HostGaussians init;
+    bool has_init = false;
+    float scale = 1.0f, translation[3] = {0.f, 0.f, 0.f};
+    ~DatasetImpl() {
+        (void)hipDeviceSynchronize();
+        for (auto& b : gt) b->release();
+    }
+    int add_camera(const float c2w[16], float fx, float fy, float cx, float cy, int w, int h, const uint8_t* rgb_host,
+                   const uint8_t* rgb_dev, bool is_test) {
+        if (w <= 0 || h <= 0 || !(fx > 0.f) || !(fy > 0.f)) throw std::invalid_argument("bad camera intrinsics");
+        TrainCamera c;
+        std::memcpy(c.c2w, c2w, sizeof(c.c2w));
+        c.fx = fx; c.fy = fy; c.cx = cx; c.cy = cy; c.width = w; c.height = h;
+        const size_t bytes = (size_t)w * h * 3;
+        gt.emplace_back(new DBuf());
+        gt.back()->ensure(bytes);
+        if (rgb_host) hip_check(hipMemcpy(gt.back()->p, rgb_host, bytes, hipMemcpyHostToDevice), "ground truth upload");
+        else hip_check(hipMemcpy(gt.back()->p, rgb_dev, bytes, hipMemcpyDeviceToDevice), "ground truth copy");
+        c.gt = gt.back()->as<uint8_t>();
+        auto& list = is_test ? test : train;
+        list.push_back(c);
+        return (int)list.size() - 1;
+    }
+};
+
+static HostGaussians synth_set(int kind, uint64_t seed, int n, int degree) {
+    HostGaussians g;
+    g.resize(n, degree);
+    if (msplat_synth_gaussians(kind, seed, n, degree, g.means.data(), g.scales.data(), g.quats.data(), g.opac.data(),
+                               g.f_dc.data(), g.f_rest.data()) != 0)
+        throw std::invalid_argument("bad synthetic Gaussian arguments");
+    return g;
+}
+
+// "synthetic:<preset>[:key=value,...]"
+static DatasetImpl* synthetic_dataset(const std::string& spec, float downscale, bool eval_mode, int test_every) {
+    std::string rest = spec.substr(std::string("synthetic:").size());
+    std::string preset = rest, opts;
+    const size_t colon = rest.find(':');
+    if (colon != std::string::npos) {
+        preset = rest.substr(0, colon);
+        opts = rest.substr(colon + 1);
+    }
+    std::map<std::string, double> kv;
+    if (preset == "c0") {
+        kv = {{"kind", 0}, {"n", 50000}, {"hidden", 20000}, {"cams", 16}, {"w", 256}, {"h", 256}, {"f", 256},
+              {"jitter", 0.0}};
+    } else if (preset == "c1") {
+        kv = {{"kind", 1}, {"n", 1500000}, {"hidden", 20000}, {"cams", 64}, {"w", 1600}, {"h", 1040}, {"f", 1300},
+              {"jitter", 0.15}};
+    } else {
+        throw std::invalid_argument("unknown synthetic preset '" + preset + "' (use c0 or c1)");
+    }
+    kv["seed"] = 1; kv["hidden_seed"] = 2; kv["cam_seed"] = 5; kv["radius"] = 3.0; kv["degree"] = 3;
+    std::stringstream ss(opts);
+    std::string item;
+    while (std::getline(ss, item, ',')) {
+        if (item.empty()) continue;
+        const size_t eq = item.find('=');
+        if (eq == std::string::npos) throw std::invalid_argument("bad synthetic option '" + item + "'");
+        const std::string k = item.substr(0, eq);
+        if (!kv.count(k)) throw std::invalid_argument("unknown synthetic option '" + k + "'");
+        kv[k] = std::stod(item.substr(eq + 1));
+    }
+    const int n = (int)kv["n"], hidden = (int)kv["hidden"], cams = (int)kv["cams"], kind = (int)kv["kind"];
+    int W = (int)kv["w"], H = (int)kv["h"];
+    float f = (float)kv["f"];
+    const int degree = (int)kv["degree"];
+    if (n <= 0 || hidden <= 0 || cams <= 0 || W <= 0 || H <= 0 || !(f > 0.f) || degree < 0 || degree > 3)
+        throw std::invalid_argument("bad synthetic sizes");
+    float cx = (float)W / 2.0f, cy = (float)H / 2.0f;
+    if (downscale > 1.0f) {
+        W = (int)((float)W / downscale);
+        H = (int)((float)H / downscale);
+        f /= downscale; cx /= downscale; cy /= downscale;
+    }
+    std::unique_ptr<DatasetImpl> ds(new DatasetImpl());
+    std::vector<float> c2w((size_t)cams * 16);
+    if (msplat_synth_cameras((uint64_t)kv["cam_seed"], cams, (float)kv["radius"], (float)kv["jitter"], c2w.data()) != 0)
+        throw std::invalid_argument("bad synthetic camera arguments");
+    // ground truth: the hidden set rendered at SH degree 3 over the default background
+    {
+        ModelOptions go;
+        go.sh_degree = 3;
+        go.sh_degree_interval = 1;
+        go.cap_max = hidden;
+        go.decimate_seeds = false;
+        const float zero[3] = {0.f, 0.f, 0.f};
+        Model hid(go, synth_set(kind, (uint64_t)kv["hidden_seed"], hidden, 3), 1.0f, zero);
+        std::vector<float> img((size_t)W * H * 3);
+        std::vector<uint8_t> q(img.size());
+        for (int i = 0; i < cams; i++) {
+            TrainCamera c;
+            std::memcpy(c.c2w, &c2w[(size_t)i * 16], sizeof(c.c2w));
+            c.fx = f; c.fy = f; c.cx = cx; c.cy = cy; c.width = W; c.height = H;
+            const float* d = hid.render(c, 1 << 20);
+            hip_check(hipStreamSynchronize(hid.engine().stream()), "render");
+            hip_check(hipMemcpy(img.data(), d, img.size() * 4, hipMemcpyDeviceToHost), "download");
+            msplat_quantize_rgb(img.data(), (int64_t)img.size(), q.data());
+            const bool is_test = eval_mode && test_every > 0 && (i % test_every == 0);
+            ds->add_camera(c.c2w, f, f, cx, cy, W, H, q.data(), nullptr, is_test);
+        }
+    }
+    ds->init = synth_set(kind, (uint64_t)kv["seed"], n, degree);
+    ds->has_init = true;
+    return ds.release();
+}
+
+// ---------------------------------------------------------------------------------------
+// Trainer
+
+static uint64_t split_mix(uint64_t& s) {
+    uint64_t x = (s += 0x9E3779B97F4A7C15ull);
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+static HostGaussians with_degree(const HostGaussians& g, int degree) {
+    if (g.degree == degree) return g;
+    HostGaussians o = g;
+    o.degree = degree;
+    const int src_b = g.rest_len() / 3, dst_b = o.rest_len() / 3, keep = std::min(src_b, dst_b);
+    o.f_rest.assign((size_t)g.n * dst_b * 3, 0.f);
+    for (int i = 0; i < g.n; i++)
+        std::memcpy(&o.f_rest[(size_t)i * dst_b * 3], &g.f_rest[(size_t)i * src_b * 3], (size_t)keep * 3 * sizeof(float));
+    return o;
+}
+
+struct TrainerImpl {
+    DatasetImpl* ds = nullptr;
+    MsplatConfig cfg;
+    std::unique_ptr<Model> model;
+    int step = 0;
+    std::vector<int> order;
+    size_t pos = 0;
+    uint64_t rng = 42;  // camera order: splitmix64 Fisher-Yates (reference: mt19937(42) + std::shuffle)
+    int last_cam = -1;
+    MsplatEngine handle{nullptr, false};
+    DBuf rgba, partials, sums;
+
+    void shuffle() {
+        for (int i = (int)order.size() - 1; i > 0; i--) {
+            const int j = (int)(split_mix(rng) % (uint64_t)(i + 1));
+            std::swap(order[i], order[j]);
+        }
+        pos = 0;
+    }
+    int next_camera() {
+        if (pos >= order.size()) shuffle();
+        return order[pos++];
+    }
+};
+
+static float window_1d[11];
+static void ssim_window(float out[11]) {
+    // createSSIMWindow's 1-D factor (reference core/src/ssim.cpp:7-24), sigma 1.5
+    float sum = 0.f;
+    for (int i = 0; i < 11; i++) {
+        const float x = (float)(i - 5);
+        out[i] = std::exp(-(x * x) / (2.0f * 1.5f * 1.5f));
+        sum += out[i];
+    }
+    for (int i = 0; i < 11; i++) out[i] /= sum;
+}
+
+static void metrics_for(TrainerImpl* T, const TrainCamera& cam, const float* img, double out3[3]) {
+    hipStream_t s = T->model->engine().stream();
+    T->partials.ensure(msk::eval_partials_bytes(cam.width, cam.height));
+    T->sums.ensure(64);
+    ssim_window(window_1d);
+    msk::eval_sums(s, img, cam.gt, cam.width, cam.height, window_1d, T->partials.as<double>(), T->sums.as<double>());
+    hip_check(hipMemcpyAsync(out3, T->sums.p, 24, hipMemcpyDeviceToHost, s), "metrics");
+    hip_check(hipStreamSynchronize(s), "metrics");
+}
+
+static MsplatPixelBuffer host_image(TrainerImpl* T, const float* dimg, int W, int H) {
+    MsplatPixelBuffer b{nullptr, 0, 0};
+    const size_t bytes = (size_t)W * H * 3 * sizeof(float);
+    float* buf = (float*)std::malloc(bytes);
+    if (!buf) throw std::bad_alloc();
+    hipStream_t s = T->model->engine().stream();
+    hipError_t e = hipMemcpyAsync(buf, dimg, bytes, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        std::free(buf);
+        hip_check(e, "image download");
+    }
+    b.data = buf;
+    b.width = W;
+    b.height = H;
+    return b;
+}
+
+// ---------------------------------------------------------------------------------------
+// Viewer
+
+struct ViewerImpl {
+    int width = 0, height = 0;
+    float fov_y = 0.f;
+    float bg[3] = {0.f, 0.f, 0.f};
+    int degree = 0, n = 0;
+    DBuf p[6];
+    Engine eng;
+    DBuf rgba;
+    float bmin[3] = {0, 0, 0}, bmax[3] = {0, 0, 0}, dcenter[3] = {0, 0, 0};
+
+    ~ViewerImpl() {
+        (void)hipStreamSynchronize(eng.stream());
+        for (auto& b : p) b.release();
+        rgba.release();
+    }
+    MsplatDeviceState state() const {
+        MsplatDeviceState st;
+        std::memset(&st, 0, sizeof(st));
+        st.N = n;
+        st.degree = degree;
+        st.means = p[0].as<float>(); st.scales = p[1].as<float>(); st.quats = p[2].as<float>();
+        st.f_dc = p[3].as<float>(); st.f_rest = p[4].as<float>(); st.opac = p[5].as<float>();
+        return st;
+    }
+    void bounds(const HostGaussians& g) {
+        if (g.n == 0) return;
+        for (int j = 0; j < 3; j++) bmin[j] = bmax[j] = g.means[j];
+        for (int i = 1; i < g.n; i++)
+            for (int j = 0; j < 3; j++) {
+                const float v = g.means[(size_t)i * 3 + j];
+                if (v < bmin[j]) bmin[j] = v;
+                if (v > bmax[j]) bmax[j] = v;
+            }
+        // densest cell of an 8x8x8 grid (reference msplat_api.mm:453-484)
+        const int G = 8;
+        std::vector<int> grid(G * G * G, 0);
+        float ext[3];
+        for (int j = 0; j < 3; j++) ext[j] = std::max(bmax[j] - bmin[j], 1e-6f);
+        for (int i = 0; i < g.n; i++) {
+            int c[3];
+            for (int j = 0; j < 3; j++)
+                c[j] = std::min((int)((g.means[(size_t)i * 3 + j] - bmin[j]) / ext[j] * G), G - 1);
+            grid[c[0] * G * G + c[1] * G + c[2]]++;
+        }
+        int best = 0, best_idx = 0;
+        for (int i = 0; i < G * G * G; i++)
+            if (grid[i] > best) { best = grid[i]; best_idx = i; }
+        const int ix = best_idx / (G * G), iy = (best_idx / G) % G, iz = best_idx % G;
+        dcenter[0] = bmin[0] + (ix + 0.5f) / G * ext[0];
+        dcenter[1] = bmin[1] + (iy + 0.5f) / G * ext[1];
+        dcenter[2] = bmin[2] + (iz + 0.5f) / G * ext[2];
+    }
+    const float* render_device(const float c2w[16]) {
+        MsplatCamera cam;
+        msplat_camera_from_fov(c2w, fov_y, width, height, &cam);
+        const int rc = eng.render(state(), cam, degree, bg);
+        if (rc != 0) throw std::runtime_error("viewer render rejected its inputs");
+        return eng.image();
+    }
+};
+
+}  // namespace msplat_hip
+
+// ---------------------------------------------------------------------------------------
+// C entry points
+
+using namespace msplat_hip;
+
+#define MS_GUARD_BEGIN try {
+#define MS_GUARD_END(ret)                                              \
+    }                                                                  \
+    catch (const std::exception& ex) {                                 \
+        std::fprintf(stderr, "msplat: %s: %s\n", __func__, ex.what()); \
+        return ret;                                                    \
+    }
+
+static TrainerImpl* TR(MsplatTrainer t) { return static_cast<TrainerImpl*>(t); }
+static DatasetImpl* DS(MsplatDataset d) { return static_cast<DatasetImpl*>(d); }
+static ViewerImpl* VW(MsplatViewer v) { return static_cast<ViewerImpl*>(v); }
+
+extern "C" {
+
+MsplatDataset msplat_dataset_create(const char* path, float downscaleFactor, bool evalMode, int testEvery) {
+    MS_GUARD_BEGIN
+    if (!path) return nullptr;
+    const std::string p(path);
+    if (p.rfind("synthetic:", 0) == 0) return synthetic_dataset(p, downscaleFactor, evalMode, testEvery);
+    throw std::runtime_error("on-disk dataset formats (COLMAP / Nerfstudio / Polycam) are not supported by this "
+                             "build; use a synthetic:<preset> path or msplat_dataset_create_empty()");
+    MS_GUARD_END(nullptr)
+}
+
+MsplatDataset msplat_dataset_create_empty(void) {
+    MS_GUARD_BEGIN
+    return new DatasetImpl();
+    MS_GUARD_END(nullptr)
+}
+
+void msplat_dataset_destroy(MsplatDataset ds) { delete DS(ds); }
+
+int msplat_dataset_num_train(MsplatDataset ds) { return ds ? (int)DS(ds)->train.size() : 0; }
+int msplat_dataset_num_test(MsplatDataset ds) { return ds ? (int)DS(ds)->test.size() : 0; }
+
+void msplat_dataset_camera_pose(MsplatDataset ds, int cameraIndex, float camToWorld[16]) {
+    if (!ds || !camToWorld) return;
+    auto& tr = DS(ds)->train;
+    if (cameraIndex >= 0 && cameraIndex < (int)tr.size()) std::memcpy(camToWorld, tr[cameraIndex].c2w, 16 * sizeof(float));
+}
+
+int msplat_dataset_add_camera(MsplatDataset ds, const float c2w[16], float fx, float fy, float cx, float cy, int width,
+                              int height, const uint8_t* rgb, bool test) {
+    MS_GUARD_BEGIN
+    if (!ds || !c2w || !rgb) return -1;
+    return DS(ds)->add_camera(c2w, fx, fy, cx, cy, width, height, rgb, nullptr, test);
+    MS_GUARD_END(-1)
+}
+
+int msplat_dataset_set_init_gaussians(MsplatDataset ds, int n, int degree, const float* means, const float* log_scales,
+                                      const float* quats, const float* opac_logits, const float* f_dc,
+                                      const float* f_rest) {
+    MS_GUARD_BEGIN
+    if (!ds || n <= 0 || degree < 0 || degree > 3 || !means || !log_scales || !quats || !opac_logits || !f_dc) return -1;
+    HostGaussians g;
+    g.resize(n, degree);
+    if (g.rest_len() > 0 && !f_rest) return -1;
+    std::memcpy(g.means.data(), means, g.means.size() * 4);
+    std::memcpy(g.scales.data(), log_scales, g.scales.size() * 4);
+    std::memcpy(g.quats.data(), quats, g.quats.size() * 4);
+    std::memcpy(g.opac.data(), opac_logits, g.opac.size() * 4);
+    std::memcpy(g.f_dc.data(), f_dc, g.f_dc.size() * 4);
+    if (g.rest_len() > 0) std::memcpy(g.f_rest.data(), f_rest, g.f_rest.size() * 4);
+    DS(ds)->init = std::move(g);
+    DS(ds)->has_init = true;
+    return 0;
+    MS_GUARD_END(-1)
+}
+
+MsplatTrainer msplat_trainer_create(MsplatDataset ds, MsplatConfig config) {
+    MS_GUARD_BEGIN
+    if (!ds) return nullptr;
+    DatasetImpl* d = DS(ds);
+    if (!d->has_init) throw std::runtime_error("dataset holds no initial Gaussians");
+    if (d->train.empty()) throw std::runtime_error("dataset holds no training cameras");
+    std::unique_ptr<TrainerImpl> T(new TrainerImpl());
+    T->ds = d;
+    T->cfg = config;
+    ModelOptions o;
+    o.sh_degree = config.shDegree;
+    o.sh_degree_interval = config.shDegreeInterval;
+    o.cap_max = config.capMax;
+    o.max_steps = config.iterations;
+    o.noise_lr = config.noiseLr;
+    o.opacity_reg = config.opacityReg;
+    o.scale_reg = config.scaleReg;
+    o.cull_radius = config.cullRadius;
+    o.keep_crs = config.keepCrs;
+    for (int c = 0; c < 3; c++) o.bg[c] = config.bgColor[c];
+    T->model.reset(new Model(o, with_degree(d->init, config.shDegree), d->scale, d->translation));
+    T->order.resize(d->train.size());
+    for (size_t i = 0; i < T->order.size(); i++) T->order[i] = (int)i;
+    T->shuffle();
+    T->handle.e = &T->model->engine();
+    T->handle.owned = false;
+    return T.release();
+    MS_GUARD_END(nullptr)
+}
+
+void msplat_trainer_destroy(MsplatTrainer t) {
+    if (!t) return;
+    TrainerImpl* T = TR(t);
+    T->rgba.release();
+    T->partials.release();
+    T->sums.release();
+    delete T;
+}
+
+MsplatStats msplat_trainer_step(MsplatTrainer t) {
+    MsplatStats s{0, 0, 0.f};
+    if (!t) return s;
+    TrainerImpl* T = TR(t);
+    MS_GUARD_BEGIN
+    T->step++;
+    const int ci = T->next_camera();
+    T->last_cam = ci;
+    const TrainCamera& cam = T->ds->train[ci];
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    T->model->full_iteration(cam, T->step, T->cfg.ssimWeight);
+    T->model->schedulers_step(T->step);
+    T->model->mcmc_after_train(T->step);
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    s.iteration = T->step;
+    s.splatCount = T->model->num_active();
+    s.msPerStep = (float)(std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count() / 1000.0);
+    return s;
+    MS_GUARD_END(s)
+}
+
+void msplat_trainer_train(MsplatTrainer t) {
+    if (!t) return;
+    while (TR(t)->step < TR(t)->cfg.iterations) {
+        const MsplatStats s = msplat_trainer_step(t);
+        if (s.iteration == 0) return;  // the step failed (reported on stderr)
+    }
+}
+
+MsplatEvalMetrics msplat_trainer_evaluate(MsplatTrainer t) {
+    MsplatEvalMetrics m{0.f, 0.f, 0.f, 0, 0};
+    if (!t) return m;
+    TrainerImpl* T = TR(t);
+    MS_GUARD_BEGIN
+    auto& cams = T->ds->test;
+    if (cams.empty()) return m;
+    double sp = 0, ss = 0, sl = 0;
+    for (const TrainCamera& cam : cams) {
+        const float* img = T->model->render(cam, T->cfg.iterations);
+        double s3[3];
+        metrics_for(T, cam, img, s3);
+        const double n = (double)cam.width * cam.height * 3;
+        const float psnr = 10.0f * (float)std::log10(1.0 / (s3[0] / n));
+        sp += psnr;
+        ss += (float)(s3[2] / n);
+        sl += (float)(s3[1] / n);
+    }
+    const int n = (int)cams.size();
+    m.psnr = (float)(sp / n);
+    m.ssim = (float)(ss / n);
+    m.l1 = (float)(sl / n);
+    m.numTest = n;
+    m.numGaussians = T->model->num_active();
+    return m;
+    MS_GUARD_END(m)
+}
+
+MsplatPixelBuffer msplat_trainer_render(MsplatTrainer t, int cameraIndex, bool useTest) {
+    MsplatPixelBuffer b{nullptr, 0, 0};
+    if (!t) return b;
+    TrainerImpl* T = TR(t);
+    MS_GUARD_BEGIN
+    auto& cams = useTest ? T->ds->test : T->ds->train;
+    if (cameraIndex < 0 || cameraIndex >= (int)cams.size()) return b;
+    const TrainCamera& cam = cams[cameraIndex];
+    const float* img = T->model->render(cam, T->step);
+    return host_image(T, img, cam.width, cam.height);
+    MS_GUARD_END(b)
+}
+
+MsplatPixelBuffer msplat_trainer_render_pose(MsplatTrainer t, const float camToWorld[16], int refCameraIndex) {
+    MsplatPixelBuffer b{nullptr, 0, 0};
+    if (!t || !camToWorld) return b;
+    TrainerImpl* T = TR(t);
+    MS_GUARD_BEGIN
+    auto& cams = T->ds->train;
+    if (refCameraIndex < 0 || refCameraIndex >= (int)cams.size()) return b;
+    TrainCamera cam = cams[refCameraIndex];
+    std::memcpy(cam.c2w, camToWorld, sizeof(cam.c2w));
+    const float* img = T->model->render(cam, T->step);
+    return host_image(T, img, cam.width, cam.height);
+    MS_GUARD_END(b)
+}
+
+void msplat_trainer_render_pose_to_buffer(MsplatTrainer t, const float camToWorld[16], int refCameraIndex,
+                                          uint8_t* outRGBA, int* outWidth, int* outHeight) {
+    if (!outWidth || !outHeight) return;
+    *outWidth = 0;
+    *outHeight = 0;
+    if (!t || !camToWorld) return;
+    TrainerImpl* T = TR(t);
+    try {
+        auto& cams = T->ds->train;
+        if (refCameraIndex < 0 || refCameraIndex >= (int)cams.size()) return;
+        TrainCamera cam = cams[refCameraIndex];
+        std::memcpy(cam.c2w, camToWorld, sizeof(cam.c2w));
+        *outWidth = cam.width;
+        *outHeight = cam.height;
+        if (!outRGBA) return;
+        const float* img = T->model->render(cam, T->step);
+        const int n = cam.width * cam.height;
+        T->rgba.ensure((size_t)n * 4);
+        hipStream_t s = T->model->engine().stream();
+        msk::to_rgba8(s, img, n, T->rgba.as<uint8_t>());
+        hip_check(hipMemcpyAsync(outRGBA, T->rgba.p, (size_t)n * 4, hipMemcpyDeviceToHost, s), "rgba download");
+        hip_check(hipStreamSynchronize(s), "rgba download");
+    } catch (const std::exception& ex) {
+        std::fprintf(stderr, "msplat: %s: %s\n", __func__, ex.what());
+    }
+}
+
+#define MS_VOID_GUARD(body)                                                 \
+    try {                                                                   \
+        body;                                                               \
+    } catch (const std::exception& ex) {                                    \
+        std::fprintf(stderr, "msplat: %s: %s\n", __func__, ex.what());      \
+    }
+
+void msplat_trainer_export_ply(MsplatTrainer t, const char* path) {
+    if (!t || !path) return;
+    MS_VOID_GUARD(TR(t)->model->save_ply(path, TR(t)->step))
+}
+
+void msplat_trainer_export_splat(MsplatTrainer t, const char* path) {
+    if (!t || !path) return;
+    MS_VOID_GUARD(TR(t)->model->save_splat(path))
+}
+
+void msplat_trainer_save_checkpoint(MsplatTrainer t, const char* path) {
+    if (!t || !path) return;
+    MS_VOID_GUARD(TR(t)->model->save_checkpoint(path, TR(t)->step))
+}
+
+int msplat_trainer_load_checkpoint(MsplatTrainer t, const char* path) {
+    if (!t || !path) return -1;
+    TrainerImpl* T = TR(t);
+    MS_GUARD_BEGIN
+    T->step = T->model->load_checkpoint(path);
+    T->shuffle();
+    return T->step;
+    MS_GUARD_END(-1)
+}
+
+int msplat_trainer_splat_count(MsplatTrainer t) { return t ? TR(t)->model->num_active() : 0; }
+int msplat_trainer_iteration(MsplatTrainer t) { return t ? TR(t)->step : 0; }
+
+int msplat_trainer_step_params(MsplatTrainer t, int cameraIndex, int step, int adamStep, MsplatStepParams* out) {
+    if (!t || !out) return -1;
+    TrainerImpl* T = TR(t);
+    if (cameraIndex < 0 || cameraIndex >= (int)T->ds->train.size()) return -1;
+    *out = T->model->step_params(T->ds->train[cameraIndex], step, T->cfg.ssimWeight, adamStep);
+    return 0;
+}
+
+int msplat_trainer_last_camera(MsplatTrainer t) { return t ? TR(t)->last_cam : -1; }
+
+double msplat_trainer_last_loss(MsplatTrainer t) { return t ? TR(t)->model->engine().loss() : 0.0; }
+
+int64_t msplat_trainer_read_group(MsplatTrainer t, int group, int which, float* dst, int64_t count) {
+    if (!t || group < 0 || group > 5 || which < 0 || which > 2) return -1;
+    TrainerImpl* T = TR(t);
+    MS_GUARD_BEGIN
+    const MsplatDeviceState st = T->model->state();
+    const float* params[6] = {st.means, st.scales, st.quats, st.f_dc, st.f_rest, st.opac};
+    const float* src = which == 0 ? params[group] : (which == 1 ? st.m[group] : st.v[group]);
+    const int64_t avail = (int64_t)st.N * row_len(group, st.degree);
+    if (dst && count > 0) {
+        const int64_t n = std::min(count, avail);
+        hipStream_t s = T->model->engine().stream();
+        hip_check(hipStreamSynchronize(s), "read");
+        if (n) hip_check(hipMemcpy(dst, src, (size_t)n * 4, hipMemcpyDeviceToHost), "read");
+    }
+    return avail;
+    MS_GUARD_END(-1)
+}
+
+int msplat_trainer_run_mcmc(MsplatTrainer t, int step, int32_t info[6]) {
+    if (!t) return -1;
+    TrainerImpl* T = TR(t);
+    MS_GUARD_BEGIN
+    const msk::McmcInfo in = T->model->mcmc_after_train(step);
+    if (info) {
+        info[0] = in.num_active; info[1] = in.dead; info[2] = in.grown; info[3] = in.nonfinite; info[4] = in.culled;
+        info[5] = (int32_t)T->model->relocation_count();
+    }
+    return 0;
+    MS_GUARD_END(-1)
+}
+
+int msplat_trainer_optim_state(MsplatTrainer t, int* adamSteps, float lr[6]) {
+    if (!t) return -1;
+    OptimState& o = TR(t)->model->optim();
+    if (adamSteps) *adamSteps = o.adam_step_count;
+    if (lr) std::memcpy(lr, o.adam_lr, sizeof(o.adam_lr));
+    return 0;
+}
+
+MsplatEngine* msplat_trainer_engine(MsplatTrainer t) { return t ? &TR(t)->handle : nullptr; }
+
+int msplat_eq9_device(int n, const float* o_old, const int* counts, float* o_new, float* s_new) {
+    if (n <= 0 || !o_old || !counts || !o_new || !s_new) return -1;
+    MS_GUARD_BEGIN
+    DBuf a, b, c, d;
+    a.ensure((size_t)n * 4); b.ensure((size_t)n * 4); c.ensure((size_t)n * 4); d.ensure((size_t)n * 4);
+    hip_check(hipMemcpy(a.p, o_old, (size_t)n * 4, hipMemcpyHostToDevice), "eq9");
+    hip_check(hipMemcpy(b.p, counts, (size_t)n * 4, hipMemcpyHostToDevice), "eq9");
+    msk::eq9_cases(nullptr, n, a.as<float>(), b.as<int>(), c.as<float>(), d.as<float>());
+    hip_check(hipDeviceSynchronize(), "eq9");
+    hip_check(hipMemcpy(o_new, c.p, (size_t)n * 4, hipMemcpyDeviceToHost), "eq9");
+    hip_check(hipMemcpy(s_new, d.p, (size_t)n * 4, hipMemcpyDeviceToHost), "eq9");
+    a.release(); b.release(); c.release(); d.release();
+    return 0;
+    MS_GUARD_END(-1)
+}
+
+void msplat_set_metallib_path(const char* path) { (void)path; }
+
+void msplat_sync(void) {
+    sync_all_engines();
+    (void)hipDeviceSynchronize();
+}
+
+void msplat_cleanup(void) { trim_all_engines(); }
+
+// ---- viewer ----------------------------------------------------------------------------
+
+MsplatViewer msplat_viewer_create(const char* plyPath, int width, int height, float fovY) {
+    MS_GUARD_BEGIN
+    if (!plyPath || width <= 0 || height <= 0) return nullptr;
+    std::unique_ptr<ViewerImpl> v(new ViewerImpl());
+    v->width = width;
+    v->height = height;
+    v->fov_y = fovY;
+    int step = 0;
+    const float zero[3] = {0.f, 0.f, 0.f};
+    HostGaussians g = read_ply(plyPath, &step, 1.0f, zero, false);
+    v->degree = g.degree;
+    v->n = g.n;
+    const std::vector<float>* src[6] = {&g.means, &g.scales, &g.quats, &g.f_dc, &g.f_rest, &g.opac};
+    for (int k = 0; k < 6; k++) {
+        const size_t bytes = src[k]->size() * 4;
+        v->p[k].ensure(bytes);
+        if (bytes) hip_check(hipMemcpy(v->p[k].p, src[k]->data(), bytes, hipMemcpyHostToDevice), "viewer upload");
+    }
+    v->bounds(g);
+    return v.release();
+    MS_GUARD_END(nullptr)
+}
+
+void msplat_viewer_resize(MsplatViewer v, int width, int height) {
+    if (!v || width <= 0 || height <= 0) return;
+    VW(v)->width = width;
+    VW(v)->height = height;
+}
+
+void msplat_viewer_set_fov(MsplatViewer v, float fovY) {
+    if (v) VW(v)->fov_y = fovY;
+}
+
+void msplat_viewer_set_background(MsplatViewer v, float r, float g, float b) {
+    if (!v) return;
+    VW(v)->bg[0] = r;
+    VW(v)->bg[1] = g;
+    VW(v)->bg[2] = b;
+}
+
+void msplat_viewer_render(MsplatViewer v, const float camToWorld[16], uint8_t* outRGBA) {
+    if (!v || !camToWorld || !outRGBA) return;
+    ViewerImpl* V = VW(v);
+    try {
+        const float* img = V->render_device(camToWorld);
+        const int n = V->width * V->height;
+        V->rgba.ensure((size_t)n * 4);
+        hipStream_t s = V->eng.stream();
+        msk::to_rgba8(s, img, n, V->rgba.as<uint8_t>());
+        hip_check(hipMemcpyAsync(outRGBA, V->rgba.p, (size_t)n * 4, hipMemcpyDeviceToHost, s), "viewer download");
+        hip_check(hipStreamSynchronize(s), "viewer download");
+    } catch (const std::exception& ex) {
+        std::fprintf(stderr, "msplat: %s: %s\n", __func__, ex.what());
+    }
+}
+
+int msplat_viewer_splat_count(MsplatViewer v) { return v ? VW(v)->n : 0; }
+
+void msplat_viewer_get_size(MsplatViewer v, int* outWidth, int* outHeight) {
+    if (!v) return;
+    if (outWidth) *outWidth = VW(v)->width;
+    if (outHeight) *outHeight = VW(v)->height;
+}
+
+void* msplat_viewer_render_buffer(MsplatViewer v, const float camToWorld[16], int* outWidth, int* outHeight) {
+    if (!v || !camToWorld) {
+        if (outWidth) *outWidth = 0;
+        if (outHeight) *outHeight = 0;
+        return nullptr;
+    }
+    ViewerImpl* V = VW(v);
+    if (outWidth) *outWidth = V->width;
+    if (outHeight) *outHeight = V->height;
+    try {
+        return (void*)V->render_device(camToWorld);
+    } catch (const std::exception& ex) {
+        std::fprintf(stderr, "msplat: %s: %s\n", __func__, ex.what());
+        return nullptr;
+    }
+}
+
+void msplat_viewer_bounds(MsplatViewer v, float boundsMin[3], float boundsMax[3]) {
+    if (!v || !boundsMin || !boundsMax) return;
+    std::memcpy(boundsMin, VW(v)->bmin, 12);
+    std::memcpy(boundsMax, VW(v)->bmax, 12);
+}
+
+void msplat_viewer_density_center(MsplatViewer v, float center[3]) {
+    if (!v || !center) return;
+    std::memcpy(center, VW(v)->dcenter, 12);
+}
+
+void* msplat_viewer_command_queue(MsplatViewer v) { return v ? (void*)VW(v)->eng.stream() : nullptr; }
+
+void msplat_viewer_destroy(MsplatViewer v) { delete VW(v); }
+
+}  // extern "C"

@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
+#include <set>
 #include <stdexcept>
 
 namespace msplat_hip {
@@ -40,6 +42,22 @@ static const char* kStageNames[ST_COUNT] = {"zero_grads",  "project_sh_fwd", "de
                                             "tile_scatter", "tile_sort_pack", "raster_fwd", "ssim_loss",
                                             "raster_bwd",  "sh_rest_adam",   "gaussian_update"};
 
+static std::mutex g_engines_mu;
+static std::set<Engine*>& engines() {
+    static std::set<Engine*> s;
+    return s;
+}
+
+void trim_all_engines() {
+    std::lock_guard<std::mutex> lk(g_engines_mu);
+    for (Engine* e : engines()) e->trim();
+}
+
+void sync_all_engines() {
+    std::lock_guard<std::mutex> lk(g_engines_mu);
+    for (Engine* e : engines()) (void)e->sync();
+}
+
 Engine::Engine() {
     hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     hip_check(hipEventCreate(&mark_[0]), "hipEventCreate");
@@ -49,9 +67,26 @@ Engine::Engine() {
     totals_.ensure(64);
     loss_.ensure(64);
     hip_check(hipMemset(loss_.p, 0, 64), "hipMemset");
+    std::lock_guard<std::mutex> lk(g_engines_mu);
+    engines().insert(this);
+}
+
+void Engine::trim() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
+                   &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
+                   &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &keys_, &ids_, &hist_, &img_,
+                   &finalT_, &finalIdx_, &vrend_, &hbuf_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
+                   &prefixT_, &afterC_};
+    for (DBuf* b : all) b->release();
+    debug_valid_ = false;
 }
 
 Engine::~Engine() {
+    {
+        std::lock_guard<std::mutex> lk(g_engines_mu);
+        engines().erase(this);
+    }
     if (stream_) hipStreamSynchronize(stream_);
     for (auto e : ev_pool_) hipEventDestroy(e);
     hipEventDestroy(mark_[0]);
@@ -341,10 +376,6 @@ void Engine::reset_stage_times() {
 // extern "C" kernel-level ABI (include/msplat_kernels.h)
 using msplat_hip::Engine;
 
-struct MsplatEngine {
-    Engine* e;
-};
-
 extern "C" {
 
 int msplat_dev_count(void) {
@@ -375,6 +406,7 @@ MsplatEngine* msplat_engine_create(void) {
     try {
         MsplatEngine* h = new MsplatEngine;
         h->e = new Engine();
+        h->owned = true;
         return h;
     } catch (const std::exception& ex) {
         fprintf(stderr, "msplat: engine creation failed: %s\n", ex.what());
@@ -384,7 +416,7 @@ MsplatEngine* msplat_engine_create(void) {
 
 void msplat_engine_destroy(MsplatEngine* e) {
     if (!e) return;
-    delete e->e;
+    if (e->owned) delete e->e;
     delete e;
 }
 

@@ -42,6 +42,8 @@ public:
     int stage_times(double* ms, int max_stages, int64_t* launches);
     void reset_stage_times();
     const float* image() const { return img_.as<float>(); }
+    // Releases every size-dependent buffer; the next step or render re-allocates them.
+    void trim();
     int last_W = 0, last_H = 0;
 
 private:
@@ -81,4 +83,15 @@ private:
     int64_t stage_n_[ST_COUNT] = {0};
 };
 
+// Every live engine (msplat_sync / msplat_cleanup act on all of them).
+void trim_all_engines();
+void sync_all_engines();
+
 }  // namespace msplat_hip
+
+// Handle type of the kernel-level C ABI; `owned` is false for an engine borrowed from a
+// trainer (msplat_engine_destroy then frees only the handle).
+struct MsplatEngine {
+    msplat_hip::Engine* e;
+    bool owned;
+};

@@ -1,0 +1,117 @@
+// model.h — trainable Gaussian state on the device and the per-step host bookkeeping
+// (reference core/include/model.hpp, core/src/model.cpp).
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "mcmc.h"
+
+namespace msplat_hip {
+
+// One camera of a dataset: OpenGL-convention camera-to-world pose (row-major), pinhole
+// intrinsics and the device-resident uint8 ground truth [H, W, 3] (reference Camera,
+// core/include/input_data.hpp).
+struct TrainCamera {
+    float c2w[16];
+    float fx = 0, fy = 0, cx = 0, cy = 0;
+    int width = 0, height = 0;
+    const uint8_t* gt = nullptr;
+};
+
+// Host copy of a Gaussian set, row-major: means[n,3], scales[n,3] (log), quats[n,4] (w,x,y,z),
+// f_dc[n,3], f_rest[n, (degree+1)^2-1, 3], opac[n] (logit).
+struct HostGaussians {
+    int n = 0, degree = 0;
+    std::vector<float> means, scales, quats, f_dc, f_rest, opac;
+    int rest_len() const { return ((degree + 1) * (degree + 1) - 1) * 3; }
+    void resize(int rows, int deg);
+};
+
+struct ModelOptions {
+    int sh_degree = 3;
+    int sh_degree_interval = 1000;
+    int cap_max = 1000000;
+    int max_steps = 30000;
+    int position_lr_max_steps = 30000;
+    float noise_lr = 5e5f, opacity_reg = 0.01f, scale_reg = 0.01f, cull_radius = 3.0f;
+    bool keep_crs = false;
+    bool decimate_seeds = true;
+    float bg[3] = {0.6130f, 0.0101f, 0.3984f};
+};
+
+// Adam state and schedule scalars (model.hpp:60-70).
+struct OptimState {
+    float adam_lr[6] = {0.00016f, 0.005f, 0.001f, 0.0025f, 0.000125f, 0.05f};
+    float means_lr_init = 0.00016f, means_lr_final = 0.0000016f;
+    int adam_step_count = 0;
+    float beta1 = 0.9f, beta2 = 0.999f, eps = 1e-8f;
+};
+
+int row_len(int group, int degree);
+
+class Model {
+public:
+    Model(const ModelOptions& opt, const HostGaussians& init, float scale, const float translation[3]);
+    ~Model();
+    Model(const Model&) = delete;
+    Model& operator=(const Model&) = delete;
+
+    // Model::fullIteration (model.cpp:901-980): one training step on `cam`.
+    void full_iteration(const TrainCamera& cam, int step, float ssim_weight);
+    // Model::schedulersStep (model.cpp:256-261).
+    void schedulers_step(int step);
+    // Model::mcmcAfterTrain (model.cpp:497-565), on the device.
+    msk::McmcInfo mcmc_after_train(int step);
+    // Model::render (model.cpp:891-899): device RGB image [H, W, 3], valid until the next call.
+    const float* render(const TrainCamera& cam, int step);
+    // The scalars full_iteration hands to the kernels for this step (no state change).
+    MsplatStepParams step_params(const TrainCamera& cam, int step, float ssim_weight, int adam_step) const;
+
+    void save_checkpoint(const std::string& path, int step);
+    int load_checkpoint(const std::string& path);
+    void save_ply(const std::string& path, int step);
+    void save_splat(const std::string& path);
+
+    HostGaussians download() const;
+    void download_moments(std::vector<float> m[6], std::vector<float> v[6]) const;
+    MsplatDeviceState state() const;
+    int num_active() const { return num_active_; }
+    int capacity() const { return cap_; }
+    int degree() const { return degree_; }
+    const ModelOptions& options() const { return opt_; }
+    OptimState& optim() { return os_; }
+    uint32_t relocation_count() const { return reloc_count_; }
+    Engine& engine() { return eng_; }
+    int degrees_to_use(int step) const;
+
+private:
+    void allocate(int degree);
+    void upload(const HostGaussians& g, const std::vector<float>* m, const std::vector<float>* v);
+
+    ModelOptions opt_;
+    OptimState os_;
+    int cap_ = 0, degree_ = 0, num_active_ = 0;
+    uint32_t reloc_count_ = 0;
+    float scale_ = 1.0f, translation_[3] = {0, 0, 0};
+    DBuf p_[6], m_[6], v_[6];
+    Engine eng_;
+    msk::McmcScratch mcmc_;
+};
+
+// On-disk formats (reference save_gaussians.cpp:10-231, model.cpp:698-843).
+struct CheckpointData {
+    uint32_t step = 0, num = 0, sh_degree = 0, adam_steps = 0;
+    float adam_lr[6] = {0}, lr_init = 0, lr_final = 0;
+    HostGaussians g;
+    std::vector<float> m[6], v[6];
+};
+void write_checkpoint(const std::string& path, const CheckpointData& c);
+CheckpointData read_checkpoint(const std::string& path);
+void write_ply(const std::string& path, const HostGaussians& g, int step, bool keep_crs, float scale,
+               const float translation[3]);
+HostGaussians read_ply(const std::string& path, int* step, float scale, const float translation[3], bool keep_crs);
+void write_splat(const std::string& path, const HostGaussians& g, bool keep_crs, float scale, const float translation[3]);
+
+}  // namespace msplat_hip

@@ -165,3 +165,46 @@ void msplat_quantize_rgb(const float* src, int64_t n, uint8_t* dst) {
 }
 
 }  // extern "C"
+
+extern "C" {
+
+void msplat_camera_from_fov(const float c2w[16], float fov_y, int width, int height, MsplatCamera* out) {
+    // intrinsics from the vertical field of view and the render size (viewer preparePose,
+    // reference core/src/msplat_api.mm:486-528)
+    const float fy = (float)height / (2.0f * std::tan(fov_y * 0.5f));
+    const float aspect = (float)width / (float)height;
+    const float fov_x = 2.0f * std::atan(std::tan(fov_y * 0.5f) * aspect);
+    const float fx = (float)width / (2.0f * std::tan(fov_x * 0.5f));
+    float R[3][3], Rinv[3][3], T[3], Tinv[3];
+    for (int i = 0; i < 3; i++) {
+        R[i][0] = c2w[i * 4 + 0];
+        R[i][1] = -c2w[i * 4 + 1];
+        R[i][2] = -c2w[i * 4 + 2];
+        T[i] = c2w[i * 4 + 3];
+    }
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Rinv[i][j] = R[j][i];
+    for (int i = 0; i < 3; i++) Tinv[i] = -(Rinv[i][0] * T[0] + Rinv[i][1] * T[1] + Rinv[i][2] * T[2]);
+    const float vm[16] = {Rinv[0][0], Rinv[0][1], Rinv[0][2], Tinv[0], Rinv[1][0], Rinv[1][1], Rinv[1][2], Tinv[1],
+                          Rinv[2][0], Rinv[2][1], Rinv[2][2], Tinv[2], 0, 0, 0, 1};
+    const float t_p = 0.001f * std::tan(0.5f * fov_y), r_p = 0.001f * std::tan(0.5f * fov_x);
+    const float pm[16] = {0.001f / r_p, 0, 0, 0, 0, 0.001f / t_p, 0, 0, 0, 0, (1000.0f + 0.001f) / (1000.0f - 0.001f),
+                          -1000.0f * 0.001f / (1000.0f - 0.001f), 0, 0, 1, 0};
+    float pvm[16] = {};
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 4; k++) pvm[i * 4 + j] += pm[i * 4 + k] * vm[k * 4 + j];
+    std::memcpy(out->viewmat, vm, sizeof(vm));
+    std::memcpy(out->projmat, pvm, sizeof(pvm));
+    out->fx = fx;
+    out->fy = fy;
+    out->cx = (float)width * 0.5f;
+    out->cy = (float)height * 0.5f;
+    out->cam_pos[0] = T[0];
+    out->cam_pos[1] = T[1];
+    out->cam_pos[2] = T[2];
+    out->width = width;
+    out->height = height;
+}
+
+}  // extern "C"

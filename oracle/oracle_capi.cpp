@@ -238,3 +238,81 @@ void oracle_sh_color_f64(int degree, const double* dir, const double* dc, const 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------------
+// Evaluation metrics: psnr / l1_loss (model:34-49) and ssim_eval (ssim:61-123), the 1-D
+// window as createSSIMWindow computes it (ssim:7-24, host expf), replicate padding, the
+// blur accumulated with fused multiply-add (clang's default contraction of
+// `sum += in * k` on the reference's arm64 targets). FP64 sums in index order.
+// Pins: SSIM(x, x) = 1 exactly; PSNR of a constant offset in closed form; scipy's
+// gaussian_filter(mode="nearest") SSIM in FP64 (tests/test_oracle.py).
+extern "C" void oracle_ssim_window(float out[11]) {
+    float sum = 0.f;
+    for (int i = 0; i < 11; i++) {
+        const float x = (float)(i - 5);
+        out[i] = std::exp(-(x * x) / (2.0f * 1.5f * 1.5f));
+        sum += out[i];
+    }
+    for (int i = 0; i < 11; i++) out[i] /= sum;
+}
+
+static void blur_replicate(const float* in, float* out, float* tmp, int H, int W, const float* k) {
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            float s = 0.f;
+            for (int t = 0; t < 11; t++) {
+                const int sx = std::min(std::max(x + t - 5, 0), W - 1);
+                s = std::fma(in[(size_t)y * W + sx], k[t], s);
+            }
+            tmp[(size_t)y * W + x] = s;
+        }
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) {
+            float s = 0.f;
+            for (int t = 0; t < 11; t++) {
+                const int sy = std::min(std::max(y + t - 5, 0), H - 1);
+                s = std::fma(tmp[(size_t)sy * W + x], k[t], s);
+            }
+            out[(size_t)y * W + x] = s;
+        }
+}
+
+// out3 = {sum (r-g)^2, sum |r-g|, sum SSIM} over H*W*3 values; g = gt_u8 / 255.
+extern "C" void oracle_eval_sums(int H, int W, const float* img, const uint8_t* gt, double* out3) {
+    const size_t P = (size_t)H * W;
+    float k[11];
+    oracle_ssim_window(k);
+    std::vector<float> r(P), g(P), rr(P), gg(P), rg(P), m1(P), m2(P), s11(P), s22(P), s12(P), tmp(P);
+    double sq = 0, ab = 0, ss = 0;
+    for (size_t i = 0; i < P * 3; i++) {
+        const float a = img[i], b = (float)gt[i] * (1.0f / 255.0f);
+        const double d = (double)(a - b);
+        sq += d * d;
+        ab += (double)std::fabs(a - b);
+    }
+    const float C1 = 0.01f * 0.01f, C2 = 0.03f * 0.03f;
+    for (int c = 0; c < 3; c++) {
+        for (size_t i = 0; i < P; i++) {
+            r[i] = img[i * 3 + c];
+            g[i] = (float)gt[i * 3 + c] * (1.0f / 255.0f);
+            rr[i] = r[i] * r[i];
+            gg[i] = g[i] * g[i];
+            rg[i] = r[i] * g[i];
+        }
+        blur_replicate(r.data(), m1.data(), tmp.data(), H, W, k);
+        blur_replicate(g.data(), m2.data(), tmp.data(), H, W, k);
+        blur_replicate(rr.data(), s11.data(), tmp.data(), H, W, k);
+        blur_replicate(gg.data(), s22.data(), tmp.data(), H, W, k);
+        blur_replicate(rg.data(), s12.data(), tmp.data(), H, W, k);
+        for (size_t i = 0; i < P; i++) {
+            const float m1sq = m1[i] * m1[i], m2sq = m2[i] * m2[i], m12 = m1[i] * m2[i];
+            const float sig1 = s11[i] - m1sq, sig2 = s22[i] - m2sq, sig12 = s12[i] - m12;
+            const float num = (2.0f * m12 + C1) * (2.0f * sig12 + C2);
+            const float den = (m1sq + m2sq + C1) * (sig1 + sig2 + C2);
+            ss += (double)(num / den);
+        }
+    }
+    out3[0] = sq;
+    out3[1] = ab;
+    out3[2] = ss;
+}

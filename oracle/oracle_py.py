@@ -92,6 +92,8 @@ def lib():
         L.oracle_project_f64.argtypes = [C.c_int, vp, vp, vp, C.POINTER(MsplatCamera), vp, vp, vp, vp]
         L.oracle_project_vjp_f64.argtypes = [C.c_int, vp, vp, vp, C.POINTER(MsplatCamera), vp, vp, vp, vp, vp, vp]
         L.oracle_sh_color_f64.argtypes = [C.c_int, vp, vp, vp, vp]
+        L.oracle_ssim_window.argtypes = [vp]
+        L.oracle_eval_sums.argtypes = [C.c_int, C.c_int, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -204,6 +206,23 @@ def train_step(st: HostState, cam, gt: np.ndarray, hp, want_ids=True):
 def render(st: HostState, cam, degrees_to_use: int, bg, want_ids=True):
     return _run("oracle_render_f64" if st.dtype == np.float64 else "oracle_render_f32", st, cam, None, None,
                 want_ids, render_args=(degrees_to_use, bg))
+
+
+def eval_sums(img: np.ndarray, gt: np.ndarray) -> np.ndarray:
+    """{sum (r-g)^2, sum |r-g|, sum SSIM} of a float render [H, W, 3] against uint8 ground truth."""
+    img = np.ascontiguousarray(img, np.float32)
+    gt = np.ascontiguousarray(gt, np.uint8)
+    out = np.zeros(3, np.float64)
+    lib().oracle_eval_sums(img.shape[0], img.shape[1], img.ctypes.data, gt.ctypes.data, out.ctypes.data)
+    return out
+
+
+def eval_metrics(img: np.ndarray, gt: np.ndarray) -> dict:
+    """psnr / ssim / l1 of one view as Trainer::evaluate computes them (float results)."""
+    s = eval_sums(img, gt)
+    n = float(img.size)
+    return {"psnr": float(np.float32(10.0) * np.float32(np.log10(1.0 / (s[0] / n)))),
+            "ssim": float(np.float32(s[2] / n)), "l1": float(np.float32(s[1] / n))}
 
 
 def set_threads(n: int) -> int:

@@ -1,0 +1,163 @@
+"""On-disk formats written and read by the GPU trainer, parsed here independently from the
+layouts the reference defines: .msplat checkpoints (core/src/model.cpp:698-843), Gaussian
+PLY and .splat (core/src/loaders/save_gaussians.cpp:10-231); plus the PLY viewer
+(msplat_api.mm:399-720) against the oracle's render of the same file."""
+import ctypes as C
+import struct
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, Viewer, scene
+from msplat_amd import _lib
+from parity_util import assert_exact, small_scene
+from test_gpu_trainer import GROUPS, _dataset, _state
+
+pytestmark = pytest.mark.gpu
+
+
+def _parse_checkpoint(path):
+    b = open(path, "rb").read()
+    off = 0
+
+    def take(fmt):
+        nonlocal off
+        v = struct.unpack_from("<" + fmt, b, off)
+        off += struct.calcsize("<" + fmt)
+        return v
+    magic, version, step, num, shdeg, adam = take("6I")
+    lr = take("6f")
+    lr_init, lr_final = take("2f")
+    tensors = []
+    for _ in range(18):
+        (nd,) = take("I")
+        shape = take(f"{nd}q")
+        (nbytes,) = take("Q")
+        data = np.frombuffer(b, np.float32, count=nbytes // 4, offset=off).reshape(shape)
+        off += nbytes
+        tensors.append(data)
+    assert off == len(b)
+    return dict(magic=magic, version=version, step=step, num=num, sh_degree=shdeg, adam=adam, lr=lr,
+                lr_init=lr_init, lr_final=lr_final, tensors=tensors)
+
+
+def _trainer(W=96, H=80, n=1500, steps=2, nan_row=False):
+    g, hidden, _ = small_scene(n=n, n_hidden=600, W=W, H=H)
+    if nan_row:
+        g["quats"][7, 1] = np.nan
+    ds, cams = _dataset(g, hidden, W, H, n_cams=3)
+    tr = GaussianTrainer(ds, TrainingConfig(iterations=100, sh_degree_interval=1, cap_max=n + 100))
+    for _ in range(steps):
+        tr.step()
+    return ds, tr, g, cams
+
+
+def test_checkpoint_layout_and_round_trip(tmp_path):
+    ds, tr, g, _ = _trainer()
+    path = str(tmp_path / "a.msplat")
+    tr.save_checkpoint(path)
+    ck = _parse_checkpoint(path)
+    n = g["means"].shape[0]
+    assert (ck["magic"], ck["version"], ck["step"], ck["num"], ck["sh_degree"], ck["adam"]) == \
+           (0x4C50534D, 1, 2, n, 3, 2)
+    st = _state(tr)
+    shapes = [(n, 3), (n, 3), (n, 4), (n, 3), (n, 15, 3), (n, 1)]
+    for k in range(18):
+        assert ck["tensors"][k].shape == shapes[k % 6]
+        key = ("", "m.", "v.")[k // 6] + GROUPS[k % 6]
+        assert_exact(key, ck["tensors"][k].reshape(-1), st[key])
+    n_adam, lr = tr.optim_state()
+    assert np.array_equal(np.array(ck["lr"], np.float32), lr)
+    assert ck["lr_init"] == np.float32(0.00016) and ck["lr_final"] == np.float32(0.0000016)
+    # resume into a fresh trainer: same state, iteration and Adam counter
+    tr2 = GaussianTrainer(ds, TrainingConfig(iterations=100, sh_degree_interval=1, cap_max=n + 100))
+    tr2.load_checkpoint(path)
+    assert tr2.iteration == 2 and tr2.splat_count == n
+    st2 = _state(tr2)
+    for key in st:
+        assert_exact(key, st2[key], st[key])
+    assert tr2.optim_state()[0] == n_adam
+    s = tr2.step()
+    assert s.iteration == 3
+    # a corrupt file is rejected without touching the state
+    bad = tmp_path / "bad.msplat"
+    bad.write_bytes(b"XXXX" + open(path, "rb").read()[4:])
+    with pytest.raises(RuntimeError):
+        tr2.load_checkpoint(str(bad))
+    assert tr2.iteration == 3
+
+
+def test_ply_layout_and_viewer(tmp_path):
+    ds, tr, g, cams = _trainer(steps=0, nan_row=True)
+    path = str(tmp_path / "s.ply")
+    tr.export_ply(path)
+    raw = open(path, "rb").read()
+    head, body = raw.split(b"end_header\n", 1)
+    lines = head.decode().split("\n")
+    assert lines[0] == "ply" and lines[1] == "format binary_little_endian 1.0"
+    assert lines[2] == "comment Generated by msplat at iteration 0"
+    n = g["means"].shape[0]
+    assert lines[3] == f"element vertex {n - 1}"  # the non-finite row is dropped
+    props = [ln.split()[-1] for ln in lines[4:] if ln.startswith("property")]
+    want = ["x", "y", "z", "nx", "ny", "nz", "f_dc_0", "f_dc_1", "f_dc_2"] + [f"f_rest_{i}" for i in range(45)] + \
+           ["opacity", "scale_0", "scale_1", "scale_2", "rot_0", "rot_1", "rot_2", "rot_3"]
+    assert props == want
+    rows = np.frombuffer(body, np.float32).reshape(n - 1, 62)
+    keep = np.setdiff1d(np.arange(n), [7])
+    assert_exact("xyz", rows[:, :3], g["means"][keep])
+    assert (rows[:, 3:6] == 0).all()
+    assert_exact("f_dc", rows[:, 6:9], g["f_dc"][keep])
+    assert_exact("f_rest channel-major", rows[:, 9:54], g["f_rest"][keep].transpose(0, 2, 1).reshape(-1, 45))
+    assert_exact("opacity", rows[:, 54], g["opac"][keep])
+    assert_exact("scales", rows[:, 55:58], g["scales"][keep])
+    assert_exact("quats", rows[:, 58:62], g["quats"][keep])
+
+    # viewer over the same file: image equals the oracle's render, RGBA8 truncated
+    W, H, fov = 120, 90, 0.9
+    v = Viewer(path, W, H, fov)
+    assert v.splat_count == n - 1 and v.size == (W, H)
+    lo, hi = v.bounds()
+    assert_exact("bounds", np.stack([lo, hi]), np.stack([rows[:, :3].min(0), rows[:, :3].max(0)]))
+    pose = ds.camera_pose(0)
+    rgba = v.render(pose)
+    cam = _lib.MsplatCamera()
+    c2w = np.ascontiguousarray(pose.reshape(16), np.float32)
+    _lib.lib().msplat_camera_from_fov(_lib.fptr(c2w), fov, W, H, cam)
+    hg = {"means": rows[:, :3], "scales": rows[:, 55:58], "quats": rows[:, 58:62], "opac": rows[:, 54],
+          "f_dc": rows[:, 6:9], "f_rest": rows[:, 9:54].reshape(-1, 3, 15).transpose(0, 2, 1), "degree": 3}
+    ref = O.render(O.HostState(hg), cam, 3, (0.0, 0.0, 0.0), want_ids=False)["img"]
+    want8 = (np.minimum(np.maximum(ref, 0.0), 1.0) * np.float32(255.0)).astype(np.uint8)
+    assert_exact("viewer rgba", rgba[..., :3], want8)
+    # the zero-copy path hands out the device image of the same render
+    w, h = C.c_int(), C.c_int()
+    L = _lib.lib()
+    dptr = L.msplat_viewer_render_buffer(v._h, _lib.fptr(c2w), C.byref(w), C.byref(h))
+    assert dptr and (w.value, h.value) == (W, H)
+    L.msplat_sync()
+    host = np.zeros((H, W, 3), np.float32)
+    assert L.msplat_dev_download(host.ctypes.data, dptr, host.nbytes) == 0
+    assert_exact("viewer buffer", host, ref)
+    v.set_background(1.0, 1.0, 1.0)
+    v.resize(64, 48)
+    assert v.render(pose).shape == (48, 64, 4)
+
+
+def test_splat_layout(tmp_path):
+    ds, tr, g, _ = _trainer(steps=0, nan_row=True)
+    path = str(tmp_path / "s.splat")
+    tr.export_splat(path)
+    rec = np.dtype([("pos", "<f4", 3), ("scale", "<f4", 3), ("rgba", "u1", 4), ("rot", "u1", 4)])
+    data = np.frombuffer(open(path, "rb").read(), rec)
+    n = g["means"].shape[0]
+    assert data.size == n - 1
+    keep = np.setdiff1d(np.arange(n), [7])
+    s = np.exp(g["scales"][keep].astype(np.float32))
+    order_key = (s[:, 0] + s[:, 1] + s[:, 2]) / (np.float32(1) + np.exp(-g["opac"][keep].astype(np.float32)))
+    order = keep[np.argsort(-order_key, kind="stable")]
+    assert_exact("positions", data["pos"], g["means"][order])
+    assert np.allclose(data["scale"], np.exp(g["scales"][order]), rtol=1e-6)
+    rgb = np.clip((g["f_dc"][order].astype(np.float64) * 0.28209479177387814 + 0.5) * 255.0, 0, 255).astype(np.uint8)
+    assert np.abs(data["rgba"][:, :3].astype(int) - rgb).max() <= 1
+    q = np.clip(g["quats"][order] * np.float32(128) + np.float32(128), 0, 255).astype(np.uint8)
+    assert np.abs(data["rot"].astype(int) - q).max() <= 1

@@ -445,3 +445,47 @@ def test_relocation_sampler_chi_square():
     exp = p / p.sum() * n_dead
     chi2 = ((counts - exp) ** 2 / exp).sum()
     assert stats.chi2.sf(chi2, n_alive - 1) > 1e-3
+
+
+def _ssim_fp64_scipy(img, gt):
+    """Replicate-padded 11-tap Gaussian SSIM (sigma 1.5) in float64 with scipy."""
+    x = np.arange(11) - 5.0
+    w = np.exp(-(x * x) / 4.5)
+    w /= w.sum()
+    r = img.astype(np.float64)
+    g = gt.astype(np.float64) / 255.0
+
+    def blur(a):
+        return ndimage.correlate1d(ndimage.correlate1d(a, w, axis=1, mode="nearest"), w, axis=0, mode="nearest")
+    tot = 0.0
+    for c in range(3):
+        a, b = r[..., c], g[..., c]
+        m1, m2 = blur(a), blur(b)
+        s11, s22, s12 = blur(a * a) - m1 * m1, blur(b * b) - m2 * m2, blur(a * b) - m1 * m2
+        c1, c2 = 1e-4, 9e-4
+        tot += (((2 * m1 * m2 + c1) * (2 * s12 + c2)) / ((m1 * m1 + m2 * m2 + c1) * (s11 + s22 + c2))).sum()
+    return tot
+
+
+def test_eval_metrics_pins():
+    """Evaluation metrics (model.cpp:34-49, ssim.cpp:61-123): SSIM(x, x) = 1 exactly, PSNR
+    of a constant offset in closed form, and SSIM against an FP64 scipy formulation with
+    replicate ('nearest') padding."""
+    rng = np.random.default_rng(3)
+    H, W = 37, 53
+    gt = rng.integers(0, 256, (H, W, 3), dtype=np.uint8)
+    same = (gt.astype(np.float32) * np.float32(1.0 / 255.0)).astype(np.float32)
+    s = O.eval_sums(same, gt)
+    assert s[0] == 0.0 and s[1] == 0.0
+    assert s[2] == H * W * 3  # every SSIM value is exactly 1
+    gt_mid = np.full((H, W, 3), 100, np.uint8)
+    off = (gt_mid.astype(np.float32) * np.float32(1.0 / 255.0) + np.float32(0.05)).astype(np.float32)
+    m = O.eval_metrics(off, gt_mid)
+    d = float(np.float32(off[0, 0, 0]) - np.float32(100 * np.float32(1.0 / 255.0)))
+    assert abs(m["psnr"] - 10.0 * np.log10(1.0 / (d * d))) < 1e-4
+    assert abs(m["l1"] - abs(d)) < 1e-7
+    img = np.clip(same + rng.normal(0, 0.08, same.shape), 0, 1).astype(np.float32)
+    s = O.eval_sums(img, gt)
+    ref = _ssim_fp64_scipy(img, gt)
+    assert abs(s[2] - ref) / abs(ref) < 2e-5, (s[2], ref)
+    assert abs(s[0] - ((img.astype(np.float64) - gt / 255.0) ** 2).sum()) / s[0] < 1e-6
