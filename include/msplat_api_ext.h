@@ -55,6 +55,12 @@ int msplat_dataset_set_init_gaussians(MsplatDataset ds, int n, int degree, const
                                       const float* quats, const float* opac_logits, const float* f_dc,
                                       const float* f_rest);
 
+/* Camera `index` of the train (test = false) or held-out list: pose, intrinsics
+ * {fx, fy, cx, cy} and size; rgb (optional) receives the uint8 ground truth [h, w, 3].
+ * Returns 0 / -1. */
+int msplat_dataset_camera_info(MsplatDataset ds, int index, bool test, float c2w[16], float intrinsics[4], int* width,
+                               int* height, uint8_t* rgb);
+
 /* ---- Trainer introspection (tests, benchmarks) --------------------------------------- */
 /* The scalars Model::fullIteration would hand to the kernels for (camera, step, adam step). */
 int msplat_trainer_step_params(MsplatTrainer t, int cameraIndex, int step, int adamStep, MsplatStepParams* out);

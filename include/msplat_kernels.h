@@ -40,6 +40,9 @@ void msplat_dev_free(void* p);
 int msplat_dev_upload(void* dst, const void* src, size_t bytes);
 int msplat_dev_download(void* dst, const void* src, size_t bytes);
 int msplat_dev_memset(void* dst, int value, size_t bytes);
+/* Device-to-device copy by a plain streaming kernel (bytes rounded down to 16); the
+ * calibration point for HBM traffic counters. Waits for completion. */
+int msplat_dev_copy_kernel(void* dst, const void* src, size_t bytes);
 
 /* Creates an engine bound to the current HIP device (buffers grow on demand and are
  * reused across steps). Replaces the reference's global MetalContext + FusedTensorCache

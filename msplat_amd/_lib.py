@@ -93,6 +93,7 @@ def lib() -> C.CDLL:
         "msplat_dev_upload": ([vp, vp, C.c_size_t], C.c_int),
         "msplat_dev_download": ([vp, vp, C.c_size_t], C.c_int),
         "msplat_dev_memset": ([vp, C.c_int, C.c_size_t], C.c_int),
+        "msplat_dev_copy_kernel": ([vp, vp, C.c_size_t], C.c_int),
         "msplat_engine_create": ([], vp),
         "msplat_engine_destroy": ([vp], None),
         "msplat_engine_train_step": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), vp,
@@ -156,6 +157,8 @@ def lib() -> C.CDLL:
         "msplat_dataset_create_empty": ([], vp),
         "msplat_dataset_add_camera": ([vp, fp, f32, f32, f32, f32, C.c_int, C.c_int, C.POINTER(C.c_uint8),
                                        C.c_bool], C.c_int),
+        "msplat_dataset_camera_info": ([vp, C.c_int, C.c_bool, fp, fp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                        C.POINTER(C.c_uint8)], C.c_int),
         "msplat_dataset_set_init_gaussians": ([vp, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp], C.c_int),
         "msplat_trainer_step_params": ([vp, C.c_int, C.c_int, C.c_int, C.POINTER(MsplatStepParams)], C.c_int),
         "msplat_trainer_last_camera": ([vp], C.c_int),

@@ -148,6 +148,24 @@ class Dataset:
     def num_test(self) -> int:
         return _lib.lib().msplat_dataset_num_test(self._h)
 
+    def camera(self, index: int, test: bool = False, image: bool = True) -> dict:
+        """Pose, intrinsics, size and (optionally) the uint8 ground truth of one camera."""
+        L = _lib.lib()
+        c2w = np.zeros(16, np.float32)
+        intr = np.zeros(4, np.float32)
+        w, h = C.c_int(), C.c_int()
+        if L.msplat_dataset_camera_info(self._h, int(index), bool(test), fptr(c2w), fptr(intr), C.byref(w),
+                                        C.byref(h), None) != 0:
+            raise IndexError(index)
+        out = {"c2w": c2w.reshape(4, 4), "fx": float(intr[0]), "fy": float(intr[1]), "cx": float(intr[2]),
+               "cy": float(intr[3]), "width": w.value, "height": h.value}
+        if image:
+            img = np.zeros((h.value, w.value, 3), np.uint8)
+            L.msplat_dataset_camera_info(self._h, int(index), bool(test), None, None, None, None,
+                                         img.ctypes.data_as(C.POINTER(C.c_uint8)))
+            out["image"] = img
+        return out
+
     def camera_pose(self, index: int) -> np.ndarray:
         out = np.zeros(16, np.float32)
         _lib.lib().msplat_dataset_camera_pose(self._h, int(index), fptr(out))

@@ -333,6 +333,22 @@ void msplat_dataset_camera_pose(MsplatDataset ds, int cameraIndex, float camToWo
     if (cameraIndex >= 0 && cameraIndex < (int)tr.size()) std::memcpy(camToWorld, tr[cameraIndex].c2w, 16 * sizeof(float));
 }
 
+int msplat_dataset_camera_info(MsplatDataset ds, int index, bool test, float c2w[16], float intrinsics[4], int* width,
+                               int* height, uint8_t* rgb) {
+    if (!ds) return -1;
+    auto& list = test ? DS(ds)->test : DS(ds)->train;
+    if (index < 0 || index >= (int)list.size()) return -1;
+    const TrainCamera& c = list[index];
+    if (c2w) std::memcpy(c2w, c.c2w, sizeof(c.c2w));
+    if (intrinsics) {
+        intrinsics[0] = c.fx; intrinsics[1] = c.fy; intrinsics[2] = c.cx; intrinsics[3] = c.cy;
+    }
+    if (width) *width = c.width;
+    if (height) *height = c.height;
+    if (rgb && hipMemcpy(rgb, c.gt, (size_t)c.width * c.height * 3, hipMemcpyDeviceToHost) != hipSuccess) return -1;
+    return 0;
+}
+
 int msplat_dataset_add_camera(MsplatDataset ds, const float c2w[16], float fx, float fy, float cx, float cy, int width,
                               int height, const uint8_t* rgb, bool test) {
     MS_GUARD_BEGIN

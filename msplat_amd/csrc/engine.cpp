@@ -400,6 +400,12 @@ int msplat_dev_memset(void* dst, int value, size_t bytes) {
     return hipMemset(dst, value, bytes) == hipSuccess ? 0 : -2;
 }
 
+int msplat_dev_copy_kernel(void* dst, const void* src, size_t bytes) {
+    if (!dst || !src) return -1;
+    msk::copy16(nullptr, src, dst, bytes);
+    return hipDeviceSynchronize() == hipSuccess ? 0 : -2;
+}
+
 MsplatEngine* msplat_engine_create(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return nullptr;

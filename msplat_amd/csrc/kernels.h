@@ -98,5 +98,6 @@ void gaussian_update(hipStream_t s, const Params& P, const MsplatCamera& cam, co
                      const MsplatStepParams& hp, uint32_t* nan_counts, float* dbg_v_mean, float* dbg_v_scale,
                      float* dbg_v_quat, float* dbg_v_opac);
 void fill_zero(hipStream_t s, void* p, size_t bytes);
+void copy16(hipStream_t s, const void* src, void* dst, size_t bytes);
 
 }  // namespace msk

@@ -106,6 +106,20 @@ void fill_zero(hipStream_t s, void* p, size_t bytes) {
     hipLaunchKernelGGL(k_fill_zero, dim3(blocks), dim3(256), 0, s, (uint32_t*)p, n);
 }
 
+// Plain 16-byte copy: the reference point that HBM traffic counters are scaled against.
+__global__ __launch_bounds__(256) void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+void copy16(hipStream_t s, const void* src, void* dst, size_t bytes) {
+    const size_t n = bytes / 16;
+    if (n == 0) return;
+    const unsigned blocks = (unsigned)std::min<size_t>(8192, (n + 255) / 256);
+    hipLaunchKernelGGL(k_copy16, dim3(blocks), dim3(256), 0, s, (const uint4*)src, (uint4*)dst, n);
+}
+
 // ------------------------------------------------------------------------------------
 // A5: project_and_sh_forward (metal:1719-1834). One thread per Gaussian; the block's
 // features_rest rows are staged through LDS with 16-byte loads (rows are 180 B apart).
