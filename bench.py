@@ -52,6 +52,9 @@ def parse():
     p.add_argument("--stage-steps", type=int, default=10)
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="launch the plain copy kernel once on this many MiB (PMC traffic calibration)")
+    p.add_argument("--suite", default="c1", choices=["c1", "c0", "c1_full", "c3_bf16", "bf16_error"],
+                   help="c1: the headline it/s line (default); the others print their own JSON line")
+    p.add_argument("--sh-bf16", action="store_true", help="bfloat16 SH storage for the c1 suite")
     return p.parse_args()
 
 
@@ -90,6 +93,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "a design note, not implemented)"}))
         sys.exit(2)
+    if a.suite != "c1":
+        import bench_suites
+        getattr(bench_suites, a.suite)(a)
+        return
     from msplat_amd import Dataset, GaussianTrainer, TrainingConfig
     from msplat_amd import _lib
 
@@ -101,6 +108,8 @@ def main():
     ds = Dataset(spec)
     cfg = TrainingConfig(iterations=30000, sh_degree=3, sh_degree_interval=1, cap_max=a.cap)
     tr = GaussianTrainer(ds, cfg)
+    if a.sh_bf16:
+        tr.set_option("sh_bf16", 1)
     eng = tr.engine_handle
     log(f"setup {time.time() - t_setup:.1f}s ({ds.num_train} cameras)")
 
@@ -159,9 +168,9 @@ def main():
 
     # algorithmic work per launch (bytes for HBM-bound kernels, FLOPs for the raster)
     work = {
-        "project_sh_fwd": ("hbm", N * (56 + 180 + 68)),
-        "gaussian_update": ("hbm", N * 408),
-        "sh_rest_adam": ("hbm", V * 1080 + N * 28),
+        "project_sh_fwd": ("hbm", N * ((50 + 90 + 68) if a.sh_bf16 else (56 + 180 + 68))),
+        "gaussian_update": ("hbm", N * (390 if a.sh_bf16 else 408)),
+        "sh_rest_adam": ("hbm", V * (810 if a.sh_bf16 else 1080) + N * 28),
         "raster_fwd": ("valu", pairs * 16),
         "raster_bwd": ("valu", pairs * 45),
     }
@@ -224,7 +233,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32+bf16-sh" if a.sh_bf16 else "f32",
         "data": "synthetic",
         "config": {"workload": "single-view 3DGS training step (C1: 64-component clustered mixture)",
                    "gaussians": N, "width": W, "height": H, "fx": a.fx, "cameras": ds.num_train,

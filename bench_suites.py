@@ -1,0 +1,215 @@
+"""Secondary benchmark suites of bench.py (`--suite ...`), each printing one JSON line:
+
+c0          the small configuration: 50k uniform-cube Gaussians, 16 cameras at 256x256,
+            200 steps of L1 + 0.2 (1 - SSIM) (SH degree 0 throughout: interval 1000), on the
+            GPU trainer and on the CPU oracle from the same initial state; it/s and final
+            loss of both.
+c1_full     C1 for 1000 steps with the plain MCMC relocation / growth at steps 500-800
+            (cap 3M, 8 of the 64 views held out): wall time, Gaussian count, held-out PSNR.
+c3_bf16     3M clustered Gaussians at 1920x1080: it/s with all-fp32 storage and with
+            bfloat16 SH coefficients + moments, per-stage times and HBM bytes saved.
+bf16_error  error against the FP64 oracle of the fp32 and the bf16-SH GPU paths, after 1
+            step (image, loss, raster gradients, updated parameters) and after 2000 steps
+            (parameters, render, loss), on a small scene (20k Gaussians, 128x128, 8 views).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+GROUPS = ("means", "scales", "quats", "f_dc", "f_rest", "opac")
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _oracle():
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as O
+    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    return O, O.set_threads(threads)
+
+
+def _state(tr):
+    return {(k, w): tr.read_group(i, w) for i, k in enumerate(GROUPS) for w in range(3)}
+
+
+def _host(O, st, n, degree=3, dtype=np.float32, sh_bf16=False):
+    nr = (degree + 1) ** 2 - 1
+    shapes = {"means": (n, 3), "scales": (n, 3), "quats": (n, 4), "f_dc": (n, 3), "f_rest": (n, nr, 3), "opac": (n,)}
+    g = {k: st[(k, 0)].reshape(shapes[k]) for k in GROUPS}
+    g["degree"] = degree
+    mom = ({k: st[(k, 1)].reshape(shapes[k]) for k in GROUPS}, {k: st[(k, 2)].reshape(shapes[k]) for k in GROUPS})
+    return O.HostState(g, dtype=dtype, moments=mom, sh_bf16=sh_bf16)
+
+
+def _camera(ds, scene, i, test=False):
+    c = ds.camera(i, test=test)
+    cam = scene.prepare_camera(c["c2w"], c["fx"], c["fy"], c["cx"], c["cy"], c["width"], c["height"])
+    return cam, c["image"]
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-30))
+
+
+def c0(a):
+    from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, _lib, scene
+    O, threads = _oracle()
+    L = _lib.lib()
+    ds = Dataset("synthetic:c0")
+    steps = 200
+    tr = GaussianTrainer(ds, TrainingConfig(iterations=steps))
+    n = tr.splat_count
+    host = _host(O, _state(tr), n)
+    cams = [_camera(ds, scene, i) for i in range(ds.num_train)]
+    plan = []
+    L.msplat_sync()
+    t0 = time.perf_counter()
+    for s in range(1, steps + 1):
+        hp = tr.step_params(0, s, s)
+        tr.step()
+        plan.append((tr.last_camera, hp))
+    L.msplat_sync()
+    gpu_s = time.perf_counter() - t0
+    gpu_loss = tr.last_loss
+    t0 = time.perf_counter()
+    cpu_loss = None
+    for ci, hp in plan:
+        out = O.train_step(host, cams[ci][0], cams[ci][1], hp, want_ids=False)
+        cpu_loss = out["loss"]
+    cpu_s = time.perf_counter() - t0
+    print(json.dumps({"suite": "c0", "gaussians": n, "steps": steps, "views": "16 x 256x256",
+                      "gpu_its": round(steps / gpu_s, 2), "cpu_its": round(steps / cpu_s, 3), "cpu_threads": threads,
+                      "speedup": round(cpu_s / gpu_s, 1), "gpu_final_loss": gpu_loss, "cpu_final_loss": cpu_loss,
+                      "final_loss_rel_diff": abs(gpu_loss - cpu_loss) / abs(cpu_loss),
+                      "note": "same initial state, same camera order and step scalars; trajectories "
+                              "separate only by float summation order"}), flush=True)
+
+
+def c1_full(a):
+    from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, _lib
+    L = _lib.lib()
+    ds = Dataset("synthetic:c1", eval_mode=True, test_every=8)
+    tr = GaussianTrainer(ds, TrainingConfig(iterations=1000, cap_max=3_000_000))
+    counts = {}
+    L.msplat_sync()
+    t0 = time.perf_counter()
+    for s in range(1, 1001):
+        st = tr.step()
+        if s % 100 == 0:
+            counts[s] = st.splat_count
+    L.msplat_sync()
+    wall = time.perf_counter() - t0
+    m = tr.evaluate()
+    print(json.dumps({"suite": "c1_full", "steps": 1000, "wall_s": round(wall, 2), "mean_its": round(1000 / wall, 2),
+                      "gaussians_by_step": counts, "train_views": ds.num_train, "test_views": ds.num_test,
+                      "heldout": m}), flush=True)
+
+
+def c3_bf16(a):
+    from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, _lib
+    L = _lib.lib()
+    n = 3_000_000
+    ds = Dataset(f"synthetic:c1:n={n},w=1920,h=1080,f=1300")
+    tr = GaussianTrainer(ds, TrainingConfig(iterations=30000, sh_degree_interval=1, cap_max=n))
+    eng = tr.engine_handle
+    res = {}
+    for mode in ("fp32", "bf16"):
+        tr.set_option("sh_bf16", 1 if mode == "bf16" else 0)
+        for _ in range(5):
+            tr.step()
+        L.msplat_sync()
+        t0 = time.perf_counter()
+        k = 30
+        for _ in range(k):
+            tr.step()
+        L.msplat_sync()
+        its = k / (time.perf_counter() - t0)
+        L.msplat_engine_reset_stage_times(eng)
+        L.msplat_engine_set_stage_timing(eng, 1)
+        for _ in range(5):
+            tr.step()
+        L.msplat_sync()
+        ms = (C.c_double * 16)()
+        nl = (C.c_int64 * 16)()
+        ns = L.msplat_engine_stage_times(eng, ms, 16, nl)
+        L.msplat_engine_set_stage_timing(eng, 0)
+        stages = {L.msplat_engine_stage_name(i).decode(): round(ms[i] / max(nl[i], 1), 4) for i in range(ns)}
+        radii = np.zeros(n, np.int32)
+        L.msplat_engine_read(eng, b"radii", radii.ctypes.data, radii.nbytes)
+        res[mode] = {"its": round(its, 2), "stages_ms": stages, "visible": int((radii > 0).sum())}
+        _log(f"{mode}: {its:.2f} it/s")
+    V = res["fp32"]["visible"]
+    per_step_saved = n * ((56 + 180) - (50 + 90)) + n * (408 - 390) + V * (1080 - 810)
+    resident_saved = n * (45 + 3) * 2 * 2 - n * (45 + 3) * 2  # moments halve, shadows add
+    print(json.dumps({"suite": "c3_bf16", "gaussians": n, "view": "1920x1080", "fp32": res["fp32"],
+                      "bf16_sh": res["bf16"], "speedup": round(res["bf16"]["its"] / res["fp32"]["its"], 3),
+                      "hbm_bytes_saved_per_step": per_step_saved,
+                      "resident_bytes_saved": resident_saved}), flush=True)
+
+
+def bf16_error(a):
+    from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, _lib, scene
+    O, threads = _oracle()
+    L = _lib.lib()
+    n, W, H = 20000, 128, 128
+    spec = f"synthetic:c1:n={n},hidden=8000,cams=8,w={W},h={H},f=128,jitter=0"
+    ds = Dataset(spec)
+    cfg = TrainingConfig(iterations=500, sh_degree_interval=1, cap_max=n)  # 500 * 0.833 < 500: no MCMC
+    trs = {"fp32": GaussianTrainer(ds, cfg), "bf16": GaussianTrainer(ds, cfg)}
+    trs["bf16"].set_option("sh_bf16", 1)
+    st0 = _state(trs["fp32"])
+    ref = _host(O, st0, n, dtype=np.float64)
+    cams = [_camera(ds, scene, i) for i in range(ds.num_train)]
+    out = {"suite": "bf16_error", "gaussians": n, "view": f"{W}x{H}", "views": ds.num_train}
+    plan = []
+    total = 2000
+    for s in range(1, total + 1):
+        hp = trs["fp32"].step_params(0, s, s)
+        for tr in trs.values():
+            tr.step()
+        ci = trs["fp32"].last_camera
+        assert trs["bf16"].last_camera == ci
+        plan.append((ci, hp))
+        if s == 1:
+            o64 = O.train_step(ref, cams[ci][0], cams[ci][1], hp, want_ids=False)
+            one = {}
+            for mode, tr in trs.items():
+                eng = tr.engine_handle
+                e = {"loss": abs(tr.last_loss - o64["loss"]) / abs(o64["loss"])}
+                for name, key, cnt in (("img", "img", W * H * 3), ("v_xy", "v_xy", 2 * n), ("v_conic", "v_conic", 3 * n),
+                                       ("v_rgb", "v_rgb", 3 * n), ("v_opac", "v_opac", n)):
+                    buf = np.zeros(cnt, np.float32)
+                    L.msplat_engine_read(eng, key.encode(), buf.ctypes.data, buf.nbytes)
+                    e[name] = _rel(buf, o64[name].reshape(-1))
+                st = _state(tr)
+                for k in GROUPS:
+                    e["param_" + k] = _rel(st[(k, 0)], ref.p[k][:n].reshape(-1))
+                one[mode] = e
+            out["after_1_step"] = one
+            _log(f"after 1 step: {one}")
+    for ci, hp in plan[1:]:
+        O.train_step(ref, cams[ci][0], cams[ci][1], hp, want_ids=False)
+    two = {}
+    eval_cam = cams[0][0]
+    r64 = O.render(ref, eval_cam, 3, scene.DEFAULT_BG, want_ids=False)["img"]
+    for mode, tr in trs.items():
+        st = _state(tr)
+        e = {k: _rel(st[(k, 0)], ref.p[k][:n].reshape(-1)) for k in GROUPS}
+        e["render"] = _rel(tr.render(0), r64)
+        two[mode] = e
+    out["after_2000_steps"] = two
+    out["oracle_threads"] = threads
+    out["note"] = "max-norm relative error per array against the FP64 oracle from the same initial state, " \
+                  "camera order and step scalars"
+    print(json.dumps(out), flush=True)

@@ -75,6 +75,12 @@ int64_t msplat_trainer_read_group(MsplatTrainer t, int group, int which, float* 
 /* Runs Model::mcmcAfterTrain(step) now (its step gate applies). info = {num_active, dead,
  * grown, nonfinite, culled, relocation counter}. Returns 0 / -1. */
 int msplat_trainer_run_mcmc(MsplatTrainer t, int step, int32_t info[6]);
+/* Trainer options beyond MsplatConfig (whose layout is fixed). Returns 0, or -1 for an
+ * unknown name or a failure.
+ *   "sh_bf16"  1 = store f_dc / f_rest coefficients and their Adam moments in bfloat16
+ *              (fp32 masters, fp32 arithmetic); 0 = all fp32 (default). Converts the
+ *              current state; checkpoints stay fp32 on disk. */
+int msplat_trainer_set_option(MsplatTrainer t, const char* name, double value);
 /* Adam step counter and current learning rates (lr[6]). */
 int msplat_trainer_optim_state(MsplatTrainer t, int* adamSteps, float lr[6]);
 /* The trainer's engine (borrowed: msplat_engine_destroy releases only the handle). */

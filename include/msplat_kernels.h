@@ -28,6 +28,12 @@ typedef struct {
     float *means, *scales, *quats, *f_dc, *f_rest, *opac;
     float* m[6];
     float* v[6];
+    /* bfloat16 SH storage (0 = everything fp32). When set, f_dc_bf16 / f_rest_bf16 hold the
+     * bf16 shadows of the fp32 masters f_dc / f_rest (the forward reads the shadows; the
+     * Adam update refreshes them), and m[3], v[3], m[4], v[4] point to bf16 arrays. */
+    int32_t sh_bf16;
+    uint16_t* f_dc_bf16;
+    uint16_t* f_rest_bf16;
 } MsplatDeviceState;
 
 typedef struct MsplatEngine MsplatEngine;

@@ -46,6 +46,7 @@ class MsplatDeviceState(C.Structure):
         ("means", C.c_void_p), ("scales", C.c_void_p), ("quats", C.c_void_p),
         ("f_dc", C.c_void_p), ("f_rest", C.c_void_p), ("opac", C.c_void_p),
         ("m", C.c_void_p * 6), ("v", C.c_void_p * 6),
+        ("sh_bf16", C.c_int32), ("f_dc_bf16", C.c_void_p), ("f_rest_bf16", C.c_void_p),
     ]
 
 
@@ -82,7 +83,13 @@ def lib() -> C.CDLL:
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(
             f"{LIB_PATH} is missing: build the HIP extension first (python -m msplat_amd.build)")
-    L = C.CDLL(LIB_PATH)
+    _lib = bind(LIB_PATH)
+    return _lib
+
+
+def bind(path: str) -> C.CDLL:
+    """Loads a library exporting the C ABI at `path` and declares every signature."""
+    L = C.CDLL(path)
     vp, i32, i64, f32, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_float, C.c_double
     fp = C.POINTER(C.c_float)
     sig = {
@@ -165,6 +172,7 @@ def lib() -> C.CDLL:
         "msplat_trainer_last_loss": ([vp], f64),
         "msplat_trainer_read_group": ([vp, C.c_int, C.c_int, fp, i64], i64),
         "msplat_trainer_run_mcmc": ([vp, C.c_int, C.POINTER(i32)], C.c_int),
+        "msplat_trainer_set_option": ([vp, C.c_char_p, f64], C.c_int),
         "msplat_trainer_optim_state": ([vp, C.POINTER(C.c_int), fp], C.c_int),
         "msplat_trainer_engine": ([vp], vp),
         "msplat_eq9_device": ([C.c_int, fp, C.POINTER(C.c_int), fp, fp], C.c_int),
@@ -173,7 +181,6 @@ def lib() -> C.CDLL:
         fn = getattr(L, name)
         fn.argtypes = argtypes
         fn.restype = restype
-    _lib = L
     return L
 
 

@@ -286,6 +286,11 @@ class GaussianTrainer:
         keys = ("num_active", "dead", "grown", "nonfinite", "culled", "relocation_count")
         return dict(zip(keys, [int(x) for x in info]))
 
+    def set_option(self, name: str, value) -> None:
+        """Options beyond TrainingConfig: "sh_bf16" (bfloat16 SH coefficients and moments)."""
+        if _lib.lib().msplat_trainer_set_option(self._h, name.encode(), float(value)) != 0:
+            raise RuntimeError(f"option {name!r} rejected (reason on stderr)")
+
     def optim_state(self):
         n = C.c_int()
         lr = np.zeros(6, np.float32)

@@ -576,19 +576,20 @@ double msplat_trainer_last_loss(MsplatTrainer t) { return t ? TR(t)->model->engi
 
 int64_t msplat_trainer_read_group(MsplatTrainer t, int group, int which, float* dst, int64_t count) {
     if (!t || group < 0 || group > 5 || which < 0 || which > 2) return -1;
-    TrainerImpl* T = TR(t);
     MS_GUARD_BEGIN
-    const MsplatDeviceState st = T->model->state();
-    const float* params[6] = {st.means, st.scales, st.quats, st.f_dc, st.f_rest, st.opac};
-    const float* src = which == 0 ? params[group] : (which == 1 ? st.m[group] : st.v[group]);
-    const int64_t avail = (int64_t)st.N * row_len(group, st.degree);
-    if (dst && count > 0) {
-        const int64_t n = std::min(count, avail);
-        hipStream_t s = T->model->engine().stream();
-        hip_check(hipStreamSynchronize(s), "read");
-        if (n) hip_check(hipMemcpy(dst, src, (size_t)n * 4, hipMemcpyDeviceToHost), "read");
+    return TR(t)->model->read_group(group, which, dst, count);
+    MS_GUARD_END(-1)
+}
+
+int msplat_trainer_set_option(MsplatTrainer t, const char* name, double value) {
+    if (!t || !name) return -1;
+    MS_GUARD_BEGIN
+    const std::string n(name);
+    if (n == "sh_bf16") {
+        TR(t)->model->set_sh_bf16(value != 0.0);
+        return 0;
     }
-    return avail;
+    throw std::invalid_argument("unknown option '" + n + "'");
     MS_GUARD_END(-1)
 }
 

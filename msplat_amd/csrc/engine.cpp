@@ -229,6 +229,9 @@ static msk::Params to_params(const MsplatDeviceState& st) {
     for (int g = 0; g < 6; g++) { P.m[g] = st.m[g]; P.v[g] = st.v[g]; }
     P.N = st.N;
     P.degree = st.degree;
+    P.sh_bf16 = st.sh_bf16;
+    P.f_dc_h = st.f_dc_bf16;
+    P.f_rest_h = st.f_rest_bf16;
     return P;
 }
 
@@ -237,6 +240,7 @@ static int nb3_of(int degree) { return degree <= 0 ? 0 : (degree == 1 ? 9 : degr
 int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt,
                        const MsplatStepParams& hp, bool debug) {
     if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0 || !gt) return -1;
+    if (st.sh_bf16 && (!st.f_dc_bf16 || (st.degree > 0 && !st.f_rest_bf16))) return -1;
     if (cam.width > 65535 * 16 || cam.height > 65535 * 16) return -1;
     if (!msk::lists_supported(st.N, ((cam.width + 15) / 16) * ((cam.height + 15) / 16))) return -1;
     const msk::Params P = to_params(st);
@@ -280,6 +284,7 @@ int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, con
 
 int Engine::render(const MsplatDeviceState& st, const MsplatCamera& cam, int degrees_to_use, const float bg[3]) {
     if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0) return -1;
+    if (st.sh_bf16 && (!st.f_dc_bf16 || (st.degree > 0 && !st.f_rest_bf16))) return -1;
     if (!msk::lists_supported(st.N, ((cam.width + 15) / 16) * ((cam.height + 15) / 16))) return -1;
     const msk::Params P = to_params(st);
     prepare(st.N, cam.width, cam.height, nb3_of(st.degree), false);

@@ -72,9 +72,12 @@ struct GradBufs {
 
 struct Params {
     float *means, *scales, *quats, *f_dc, *f_rest, *opac;
-    float* m[6];
+    float* m[6];   // groups 3 and 4 hold bf16 data when sh_bf16 is set
     float* v[6];
     int N, degree;
+    int sh_bf16 = 0;
+    uint16_t* f_dc_h = nullptr;    // bf16 shadows of f_dc / f_rest (sh_bf16 only)
+    uint16_t* f_rest_h = nullptr;
 };
 
 void project_forward(hipStream_t s, const Params& P, const MsplatCamera& cam, int degrees_to_use, ProjOut& o,
@@ -99,5 +102,10 @@ void gaussian_update(hipStream_t s, const Params& P, const MsplatCamera& cam, co
                      float* dbg_v_quat, float* dbg_v_opac);
 void fill_zero(hipStream_t s, void* p, size_t bytes);
 void copy16(hipStream_t s, const void* src, void* dst, size_t bytes);
+// bf16 storage helpers: shadows of f_dc / f_rest from the fp32 masters (rows [0, N)), and
+// element-wise conversions.
+void sh_refresh_shadows(hipStream_t s, const Params& P);
+void f32_to_bf16(hipStream_t s, const float* src, uint16_t* dst, size_t n);
+void bf16_to_f32(hipStream_t s, const uint16_t* src, float* dst, size_t n);
 
 }  // namespace msk

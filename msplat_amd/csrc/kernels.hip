@@ -157,24 +157,33 @@ __device__ __forceinline__ void cov3d_of(const float s[3], const float q[4], flo
     cov[5] = M[2][0] * M[2][0] + M[2][1] * M[2][1] + M[2][2] * M[2][2];
 }
 
+__device__ __forceinline__ float sh_ld(const float* p, int k) { return p[k]; }
+__device__ __forceinline__ float sh_ld(const uint16_t* p, int k) { return ms_bf16_float(p[k]); }
+
+// SHT = float (fp32 SH) or uint16_t (bf16 shadows of the fp32 masters).
+template <typename SHT>
 __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
     int N, const float* __restrict__ means, const float* __restrict__ scales, const float* __restrict__ quats,
-    const float* __restrict__ f_dc, const float* __restrict__ f_rest, const float* __restrict__ opac, int nb3,
+    const SHT* __restrict__ f_dc, const SHT* __restrict__ f_rest, const float* __restrict__ opac, int nb3,
     int deg_use, MsplatCamera cam, int tbx, int tby, int32_t* __restrict__ radii_o, int32_t* __restrict__ tiles_o,
     float* __restrict__ depth_o, ushort4* __restrict__ rect_o, float2* __restrict__ aabb_o, float4* __restrict__ rA,
     float4* __restrict__ rB, float2* __restrict__ rC, uint32_t* __restrict__ nanc) {
-    extern __shared__ float s_rest[];  // PROJ_BLOCK * nb3 floats when deg_use > 0
+    extern __shared__ uint4 s_raw[];  // PROJ_BLOCK * nb3 coefficients when deg_use > 0
+    SHT* s_rest = reinterpret_cast<SHT*>(s_raw);
     const int i0 = blockIdx.x * PROJ_BLOCK;
     const int i = i0 + threadIdx.x;
     if (deg_use > 0 && nb3 > 0) {
+        // the block's rows are contiguous and start 16-byte aligned (i0 * nb3 * sizeof(SHT)
+        // is a multiple of 16 for PROJ_BLOCK = 128)
         const int rows = min(PROJ_BLOCK, N - i0);
-        const int nf = rows * nb3;
-        const float* src = f_rest + (size_t)i0 * nb3;  // 16-byte aligned: i0 * nb3 is a multiple of 4
-        const int n4 = nf >> 2;
-        const float4* src4 = reinterpret_cast<const float4*>(src);
-        float4* dst4 = reinterpret_cast<float4*>(s_rest);
-        for (int k = threadIdx.x; k < n4; k += PROJ_BLOCK) dst4[k] = src4[k];
-        for (int k = (n4 << 2) + threadIdx.x; k < nf; k += PROJ_BLOCK) s_rest[k] = src[k];
+        const int nbytes = rows * nb3 * (int)sizeof(SHT);
+        const uint8_t* src = reinterpret_cast<const uint8_t*>(f_rest + (size_t)i0 * nb3);
+        const int n16 = nbytes >> 4;
+        const uint4* src16 = reinterpret_cast<const uint4*>(src);
+        for (int k = threadIdx.x; k < n16; k += PROJ_BLOCK) s_raw[k] = src16[k];
+        const uint16_t* src2 = reinterpret_cast<const uint16_t*>(src);
+        uint16_t* dst2 = reinterpret_cast<uint16_t*>(s_raw);
+        for (int k = (n16 << 3) + threadIdx.x; k < (nbytes >> 1); k += PROJ_BLOCK) dst2[k] = src2[k];
         __syncthreads();
     }
     if (i >= N) return;
@@ -259,24 +268,24 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
                 const float x = d0 * inv, y = d1 * inv, z = d2 * inv;
                 float col[3];
 #pragma unroll
-                for (int c = 0; c < 3; c++) col[c] = SH_C0 * f_dc[3 * (size_t)i + c];
+                for (int c = 0; c < 3; c++) col[c] = SH_C0 * sh_ld(f_dc + 3 * (size_t)i, c);
                 if (deg_use >= 1) {
-                    const float* rs = s_rest + threadIdx.x * nb3;
+                    const SHT* rs = s_rest + threadIdx.x * nb3;
                     const float xx = x * x, xy = x * y, xz = x * z, yy = y * y, yz = y * z, zz = z * z;
 #pragma unroll
                     for (int c = 0; c < 3; c++) {
-                        col[c] += SH_C1 * (-y * rs[0 * 3 + c] + z * rs[1 * 3 + c] - x * rs[2 * 3 + c]);
+                        col[c] += SH_C1 * (-y * sh_ld(rs, 0 * 3 + c) + z * sh_ld(rs, 1 * 3 + c) - x * sh_ld(rs, 2 * 3 + c));
                         if (deg_use < 2) continue;
-                        col[c] += (SH_C2_0 * xy * rs[3 * 3 + c] + SH_C2_1 * yz * rs[4 * 3 + c] +
-                                   SH_C2_2 * (2.f * zz - xx - yy) * rs[5 * 3 + c] + SH_C2_3 * xz * rs[6 * 3 + c] +
-                                   SH_C2_4 * (xx - yy) * rs[7 * 3 + c]);
+                        col[c] += (SH_C2_0 * xy * sh_ld(rs, 3 * 3 + c) + SH_C2_1 * yz * sh_ld(rs, 4 * 3 + c) +
+                                   SH_C2_2 * (2.f * zz - xx - yy) * sh_ld(rs, 5 * 3 + c) + SH_C2_3 * xz * sh_ld(rs, 6 * 3 + c) +
+                                   SH_C2_4 * (xx - yy) * sh_ld(rs, 7 * 3 + c));
                         if (deg_use < 3) continue;
-                        col[c] += (SH_C3_0 * y * (3.f * xx - yy) * rs[8 * 3 + c] + SH_C3_1 * xy * z * rs[9 * 3 + c] +
-                                   SH_C3_2 * y * (4.f * zz - xx - yy) * rs[10 * 3 + c] +
-                                   SH_C3_3 * z * (2.f * zz - 3.f * xx - 3.f * yy) * rs[11 * 3 + c] +
-                                   SH_C3_4 * x * (4.f * zz - xx - yy) * rs[12 * 3 + c] +
-                                   SH_C3_5 * z * (xx - yy) * rs[13 * 3 + c] +
-                                   SH_C3_6 * x * (xx - 3.f * yy) * rs[14 * 3 + c]);
+                        col[c] += (SH_C3_0 * y * (3.f * xx - yy) * sh_ld(rs, 8 * 3 + c) + SH_C3_1 * xy * z * sh_ld(rs, 9 * 3 + c) +
+                                   SH_C3_2 * y * (4.f * zz - xx - yy) * sh_ld(rs, 10 * 3 + c) +
+                                   SH_C3_3 * z * (2.f * zz - 3.f * xx - 3.f * yy) * sh_ld(rs, 11 * 3 + c) +
+                                   SH_C3_4 * x * (4.f * zz - xx - yy) * sh_ld(rs, 12 * 3 + c) +
+                                   SH_C3_5 * z * (xx - yy) * sh_ld(rs, 13 * 3 + c) +
+                                   SH_C3_6 * x * (xx - 3.f * yy) * sh_ld(rs, 14 * 3 + c));
                     }
                 }
                 // zombie gate (metal:1818-1833)
@@ -327,10 +336,16 @@ void project_forward(hipStream_t s, const Params& P, const MsplatCamera& cam, in
     const int nb3 = (P.degree <= 0 ? 0 : (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15)) * 3;
     const int tbx = (cam.width + TILE - 1) / TILE, tby = (cam.height + TILE - 1) / TILE;
     const int du = degrees_to_use < P.degree ? degrees_to_use : P.degree;
-    size_t lds = (du > 0 && nb3 > 0) ? (size_t)PROJ_BLOCK * nb3 * sizeof(float) : 0;
-    hipLaunchKernelGGL(k_project, dim3(cdiv(P.N, PROJ_BLOCK)), dim3(PROJ_BLOCK), lds, s, P.N, P.means, P.scales,
-                       P.quats, P.f_dc, P.f_rest, P.opac, nb3, du, cam, tbx, tby, o.radii, o.tiles_hit, o.depths,
-                       o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
+    const size_t esz = P.sh_bf16 ? 2 : 4;
+    size_t lds = (du > 0 && nb3 > 0) ? (size_t)PROJ_BLOCK * nb3 * esz : 0;
+    if (P.sh_bf16)
+        hipLaunchKernelGGL(k_project<uint16_t>, dim3(cdiv(P.N, PROJ_BLOCK)), dim3(PROJ_BLOCK), lds, s, P.N, P.means,
+                           P.scales, P.quats, P.f_dc_h, P.f_rest_h, P.opac, nb3, du, cam, tbx, tby, o.radii,
+                           o.tiles_hit, o.depths, o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
+    else
+        hipLaunchKernelGGL(k_project<float>, dim3(cdiv(P.N, PROJ_BLOCK)), dim3(PROJ_BLOCK), lds, s, P.N, P.means,
+                           P.scales, P.quats, P.f_dc, P.f_rest, P.opac, nb3, du, cam, tbx, tby, o.radii, o.tiles_hit,
+                           o.depths, o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1300,12 +1315,45 @@ __device__ __forceinline__ void sh_basis_all(float x, float y, float z, float* b
 // coefficients with 16-byte loads and stores of parameter, exp_avg and exp_avg_sq.
 constexpr int SH_EPB = 1024;
 
-template <int NB3>
+// Moment storage of the SH groups: fp32, or bf16 (rounded to nearest even on store).
+template <bool BF> struct Mom;
+template <> struct Mom<false> {
+    using T = float;
+    static __device__ __forceinline__ void ld4(const float* p, size_t e, float o[4]) {
+        const float4 a = reinterpret_cast<const float4*>(p)[e >> 2];
+        o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
+    }
+    static __device__ __forceinline__ void st4(float* p, size_t e, const float o[4]) {
+        reinterpret_cast<float4*>(p)[e >> 2] = make_float4(o[0], o[1], o[2], o[3]);
+    }
+    static __device__ __forceinline__ float ld(const float* p, size_t e) { return p[e]; }
+    static __device__ __forceinline__ void st(float* p, size_t e, float x) { p[e] = x; }
+};
+template <> struct Mom<true> {
+    using T = uint16_t;
+    static __device__ __forceinline__ void ld4(const uint16_t* p, size_t e, float o[4]) {
+        const uint2 a = reinterpret_cast<const uint2*>(p)[e >> 2];
+        o[0] = ms_bits_to_float(a.x << 16); o[1] = ms_bits_to_float(a.x & 0xFFFF0000u);
+        o[2] = ms_bits_to_float(a.y << 16); o[3] = ms_bits_to_float(a.y & 0xFFFF0000u);
+    }
+    static __device__ __forceinline__ void st4(uint16_t* p, size_t e, const float o[4]) {
+        uint2 a;
+        a.x = (uint32_t)ms_bf16_bits(o[0]) | ((uint32_t)ms_bf16_bits(o[1]) << 16);
+        a.y = (uint32_t)ms_bf16_bits(o[2]) | ((uint32_t)ms_bf16_bits(o[3]) << 16);
+        reinterpret_cast<uint2*>(p)[e >> 2] = a;
+    }
+    static __device__ __forceinline__ float ld(const uint16_t* p, size_t e) { return ms_bf16_float(p[e]); }
+    static __device__ __forceinline__ void st(uint16_t* p, size_t e, float x) { p[e] = ms_bf16_bits(x); }
+};
+
+template <int NB3, bool BF>
 __global__ __launch_bounds__(256) void k_sh_rest_adam(int N, int nuse3, const float* __restrict__ means, float cpx,
                                                       float cpy, float cpz, const int32_t* __restrict__ radii,
                                                       const float* __restrict__ v_rgb, float* __restrict__ f_rest,
-                                                      float* __restrict__ m4, float* __restrict__ v4, float step,
-                                                      float bc2s, float b1, float b2, float eps) {
+                                                      typename Mom<BF>::T* __restrict__ m4,
+                                                      typename Mom<BF>::T* __restrict__ v4,
+                                                      uint16_t* __restrict__ shadow, float step, float bc2s, float b1,
+                                                      float b2, float eps) {
     constexpr int MAXG = SH_EPB / NB3 + 2;
     __shared__ float s_b[MAXG][16];  // 15 basis values + visibility flag
     __shared__ float s_vc[MAXG][3];
@@ -1334,17 +1382,18 @@ __global__ __launch_bounds__(256) void k_sh_rest_adam(int N, int nuse3, const fl
     const bool vec = e + 4 <= total;
     if (vec) {
         const float4 a = reinterpret_cast<const float4*>(f_rest)[e >> 2];
-        const float4 bm = reinterpret_cast<const float4*>(m4)[e >> 2];
-        const float4 bv = reinterpret_cast<const float4*>(v4)[e >> 2];
         p[0] = a.x; p[1] = a.y; p[2] = a.z; p[3] = a.w;
-        m[0] = bm.x; m[1] = bm.y; m[2] = bm.z; m[3] = bm.w;
-        v[0] = bv.x; v[1] = bv.y; v[2] = bv.z; v[3] = bv.w;
+        Mom<BF>::ld4(m4, e, m);
+        Mom<BF>::ld4(v4, e, v);
     } else {
         for (int q = 0; q < 4; q++) {
             const bool in = e + q < total;
-            p[q] = in ? f_rest[e + q] : 0.f; m[q] = in ? m4[e + q] : 0.f; v[q] = in ? v4[e + q] : 0.f;
+            p[q] = in ? f_rest[e + q] : 0.f;
+            m[q] = in ? Mom<BF>::ld(m4, e + q) : 0.f;
+            v[q] = in ? Mom<BF>::ld(v4, e + q) : 0.f;
         }
     }
+    bool touched = false;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const size_t el = e + q;
@@ -1356,14 +1405,27 @@ __global__ __launch_bounds__(256) void k_sh_rest_adam(int N, int nuse3, const fl
         const int b = r / 3, c = r - 3 * b;
         const float g_el = s_b[k][b] * s_vc[k][c];
         adam_el(p[q], m[q], v[q], g_el, step, b1, b2, bc2s, eps);
+        touched = true;
     }
+    if (!touched) return;  // nothing of these four elements was updated
     if (vec) {
         reinterpret_cast<float4*>(f_rest)[e >> 2] = make_float4(p[0], p[1], p[2], p[3]);
-        reinterpret_cast<float4*>(m4)[e >> 2] = make_float4(m[0], m[1], m[2], m[3]);
-        reinterpret_cast<float4*>(v4)[e >> 2] = make_float4(v[0], v[1], v[2], v[3]);
+        Mom<BF>::st4(m4, e, m);
+        Mom<BF>::st4(v4, e, v);
+        if (BF) {
+            uint2 h;
+            h.x = (uint32_t)ms_bf16_bits(p[0]) | ((uint32_t)ms_bf16_bits(p[1]) << 16);
+            h.y = (uint32_t)ms_bf16_bits(p[2]) | ((uint32_t)ms_bf16_bits(p[3]) << 16);
+            reinterpret_cast<uint2*>(shadow)[e >> 2] = h;
+        }
     } else {
         for (int q = 0; q < 4; q++)
-            if (e + q < total) { f_rest[e + q] = p[q]; m4[e + q] = m[q]; v4[e + q] = v[q]; }
+            if (e + q < total) {
+                f_rest[e + q] = p[q];
+                Mom<BF>::st(m4, e + q, m[q]);
+                Mom<BF>::st(v4, e + q, v[q]);
+                if (BF) shadow[e + q] = ms_bf16_bits(p[q]);
+            }
     }
 }
 
@@ -1374,13 +1436,19 @@ void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const
     const int nuse3 = (du >= 3 ? 15 : du == 2 ? 8 : 3) * 3;
     const int nb3 = (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15) * 3;
     const size_t total = (size_t)P.N * nb3;
-#define LAUNCH_SH(NB)                                                                                              \
-    hipLaunchKernelGGL(k_sh_rest_adam<NB>, dim3(cdiv(total, SH_EPB)), dim3(256), 0, s, P.N, nuse3, P.means, cam.cam_pos[0], \
-                       cam.cam_pos[1], cam.cam_pos[2], radii, v_rgb, P.f_rest, P.m[4], P.v[4], hp.step_size[4],     \
-                       hp.bc2_sqrt[4], hp.beta1, hp.beta2, hp.eps)
-    if (nb3 == 9) LAUNCH_SH(9);
-    else if (nb3 == 24) LAUNCH_SH(24);
-    else LAUNCH_SH(45);
+#define LAUNCH_SH(NB, BF, MT)                                                                                      \
+    hipLaunchKernelGGL((k_sh_rest_adam<NB, BF>), dim3(cdiv(total, SH_EPB)), dim3(256), 0, s, P.N, nuse3, P.means,  \
+                       cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2], radii, v_rgb, P.f_rest, (MT*)P.m[4],        \
+                       (MT*)P.v[4], P.f_rest_h, hp.step_size[4], hp.bc2_sqrt[4], hp.beta1, hp.beta2, hp.eps)
+    if (P.sh_bf16) {
+        if (nb3 == 9) LAUNCH_SH(9, true, uint16_t);
+        else if (nb3 == 24) LAUNCH_SH(24, true, uint16_t);
+        else LAUNCH_SH(45, true, uint16_t);
+    } else {
+        if (nb3 == 9) LAUNCH_SH(9, false, float);
+        else if (nb3 == 24) LAUNCH_SH(24, false, float);
+        else LAUNCH_SH(45, false, float);
+    }
 #undef LAUNCH_SH
 }
 
@@ -1515,11 +1583,13 @@ __device__ __forceinline__ uint32_t pcg_hash32(uint32_t seed, uint32_t idx) {
 }
 __device__ __forceinline__ float pcg_unit(uint32_t h) { return ((float)(h >> 9) + 0.5f) * (1.0f / 8388608.0f); }
 
+template <bool BF>
 __global__ __launch_bounds__(256) void k_gaussian_update(
     int N, float* __restrict__ means, float* __restrict__ scales, float* __restrict__ quats, float* __restrict__ f_dc,
     float* __restrict__ opac, float* __restrict__ m0, float* __restrict__ v0, float* __restrict__ m1,
-    float* __restrict__ v1, float* __restrict__ m2, float* __restrict__ v2, float* __restrict__ m3,
-    float* __restrict__ v3, float* __restrict__ m5, float* __restrict__ v5, MsplatCamera cam,
+    float* __restrict__ v1, float* __restrict__ m2, float* __restrict__ v2, typename Mom<BF>::T* __restrict__ m3,
+    typename Mom<BF>::T* __restrict__ v3, uint16_t* __restrict__ dc_shadow, float* __restrict__ m5,
+    float* __restrict__ v5, MsplatCamera cam,
     const int32_t* __restrict__ radii, const float4* __restrict__ rA, const float4* __restrict__ rB, GradBufs G,
     MsplatStepParams hp, uint32_t* __restrict__ nanc, float* __restrict__ dbg_vm, float* __restrict__ dbg_vs,
     float* __restrict__ dbg_vq, float* __restrict__ dbg_vo) {
@@ -1542,9 +1612,12 @@ __global__ __launch_bounds__(256) void k_gaussian_update(
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const size_t k = 3 * (size_t)i + c;
-                float pp = f_dc[k], mm_ = m3[k], vv = v3[k];
+                float pp = f_dc[k], mm_ = Mom<BF>::ld(m3, k), vv = Mom<BF>::ld(v3, k);
                 adam_el(pp, mm_, vv, SH_C0 * vc[c], hp.step_size[3], b1, b2, hp.bc2_sqrt[3], eps);
-                f_dc[k] = pp; m3[k] = mm_; v3[k] = vv;
+                f_dc[k] = pp;
+                Mom<BF>::st(m3, k, mm_);
+                Mom<BF>::st(v3, k, vv);
+                if (BF) dc_shadow[k] = ms_bf16_bits(pp);
             }
         } else {
             atomicAdd(&nanc[3], 1u);
@@ -1552,7 +1625,8 @@ __global__ __launch_bounds__(256) void k_gaussian_update(
     }
     float vo = G.v_opac[i];
     if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
-        const float vdc = v3[3 * (size_t)i] + v3[3 * (size_t)i + 1] + v3[3 * (size_t)i + 2];
+        const float vdc = Mom<BF>::ld(v3, 3 * (size_t)i) + Mom<BF>::ld(v3, 3 * (size_t)i + 1) +
+                          Mom<BF>::ld(v3, 3 * (size_t)i + 2);
         const float duty = (vdc == 0.0f) ? 1.0f : hp.reg_invis_scale;
         const float invN = duty / (float)N;
         const float sig = 1.0f / (1.0f + ms_expf(-opac[i]));
@@ -1646,9 +1720,39 @@ void gaussian_update(hipStream_t s, const Params& P, const MsplatCamera& cam, co
                      const MsplatStepParams& hp, uint32_t* nan_counts, float* dbg_v_mean, float* dbg_v_scale,
                      float* dbg_v_quat, float* dbg_v_opac) {
     if (P.N <= 0) return;
-    hipLaunchKernelGGL(k_gaussian_update, dim3(cdiv(P.N, 256)), dim3(256), 0, s, P.N, P.means, P.scales, P.quats, P.f_dc,
-                       P.opac, P.m[0], P.v[0], P.m[1], P.v[1], P.m[2], P.v[2], P.m[3], P.v[3], P.m[5], P.v[5], cam,
-                       pj.radii, pj.rec.A, pj.rec.B, G, hp, nan_counts, dbg_v_mean, dbg_v_scale, dbg_v_quat, dbg_v_opac);
+    if (P.sh_bf16)
+        hipLaunchKernelGGL(k_gaussian_update<true>, dim3(cdiv(P.N, 256)), dim3(256), 0, s, P.N, P.means, P.scales,
+                           P.quats, P.f_dc, P.opac, P.m[0], P.v[0], P.m[1], P.v[1], P.m[2], P.v[2], (uint16_t*)P.m[3],
+                           (uint16_t*)P.v[3], P.f_dc_h, P.m[5], P.v[5], cam, pj.radii, pj.rec.A, pj.rec.B, G, hp,
+                           nan_counts, dbg_v_mean, dbg_v_scale, dbg_v_quat, dbg_v_opac);
+    else
+        hipLaunchKernelGGL(k_gaussian_update<false>, dim3(cdiv(P.N, 256)), dim3(256), 0, s, P.N, P.means, P.scales,
+                           P.quats, P.f_dc, P.opac, P.m[0], P.v[0], P.m[1], P.v[1], P.m[2], P.v[2], P.m[3], P.v[3],
+                           (uint16_t*)nullptr, P.m[5], P.v[5], cam, pj.radii, pj.rec.A, pj.rec.B, G, hp, nan_counts,
+                           dbg_v_mean, dbg_v_scale, dbg_v_quat, dbg_v_opac);
+}
+
+__global__ void k_f32_to_bf16(const float* __restrict__ src, uint16_t* __restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = ms_bf16_bits(src[i]);
+}
+__global__ void k_bf16_to_f32(const uint16_t* __restrict__ src, float* __restrict__ dst, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = ms_bf16_float(src[i]);
+}
+
+void f32_to_bf16(hipStream_t s, const float* src, uint16_t* dst, size_t n) {
+    if (n) hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+}
+void bf16_to_f32(hipStream_t s, const uint16_t* src, float* dst, size_t n) {
+    if (n) hipLaunchKernelGGL(k_bf16_to_f32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+}
+
+void sh_refresh_shadows(hipStream_t s, const Params& P) {
+    if (!P.sh_bf16 || P.N <= 0) return;
+    const int nb3 = (P.degree <= 0 ? 0 : (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15)) * 3;
+    f32_to_bf16(s, P.f_dc, P.f_dc_h, (size_t)P.N * 3);
+    f32_to_bf16(s, P.f_rest, P.f_rest_h, (size_t)P.N * nb3);
 }
 
 }  // namespace msk

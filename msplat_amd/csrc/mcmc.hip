@@ -115,19 +115,31 @@ __device__ void eq9(float o_old, const float s_old[3], int n, float* o_new_out, 
     *o_new_out = o_new;
 }
 
+// Row-wise view of the trainable state: the six parameter groups, the two bf16 SH shadows
+// (absent in fp32 mode) and the Adam moments, each with its row size in bytes (all even).
 struct Rows {
-    float* p[6];
-    float* m[6];
-    float* v[6];
-    int len[6];
+    void* p[8];
+    void* m[6];
+    void* v[6];
+    int pb[8];
+    int mb[6];
 };
 
-__device__ __forceinline__ void zero_row(const Rows& R, int idx) {
-    for (int g = 0; g < 6; g++)
-        for (int k = 0; k < R.len[g]; k++) {
-            R.m[g][(size_t)idx * R.len[g] + k] = 0.f;
-            R.v[g][(size_t)idx * R.len[g] + k] = 0.f;
-        }
+__device__ __forceinline__ void zero_row(void* base, int rb, int idx) {
+    uint16_t* r = reinterpret_cast<uint16_t*>(base) + (size_t)idx * (rb >> 1);
+    for (int k = 0; k < (rb >> 1); k++) r[k] = 0;
+}
+__device__ __forceinline__ void copy_row(void* base, int rb, int dst, int src) {
+    uint16_t* b = reinterpret_cast<uint16_t*>(base);
+    const size_t n = (size_t)(rb >> 1);
+    for (size_t k = 0; k < n; k++) b[(size_t)dst * n + k] = b[(size_t)src * n + k];
+}
+__device__ __forceinline__ void zero_moments(const Rows& R, int idx) {
+    for (int g = 0; g < 6; g++) {
+        if (!R.mb[g]) continue;
+        zero_row(R.m[g], R.mb[g], idx);
+        zero_row(R.v[g], R.mb[g], idx);
+    }
 }
 
 __global__ void k_eq9_sources(int ncand, const uint32_t* __restrict__ cand_list, const uint32_t* __restrict__ counts,
@@ -135,8 +147,8 @@ __global__ void k_eq9_sources(int ncand, const uint32_t* __restrict__ cand_list,
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= ncand || counts[k] == 0) return;
     const int src = cand_list ? (int)cand_list[k] : k;
-    float* op = R.p[5];
-    float* sc = R.p[1];
+    float* op = reinterpret_cast<float*>(R.p[5]);
+    float* sc = reinterpret_cast<float*>(R.p[1]);
     const float o_old = 1.0f / (1.0f + ms_expf(-op[src]));
     const float s_old[3] = {ms_expf(sc[3 * (size_t)src]), ms_expf(sc[3 * (size_t)src + 1]), ms_expf(sc[3 * (size_t)src + 2])};
     float o_new, s_new[3];
@@ -144,7 +156,7 @@ __global__ void k_eq9_sources(int ncand, const uint32_t* __restrict__ cand_list,
     o_new = fminf(fmaxf(o_new, 1e-6f), 1.0f - 1e-6f);
     op[src] = (float)log((double)o_new / (1.0 - (double)o_new));
     for (int j = 0; j < 3; j++) sc[3 * (size_t)src + j] = (float)log((double)fmaxf(s_new[j], 1e-10f));
-    zero_row(R, src);
+    zero_moments(R, src);
 }
 
 __global__ void k_copy_rows(int num, const uint32_t* __restrict__ dst_list, int dst_base,
@@ -153,9 +165,9 @@ __global__ void k_copy_rows(int num, const uint32_t* __restrict__ dst_list, int 
     if (j >= num) return;
     const int dst = dst_list ? (int)dst_list[j] : dst_base + j;
     const int src = cand_list ? (int)cand_list[chosen[j]] : (int)chosen[j];
-    for (int g = 0; g < 6; g++)
-        for (int k = 0; k < R.len[g]; k++) R.p[g][(size_t)dst * R.len[g] + k] = R.p[g][(size_t)src * R.len[g] + k];
-    zero_row(R, dst);
+    for (int g = 0; g < 8; g++)
+        if (R.pb[g]) copy_row(R.p[g], R.pb[g], dst, src);
+    zero_moments(R, dst);
 }
 
 // Eq. 9 on explicit (opacity, count) pairs, for the known-answer tests.
@@ -200,13 +212,19 @@ static Rows make_rows(const Params& P) {
     Rows R;
     const int nr = (P.degree <= 0 ? 0 : (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15)) * 3;
     const int lens[6] = {3, 3, 4, 3, nr, 1};
-    float* ps[6] = {P.means, P.scales, P.quats, P.f_dc, P.f_rest, P.opac};
+    void* ps[6] = {P.means, P.scales, P.quats, P.f_dc, P.f_rest, P.opac};
     for (int g = 0; g < 6; g++) {
         R.p[g] = ps[g];
+        R.pb[g] = lens[g] * 4;
         R.m[g] = P.m[g];
         R.v[g] = P.v[g];
-        R.len[g] = lens[g];
+        const bool half = P.sh_bf16 && (g == 3 || g == 4);
+        R.mb[g] = lens[g] * (half ? 2 : 4);
     }
+    R.p[6] = P.sh_bf16 ? (void*)P.f_dc_h : nullptr;
+    R.pb[6] = P.sh_bf16 ? 6 : 0;
+    R.p[7] = P.sh_bf16 ? (void*)P.f_rest_h : nullptr;
+    R.pb[7] = (P.sh_bf16 && nr) ? nr * 2 : 0;
     return R;
 }
 
@@ -241,7 +259,7 @@ McmcInfo mcmc_relocate_and_grow(hipStream_t s, const Params& P, int cap_max, flo
     info.num_active = N;
     if (N <= 0) return info;
     const Rows R = make_rows(P);
-    const int nr = R.len[4];
+    const int nr = R.pb[4] / 4;
     uint32_t* dead = grow<uint32_t>(S, 0, N);
     uint32_t* alive = grow<uint32_t>(S, 1, N);
     uint64_t* w = grow<uint64_t>(S, 2, N);

@@ -14,6 +14,7 @@
 #include <stdexcept>
 
 #include "../../include/msplat_api_ext.h"
+#include "ms_math.h"
 
 namespace msplat_hip {
 
@@ -105,39 +106,106 @@ Model::~Model() {
         m_[k].release();
         v_[k].release();
     }
+    dc_h_.release();
+    rest_h_.release();
     mcmc_.release();
 }
 
 void Model::allocate(int degree) {
     degree_ = degree;
+    hipStream_t s = eng_.stream();
     for (int k = 0; k < 6; k++) {
-        const size_t bytes = (size_t)cap_ * std::max(row_len(k, degree), 1) * sizeof(float);
+        const size_t rows = (size_t)cap_ * std::max(row_len(k, degree), 1);
         p_[k].release();
         m_[k].release();
         v_[k].release();
-        p_[k].ensure(bytes);
-        m_[k].ensure(bytes);
-        v_[k].ensure(bytes);
-        msk::fill_zero(eng_.stream(), p_[k].p, bytes);
-        msk::fill_zero(eng_.stream(), m_[k].p, bytes);
-        msk::fill_zero(eng_.stream(), v_[k].p, bytes);
+        p_[k].ensure(rows * 4);
+        m_[k].ensure(rows * moment_elem_bytes(k));
+        v_[k].ensure(rows * moment_elem_bytes(k));
+        msk::fill_zero(s, p_[k].p, rows * 4);
+        msk::fill_zero(s, m_[k].p, rows * moment_elem_bytes(k));
+        msk::fill_zero(s, v_[k].p, rows * moment_elem_bytes(k));
     }
-    hip_check(hipStreamSynchronize(eng_.stream()), "allocate");
+    dc_h_.release();
+    rest_h_.release();
+    if (sh_bf16_) {
+        dc_h_.ensure((size_t)cap_ * 3 * 2);
+        rest_h_.ensure((size_t)cap_ * std::max(row_len(4, degree), 1) * 2);
+        msk::fill_zero(s, dc_h_.p, (size_t)cap_ * 3 * 2);
+        msk::fill_zero(s, rest_h_.p, (size_t)cap_ * std::max(row_len(4, degree), 1) * 2);
+    }
+    hip_check(hipStreamSynchronize(s), "allocate");
+}
+
+void Model::refresh_shadows() {
+    if (!sh_bf16_) return;
+    msk::Params P;
+    P.f_dc = p_[3].as<float>();
+    P.f_rest = p_[4].as<float>();
+    P.f_dc_h = dc_h_.as<uint16_t>();
+    P.f_rest_h = rest_h_.as<uint16_t>();
+    P.N = num_active_;
+    P.degree = degree_;
+    P.sh_bf16 = 1;
+    msk::sh_refresh_shadows(eng_.stream(), P);
+}
+
+void Model::set_sh_bf16(bool on) {
+    if (on == sh_bf16_) return;
+    hipStream_t s = eng_.stream();
+    for (int g = 3; g <= 4; g++) {
+        const size_t n = (size_t)cap_ * std::max(row_len(g, degree_), 1);
+        DBuf* bufs[2] = {&m_[g], &v_[g]};
+        for (DBuf* b : bufs) {
+            DBuf nb;
+            nb.ensure(n * (on ? 2 : 4));
+            if (on) msk::f32_to_bf16(s, b->as<float>(), nb.as<uint16_t>(), n);
+            else msk::bf16_to_f32(s, b->as<uint16_t>(), nb.as<float>(), n);
+            hip_check(hipStreamSynchronize(s), "moment conversion");
+            b->release();
+            *b = nb;
+        }
+    }
+    sh_bf16_ = on;
+    dc_h_.release();
+    rest_h_.release();
+    if (on) {
+        dc_h_.ensure((size_t)cap_ * 3 * 2);
+        rest_h_.ensure((size_t)cap_ * std::max(row_len(4, degree_), 1) * 2);
+        msk::fill_zero(s, dc_h_.p, (size_t)cap_ * 3 * 2);
+        msk::fill_zero(s, rest_h_.p, (size_t)cap_ * std::max(row_len(4, degree_), 1) * 2);
+        refresh_shadows();
+    }
+    hip_check(hipStreamSynchronize(s), "sh storage switch");
 }
 
 void Model::upload(const HostGaussians& g, const std::vector<float>* m, const std::vector<float>* v) {
     if (g.n > cap_) throw std::runtime_error("more Gaussians than cap_max");
     if (g.degree != degree_) allocate(g.degree);
     hipStream_t s = eng_.stream();
+    std::vector<uint16_t> half;
     for (int k = 0; k < 6; k++) {
-        const size_t bytes = (size_t)g.n * row_len(k, degree_) * sizeof(float);
-        if (bytes == 0) continue;
-        hip_check(hipMemcpyAsync(p_[k].p, group_vec(g, k)->data(), bytes, hipMemcpyHostToDevice, s), "upload");
-        if (m) hip_check(hipMemcpyAsync(m_[k].p, m[k].data(), bytes, hipMemcpyHostToDevice, s), "upload");
-        if (v) hip_check(hipMemcpyAsync(v_[k].p, v[k].data(), bytes, hipMemcpyHostToDevice, s), "upload");
+        const size_t count = (size_t)g.n * row_len(k, degree_);
+        if (count == 0) continue;
+        hip_check(hipMemcpyAsync(p_[k].p, group_vec(g, k)->data(), count * 4, hipMemcpyHostToDevice, s), "upload");
+        const std::vector<float>* src[2] = {m ? &m[k] : nullptr, v ? &v[k] : nullptr};
+        DBuf* dst[2] = {&m_[k], &v_[k]};
+        for (int w = 0; w < 2; w++) {
+            if (!src[w]) continue;
+            if (moment_elem_bytes(k) == 2) {
+                half.resize(count);
+                for (size_t e = 0; e < count; e++) half[e] = ms_bf16_bits((*src[w])[e]);
+                hip_check(hipMemcpyAsync(dst[w]->p, half.data(), count * 2, hipMemcpyHostToDevice, s), "upload");
+                hip_check(hipStreamSynchronize(s), "upload");
+            } else {
+                hip_check(hipMemcpyAsync(dst[w]->p, src[w]->data(), count * 4, hipMemcpyHostToDevice, s), "upload");
+            }
+        }
     }
     hip_check(hipStreamSynchronize(s), "upload");
     num_active_ = g.n;
+    refresh_shadows();
+    hip_check(hipStreamSynchronize(s), "upload");
 }
 
 MsplatDeviceState Model::state() const {
@@ -154,6 +222,9 @@ MsplatDeviceState Model::state() const {
         st.m[k] = m_[k].as<float>();
         st.v[k] = v_[k].as<float>();
     }
+    st.sh_bf16 = sh_bf16_ ? 1 : 0;
+    st.f_dc_bf16 = sh_bf16_ ? dc_h_.as<uint16_t>() : nullptr;
+    st.f_rest_bf16 = sh_bf16_ ? rest_h_.as<uint16_t>() : nullptr;
     return st;
 }
 
@@ -223,6 +294,9 @@ msk::McmcInfo Model::mcmc_after_train(int step) {
     for (int g = 0; g < 6; g++) { P.m[g] = st.m[g]; P.v[g] = st.v[g]; }
     P.N = num_active_;
     P.degree = degree_;
+    P.sh_bf16 = st.sh_bf16;
+    P.f_dc_h = st.f_dc_bf16;
+    P.f_rest_h = st.f_rest_bf16;
     info = msk::mcmc_relocate_and_grow(eng_.stream(), P, cap_, opt_.cull_radius, reloc_count_, mcmc_);
     num_active_ = info.num_active;
     return info;
@@ -247,15 +321,30 @@ HostGaussians Model::download() const {
     return g;
 }
 
+int64_t Model::read_group(int g, int which, float* dst, int64_t count) const {
+    const int64_t avail = (int64_t)num_active_ * row_len(g, degree_);
+    if (!dst || count <= 0 || avail == 0) return avail;
+    const int64_t n = std::min(count, avail);
+    hip_check(hipStreamSynchronize(const_cast<Engine&>(eng_).stream()), "read");
+    const DBuf& b = which == 0 ? p_[g] : (which == 1 ? m_[g] : v_[g]);
+    if (which != 0 && moment_elem_bytes(g) == 2) {
+        std::vector<uint16_t> h((size_t)n);
+        hip_check(hipMemcpy(h.data(), b.p, (size_t)n * 2, hipMemcpyDeviceToHost), "read");
+        for (int64_t e = 0; e < n; e++) dst[e] = ms_bf16_float(h[(size_t)e]);
+    } else {
+        hip_check(hipMemcpy(dst, b.p, (size_t)n * 4, hipMemcpyDeviceToHost), "read");
+    }
+    return avail;
+}
+
 void Model::download_moments(std::vector<float> m[6], std::vector<float> v[6]) const {
-    hip_check(hipStreamSynchronize(const_cast<Engine&>(eng_).stream()), "download");
     for (int k = 0; k < 6; k++) {
         const size_t count = (size_t)num_active_ * row_len(k, degree_);
         m[k].assign(count, 0.f);
         v[k].assign(count, 0.f);
         if (count) {
-            hip_check(hipMemcpy(m[k].data(), m_[k].p, count * 4, hipMemcpyDeviceToHost), "download");
-            hip_check(hipMemcpy(v[k].data(), v_[k].p, count * 4, hipMemcpyDeviceToHost), "download");
+            read_group(k, 1, m[k].data(), (int64_t)count);
+            read_group(k, 2, v[k].data(), (int64_t)count);
         }
     }
 }
@@ -281,10 +370,10 @@ int Model::load_checkpoint(const std::string& path) {
     hipStream_t s = eng_.stream();
     if (c.g.degree != degree_) allocate(c.g.degree);
     for (int k = 0; k < 6; k++) {
-        const size_t bytes = (size_t)cap_ * std::max(row_len(k, degree_), 1) * sizeof(float);
-        msk::fill_zero(s, p_[k].p, bytes);
-        msk::fill_zero(s, m_[k].p, bytes);
-        msk::fill_zero(s, v_[k].p, bytes);
+        const size_t rows = (size_t)cap_ * std::max(row_len(k, degree_), 1);
+        msk::fill_zero(s, p_[k].p, rows * 4);
+        msk::fill_zero(s, m_[k].p, rows * moment_elem_bytes(k));
+        msk::fill_zero(s, v_[k].p, rows * moment_elem_bytes(k));
     }
     upload(c.g, c.m, c.v);
     std::memcpy(os_.adam_lr, c.adam_lr, sizeof(os_.adam_lr));

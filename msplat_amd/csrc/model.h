@@ -76,6 +76,13 @@ public:
 
     HostGaussians download() const;
     void download_moments(std::vector<float> m[6], std::vector<float> v[6]) const;
+    // Copies up to `count` floats of the active rows of group g (which: 0 parameter,
+    // 1 exp_avg, 2 exp_avg_sq), converting bf16 storage to float; returns the row-major size.
+    int64_t read_group(int g, int which, float* dst, int64_t count) const;
+    // SH coefficients and their Adam moments in bfloat16 storage with fp32 masters
+    // (converts the current state in place).
+    void set_sh_bf16(bool on);
+    bool sh_bf16() const { return sh_bf16_; }
     MsplatDeviceState state() const;
     int num_active() const { return num_active_; }
     int capacity() const { return cap_; }
@@ -88,6 +95,8 @@ public:
 
 private:
     void allocate(int degree);
+    size_t moment_elem_bytes(int g) const { return (sh_bf16_ && (g == 3 || g == 4)) ? 2 : 4; }
+    void refresh_shadows();
     void upload(const HostGaussians& g, const std::vector<float>* m, const std::vector<float>* v);
 
     ModelOptions opt_;
@@ -96,6 +105,8 @@ private:
     uint32_t reloc_count_ = 0;
     float scale_ = 1.0f, translation_[3] = {0, 0, 0};
     DBuf p_[6], m_[6], v_[6];
+    bool sh_bf16_ = false;
+    DBuf dc_h_, rest_h_;  // bf16 shadows of f_dc / f_rest (sh_bf16_ only)
     Engine eng_;
     msk::McmcScratch mcmc_;
 };

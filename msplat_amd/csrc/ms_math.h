@@ -140,3 +140,17 @@ MS_HD void ms_sincosf(float a, float* s_out, float* c_out) {
 
 // Logistic sigmoid in the reference's form 1 / (1 + exp(-x)).
 MS_HD float ms_sigmoid(float x) { return 1.0f / (1.0f + ms_expf(-x)); }
+
+// bfloat16 storage: round to nearest, ties to even, on the upper 16 bits of the float;
+// NaN stays NaN (quiet), infinities stay infinite. Shared by the kernels and the oracle.
+MS_HD uint16_t ms_bf16_bits(float x) {
+    const uint32_t u = ms_float_to_bits(x);
+    if ((u & 0x7F800000u) == 0x7F800000u) {
+        if (u & 0x007FFFFFu) return (uint16_t)((u >> 16) | 0x0040u);
+        return (uint16_t)(u >> 16);
+    }
+    const uint32_t r = u + 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(r >> 16);
+}
+MS_HD float ms_bf16_float(uint16_t h) { return ms_bits_to_float((uint32_t)h << 16); }
+MS_HD float ms_bf16_round(float x) { return ms_bf16_float(ms_bf16_bits(x)); }

@@ -960,7 +960,21 @@ template <class T> struct State {
     T *means, *scales, *quats, *f_dc, *f_rest, *opac;
     T* m[6];
     T* v[6];
+    // SH storage in bfloat16 (FP32 only): the forward reads f_dc / f_rest rounded to bf16
+    // (the device keeps a bf16 shadow of the fp32 master), and the Adam moments of groups
+    // 3 and 4 are stored rounded to bf16 after every update. Masters stay fp32.
+    int sh_bf16 = 0;
 };
+
+template <class T> inline T sh_store(T x, int bf16) { return x; }
+template <> inline float sh_store<float>(float x, int bf16) { return bf16 ? ms_bf16_round(x) : x; }
+
+template <class T> inline const T* sh_view(const State<T>& S, const T* src, size_t n, std::vector<T>& tmp) {
+    if (!S.sh_bf16) return src;
+    tmp.resize(n);
+    for (size_t k = 0; k < n; k++) tmp[k] = sh_store<T>(src[k], 1);
+    return tmp.data();
+}
 
 template <class T> struct StepOut {
     Proj<T> proj;
@@ -977,7 +991,11 @@ template <class T>
 void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const MsplatStepParams& hp, StepOut<T>& O) {
     const int N = S.N, W = cam.width, H = cam.height;
     const int nb = num_sh_bases(S.degree) - 1;
-    project_forward<T>(N, S.means, S.scales, S.quats, S.f_dc, S.f_rest, S.opac, S.degree, hp.degrees_to_use, cam, O.proj);
+    std::vector<T> dc_h, rest_h;
+    const T* f_dc_fwd = sh_view(S, S.f_dc, 3 * (size_t)N, dc_h);
+    const T* f_rest_fwd = sh_view(S, S.f_rest, (size_t)N * nb * 3, rest_h);
+    project_forward<T>(N, S.means, S.scales, S.quats, f_dc_fwd, f_rest_fwd, S.opac, S.degree, hp.degrees_to_use, cam,
+                       O.proj);
     build_lists<T>(N, O.proj, W, H, O.lists);
     uint32_t K = k_max_rule(O.lists.capacity, O.lists.num_tiles);
     raster_forward<T>(O.lists, W, H, hp.background, K, O.raster);
@@ -1008,6 +1026,8 @@ void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const M
                     size_t k = 3 * (size_t)i + c;
                     adam<T>(S.f_dc[k], S.m[3][k], S.v[3][k], T(SH_C0) * vc[c], T(hp.step_size[3]), b1, b2,
                             T(hp.bc2_sqrt[3]), eps);
+                    S.m[3][k] = sh_store<T>(S.m[3][k], S.sh_bf16);
+                    S.v[3][k] = sh_store<T>(S.v[3][k], S.sh_bf16);
                 }
                 if (hp.degrees_to_use >= 1) {
                     T bs[15];
@@ -1018,6 +1038,8 @@ void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const M
                             size_t k = (size_t)nb * 3 * i + b * 3 + c;
                             adam<T>(S.f_rest[k], S.m[4][k], S.v[4][k], bs[b] * vc[c], T(hp.step_size[4]), b1, b2,
                                     T(hp.bc2_sqrt[4]), eps);
+                            S.m[4][k] = sh_store<T>(S.m[4][k], S.sh_bf16);
+                            S.v[4][k] = sh_store<T>(S.v[4][k], S.sh_bf16);
                         }
                 }
             } else {
@@ -1062,7 +1084,12 @@ void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const M
 // Forward only (msplat_render, mm:1099-1116)
 template <class T>
 void render(const State<T>& S, const MsplatCamera& cam, int degrees_to_use, const float bg[3], StepOut<T>& O) {
-    project_forward<T>(S.N, S.means, S.scales, S.quats, S.f_dc, S.f_rest, S.opac, S.degree, degrees_to_use, cam, O.proj);
+    std::vector<T> dc_h, rest_h;
+    const int nb = num_sh_bases(S.degree) - 1;
+    const T* f_dc_fwd = sh_view(S, S.f_dc, 3 * (size_t)S.N, dc_h);
+    const T* f_rest_fwd = sh_view(S, S.f_rest, (size_t)S.N * nb * 3, rest_h);
+    project_forward<T>(S.N, S.means, S.scales, S.quats, f_dc_fwd, f_rest_fwd, S.opac, S.degree, degrees_to_use, cam,
+                       O.proj);
     build_lists<T>(S.N, O.proj, cam.width, cam.height, O.lists);
     uint32_t K = k_max_rule(O.lists.capacity, O.lists.num_tiles);
     raster_forward<T>(O.lists, cam.width, cam.height, bg, K, O.raster);

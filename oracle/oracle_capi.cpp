@@ -13,6 +13,7 @@ using namespace oracle;
         FT *means, *scales, *quats, *f_dc, *f_rest, *opac;                                       \
         FT* m[6];                                                                                \
         FT* v[6];                                                                                \
+        int32_t sh_bf16;                                                                         \
     } OracleState##SUF;                                                                          \
     typedef struct {                                                                             \
         FT *xys, *depths, *conics, *colors, *aabb;                                               \
@@ -41,6 +42,7 @@ template <class T, class SS> static State<T> make_state(SS* s) {
     S.means = s->means; S.scales = s->scales; S.quats = s->quats;
     S.f_dc = s->f_dc; S.f_rest = s->f_rest; S.opac = s->opac;
     for (int g = 0; g < 6; g++) { S.m[g] = s->m[g]; S.v[g] = s->v[g]; }
+    S.sh_bf16 = s->sh_bf16;
     return S;
 }
 

@@ -25,7 +25,7 @@ def _state_struct(ft):
 
     class S(C.Structure):
         _fields_ = [("N", C.c_int32), ("degree", C.c_int32), ("means", P), ("scales", P), ("quats", P),
-                    ("f_dc", P), ("f_rest", P), ("opac", P), ("m", P * 6), ("v", P * 6)]
+                    ("f_dc", P), ("f_rest", P), ("opac", P), ("m", P * 6), ("v", P * 6), ("sh_bf16", C.c_int32)]
     return S
 
 
@@ -104,8 +104,9 @@ GROUP_SHAPES = ("means", "scales", "quats", "f_dc", "f_rest", "opac")
 class HostState:
     """Host copy of the trainable state (params + Adam moments), numpy arrays of a given dtype."""
 
-    def __init__(self, g: dict, dtype=np.float32, moments=None, capacity=None):
+    def __init__(self, g: dict, dtype=np.float32, moments=None, capacity=None, sh_bf16=False):
         self.dtype = dtype
+        self.sh_bf16 = bool(sh_bf16)
         self.degree = int(g["degree"])
         n = g["means"].shape[0]
         self.N = n
@@ -130,6 +131,7 @@ class HostState:
         ft = C.c_float if self.dtype == np.float32 else C.c_double
         s = S()
         s.N, s.degree = self.N, self.degree
+        s.sh_bf16 = 1 if self.sh_bf16 else 0
 
         def ptr(a):
             return a.ctypes.data_as(C.POINTER(ft))

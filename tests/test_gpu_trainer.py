@@ -54,7 +54,7 @@ def test_trainer_steps_match_oracle():
     g, hidden, _ = small_scene(W=W, H=H)
     ds, cams = _dataset(g, hidden, W, H)
     train_cams = [c for c in cams if not c["test"]]
-    cfg = TrainingConfig(iterations=1000, sh_degree_interval=1, cap_max=g["means"].shape[0] + 64)
+    cfg = TrainingConfig(iterations=1000, sh_degree_interval=1, cap_max=2 * g["means"].shape[0])
     tr = GaussianTrainer(ds, cfg)
     for step in (1, 2, 3, 4):
         before = _state(tr)
@@ -88,28 +88,37 @@ def _mcmc_scene(n=4000, degree=3, seed=5):
     return g
 
 
-@pytest.mark.parametrize("cap_extra", [0.10, 0.02], ids=["grow_5pct", "grow_to_cap"])
-def test_gpu_mcmc_matches_oracle(cap_extra):
+@pytest.mark.parametrize("grow_to_cap", [False, True], ids=["grow_5pct", "grow_to_cap"])
+def test_gpu_mcmc_matches_oracle(grow_to_cap):
+    """Relocation + growth on the device against the oracle from the same pre-pass state.
+    The initial set is 75% of the cap (above that, fresh seeds are decimated first,
+    model.cpp:176-204); "grow_to_cap" runs passes until the next growth hits the cap."""
     W, H = 96, 80
     g = _mcmc_scene()
     n = g["means"].shape[0]
     hidden = scene.synth_gaussians(0, 99, 500, 3)
     ds, _ = _dataset(g, hidden, W, H, n_cams=2)
-    cap = int(n * (1 + cap_extra))
-    tr = GaussianTrainer(ds, TrainingConfig(iterations=1000, cap_max=cap))
-    # non-zero moments so that zeroing is observable
-    tr.step()
+    cap = (4 * n) // 3 + 2  # n <= 0.75 cap: no seed decimation
+    tr = GaussianTrainer(ds, TrainingConfig(iterations=10000, cap_max=cap))
+    tr.step()  # non-zero moments so that zeroing is observable
+    step, count = 500, 0
+    if grow_to_cap:
+        while int(np.float32(1.05) * np.float32(tr.splat_count)) < cap:
+            count = tr.run_mcmc(step)["relocation_count"]
+            step += 100
+            tr.step()
+    n0 = tr.splat_count
     before = _state(tr)
     host = _host_state(before, 3, cap=cap)
     info_o = (C.c_int32 * 6)()
-    L = O.lib()
-    L.oracle_mcmc_step(host.struct(), cap, 3.0, 0, info_o)
-    info = tr.run_mcmc(500)
+    O.lib().oracle_mcmc_step(host.struct(), cap, 3.0, count, info_o)
+    info = tr.run_mcmc(step)
     want = dict(zip(("num_active", "dead", "grown", "nonfinite", "culled", "relocation_count"), list(info_o)))
     assert info == want
-    assert info["dead"] > n // 5 and info["culled"] == 30 - 3 and info["nonfinite"] == 3 + 5
+    if not grow_to_cap:
+        assert info["dead"] > n // 5 and info["culled"] == 30 - 3 and info["nonfinite"] == 3 + 5
     n2 = info["num_active"]
-    assert n2 == min(cap, int(np.float32(1.05) * np.float32(n)))
+    assert n2 == min(cap, int(np.float32(1.05) * np.float32(n0)))
     after = _state(tr)
     for k in GROUPS:
         got = after[k].reshape(n2, -1)
@@ -122,9 +131,9 @@ def test_gpu_mcmc_matches_oracle(cap_extra):
         assert_exact("m." + k, after["m." + k].reshape(n2, -1), host.m[k][:n2].reshape(n2, -1))
         assert_exact("v." + k, after["v." + k].reshape(n2, -1), host.v[k][:n2].reshape(n2, -1))
     # the relocation counter advanced once per draw (relocation + growth)
-    assert info["relocation_count"] == 2
+    assert info["relocation_count"] == count + 2
     # outside the schedule nothing happens (model.cpp:498-501)
-    assert tr.run_mcmc(550)["num_active"] == n2
+    assert tr.run_mcmc(step + 50)["num_active"] == n2
 
 
 def test_gpu_eq9_known_answers():
