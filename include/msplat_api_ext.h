@@ -61,6 +61,22 @@ int msplat_dataset_set_init_gaussians(MsplatDataset ds, int n, int degree, const
 int msplat_dataset_camera_info(MsplatDataset ds, int index, bool test, float c2w[16], float intrinsics[4], int* width,
                                int* height, uint8_t* rgb);
 
+/* ---- Multi-view batched steps (no counterpart in the reference, which trains one view per
+ * step) ---------------------------------------------------------------------------------
+ * One iteration over the next `views` training cameras of the shuffled order: every view
+ * renders the same parameters, the gradients of the mean of the per-view losses are summed
+ * over the views, and one Adam update, the regulariser and SGLD follow; the LR schedule and
+ * MCMC relocation then advance by one iteration. Views may differ in size. views <= 64. */
+MsplatStats msplat_trainer_step_batch(MsplatTrainer t, int views);
+/* Cameras (train indices) and per-view losses of the most recent batched step; returns the
+ * number of views, or -1. Either output may be NULL. Waits for the device when losses is set. */
+int msplat_trainer_last_batch(MsplatTrainer t, int32_t* cameras, double* losses, int max_views);
+/* Recorded-graph counters of the batched steps: out = {graphs recorded, steps replayed from
+ * a recorded graph, launch width, launch height}. Batched steps are recorded once per launch
+ * configuration and replayed when msplat_trainer_set_option(t, "batch_graph", 1) is set.
+ * Returns 0 / -1. */
+int msplat_trainer_batch_info(MsplatTrainer t, int64_t out[4]);
+
 /* ---- Trainer introspection (tests, benchmarks) --------------------------------------- */
 /* The scalars Model::fullIteration would hand to the kernels for (camera, step, adam step). */
 int msplat_trainer_step_params(MsplatTrainer t, int cameraIndex, int step, int adamStep, MsplatStepParams* out);

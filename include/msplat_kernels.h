@@ -109,6 +109,20 @@ int msplat_engine_stage_times(MsplatEngine* e, double* ms, int max_stages, int64
 const char* msplat_engine_stage_name(int k);
 void msplat_engine_reset_stage_times(MsplatEngine* e);
 
+/* ---- Learned relocation proposal: the network's kernels on device buffers (no
+ * counterpart in the reference; used by the trainer's learned relocation and by tests).
+ * Parameters theta: 5508 floats, W1 [64 16], b1 [64], W2 [64 64], b2 [64], W3 [4 64],
+ * b3 [4] (row-major [out][in]); inputs X: [N 16] bfloat16 bit patterns; targets: [N 8]
+ * floats (y, logit weight, offset target x3, offset weight, 2 unused). All pointers are
+ * device pointers; every call waits for its kernels. Return 0, -1 (bad argument) or -2. */
+size_t msplat_mlp_packed_bytes(void);
+int msplat_mlp_pack(const float* theta, void* packed);
+/* out [N 4] = (logit, offset x, y, z). */
+int msplat_mlp_forward(int n, const uint16_t* X, const void* packed, float* out);
+/* grad [5508] += d loss / d theta, loss2 [2] += (logit term, offset term). */
+int msplat_mlp_backward(int n, const uint16_t* X, const float* targets, const void* packed, float* grad,
+                        double* loss2);
+
 #ifdef __cplusplus
 }
 #endif

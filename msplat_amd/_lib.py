@@ -169,6 +169,9 @@ def bind(path: str) -> C.CDLL:
         "msplat_dataset_set_init_gaussians": ([vp, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp], C.c_int),
         "msplat_trainer_step_params": ([vp, C.c_int, C.c_int, C.c_int, C.POINTER(MsplatStepParams)], C.c_int),
         "msplat_trainer_last_camera": ([vp], C.c_int),
+        "msplat_trainer_step_batch": ([vp, C.c_int], MsplatStats),
+        "msplat_trainer_last_batch": ([vp, C.POINTER(i32), C.POINTER(f64), C.c_int], C.c_int),
+        "msplat_trainer_batch_info": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "msplat_trainer_last_loss": ([vp], f64),
         "msplat_trainer_read_group": ([vp, C.c_int, C.c_int, fp, i64], i64),
         "msplat_trainer_run_mcmc": ([vp, C.c_int, C.POINTER(i32)], C.c_int),

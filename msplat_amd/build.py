@@ -21,8 +21,11 @@ LIB = os.path.join(HERE, "libmsplat_core.so")
 SOURCES = [
     "kernels.hip",
     "mcmc.hip",
+    "mlp.hip",
     "eval.hip",
     "engine.cpp",
+    "batch.cpp",
+    "proposal.cpp",
     "model.cpp",
     "api.cpp",
     "scene.cpp",

@@ -204,6 +204,29 @@ class GaussianTrainer:
             raise RuntimeError("training step failed (reason on stderr)")
         return TrainingStats(s)
 
+    def step_batch(self, views: int) -> TrainingStats:
+        """One multi-view iteration over the next `views` cameras (msplat_trainer_step_batch)."""
+        s = _lib.lib().msplat_trainer_step_batch(self._h, int(views))
+        if s.iteration == 0:
+            raise RuntimeError("batched step failed (reason on stderr)")
+        return TrainingStats(s)
+
+    def last_batch(self, losses: bool = False):
+        """(camera indices, per-view losses or None) of the most recent batched step."""
+        L = _lib.lib()
+        n = L.msplat_trainer_last_batch(self._h, None, None, 0)
+        cams = (C.c_int32 * max(n, 1))()
+        ls = (C.c_double * max(n, 1))()
+        L.msplat_trainer_last_batch(self._h, cams, ls if losses else None, n)
+        return [int(c) for c in cams[:n]], ([float(x) for x in ls[:n]] if losses else None)
+
+    def batch_info(self) -> dict:
+        """Recorded-graph counters of the batched steps (msplat_trainer_batch_info)."""
+        out = (C.c_int64 * 4)()
+        if _lib.lib().msplat_trainer_batch_info(self._h, out) != 0:
+            raise RuntimeError("batch_info failed")
+        return {"captures": int(out[0]), "replays": int(out[1]), "launch": (int(out[2]), int(out[3]))}
+
     def train(self, callback=None, callback_every: int = 100) -> None:
         while self.iteration < self._cfg.iterations:
             s = self.step()
@@ -287,7 +310,9 @@ class GaussianTrainer:
         return dict(zip(keys, [int(x) for x in info]))
 
     def set_option(self, name: str, value) -> None:
-        """Options beyond TrainingConfig: "sh_bf16" (bfloat16 SH coefficients and moments)."""
+        """Options beyond TrainingConfig: "sh_bf16" (bfloat16 SH coefficients and moments),
+        "batch_graph" (record batched steps as HIP graphs and replay them), "persist_grid"
+        (resident workgroups of the persistent raster kernels, 0 = default)."""
         if _lib.lib().msplat_trainer_set_option(self._h, name.encode(), float(value)) != 0:
             raise RuntimeError(f"option {name!r} rejected (reason on stderr)")
 

@@ -165,6 +165,7 @@ struct TrainerImpl {
     size_t pos = 0;
     uint64_t rng = 42;  // camera order: splitmix64 Fisher-Yates (reference: mt19937(42) + std::shuffle)
     int last_cam = -1;
+    std::vector<int> last_batch;
     MsplatEngine handle{nullptr, false};
     DBuf rgba, partials, sums;
 
@@ -437,6 +438,57 @@ MsplatStats msplat_trainer_step(MsplatTrainer t) {
     MS_GUARD_END(s)
 }
 
+MsplatStats msplat_trainer_step_batch(MsplatTrainer t, int views) {
+    MsplatStats s{0, 0, 0.f};
+    if (!t || views <= 0) return s;
+    TrainerImpl* T = TR(t);
+    MS_GUARD_BEGIN
+    const auto t0 = std::chrono::high_resolution_clock::now();
+    T->step++;
+    T->last_batch.clear();
+    std::vector<const TrainCamera*> cams;
+    for (int v = 0; v < views; v++) {
+        const int ci = T->next_camera();
+        T->last_batch.push_back(ci);
+        cams.push_back(&T->ds->train[ci]);
+    }
+    T->last_cam = T->last_batch.back();
+    T->model->full_iteration_batch(cams.data(), views, T->step, T->cfg.ssimWeight);
+    T->model->schedulers_step(T->step);
+    T->model->mcmc_after_train(T->step);
+    const auto t1 = std::chrono::high_resolution_clock::now();
+    s.iteration = T->step;
+    s.splatCount = T->model->num_active();
+    s.msPerStep = (float)(std::chrono::duration_cast<std::chrono::microseconds>(t1 - t0).count() / 1000.0);
+    return s;
+    MS_GUARD_END(s)
+}
+
+int msplat_trainer_last_batch(MsplatTrainer t, int32_t* cameras, double* losses, int max_views) {
+    if (!t) return -1;
+    TrainerImpl* T = TR(t);
+    MS_GUARD_BEGIN
+    const int n = (int)T->last_batch.size();
+    std::vector<double> l;
+    if (losses) l = T->model->batch_losses();
+    for (int v = 0; v < n && v < max_views; v++) {
+        if (cameras) cameras[v] = T->last_batch[v];
+        if (losses && v < (int)l.size()) losses[v] = l[v];
+    }
+    return n;
+    MS_GUARD_END(-1)
+}
+
+int msplat_trainer_batch_info(MsplatTrainer t, int64_t out[4]) {
+    if (!t || !out) return -1;
+    msplat_hip::BatchRunner* b = TR(t)->model->batch_runner();
+    out[0] = b ? b->captures() : 0;
+    out[1] = b ? b->replays() : 0;
+    out[2] = b ? b->launch_geom().W : 0;
+    out[3] = b ? b->launch_geom().H : 0;
+    return 0;
+}
+
 void msplat_trainer_train(MsplatTrainer t) {
     if (!t) return;
     while (TR(t)->step < TR(t)->cfg.iterations) {
@@ -587,6 +639,16 @@ int msplat_trainer_set_option(MsplatTrainer t, const char* name, double value) {
     const std::string n(name);
     if (n == "sh_bf16") {
         TR(t)->model->set_sh_bf16(value != 0.0);
+        return 0;
+    }
+    if (n == "batch_graph") {
+        TR(t)->model->set_batch_graph(value != 0.0);
+        return 0;
+    }
+    if (n == "persist_grid") {
+        if (value < 0.0 || value > 1 << 20) throw std::invalid_argument("persist_grid out of range");
+        TR(t)->model->engine().set_persist_grid(value > 0.0 ? (int)value : msplat_hip::default_persist_grid());
+        TR(t)->model->set_persist_grid((int)value);
         return 0;
     }
     throw std::invalid_argument("unknown option '" + n + "'");

@@ -1,0 +1,209 @@
+// batch.cpp — multi-view batched step (no counterpart in the reference, which trains one
+// view per step: Model::fullIteration, core/src/model.cpp:901-980). Semantics: every view
+// runs the single-view forward/backward on the same parameters; the projection VJPs and SH
+// colour gradients are summed in view order; one Adam/SGLD update follows, SH Adam for the
+// Gaussians that had a finite colour gradient in at least one view.
+#include "batch.h"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+namespace msplat_hip {
+
+bool BatchRunner::Key::operator==(const Key& o) const {
+    if (N != o.N || B != o.B || du != o.du || degree != o.degree || bf16 != o.bf16 || grid != o.grid) return false;
+    if (g.W != o.g.W || g.H != o.g.H || g.T != o.g.T || gens != o.gens || nanc != o.nanc) return false;
+    for (int k = 0; k < 8; k++)
+        if (state[k] != o.state[k]) return false;
+    for (int k = 0; k < 7; k++)
+        if (acc[k] != o.acc[k]) return false;
+    for (int k = 0; k < 12; k++)
+        if (mom[k] != o.mom[k]) return false;
+    for (int k = 0; k < 7; k++)
+        if (own[k] != o.own[k]) return false;
+    return true;
+}
+
+BatchRunner::BatchRunner(hipStream_t s) : s_(s) { persist_grid_ = default_persist_grid(); }
+
+BatchRunner::~BatchRunner() {
+    (void)hipStreamSynchronize(s_);
+    release_graph();
+    DBuf* all[] = {&block_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_};
+    for (DBuf* b : all) b->release();
+}
+
+void BatchRunner::release_graph() {
+    if (exec_) (void)hipGraphExecDestroy(exec_);
+    exec_ = nullptr;
+    key_ = Key();
+}
+
+void BatchRunner::set_persist_grid(int g) {
+    persist_grid_ = g > 0 ? g : default_persist_grid();
+    for (auto& e : slots_) e->set_persist_grid(persist_grid_);
+}
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// The launch sequence of one batched step (recorded once per key in graph mode).
+void BatchRunner::enqueue(const msk::Params& P, int B, int du, int Tmax, msk::BatchGrads A, uint32_t* nan_counts) {
+    const size_t n = (size_t)(P.N > 0 ? P.N : 1);
+    const int nb3 = P.degree <= 0 ? 0 : (P.degree == 1 ? 9 : P.degree == 2 ? 24 : 45);
+    msk::fill_zero(s_, A.v_mean, n * 12);
+    msk::fill_zero(s_, A.v_scale, n * 12);
+    msk::fill_zero(s_, A.v_quat, n * 16);
+    msk::fill_zero(s_, A.v_opac, n * 4);
+    msk::fill_zero(s_, A.v_dc, n * 12);
+    msk::fill_zero(s_, A.v_rest, n * (size_t)std::max(nb3, 1) * 4);
+    msk::fill_zero(s_, A.vis, n * 4);
+    for (int v = 0; v < B; v++) slots_[v]->front(P, du, true);
+    msk::RasterWork w{};
+    w.views = reinterpret_cast<const msk::RasterView*>(block_.as<uint8_t>() + align256(sizeof(MsplatStepParams)) +
+                                                       align256(sizeof(msk::ViewConsts) * B));
+    w.B = B;
+    w.Tmax = Tmax;
+    w.keys_a = wka_.as<uint32_t>();
+    w.vals_a = wva_.as<uint32_t>();
+    w.keys_b = wkb_.as<uint32_t>();
+    w.vals_b = wvb_.as<uint32_t>();
+    w.temp = wtemp_.p;
+    w.n_work = wmisc_.as<uint32_t>();
+    w.ctr = wmisc_.as<uint32_t>() + 1;
+    w.grid = persist_grid_;
+    for (int v = 0; v < B; v++)
+        if (slots_[v]->chunked()) slots_[v]->raster_fwd_own();
+    msk::raster_work_build(s_, w);
+    msk::raster_forward_views(s_, w);
+    for (int v = 0; v < B; v++) slots_[v]->loss_to(0);
+    for (int v = 0; v < B; v++)
+        if (slots_[v]->chunked()) slots_[v]->raster_bwd_own();
+    msk::raster_backward_views(s_, w);
+    for (int v = 0; v < B; v++) {
+        MsplatDeviceState st{};
+        st.means = P.means; st.scales = P.scales; st.quats = P.quats; st.f_dc = P.f_dc; st.f_rest = P.f_rest;
+        st.opac = P.opac;
+        st.N = P.N;
+        st.degree = P.degree;
+        st.sh_bf16 = P.sh_bf16;
+        st.f_dc_bf16 = P.f_dc_h;
+        st.f_rest_bf16 = P.f_rest_h;
+        slots_[v]->accumulate(st, du, A);
+    }
+    msk::batch_update(s_, P, A, block_.as<MsplatStepParams>(), du, nan_counts);
+}
+
+void BatchRunner::run(const MsplatDeviceState& st, const BatchView* views, int B, const MsplatStepParams& hp,
+                      msk::BatchGrads A, uint32_t* nan_counts) {
+    if (B <= 0 || B > Engine::kLossSlots) throw std::invalid_argument("batch size must be in [1, 64]");
+    const msk::Params P = params_of(st);
+    const int du = hp.degrees_to_use < st.degree ? hp.degrees_to_use : st.degree;
+    while ((int)slots_.size() < B) {
+        slots_.emplace_back(new Engine(s_));
+        slots_.back()->set_persist_grid(persist_grid_);
+    }
+    sizes_.assign(B, {0, 0});
+    bool graphable = graph_on_;
+    msk::Geom gmax{};
+    for (int v = 0; v < B; v++) {
+        const MsplatCamera& c = views[v].cam;
+        if (c.width <= 0 || c.height <= 0 || !views[v].gt) throw std::invalid_argument("batched view without image");
+        const msk::Geom g = msk::geom_of(c.width, c.height);
+        if (!msk::lists_supported(st.N, g.T)) throw std::invalid_argument("view too large for the tile lists");
+        sizes_[v] = {c.width, c.height};
+        gmax.W = std::max(gmax.W, g.W);
+        gmax.H = std::max(gmax.H, g.H);
+    }
+    if (graphable) {
+        hw_.W = std::max(hw_.W, gmax.W);
+        hw_.H = std::max(hw_.H, gmax.H);
+        hw_ = msk::geom_of(hw_.W, hw_.H);
+        if (!msk::lists_supported(st.N, hw_.T)) graphable = false;
+    }
+    // per-view buffers: the recorded geometry (graph) or each view's own
+    for (int v = 0; v < B; v++) {
+        const msk::Geom g = msk::geom_of(sizes_[v].first, sizes_[v].second);
+        slots_[v]->prepare(st.N, graphable ? hw_ : g, g, true);
+    }
+    int Tmax = 0;
+    for (int v = 0; v < B; v++)
+        Tmax = std::max(Tmax, graphable ? hw_.T : msk::geom_of(sizes_[v].first, sizes_[v].second).T);
+    const size_t items = (size_t)B * Tmax;
+    wka_.ensure(items * 4); wva_.ensure(items * 4); wkb_.ensure(items * 4); wvb_.ensure(items * 4);
+    wtemp_.ensure(msk::work_temp_bytes((int)items));
+    wmisc_.ensure(64);
+    const size_t off_vc = align256(sizeof(MsplatStepParams));
+    const size_t off_rv = off_vc + align256(sizeof(msk::ViewConsts) * B);
+    const size_t bytes = off_rv + sizeof(msk::RasterView) * B;
+    block_.ensure(bytes);
+    // host image of the step block
+    std::vector<uint8_t> h(bytes, 0);
+    std::memcpy(h.data(), &hp, sizeof(hp));
+    for (int v = 0; v < B; v++) {
+        const msk::ViewConsts vc =
+            Engine::make_view(views[v].cam, views[v].gt, hp.background, hp.ssim_weight, views[v].inv_n);
+        const msk::ViewConsts* d = reinterpret_cast<const msk::ViewConsts*>(block_.as<uint8_t>() + off_vc) + v;
+        slots_[v]->bind_view(vc, d);
+        std::memcpy(h.data() + off_vc + sizeof(vc) * v, &vc, sizeof(vc));
+        const msk::RasterView rv = slots_[v]->raster_view(true);
+        std::memcpy(h.data() + off_rv + sizeof(rv) * v, &rv, sizeof(rv));
+    }
+    ring_.upload(s_, block_.p, h.data(), bytes);
+    if (!graphable) {
+        enqueue(P, B, du, Tmax, A, nan_counts);
+        hip_check(hipGetLastError(), "batched step");
+        return;
+    }
+    Key k;
+    k.N = st.N;
+    k.B = B;
+    k.du = du;
+    k.degree = st.degree;
+    k.bf16 = st.sh_bf16;
+    k.grid = persist_grid_;
+    k.g = hw_;
+    int64_t gens = 0;
+    // per slot: buffer generation and depth-chunk count (chunked views record their own path)
+    for (int v = 0; v < B; v++) gens = gens * 1000003 + slots_[v]->generation() * 4096 + slots_[v]->chunks();
+    k.gens = gens;
+    const DBuf* own[7] = {&block_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_};
+    for (int j = 0; j < 7; j++) k.own[j] = own[j]->p;
+    const void* ptrs[8] = {st.means, st.scales, st.quats, st.f_dc, st.f_rest, st.opac, st.f_dc_bf16, st.f_rest_bf16};
+    for (int j = 0; j < 8; j++) k.state[j] = ptrs[j];
+    for (int j = 0; j < 6; j++) {
+        k.mom[j] = st.m[j];
+        k.mom[6 + j] = st.v[j];
+    }
+    const void* accs[7] = {A.v_mean, A.v_scale, A.v_quat, A.v_opac, A.v_dc, A.v_rest, A.vis};
+    for (int j = 0; j < 7; j++) k.acc[j] = accs[j];
+    k.nanc = nan_counts;
+    if (!(exec_ && key_ == k)) {
+        release_graph();
+        hipGraph_t graph = nullptr;
+        hip_check(hipStreamBeginCapture(s_, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+        try {
+            enqueue(P, B, du, Tmax, A, nan_counts);
+        } catch (...) {
+            (void)hipStreamEndCapture(s_, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(s_, &graph), "hipStreamEndCapture");
+        const hipError_t e = hipGraphInstantiate(&exec_, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hip_check(e, "hipGraphInstantiate");
+        key_ = k;
+        captures_++;
+    } else {
+        replays_++;
+    }
+    hip_check(hipGraphLaunch(exec_, s_), "hipGraphLaunch");
+}
+
+double BatchRunner::loss(int v) {
+    if (v < 0 || v >= (int)sizes_.size()) return 0.0;
+    return slots_[v]->loss_slot(0, sizes_[v].first, sizes_[v].second);
+}
+
+}  // namespace msplat_hip

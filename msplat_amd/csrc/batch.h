@@ -1,0 +1,73 @@
+// batch.h — the multi-view batched training step: B views per step, each with its own
+// engine slot (lists, images, gradients), one persistent raster launch over all views'
+// tiles in each direction, gradient sums in view order and one update. The whole
+// iteration can be recorded once as a HIP graph and replayed: every per-step input the
+// kernels read (cameras, targets, step scalars, the raster view table) sits in one device
+// block that is rewritten before each replay.
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "engine.h"
+
+namespace msplat_hip {
+
+struct BatchView {
+    MsplatCamera cam;
+    const uint8_t* gt;   // device [H W 3]
+    float inv_n;         // loss normalisation of this view
+};
+
+class BatchRunner {
+public:
+    explicit BatchRunner(hipStream_t s);
+    ~BatchRunner();
+    BatchRunner(const BatchRunner&) = delete;
+    BatchRunner& operator=(const BatchRunner&) = delete;
+
+    // One step over views[0..B). `A` holds the cap-sized gradient sums (zeroed here).
+    void run(const MsplatDeviceState& st, const BatchView* views, int B, const MsplatStepParams& hp,
+             msk::BatchGrads A, uint32_t* nan_counts);
+    // Mean loss of view v of the last step (waits for the stream).
+    double loss(int v);
+    void set_graph(bool on) { graph_on_ = on; }
+    bool graph() const { return graph_on_; }
+    // Work-queue residency of the persistent raster kernels (0 = default).
+    void set_persist_grid(int g);
+    int persist_grid() const { return persist_grid_; }
+    // Counters: steps replayed from a recorded graph, graphs recorded.
+    int64_t replays() const { return replays_; }
+    int64_t captures() const { return captures_; }
+    // Tile-count geometry the recorded graph is launched with (grows monotonically).
+    msk::Geom launch_geom() const { return hw_; }
+
+private:
+    struct Key {
+        int N = -1, B = 0, du = -1, degree = -1, bf16 = 0, grid = 0;
+        msk::Geom g{};
+        int64_t gens = 0;
+        const void* state[8] = {};
+        const void* mom[12] = {};
+        const void* own[7] = {};
+        const void* acc[7] = {};
+        const void* nanc = nullptr;
+        bool operator==(const Key& o) const;
+    };
+    void enqueue(const msk::Params& P, int B, int du, int Tmax, msk::BatchGrads A, uint32_t* nan_counts);
+    void release_graph();
+
+    hipStream_t s_;
+    std::vector<std::unique_ptr<Engine>> slots_;
+    std::vector<std::pair<int, int>> sizes_;
+    DBuf block_;                                     // [MsplatStepParams | ViewConsts x B | RasterView x B]
+    DBuf wka_, wva_, wkb_, wvb_, wtemp_, wmisc_;     // batch work queue
+    ConstRing ring_;
+    bool graph_on_ = false;
+    int persist_grid_ = 0;
+    msk::Geom hw_{};                                 // high-water launch geometry (graph mode)
+    hipGraphExec_t exec_ = nullptr;
+    Key key_;
+    int64_t replays_ = 0, captures_ = 0;
+};
+
+}  // namespace msplat_hip

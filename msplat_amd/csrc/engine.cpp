@@ -58,15 +58,60 @@ void sync_all_engines() {
     for (Engine* e : engines()) (void)e->sync();
 }
 
-Engine::Engine() {
-    hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+ConstRing::ConstRing() {}
+
+ConstRing::~ConstRing() {
+    for (int k = 0; k < kSlots; k++) {
+        if (armed_[k]) (void)hipEventSynchronize(ev_[k]);
+        if (ev_[k]) (void)hipEventDestroy(ev_[k]);
+    }
+    if (host_) (void)hipHostFree(host_);
+}
+
+void ConstRing::upload(hipStream_t s, void* dst, const void* src, size_t bytes) {
+    if (bytes > kSlotBytes) throw std::invalid_argument("constant block larger than a staging slot");
+    if (!host_) {  // on first use: engines that only run views bound by a batch never upload
+        hip_check(hipHostMalloc((void**)&host_, kSlots * kSlotBytes, hipHostMallocDefault), "hipHostMalloc");
+        for (int j = 0; j < kSlots; j++)
+            hip_check(hipEventCreateWithFlags(&ev_[j], hipEventDisableTiming), "hipEventCreate");
+    }
+    const int k = next_;
+    next_ = (next_ + 1) % kSlots;
+    if (armed_[k]) hip_check(hipEventSynchronize(ev_[k]), "hipEventSynchronize");
+    uint8_t* h = host_ + (size_t)k * kSlotBytes;
+    std::memcpy(h, src, bytes);
+    hip_check(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+    hip_check(hipEventRecord(ev_[k], s), "hipEventRecord");
+    armed_[k] = true;
+}
+
+int default_persist_grid() {
+    static int g = 0;
+    if (g == 0) {
+        int dev = 0, cus = 0;
+        if (hipGetDevice(&dev) != hipSuccess ||
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+            cus = 256;
+        g = cus * 8;
+    }
+    return g;
+}
+
+Engine::Engine(hipStream_t shared) {
+    if (shared) {
+        stream_ = shared;
+        owns_stream_ = false;
+    } else {
+        hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    persist_grid_ = default_persist_grid();
     hip_check(hipEventCreate(&mark_[0]), "hipEventCreate");
     hip_check(hipEventCreate(&mark_[1]), "hipEventCreate");
     nanc_.ensure(64);
     hip_check(hipMemset(nanc_.p, 0, 64), "hipMemset");
     totals_.ensure(64);
-    loss_.ensure(64);
-    hip_check(hipMemset(loss_.p, 0, 64), "hipMemset");
+    loss_.ensure(kLossSlots * 8);
+    hip_check(hipMemset(loss_.p, 0, kLossSlots * 8), "hipMemset");
     std::lock_guard<std::mutex> lk(g_engines_mu);
     engines().insert(this);
 }
@@ -77,9 +122,10 @@ void Engine::trim() {
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &keys_, &ids_, &hist_, &img_,
                    &finalT_, &finalIdx_, &vrend_, &hbuf_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
-                   &prefixT_, &afterC_};
+                   &prefixT_, &afterC_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_};
     for (DBuf* b : all) b->release();
     debug_valid_ = false;
+    gen_++;
 }
 
 Engine::~Engine() {
@@ -95,9 +141,9 @@ Engine::~Engine() {
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &keys_, &ids_, &hist_, &img_, &finalT_, &finalIdx_,
                    &vrend_, &hbuf_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
-                   &afterC_, &nanc_};
+                   &afterC_, &nanc_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_};
     for (DBuf* b : all) b->release();
-    if (stream_) hipStreamDestroy(stream_);
+    if (stream_ && owns_stream_) hipStreamDestroy(stream_);
 }
 
 // K_max rule of the depth-chunked raster path (reference msplat_metal.mm:1204-1224).
@@ -112,37 +158,115 @@ static uint32_t k_max_rule(int64_t capacity, int num_tiles) {
     return K;
 }
 
-void Engine::prepare(int N, int W, int H, int nb3, bool train) {
-    (void)nb3;
+void Engine::ensure(DBuf& b, size_t bytes) {
+    const size_t cap = b.cap;
+    b.ensure(bytes);
+    if (b.cap != cap) gen_++;
+}
+
+void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool train) {
     N_ = N;
-    W_ = W;
-    H_ = H;
-    tbx_ = (W + msk::TILE - 1) / msk::TILE;
-    tby_ = (H + msk::TILE - 1) / msk::TILE;
-    const int T = tbx_ * tby_;
-    const size_t n = (size_t)(N > 0 ? N : 1), P = (size_t)W * H;
+    launch_ = launch;
+    view_ = view;
+    W_ = view.W;
+    H_ = view.H;
+    tbx_ = view.tbx;
+    tby_ = view.tby;
+    const int T = launch.T;
+    const size_t n = (size_t)(N > 0 ? N : 1), P = (size_t)launch.W * launch.H;
     capacity_ = (int64_t)N * 48;  // 48 N slots (reference msplat_metal.mm:518, 1142)
     if (capacity_ < 64) capacity_ = 64;
-    K_ = k_max_rule(capacity_, T);
-    radii_.ensure(n * 4); tiles_.ensure(n * 4); depths_.ensure(n * 4); rect_.ensure(n * 8); aabb_.ensure(n * 8);
-    recA_.ensure((n + 1) * 16); recB_.ensure((n + 1) * 16); recC_.ensure((n + 1) * 8);  // row N: zero record
-    counts_.ensure((size_t)T * 4); exact_.ensure((size_t)T * 4); bins_.ensure((size_t)T * 8);
-    keys_.ensure((size_t)capacity_ * 4);
-    dkeys_.ensure(n * 4); dkeys_out_.ensure(n * 4); dvals_.ensure(n * 4); sorted_ids_.ensure(n * 4);
-    temp_.ensure(msk::lists_temp_bytes(N));
-    ids_.ensure((size_t)capacity_ * 4);
-    hist_.ensure(msk::lists_hist_bytes(N, T));
-    img_.ensure(P * 12); finalT_.ensure(P * 4); finalIdx_.ensure(P * 4);
+    K_ = k_max_rule(capacity_, view.T);
+    ensure(radii_, n * 4); ensure(tiles_, n * 4); ensure(depths_, n * 4); ensure(rect_, n * 8); ensure(aabb_, n * 8);
+    ensure(recA_, (n + 1) * 16); ensure(recB_, (n + 1) * 16); ensure(recC_, (n + 1) * 8);  // row N: zero record
+    ensure(counts_, (size_t)T * 4); ensure(exact_, (size_t)T * 4); ensure(bins_, (size_t)T * 8);
+    ensure(keys_, (size_t)capacity_ * 4);
+    ensure(dkeys_, n * 4); ensure(dkeys_out_, n * 4); ensure(dvals_, n * 4); ensure(sorted_ids_, n * 4);
+    ensure(temp_, msk::lists_temp_bytes(N));
+    ensure(ids_, (size_t)capacity_ * 4);
+    ensure(hist_, msk::lists_hist_bytes(N, T));
+    ensure(img_, P * 12); ensure(finalT_, P * 4); ensure(finalIdx_, P * 4);
+    ensure(wka_, (size_t)T * 4); ensure(wva_, (size_t)T * 4); ensure(wkb_, (size_t)T * 4); ensure(wvb_, (size_t)T * 4);
+    ensure(wtemp_, msk::work_temp_bytes(T));
+    ensure(wmisc_, 64);
+    ensure(consts_, sizeof(msk::ViewConsts) + sizeof(msk::RasterView) + 64);
     if (K_ > 1) {
-        chunkT_.ensure((size_t)K_ * P * 4); chunkC_.ensure((size_t)K_ * P * 12); chunkIdx_.ensure((size_t)K_ * P * 4);
-        if (train) { prefixT_.ensure((size_t)K_ * P * 4); afterC_.ensure((size_t)K_ * P * 12); }
+        const size_t Pv = (size_t)view.W * view.H;
+        ensure(chunkT_, (size_t)K_ * Pv * 4); ensure(chunkC_, (size_t)K_ * Pv * 12); ensure(chunkIdx_, (size_t)K_ * Pv * 4);
+        if (train) { ensure(prefixT_, (size_t)K_ * Pv * 4); ensure(afterC_, (size_t)K_ * Pv * 12); }
     }
     if (train) {
-        v_xy_.ensure(n * 8); v_conic_.ensure(n * 12); v_rgb_.ensure(n * 12); v_opac_.ensure(n * 4);
-        vrend_.ensure(P * 12); hbuf_.ensure(P * 60); dbuf_.ensure(P * 36);
-        const size_t nblk = (size_t)((W + 15) / 16) * ((H + 15) / 16);
-        partials_.ensure(nblk * 4);
+        ensure(v_xy_, n * 8); ensure(v_conic_, n * 12); ensure(v_rgb_, n * 12); ensure(v_opac_, n * 4);
+        ensure(vrend_, P * 12); ensure(hbuf_, P * 60); ensure(dbuf_, P * 36);
+        const size_t nblk = (size_t)((launch.W + 15) / 16) * ((launch.H + 15) / 16);
+        ensure(partials_, nblk * 4);
     }
+}
+
+msk::ViewConsts Engine::make_view(const MsplatCamera& cam, const uint8_t* gt, const float bg[3], float ssim_w,
+                                  float inv_n) {
+    msk::ViewConsts v{};
+    v.cam = cam;
+    const msk::Geom g = msk::geom_of(cam.width, cam.height);
+    v.W = g.W;
+    v.H = g.H;
+    v.tbx = g.tbx;
+    v.tby = g.tby;
+    v.T = g.T;
+    for (int c = 0; c < 3; c++) v.bg[c] = bg[c];
+    v.ssim_w = ssim_w;
+    v.inv_n = inv_n;
+    v.gt = gt;
+    return v;
+}
+
+void Engine::bind_view(const msk::ViewConsts& h, const msk::ViewConsts* d) {
+    vch_ = h;
+    vcd_ = d;
+}
+
+msk::RasterView Engine::raster_view(bool train) const {
+    msk::RasterView r{};
+    r.vc = vcd_;
+    r.bins = bins_.as<int2>();
+    r.pk = {ids_.as<uint32_t>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
+    r.img = img_.as<float>();
+    r.final_T = finalT_.as<float>();
+    r.final_idx = finalIdx_.as<int32_t>();
+    r.v_rend = train ? vrend_.as<float>() : nullptr;
+    r.G = train ? msk::GradBufs{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()}
+                : msk::GradBufs{nullptr, nullptr, nullptr, nullptr};
+    r.chunked = K_ > 1 ? 1 : 0;
+    return r;
+}
+
+// The engine's own constant block: [ViewConsts | RasterView of this view], one copy.
+void Engine::upload_view(const msk::ViewConsts& h, bool train) {
+    vch_ = h;
+    vcd_ = consts_.as<msk::ViewConsts>();
+    struct {
+        msk::ViewConsts v;
+        msk::RasterView r;
+    } blk;
+    blk.v = h;
+    blk.r = raster_view(train);
+    ring_.upload(stream_, consts_.p, &blk, sizeof(blk));
+}
+
+msk::RasterWork Engine::own_work() {
+    msk::RasterWork w{};
+    w.views = reinterpret_cast<const msk::RasterView*>(consts_.as<uint8_t>() + sizeof(msk::ViewConsts));
+    w.B = 1;
+    w.Tmax = launch_.T;
+    w.keys_a = wka_.as<uint32_t>();
+    w.vals_a = wva_.as<uint32_t>();
+    w.keys_b = wkb_.as<uint32_t>();
+    w.vals_b = wvb_.as<uint32_t>();
+    w.temp = wtemp_.p;
+    w.n_work = wmisc_.as<uint32_t>();
+    w.ctr = wmisc_.as<uint32_t>() + 1;
+    w.grid = persist_grid_;
+    return w;
 }
 
 void Engine::stage_event(int stage_end) {
@@ -176,10 +300,19 @@ void Engine::flush_events() {
     ev_used_ = 0;
 }
 
-void Engine::forward(const msk::Params& P, const MsplatCamera& cam, int degrees_to_use, const float bg[3]) {
+void Engine::front(const msk::Params& P, int degrees_to_use, bool train) {
+    const size_t n = (size_t)(P.N > 0 ? P.N : 1);
+    if (train) {
+        // accumulators of the raster backward (the reference blit-zeroes them, mm:1496-1512)
+        msk::fill_zero(stream_, v_xy_.p, n * 8);
+        msk::fill_zero(stream_, v_conic_.p, n * 12);
+        msk::fill_zero(stream_, v_rgb_.p, n * 12);
+        msk::fill_zero(stream_, v_opac_.p, n * 4);
+        stage_event(ST_ZERO);
+    }
     msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
                     aabb_.as<float2>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
-    msk::project_forward(stream_, P, cam, degrees_to_use, pj, nanc_.as<uint32_t>());
+    msk::project_forward(stream_, P, vcd_, degrees_to_use, pj, nanc_.as<uint32_t>());
     // the transparent record N
     msk::fill_zero(stream_, recA_.as<float4>() + P.N, 16);
     msk::fill_zero(stream_, recB_.as<float4>() + P.N, 16);
@@ -189,16 +322,36 @@ void Engine::forward(const msk::Params& P, const MsplatCamera& cam, int degrees_
     msk::ListBufs& L = lists_;
     msk::lists_rank(stream_, P.N, pj, L);
     stage_event(ST_RANK);
-    msk::lists_count(stream_, P.N, pj, tbx_, tby_, L);
+    msk::lists_count(stream_, P.N, pj, vcd_, launch_, L);
     stage_event(ST_COUNTS);
-    msk::lists_scatter(stream_, P.N, pj, tbx_, L);
+    msk::lists_scatter(stream_, P.N, pj, vcd_, launch_, L);
     stage_event(ST_SCATTER);
-    msk::lists_sort_pack(stream_, P.N, pj, tbx_, tby_, L);
+    msk::lists_sort_pack(stream_, P.N, vcd_, launch_, L);
     stage_event(ST_SORT);
+}
+
+void Engine::raster_fwd_own() {
     msk::RasterBufs R{img_.as<float>(), finalT_.as<float>(), finalIdx_.as<int32_t>(), chunkT_.as<float>(),
                       chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
-    msk::raster_forward(stream_, L, W_, H_, tbx_, tby_, bg, R);
+    const msk::RasterWork w = own_work();
+    if (K_ <= 1) msk::raster_work_build(stream_, w);
+    msk::raster_forward(stream_, lists_, vcd_, launch_, R, &w);
     stage_event(ST_RASTER_FWD);
+}
+
+void Engine::loss_to(int slot) {
+    msk::ssim_loss(stream_, vcd_, launch_, img_.as<float>(), hbuf_.as<float>(), dbuf_.as<float>(), vrend_.as<float>(),
+                   partials_.as<float>(), loss_.as<double>() + slot, nanc_.as<uint32_t>());
+    stage_event(ST_SSIM);
+}
+
+void Engine::raster_bwd_own() {
+    msk::RasterBufs R{img_.as<float>(), finalT_.as<float>(), finalIdx_.as<int32_t>(), chunkT_.as<float>(),
+                      chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
+    msk::GradBufs G{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
+    const msk::RasterWork w = own_work();
+    msk::raster_backward(stream_, lists_, R, vcd_, launch_, vrend_.as<float>(), G, &w);
+    stage_event(ST_RASTER_BWD);
 }
 
 msk::ListBufs Engine::list_bufs() {
@@ -222,7 +375,7 @@ msk::ListBufs Engine::list_bufs() {
     return L;
 }
 
-static msk::Params to_params(const MsplatDeviceState& st) {
+msk::Params params_of(const MsplatDeviceState& st) {
     msk::Params P;
     P.means = st.means; P.scales = st.scales; P.quats = st.quats; P.f_dc = st.f_dc; P.f_rest = st.f_rest;
     P.opac = st.opac;
@@ -235,37 +388,53 @@ static msk::Params to_params(const MsplatDeviceState& st) {
     return P;
 }
 
-static int nb3_of(int degree) { return degree <= 0 ? 0 : (degree == 1 ? 9 : degree == 2 ? 24 : 45); }
+
+int Engine::forward_backward(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt,
+                             const MsplatStepParams& hp, int slot) {
+    if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0 || !gt) return -1;
+    if (slot < 0 || slot >= kLossSlots) return -1;
+    if (st.sh_bf16 && (!st.f_dc_bf16 || (st.degree > 0 && !st.f_rest_bf16))) return -1;
+    if (cam.width > 65535 * 16 || cam.height > 65535 * 16) return -1;
+    const msk::Geom g = msk::geom_of(cam.width, cam.height);
+    if (!msk::lists_supported(st.N, g.T)) return -1;
+    const msk::Params P = params_of(st);
+    prepare(st.N, g, g, true);
+    last_W = cam.width;
+    last_H = cam.height;
+    stage_event(-1);
+    upload_view(make_view(cam, gt, hp.background, hp.ssim_weight, hp.loss_inv_n), true);
+    front(P, hp.degrees_to_use, true);
+    raster_fwd_own();
+    loss_to(slot);
+    raster_bwd_own();
+    hipError_t err = hipGetLastError();
+    return err == hipSuccess ? 0 : -2;
+}
+
+void Engine::accumulate(const MsplatDeviceState& st, int degrees_to_use, msk::BatchGrads& acc) {
+    const msk::Params P = params_of(st);
+    msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
+                    aabb_.as<float2>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
+    msk::GradBufs G{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
+    msk::view_accumulate(stream_, P, vcd_, degrees_to_use, pj, G, acc, nanc_.as<uint32_t>());
+    stage_event(ST_UPDATE);
+}
+
+double Engine::loss_slot(int k, int W, int H) {
+    if (k < 0 || k >= kLossSlots) return 0.0;
+    double l = 0.0;
+    hipMemcpyAsync(&l, loss_.as<double>() + k, 8, hipMemcpyDeviceToHost, stream_);
+    hipStreamSynchronize(stream_);
+    return (W > 0 && H > 0) ? l / ((double)W * H) : 0.0;
+}
 
 int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt,
                        const MsplatStepParams& hp, bool debug) {
-    if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0 || !gt) return -1;
-    if (st.sh_bf16 && (!st.f_dc_bf16 || (st.degree > 0 && !st.f_rest_bf16))) return -1;
-    if (cam.width > 65535 * 16 || cam.height > 65535 * 16) return -1;
-    if (!msk::lists_supported(st.N, ((cam.width + 15) / 16) * ((cam.height + 15) / 16))) return -1;
-    const msk::Params P = to_params(st);
-    prepare(st.N, cam.width, cam.height, nb3_of(st.degree), true);
-    last_W = cam.width;
-    last_H = cam.height;
+    const int rc = forward_backward(st, cam, gt, hp, 0);
+    if (rc != 0) return rc;
+    const msk::Params P = params_of(st);
     const size_t n = (size_t)(st.N > 0 ? st.N : 1);
-    stage_event(-1);
-    // accumulators of the raster backward (the reference blit-zeroes them, mm:1496-1512)
-    msk::fill_zero(stream_, v_xy_.p, n * 8);
-    msk::fill_zero(stream_, v_conic_.p, n * 12);
-    msk::fill_zero(stream_, v_rgb_.p, n * 12);
-    msk::fill_zero(stream_, v_opac_.p, n * 4);
-    stage_event(ST_ZERO);
-    forward(P, cam, hp.degrees_to_use, hp.background);
-    msk::ssim_loss(stream_, W_, H_, img_.as<float>(), gt, hp.ssim_weight, hp.loss_inv_n, hbuf_.as<float>(),
-                   dbuf_.as<float>(), vrend_.as<float>(), partials_.as<float>(), loss_.as<double>(),
-                   nanc_.as<uint32_t>());
-    stage_event(ST_SSIM);
-    msk::RasterBufs R{img_.as<float>(), finalT_.as<float>(), finalIdx_.as<int32_t>(), chunkT_.as<float>(),
-                      chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
-    msk::ListBufs& L = lists_;
     msk::GradBufs G{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
-    msk::raster_backward(stream_, L, R, W_, H_, tbx_, tby_, hp.background, vrend_.as<float>(), G);
-    stage_event(ST_RASTER_BWD);
     msk::sh_rest_adam(stream_, P, cam, radii_.as<int32_t>(), v_rgb_.as<float>(), hp);
     stage_event(ST_SH_ADAM);
     msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
@@ -285,13 +454,17 @@ int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, con
 int Engine::render(const MsplatDeviceState& st, const MsplatCamera& cam, int degrees_to_use, const float bg[3]) {
     if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0) return -1;
     if (st.sh_bf16 && (!st.f_dc_bf16 || (st.degree > 0 && !st.f_rest_bf16))) return -1;
-    if (!msk::lists_supported(st.N, ((cam.width + 15) / 16) * ((cam.height + 15) / 16))) return -1;
-    const msk::Params P = to_params(st);
-    prepare(st.N, cam.width, cam.height, nb3_of(st.degree), false);
+    if (cam.width > 65535 * 16 || cam.height > 65535 * 16) return -1;
+    const msk::Geom g = msk::geom_of(cam.width, cam.height);
+    if (!msk::lists_supported(st.N, g.T)) return -1;
+    const msk::Params P = params_of(st);
+    prepare(st.N, g, g, false);
     last_W = cam.width;
     last_H = cam.height;
     stage_event(-1);
-    forward(P, cam, degrees_to_use, bg);
+    upload_view(make_view(cam, nullptr, bg, 0.f, 0.f), false);
+    front(P, degrees_to_use, false);
+    raster_fwd_own();
     hipError_t err = hipGetLastError();
     return err == hipSuccess ? 0 : -2;
 }

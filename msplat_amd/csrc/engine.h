@@ -24,12 +24,42 @@ enum Stage {
     ST_SH_ADAM, ST_UPDATE, ST_COUNT
 };
 
+// Pinned staging ring for the small per-step constants (view constants, step scalars):
+// slot k is refilled only after the event recorded behind its previous copy has completed,
+// so the host never overwrites bytes a copy has not read yet.
+class ConstRing {
+public:
+    ConstRing();
+    ~ConstRing();
+    void upload(hipStream_t s, void* dst, const void* src, size_t bytes);
+    static constexpr int kSlots = 64;
+    static constexpr size_t kSlotBytes = 8192;
+
+private:
+    uint8_t* host_ = nullptr;
+    hipEvent_t ev_[kSlots] = {};
+    bool armed_[kSlots] = {};
+    int next_ = 0;
+};
+
 class Engine {
 public:
-    Engine();
+    // shared: run on this stream (owned by the caller) instead of a stream of its own.
+    explicit Engine(hipStream_t shared = nullptr);
     ~Engine();
     int train_step(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt, const MsplatStepParams& hp,
                    bool debug);
+    // The view-dependent part of a step: gradient accumulators cleared, forward, loss (into
+    // loss slot `slot`), raster backward. Leaves the projection, records and raster
+    // gradients of this view in the engine's buffers.
+    int forward_backward(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt,
+                         const MsplatStepParams& hp, int slot);
+    // Adds this view's projection VJP and SH colour gradients (after its backward).
+    void accumulate(const MsplatDeviceState& st, int degrees_to_use, msk::BatchGrads& acc);
+    // Loss slot k as the mean over its view's pixels (waits for the stream).
+    double loss_slot(int k, int W, int H);
+    static constexpr int kLossSlots = 64;
+    uint32_t* nan_counts() { return nanc_.as<uint32_t>(); }
     int render(const MsplatDeviceState& st, const MsplatCamera& cam, int degrees_to_use, const float bg[3]);
     int buffer(const std::string& name, void** p, size_t* bytes);
     int sync();
@@ -46,17 +76,47 @@ public:
     void trim();
     int last_W = 0, last_H = 0;
 
+    // ---- one view of a batched step (the batch runner drives these per view) ----
+    // Buffers for N Gaussians and launch geometry `launch`; `view` is this view's own.
+    void prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool train);
+    static msk::ViewConsts make_view(const MsplatCamera& cam, const uint8_t* gt, const float bg[3], float ssim_w,
+                                     float inv_n);
+    // The view's constants live at `d` (already on the device, written by the caller).
+    void bind_view(const msk::ViewConsts& h, const msk::ViewConsts* d);
+    // Gradient accumulators cleared (train), projection, tile lists.
+    void front(const msk::Params& P, int degrees_to_use, bool train);
+    void loss_to(int slot);
+    bool chunked() const { return K_ > 1; }
+    int chunks() const { return (int)K_; }
+    // Raster forward / backward of this view alone (chunked views always go this way).
+    void raster_fwd_own();
+    void raster_bwd_own();
+    msk::RasterView raster_view(bool train) const;
+    int64_t generation() const { return gen_; }
+    void set_persist_grid(int g) { persist_grid_ = g; }
+    int persist_grid() const { return persist_grid_; }
+
 private:
-    void prepare(int N, int W, int H, int nb3, bool train);
-    void forward(const msk::Params& P, const MsplatCamera& cam, int degrees_to_use, const float bg[3]);
+    void upload_view(const msk::ViewConsts& h, bool train);
     msk::ListBufs list_bufs();
+    msk::RasterWork own_work();
     void stage_event(int stage_end);
     void flush_events();
+    void ensure(DBuf& b, size_t bytes);
 
     hipStream_t stream_ = nullptr;
+    bool owns_stream_ = true;
     int N_ = 0, W_ = 0, H_ = 0, tbx_ = 0, tby_ = 0;
+    msk::Geom launch_{}, view_{};
     int64_t capacity_ = 0;
     uint32_t K_ = 1;
+    int64_t gen_ = 0;   // bumps whenever a buffer is re-allocated
+    int persist_grid_ = 0;
+    // view constants: host copy, device copy in use, own device block [ViewConsts | RasterView]
+    msk::ViewConsts vch_{};
+    const msk::ViewConsts* vcd_ = nullptr;
+    DBuf consts_;
+    ConstRing ring_;
     // per-Gaussian
     DBuf radii_, tiles_, depths_, rect_, aabb_, recA_, recB_, recC_;
     DBuf v_xy_, v_conic_, v_rgb_, v_opac_;
@@ -64,6 +124,7 @@ private:
     // per-tile
     DBuf counts_, exact_, bins_, totals_;
     DBuf dkeys_, dkeys_out_, dvals_, sorted_ids_, temp_;
+    DBuf wka_, wva_, wkb_, wvb_, wtemp_, wmisc_;   // own persistent-raster work queue
     // per-intersection
     DBuf keys_, ids_, hist_;
     // per-pixel
@@ -82,6 +143,10 @@ private:
     double stage_ms_[ST_COUNT] = {0};
     int64_t stage_n_[ST_COUNT] = {0};
 };
+
+// Resident workgroups of a persistent raster launch by default: 8 per CU.
+int default_persist_grid();
+msk::Params params_of(const MsplatDeviceState& st);
 
 // Every live engine (msplat_sync / msplat_cleanup act on all of them).
 void trim_all_engines();

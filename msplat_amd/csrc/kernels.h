@@ -54,6 +54,24 @@ struct ListBufs {
     size_t temp_bytes;
 };
 
+// Constants of one view of a step, read by the kernels from device memory, so that a
+// recorded launch sequence (a captured graph) replays for any camera and target whose
+// extents fit its launch geometry.
+struct ViewConsts {
+    MsplatCamera cam;
+    int W, H, tbx, tby, T;
+    float bg[3];
+    float ssim_w, inv_n;
+    const uint8_t* gt;   // [H W 3] target, uint8
+};
+
+// Launch geometry: a view's own extents (eager launches) or the largest extents of the
+// views a recorded sequence serves (workgroups past a view's extents exit at once).
+struct Geom {
+    int W, H, tbx, tby, T;
+};
+Geom geom_of(int W, int H);
+
 struct RasterBufs {
     float* img;       // [H W 3]
     float* final_T;   // [H W]
@@ -80,26 +98,80 @@ struct Params {
     uint16_t* f_rest_h = nullptr;
 };
 
-void project_forward(hipStream_t s, const Params& P, const MsplatCamera& cam, int degrees_to_use, ProjOut& o,
+// One view of the persistent raster kernels: its lists, records, outputs and gradients.
+struct RasterView {
+    const ViewConsts* vc;
+    const int2* bins;
+    EntryView pk;
+    float* img;
+    float* final_T;
+    int32_t* final_idx;
+    const float* v_rend;
+    GradBufs G;
+    int chunked;   // rasterised by the chunked kernels, not through the work queue
+};
+
+// Work queue of the persistent raster kernels over B views of at most Tmax tiles each.
+struct RasterWork {
+    const RasterView* views;   // device [B]
+    int B, Tmax;
+    uint32_t *keys_a, *vals_a, *keys_b, *vals_b;   // [B Tmax] each
+    void* temp;                // work_temp_bytes(B Tmax)
+    uint32_t* n_work;          // [1]
+    uint32_t* ctr;             // [2] forward / backward queue heads
+    int grid;                  // resident workgroups
+};
+
+void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, ProjOut& o,
                      uint32_t* nan_counts);
 size_t lists_temp_bytes(int N);
 size_t lists_hist_bytes(int N, int T);
 bool lists_supported(int N, int T);
 void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L);
-void lists_count(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L);
-void lists_scatter(hipStream_t s, int N, const ProjOut& pj, int tbx, ListBufs& L);
-void lists_sort_pack(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L);
-void raster_forward(hipStream_t s, const ListBufs& L, int W, int H, int tbx, int tby, const float bg[3], RasterBufs& R);
-void ssim_loss(hipStream_t s, int W, int H, const float* img, const uint8_t* gt, float ssim_w, float inv_n,
-               float* hbuf /*[H W 15]*/, float* dbuf /*[H W 9]*/, float* v_rend, float* loss_partials,
-               double* loss_out, uint32_t* nan_counts);
-void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, int W, int H, int tbx, int tby,
-                     const float bg[3], const float* v_rend, GradBufs& G);
+void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);
+void lists_scatter(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);
+void lists_sort_pack(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L);
+// Stable LSD radix sort of n (key, value) pairs over the low 8 * passes key bits; with an
+// even number of passes the result is back in (ka, va). temp: lists_temp_bytes(n).
+void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, void* temp,
+                      int passes);
+size_t work_temp_bytes(int items);
+// Fills the persistent raster work queue (ordered by decreasing tile list length) and
+// resets its heads; the views' lists must be built.
+void raster_work_build(hipStream_t s, const RasterWork& w);
+// Monolithic views (K <= 1) rasterise through the persistent kernel over `work`; chunked
+// views through the per-(tile, chunk) kernels.
+void raster_forward(hipStream_t s, const ListBufs& L, const ViewConsts* vc, const Geom& g, RasterBufs& R,
+                    const RasterWork* work);
+void raster_forward_views(hipStream_t s, const RasterWork& w);
+void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& g, const float* img, float* hbuf /*[H W 15]*/,
+               float* dbuf /*[H W 9]*/, float* v_rend, float* loss_partials, double* loss_out, uint32_t* nan_counts);
+void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, const ViewConsts* vc, const Geom& g,
+                     const float* v_rend, GradBufs& G, const RasterWork* work);
+void raster_backward_views(hipStream_t s, const RasterWork& w);
 void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const int32_t* radii, const float* v_rgb,
                   const MsplatStepParams& hp);
 void gaussian_update(hipStream_t s, const Params& P, const MsplatCamera& cam, const ProjOut& pj, const GradBufs& G,
                      const MsplatStepParams& hp, uint32_t* nan_counts, float* dbg_v_mean, float* dbg_v_scale,
                      float* dbg_v_quat, float* dbg_v_opac);
+// Gradients of a multi-view batched step, summed over its views (cap-sized, row-major).
+struct BatchGrads {
+    float* v_mean;    // [N 3]
+    float* v_scale;   // [N 3]
+    float* v_quat;    // [N 4]
+    float* v_opac;    // [N]
+    float* v_dc;      // [N 3]
+    float* v_rest;    // [N nb3]
+    uint32_t* vis;    // [N] 1 = a view gave a finite colour gradient
+};
+// One view's projection VJP and SH colour gradients added into A (views run in order on one
+// stream, so the sums need no atomics).
+void view_accumulate(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, const ProjOut& pj,
+                     const GradBufs& G, BatchGrads& A, uint32_t* nan_counts);
+// The batch's single update: SH Adam for Gaussians with vis, regulariser, Adam on the other
+// groups, SGLD.
+void batch_update(hipStream_t s, const Params& P, const BatchGrads& A, const MsplatStepParams* hpd, int degrees_to_use,
+                  uint32_t* nan_counts);
 void fill_zero(hipStream_t s, void* p, size_t bytes);
 void copy16(hipStream_t s, const void* src, void* dst, size_t bytes);
 // bf16 storage helpers: shadows of f_dc / f_rest from the fp32 masters (rows [0, N)), and

@@ -10,7 +10,7 @@
 // same formulas bit for bit.
 //
 // Reference kernels are cited as metal:<line> (core/metal/msplat_metal.metal).
-#include <hipcub/hipcub.hpp>
+#include <utility>
 
 #include "kernels.h"
 #include "ms_math.h"
@@ -165,7 +165,7 @@ template <typename SHT>
 __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
     int N, const float* __restrict__ means, const float* __restrict__ scales, const float* __restrict__ quats,
     const SHT* __restrict__ f_dc, const SHT* __restrict__ f_rest, const float* __restrict__ opac, int nb3,
-    int deg_use, MsplatCamera cam, int tbx, int tby, int32_t* __restrict__ radii_o, int32_t* __restrict__ tiles_o,
+    int deg_use, const ViewConsts* __restrict__ vc, int32_t* __restrict__ radii_o, int32_t* __restrict__ tiles_o,
     float* __restrict__ depth_o, ushort4* __restrict__ rect_o, float2* __restrict__ aabb_o, float4* __restrict__ rA,
     float4* __restrict__ rB, float2* __restrict__ rC, uint32_t* __restrict__ nanc) {
     extern __shared__ uint4 s_raw[];  // PROJ_BLOCK * nb3 coefficients when deg_use > 0
@@ -187,6 +187,8 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
         __syncthreads();
     }
     if (i >= N) return;
+    const MsplatCamera cam = vc->cam;
+    const int tbx = vc->tbx, tby = vc->tby;
 
     int32_t rad = 0, area = 0;
     float depth = 0.f;
@@ -330,22 +332,31 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
     rC[i] = C;
 }
 
-void project_forward(hipStream_t s, const Params& P, const MsplatCamera& cam, int degrees_to_use, ProjOut& o,
+void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, ProjOut& o,
                      uint32_t* nan_counts) {
     if (P.N <= 0) return;
     const int nb3 = (P.degree <= 0 ? 0 : (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15)) * 3;
-    const int tbx = (cam.width + TILE - 1) / TILE, tby = (cam.height + TILE - 1) / TILE;
     const int du = degrees_to_use < P.degree ? degrees_to_use : P.degree;
     const size_t esz = P.sh_bf16 ? 2 : 4;
     size_t lds = (du > 0 && nb3 > 0) ? (size_t)PROJ_BLOCK * nb3 * esz : 0;
     if (P.sh_bf16)
         hipLaunchKernelGGL(k_project<uint16_t>, dim3(cdiv(P.N, PROJ_BLOCK)), dim3(PROJ_BLOCK), lds, s, P.N, P.means,
-                           P.scales, P.quats, P.f_dc_h, P.f_rest_h, P.opac, nb3, du, cam, tbx, tby, o.radii,
-                           o.tiles_hit, o.depths, o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
+                           P.scales, P.quats, P.f_dc_h, P.f_rest_h, P.opac, nb3, du, vc, o.radii, o.tiles_hit,
+                           o.depths, o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
     else
         hipLaunchKernelGGL(k_project<float>, dim3(cdiv(P.N, PROJ_BLOCK)), dim3(PROJ_BLOCK), lds, s, P.N, P.means,
-                           P.scales, P.quats, P.f_dc, P.f_rest, P.opac, nb3, du, cam, tbx, tby, o.radii, o.tiles_hit,
-                           o.depths, o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
+                           P.scales, P.quats, P.f_dc, P.f_rest, P.opac, nb3, du, vc, o.radii, o.tiles_hit, o.depths,
+                           o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
+}
+
+Geom geom_of(int W, int H) {
+    Geom g;
+    g.W = W;
+    g.H = H;
+    g.tbx = (W + TILE - 1) / TILE;
+    g.tby = (H + TILE - 1) / TILE;
+    g.T = g.tbx * g.tby;
+    return g;
 }
 
 // ------------------------------------------------------------------------------------
@@ -385,9 +396,11 @@ constexpr int MAX_CHUNK = 1984;  // Gaussians per chunk: every window fits 2048 
 
 __global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const uint32_t* __restrict__ sorted_ids,
                                                             const int32_t* __restrict__ radii,
-                                                            const ushort4* __restrict__ rect, int tbx, int T,
+                                                            const ushort4* __restrict__ rect,
+                                                            const ViewConsts* __restrict__ vc,
                                                             uint32_t* __restrict__ mat) {
     extern __shared__ uint32_t hist[];
+    const int tbx = vc->tbx, T = vc->T;
     for (int t = threadIdx.x; t < T; t += HIST_THREADS) hist[t] = 0u;
     __syncthreads();
     const int r0 = blockIdx.x * G, r1 = min(N, r0 + G);
@@ -404,7 +417,9 @@ __global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const 
 }
 
 // Exclusive scan of each tile's column over the chunks; the column total is the tile count.
-__global__ void k_col_scan(int T, int nblk, uint32_t* __restrict__ mat, uint32_t* __restrict__ counts) {
+__global__ void k_col_scan(const ViewConsts* __restrict__ vc, int nblk, uint32_t* __restrict__ mat,
+                           uint32_t* __restrict__ counts) {
+    const int T = vc->T;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     uint32_t run = 0;
@@ -428,10 +443,11 @@ __global__ void k_col_scan(int T, int nblk, uint32_t* __restrict__ mat, uint32_t
     counts[t] = run;
 }
 
-__global__ __launch_bounds__(1024) void k_prefix(int T, const uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ vc, const uint32_t* __restrict__ counts,
                                                  uint32_t* __restrict__ exact_off, int2* __restrict__ bins,
                                                  uint32_t* __restrict__ totals, uint32_t cap) {
     __shared__ uint32_t lw[17];
+    const int T = vc->T;
     uint32_t run_ex = 0, run_pd = 0, mx = 0;
     for (int c0 = 0; c0 < T; c0 += 1024) {
         const int t = c0 + threadIdx.x;
@@ -464,11 +480,13 @@ __global__ __launch_bounds__(1024) void k_prefix(int T, const uint32_t* __restri
 
 __global__ __launch_bounds__(HIST_THREADS) void k_tile_scatter(int N, int G, const uint32_t* __restrict__ sorted_ids,
                                                                const int32_t* __restrict__ radii,
-                                                               const ushort4* __restrict__ rect, int tbx, int T,
+                                                               const ushort4* __restrict__ rect,
+                                                               const ViewConsts* __restrict__ vc,
                                                                const uint32_t* __restrict__ mat,
                                                                const uint32_t* __restrict__ exact_off,
                                                                uint32_t* __restrict__ keys, uint32_t cap) {
     extern __shared__ uint32_t cur[];
+    const int tbx = vc->tbx, T = vc->T;
     const uint32_t* row = mat + (size_t)blockIdx.x * T;
     for (int t = threadIdx.x; t < T; t += HIST_THREADS) cur[t] = exact_off[t] + row[t];
     __syncthreads();
@@ -538,7 +556,8 @@ constexpr int SORT_THREADS = 256;
 constexpr int WIN = 64;
 constexpr int WAVE_LDS_KEYS = 2048;
 
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(int T, int nblk, const uint32_t* __restrict__ counts,
+__global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(const ViewConsts* __restrict__ vc, int nblk,
+                                                            const uint32_t* __restrict__ counts,
                                                             const uint32_t* __restrict__ exact_off,
                                                             const uint32_t* __restrict__ mat,
                                                             const uint32_t* __restrict__ keys,
@@ -547,7 +566,9 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(int T, int nblk, con
     extern __shared__ uint32_t sm[];
     uint32_t* off = sm;                          // [nblk + 1] segment starts of this tile
     uint32_t* wl = sm + ((nblk + 4) & ~3);       // [4][WAVE_LDS_KEYS]
+    const int T = vc->T;
     const int t = blockIdx.x;
+    if (t >= T) return;
     const uint32_t count = counts[t];
     if (count == 0) return;
     const uint32_t base = exact_off[t];
@@ -612,12 +633,87 @@ static int hist_blocks(int N) {
     return nb < 1 ? 1 : nb;
 }
 
-size_t lists_temp_bytes(int N) {
-    size_t bytes = 0;
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, (uint32_t*)nullptr, (uint32_t*)nullptr,
-                                             (uint32_t*)nullptr, (uint32_t*)nullptr, N > 0 ? N : 1, 0, 32);
-    return bytes;
+// Stable LSD radix sort of (key, value) u32 pairs, 8-bit digits, 4 passes: per-workgroup LDS
+// digit histograms, one scan of the digit-major (digit, workgroup) count matrix, and a
+// scatter that ranks each item among equal digits of earlier items with wave ballots. Only
+// kernels: no memset or host synchronisation, so a step that uses it can be captured.
+constexpr int RS_THREADS = 256, RS_ITEMS = 16, RS_TILE = RS_THREADS * RS_ITEMS;
+
+static int rs_blocks(int N) { return (N + RS_TILE - 1) / RS_TILE; }
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_hist(int N, const uint32_t* __restrict__ keys, int shift,
+                                                        uint32_t* __restrict__ counts, int nblk) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const int base = blockIdx.x * RS_TILE + threadIdx.x;
+#pragma unroll 4
+    for (int k = 0; k < RS_ITEMS; k++) {
+        const int i = base + k * RS_THREADS;
+        if (i < N) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    counts[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
+
+// In-place exclusive scan of n counts by one workgroup of 1024.
+__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ counts, int n) {
+    __shared__ uint32_t lw[17];
+    const int per = (n + 1023) / 1024;
+    const int b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+    uint32_t sum = 0;
+    for (int i = b0; i < b1; i++) sum += counts[i];
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(sum, lw, &tot);
+    for (int i = b0; i < b1; i++) {
+        const uint32_t v = counts[i];
+        counts[i] = run;
+        run += v;
+    }
+}
+
+__global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t* __restrict__ kin,
+                                                           const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+                                                           uint32_t* __restrict__ vout, int shift,
+                                                           const uint32_t* __restrict__ counts, int nblk) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    run[threadIdx.x] = counts[(size_t)threadIdx.x * nblk + blockIdx.x];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int r = 0; r < RS_ITEMS; r++) {
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][threadIdx.x] = 0u;
+        __syncthreads();
+        const int i = blockIdx.x * RS_TILE + r * RS_THREADS + threadIdx.x;
+        const bool valid = i < N;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t d = (key >> shift) & 255u;
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint64_t m = __ballot((d >> b) & 1u);
+            peers &= ((d >> b) & 1u) ? m : ~m;
+        }
+        const uint32_t rank = (uint32_t)__builtin_popcountll(peers & lt);
+        if (valid && rank == 0u) wcnt[wid][d] = (uint32_t)__builtin_popcountll(peers);
+        __syncthreads();
+        if (valid) {
+            uint32_t pos = run[d] + rank;
+            for (int w = 0; w < wid; w++) pos += wcnt[w][d];
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+        uint32_t add = 0u;
+#pragma unroll
+        for (int w = 0; w < RS_THREADS / 64; w++) add += wcnt[w][threadIdx.x];
+        run[threadIdx.x] += add;
+        __syncthreads();
+    }
+}
+
+size_t lists_temp_bytes(int N) { return ((size_t)256 * (size_t)rs_blocks(N > 0 ? N : 1) + 64) * 4; }
 
 size_t lists_hist_bytes(int N, int T) { return (size_t)hist_blocks(N) * (size_t)T * 4; }
 
@@ -638,47 +734,59 @@ bool lists_supported(int N, int T) {
     return (size_t)T * 4 <= lds_limit() && sort_lds <= lds_limit();
 }
 
-void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L) {
-    if (N <= 0) return;
-    hipLaunchKernelGGL(k_depth_keys, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.depths, L.dkeys, L.dvals);
-    size_t bytes = L.temp_bytes;
-    (void)hipcub::DeviceRadixSort::SortPairs(L.temp, bytes, L.dkeys, L.dkeys_out, L.dvals, L.sorted_ids, N, 0, 32, s);
+void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, void* temp,
+                      int passes) {
+    if (n <= 0) return;
+    const int nblk = rs_blocks(n);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(temp);
+    for (int pass = 0; pass < passes; pass++) {
+        const int shift = 8 * pass;
+        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, s, n, ka, shift, counts, nblk);
+        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, s, counts, 256 * nblk);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, s, n, ka, va, kb, vb, shift, counts, nblk);
+        std::swap(ka, kb);
+        std::swap(va, vb);
+    }
 }
 
-void lists_count(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L) {
-    const int T = tbx * tby;
+void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L) {
+    if (N <= 0) return;
+    // keys start in (dkeys_out, sorted_ids); four passes ping-pong and end there
+    hipLaunchKernelGGL(k_depth_keys, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.depths, L.dkeys_out,
+                       L.sorted_ids);
+    radix_sort_pairs(s, N, L.dkeys_out, L.sorted_ids, L.dkeys, L.dvals, L.temp, 4);
+}
+
+void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N > 0) {
         const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
-        const size_t lds = (size_t)T * 4;
+        const size_t lds = (size_t)g.T * 4;
         if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_hist, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, pj.radii, pj.rect,
-                           tbx, T, L.hist);
-        hipLaunchKernelGGL(k_col_scan, dim3(cdiv(T, 64)), dim3(64), 0, s, T, nblk, L.hist, L.tile_counts);
+                           vc, L.hist);
+        hipLaunchKernelGGL(k_col_scan, dim3(cdiv(g.T, 64)), dim3(64), 0, s, vc, nblk, L.hist, L.tile_counts);
     } else {
-        fill_zero(s, L.tile_counts, (size_t)T * 4);
+        fill_zero(s, L.tile_counts, (size_t)g.T * 4);
     }
-    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, T, L.tile_counts, L.exact_off, L.bins, L.totals,
+    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, vc, L.tile_counts, L.exact_off, L.bins, L.totals,
                        (uint32_t)L.capacity);
 }
 
-void lists_scatter(hipStream_t s, int N, const ProjOut& pj, int tbx, ListBufs& L) {
+void lists_scatter(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N <= 0) return;
-    const int T = L.num_tiles;
     const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
-    const size_t lds = (size_t)T * 4;
+    const size_t lds = (size_t)g.T * 4;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_tile_scatter, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, pj.radii, pj.rect, tbx,
-                       T, L.hist, L.exact_off, L.keys, (uint32_t)L.capacity);
+    hipLaunchKernelGGL(k_tile_scatter, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, pj.radii, pj.rect, vc,
+                       L.hist, L.exact_off, L.keys, (uint32_t)L.capacity);
 }
 
-void lists_sort_pack(hipStream_t s, int N, const ProjOut& pj, int tbx, int tby, ListBufs& L) {
-    (void)pj;
+void lists_sort_pack(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N <= 0) return;
-    const int T = tbx * tby;
     const int nblk = hist_blocks(N);
     const size_t lds = ((size_t)(nblk + 4) & ~(size_t)3) * 4 + 4 * WAVE_LDS_KEYS * 4;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_sort_pack, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_sort_pack, dim3(T), dim3(SORT_THREADS), lds, s, T, nblk, L.tile_counts, L.exact_off, L.hist,
+    hipLaunchKernelGGL(k_sort_pack, dim3(g.T), dim3(SORT_THREADS), lds, s, vc, nblk, L.tile_counts, L.exact_off, L.hist,
                        L.keys, L.sorted_ids, L.ids, N, (uint32_t)L.capacity);
 }
 
@@ -738,31 +846,82 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
     }
 }
 
-__global__ __launch_bounds__(RB) void k_raster_fwd(int W, int H, int tbx, const int2* __restrict__ bins, EntryView pk,
-                                                   float bg0, float bg1, float bg2, float* __restrict__ img,
-                                                   float* __restrict__ final_T, int32_t* __restrict__ final_idx) {
-    const int t = blockIdx.x;
-    const int tx = t % tbx, ty = t / tbx;
-    const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
-    const bool inside = i < H && j < W;
-    const int2 r = bins[t];
-    float T, C[3];
-    int last;
-    composite_range(r.x, r.y, (float)j, (float)i, inside, pk, T, C, last);
-    if (inside) {
-        const size_t pid = (size_t)i * W + j;
-        final_T[pid] = T;
-        final_idx[pid] = last;
-        img[3 * pid + 0] = fminf(fmaxf(fmaf(bg0, T, C[0]), 0.f), 1.f);
-        img[3 * pid + 1] = fminf(fmaxf(fmaf(bg1, T, C[1]), 0.f), 1.f);
-        img[3 * pid + 2] = fminf(fmaxf(fmaf(bg2, T, C[2]), 0.f), 1.f);
+// Work items of the persistent raster kernels: (view << 20) | tile, ordered by decreasing
+// tile list length so that the longest tiles start first and the launch ends on short ones.
+constexpr int WORK_TILE_BITS = 20;
+
+__global__ void k_work_keys(const RasterView* __restrict__ views, int B, int Tmax, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ vals, uint32_t* __restrict__ n_work, uint32_t* __restrict__ ctr) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        uint32_t n = 0;
+        for (int v = 0; v < B; v++) n += views[v].chunked ? 0u : (uint32_t)views[v].vc->T;
+        *n_work = n;
+        ctr[0] = 0u;
+        ctr[1] = 0u;
+    }
+    if (k >= B * Tmax) return;
+    const int v = k / Tmax, t = k % Tmax;
+    const RasterView& rv = views[v];
+    if (rv.chunked || t >= rv.vc->T) {
+        keys[k] = 0xFFFFu;
+        vals[k] = 0xFFFFFFFFu;
+        return;
+    }
+    const int2 r = rv.bins[t];
+    const uint32_t len = (uint32_t)max(0, r.y - r.x);
+    keys[k] = 0xFFFEu - min(len, 0xFFFEu);
+    vals[k] = ((uint32_t)v << WORK_TILE_BITS) | (uint32_t)t;
+}
+
+// Next work item of the calling workgroup, or >= n_work when the queue is drained.
+__device__ __forceinline__ uint32_t next_item(uint32_t* ctr) {
+    __shared__ uint32_t s_item;
+    if (threadIdx.x == 0) s_item = atomicAdd(ctr, 1u);
+    __syncthreads();
+    const uint32_t it = s_item;
+    __syncthreads();
+    return it;
+}
+
+__global__ __launch_bounds__(RB) void k_raster_fwd_persist(const RasterView* __restrict__ views,
+                                                           const uint32_t* __restrict__ work,
+                                                           const uint32_t* __restrict__ n_work,
+                                                           uint32_t* __restrict__ ctr) {
+    const uint32_t total = *n_work;
+    for (;;) {
+        const uint32_t it = next_item(ctr);
+        if (it >= total) break;
+        const uint32_t code = work[it];
+        const RasterView& rv = views[code >> WORK_TILE_BITS];
+        const ViewConsts& vc = *rv.vc;
+        const int t = (int)(code & ((1u << WORK_TILE_BITS) - 1u));
+        const int W = vc.W, H = vc.H, tbx = vc.tbx;
+        const int tx = t % tbx, ty = t / tbx;
+        const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
+        const bool inside = i < H && j < W;
+        const int2 r = rv.bins[t];
+        float T, C[3];
+        int last;
+        composite_range(r.x, r.y, (float)j, (float)i, inside, rv.pk, T, C, last);
+        if (inside) {
+            const size_t pid = (size_t)i * W + j;
+            rv.final_T[pid] = T;
+            rv.final_idx[pid] = last;
+            rv.img[3 * pid + 0] = fminf(fmaxf(fmaf(vc.bg[0], T, C[0]), 0.f), 1.f);
+            rv.img[3 * pid + 1] = fminf(fmaxf(fmaf(vc.bg[1], T, C[1]), 0.f), 1.f);
+            rv.img[3 * pid + 2] = fminf(fmaxf(fmaf(vc.bg[2], T, C[2]), 0.f), 1.f);
+        }
     }
 }
 
-__global__ __launch_bounds__(RB) void k_raster_fwd_chunk(int W, int H, int tbx, const int2* __restrict__ bins,
-                                                         EntryView pk, float* __restrict__ chunk_T,
-                                                         float* __restrict__ chunk_C, int32_t* __restrict__ chunk_idx) {
+__global__ __launch_bounds__(RB) void k_raster_fwd_chunk(const ViewConsts* __restrict__ vc,
+                                                         const int2* __restrict__ bins, EntryView pk,
+                                                         float* __restrict__ chunk_T, float* __restrict__ chunk_C,
+                                                         int32_t* __restrict__ chunk_idx) {
+    const int W = vc->W, H = vc->H, tbx = vc->tbx;
     const int t = blockIdx.x, k = blockIdx.y;
+    if (t >= vc->T) return;
     const int tx = t % tbx, ty = t / tbx;
     const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
     const bool inside = i < H && j < W;
@@ -788,12 +947,13 @@ __global__ __launch_bounds__(RB) void k_raster_fwd_chunk(int W, int H, int tbx, 
     }
 }
 
-__global__ void k_raster_merge(int W, int H, int K, const float* __restrict__ chunk_T, const float* __restrict__ chunk_C,
-                               int32_t* __restrict__ chunk_idx, float bg0, float bg1, float bg2,
+__global__ void k_raster_merge(const ViewConsts* __restrict__ vc, int K, const float* __restrict__ chunk_T,
+                               const float* __restrict__ chunk_C, int32_t* __restrict__ chunk_idx,
                                float* __restrict__ img, float* __restrict__ final_T, int32_t* __restrict__ final_idx) {
-    const size_t P = (size_t)W * H;
+    const size_t P = (size_t)vc->W * vc->H;
     const size_t pid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= P) return;
+    const float bg0 = vc->bg[0], bg1 = vc->bg[1], bg2 = vc->bg[2];
     float Tr = 1.f, C0 = 0.f, C1 = 0.f, C2 = 0.f;
     int last = -1, cutoff = K;
     for (int k = 0; k < K; k++) {
@@ -816,17 +976,32 @@ __global__ void k_raster_merge(int W, int H, int K, const float* __restrict__ ch
     img[3 * pid + 2] = fminf(fmaxf(fmaf(bg2, Tr, C2), 0.f), 1.f);
 }
 
-void raster_forward(hipStream_t s, const ListBufs& L, int W, int H, int tbx, int tby, const float bg[3], RasterBufs& R) {
-    const int T = tbx * tby;
+size_t work_temp_bytes(int items) { return lists_temp_bytes(items); }
+
+void raster_work_build(hipStream_t s, const RasterWork& w) {
+    const int n = w.B * w.Tmax;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_work_keys, dim3(cdiv(n, 256)), dim3(256), 0, s, w.views, w.B, w.Tmax, w.keys_a, w.vals_a,
+                       w.n_work, w.ctr);
+    // keys are 16 bits: two stable 8-bit passes leave the order in (keys_a, vals_a)
+    radix_sort_pairs(s, n, w.keys_a, w.vals_a, w.keys_b, w.vals_b, w.temp, 2);
+}
+
+void raster_forward(hipStream_t s, const ListBufs& L, const ViewConsts* vc, const Geom& g, RasterBufs& R,
+                    const RasterWork* work) {
     if (R.K <= 1) {
-        hipLaunchKernelGGL(k_raster_fwd, dim3(T), dim3(RB), 0, s, W, H, tbx, L.bins, L.view, bg[0], bg[1], bg[2], R.img,
-                           R.final_T, R.final_idx);
+        hipLaunchKernelGGL(k_raster_fwd_persist, dim3(work->grid), dim3(RB), 0, s, work->views, work->vals_a,
+                           work->n_work, work->ctr);
         return;
     }
-    hipLaunchKernelGGL(k_raster_fwd_chunk, dim3(T, R.K), dim3(RB), 0, s, W, H, tbx, L.bins, L.view, R.chunk_T, R.chunk_C,
+    hipLaunchKernelGGL(k_raster_fwd_chunk, dim3(g.T, R.K), dim3(RB), 0, s, vc, L.bins, L.view, R.chunk_T, R.chunk_C,
                        R.chunk_idx);
-    hipLaunchKernelGGL(k_raster_merge, dim3(cdiv((size_t)W * H, 256)), dim3(256), 0, s, W, H, (int)R.K, R.chunk_T,
-                       R.chunk_C, R.chunk_idx, bg[0], bg[1], bg[2], R.img, R.final_T, R.final_idx);
+    hipLaunchKernelGGL(k_raster_merge, dim3(cdiv((size_t)g.W * g.H, 256)), dim3(256), 0, s, vc, (int)R.K, R.chunk_T,
+                       R.chunk_C, R.chunk_idx, R.img, R.final_T, R.final_idx);
+}
+
+void raster_forward_views(hipStream_t s, const RasterWork& w) {
+    hipLaunchKernelGGL(k_raster_fwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
 }
 
 // ------------------------------------------------------------------------------------
@@ -835,10 +1010,13 @@ void raster_forward(hipStream_t s, const ListBufs& L, int W, int H, int tbx, int
 // contiguous 256-byte run.
 constexpr int SB = 16;
 
-__global__ __launch_bounds__(256) void k_ssim_h(int W, int H, const float* __restrict__ img,
-                                                const uint8_t* __restrict__ gt, float* __restrict__ hbuf) {
+__global__ __launch_bounds__(256) void k_ssim_h(const ViewConsts* __restrict__ vc, const float* __restrict__ img,
+                                                float* __restrict__ hbuf) {
     __shared__ float sg[3][SB][SB + 10];
     __shared__ float sr[3][SB][SB + 10];
+    const int W = vc->W, H = vc->H;
+    if ((int)blockIdx.x * SB >= W || (int)blockIdx.y * SB >= H) return;
+    const uint8_t* __restrict__ gt = vc->gt;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int bx = blockIdx.x * SB - 5, by = blockIdx.y * SB;
     for (int c = 0; c < 3; c++)
@@ -878,10 +1056,14 @@ __global__ __launch_bounds__(256) void k_ssim_h(int W, int H, const float* __res
     }
 }
 
-__global__ __launch_bounds__(256) void k_ssim_fused(int W, int H, const float* __restrict__ img,
-                                                    const uint8_t* __restrict__ gt, const float* __restrict__ hbuf,
-                                                    float ssim_w, float* __restrict__ dbuf,
+__global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict__ vc, const float* __restrict__ img,
+                                                    const float* __restrict__ hbuf, float* __restrict__ dbuf,
                                                     float* __restrict__ partials, uint32_t* __restrict__ nanc) {
+    const int W = vc->W, H = vc->H;
+    const int gx = (W + SB - 1) / SB;
+    if ((int)blockIdx.x >= gx || (int)blockIdx.y * SB >= H) return;
+    const uint8_t* __restrict__ gt = vc->gt;
+    const float ssim_w = vc->ssim_w;
     constexpr int TD = SB + 10;  // 26
     __shared__ float shp[TD][TD][5];
     __shared__ float sf[3][SB][TD];
@@ -973,15 +1155,18 @@ __global__ __launch_bounds__(256) void k_ssim_fused(int W, int H, const float* _
         if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = red[0];
+    if (threadIdx.x == 0) partials[blockIdx.y * gx + blockIdx.x] = red[0];
 }
 
-__global__ __launch_bounds__(256) void k_ssim_vbwd(int W, int H, const float* __restrict__ img,
-                                                   const uint8_t* __restrict__ gt, const float* __restrict__ dbuf,
-                                                   float ssim_w, float inv_n, float* __restrict__ v_rend,
+__global__ __launch_bounds__(256) void k_ssim_vbwd(const ViewConsts* __restrict__ vc, const float* __restrict__ img,
+                                                   const float* __restrict__ dbuf, float* __restrict__ v_rend,
                                                    uint32_t* __restrict__ nanc) {
     constexpr int TH = SB + 10;
     __shared__ float sh[9][TH][SB];
+    const int W = vc->W, H = vc->H;
+    if ((int)blockIdx.x * SB >= W || (int)blockIdx.y * SB >= H) return;
+    const uint8_t* __restrict__ gt = vc->gt;
+    const float ssim_w = vc->ssim_w, inv_n = vc->inv_n;
     const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
     const int bx = blockIdx.x * SB, by = blockIdx.y * SB - 5;
     const size_t P = (size_t)W * H;
@@ -1019,8 +1204,10 @@ __global__ __launch_bounds__(256) void k_ssim_vbwd(int W, int H, const float* __
     }
 }
 
-__global__ __launch_bounds__(256) void k_loss_final(int n, const float* __restrict__ partials, double* __restrict__ out) {
+__global__ __launch_bounds__(256) void k_loss_final(const ViewConsts* __restrict__ vc, const float* __restrict__ partials,
+                                                    double* __restrict__ out) {
     __shared__ double red[256];
+    const int n = ((vc->W + SB - 1) / SB) * ((vc->H + SB - 1) / SB);
     double acc = 0.0;
     for (int k = threadIdx.x; k < n; k += 256) acc += (double)partials[k];
     red[threadIdx.x] = acc;
@@ -1032,13 +1219,13 @@ __global__ __launch_bounds__(256) void k_loss_final(int n, const float* __restri
     if (threadIdx.x == 0) *out = red[0];
 }
 
-void ssim_loss(hipStream_t s, int W, int H, const float* img, const uint8_t* gt, float ssim_w, float inv_n, float* hbuf,
-               float* dbuf, float* v_rend, float* loss_partials, double* loss_out, uint32_t* nan_counts) {
-    dim3 g((W + SB - 1) / SB, (H + SB - 1) / SB);
-    hipLaunchKernelGGL(k_ssim_h, g, dim3(256), 0, s, W, H, img, gt, hbuf);
-    hipLaunchKernelGGL(k_ssim_fused, g, dim3(256), 0, s, W, H, img, gt, hbuf, ssim_w, dbuf, loss_partials, nan_counts);
-    hipLaunchKernelGGL(k_ssim_vbwd, g, dim3(256), 0, s, W, H, img, gt, dbuf, ssim_w, inv_n, v_rend, nan_counts);
-    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, s, (int)(g.x * g.y), loss_partials, loss_out);
+void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& gm, const float* img, float* hbuf, float* dbuf,
+               float* v_rend, float* loss_partials, double* loss_out, uint32_t* nan_counts) {
+    dim3 g((gm.W + SB - 1) / SB, (gm.H + SB - 1) / SB);
+    hipLaunchKernelGGL(k_ssim_h, g, dim3(256), 0, s, vc, img, hbuf);
+    hipLaunchKernelGGL(k_ssim_fused, g, dim3(256), 0, s, vc, img, hbuf, dbuf, loss_partials, nan_counts);
+    hipLaunchKernelGGL(k_ssim_vbwd, g, dim3(256), 0, s, vc, img, dbuf, v_rend, nan_counts);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, s, vc, loss_partials, loss_out);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1167,30 +1354,41 @@ __device__ __forceinline__ int block_max_int(int v) {
     return r;
 }
 
-__global__ __launch_bounds__(RB) void k_raster_bwd(int W, int H, int tbx, const int2* __restrict__ bins, EntryView pk,
-                                                   float bg0, float bg1, float bg2, const float* __restrict__ final_T,
-                                                   const int32_t* __restrict__ final_idx,
-                                                   const float* __restrict__ v_rend, GradBufs G) {
-    const int t = blockIdx.x;
-    const int tx = t % tbx, ty = t / tbx;
-    const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
-    const bool inside = i < H && j < W;
-    const int2 r = bins[t];
-    if (r.y <= r.x) return;
-    const size_t pid = inside ? (size_t)i * W + j : 0;
-    const float Tf = inside ? final_T[pid] : 1.f;
-    const int bf = inside ? final_idx[pid] : -1;
-    const float vo0 = inside ? v_rend[3 * pid] : 0.f, vo1 = inside ? v_rend[3 * pid + 1] : 0.f,
-                vo2 = inside ? v_rend[3 * pid + 2] : 0.f;
-    const int tmax = block_max_int(inside ? bf : INT_MIN);
-    const float bg[3] = {bg0, bg1, bg2};
-    backward_range(r.x, r.y, bf, inside, (float)j, (float)i, Tf, 0.f, 0.f, 0.f, Tf, bg, vo0, vo1, vo2, pk, G, tmax);
+__global__ __launch_bounds__(RB) void k_raster_bwd_persist(const RasterView* __restrict__ views,
+                                                           const uint32_t* __restrict__ work,
+                                                           const uint32_t* __restrict__ n_work,
+                                                           uint32_t* __restrict__ ctr) {
+    const uint32_t total = *n_work;
+    for (;;) {
+        const uint32_t it = next_item(ctr + 1);
+        if (it >= total) break;
+        const uint32_t code = work[it];
+        const RasterView& rv = views[code >> WORK_TILE_BITS];
+        const ViewConsts& vc = *rv.vc;
+        const int t = (int)(code & ((1u << WORK_TILE_BITS) - 1u));
+        const int W = vc.W, H = vc.H, tbx = vc.tbx;
+        const int tx = t % tbx, ty = t / tbx;
+        const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
+        const bool inside = i < H && j < W;
+        const int2 r = rv.bins[t];
+        if (r.y <= r.x) continue;  // uniform over the workgroup
+        const size_t pid = inside ? (size_t)i * W + j : 0;
+        const float Tf = inside ? rv.final_T[pid] : 1.f;
+        const int bf = inside ? rv.final_idx[pid] : -1;
+        const float vo0 = inside ? rv.v_rend[3 * pid] : 0.f, vo1 = inside ? rv.v_rend[3 * pid + 1] : 0.f,
+                    vo2 = inside ? rv.v_rend[3 * pid + 2] : 0.f;
+        const int tmax = block_max_int(inside ? bf : INT_MIN);
+        const float bg[3] = {vc.bg[0], vc.bg[1], vc.bg[2]};
+        GradBufs G = rv.G;
+        backward_range(r.x, r.y, bf, inside, (float)j, (float)i, Tf, 0.f, 0.f, 0.f, Tf, bg, vo0, vo1, vo2, rv.pk, G,
+                       tmax);
+    }
 }
 
-__global__ void k_chunk_prefix_suffix(int W, int H, int K, const float* __restrict__ chunk_T,
+__global__ void k_chunk_prefix_suffix(const ViewConsts* __restrict__ vc, int K, const float* __restrict__ chunk_T,
                                       const float* __restrict__ chunk_C, float* __restrict__ prefix_T,
                                       float* __restrict__ after_C) {
-    const size_t P = (size_t)W * H;
+    const size_t P = (size_t)vc->W * vc->H;
     const size_t pid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (pid >= P) return;
     float pT = 1.f;
@@ -1210,15 +1408,17 @@ __global__ void k_chunk_prefix_suffix(int W, int H, int K, const float* __restri
     }
 }
 
-__global__ __launch_bounds__(RB) void k_raster_bwd_chunk(int W, int H, int tbx, const int2* __restrict__ bins,
-                                                         EntryView pk, float bg0, float bg1, float bg2,
+__global__ __launch_bounds__(RB) void k_raster_bwd_chunk(const ViewConsts* __restrict__ vc,
+                                                         const int2* __restrict__ bins, EntryView pk,
                                                          const float* __restrict__ final_T,
                                                          const int32_t* __restrict__ chunk_idx,
                                                          const float* __restrict__ prefix_T,
                                                          const float* __restrict__ chunk_T,
                                                          const float* __restrict__ after_C,
                                                          const float* __restrict__ v_rend, GradBufs G) {
+    const int W = vc->W, H = vc->H, tbx = vc->tbx;
     const int t = blockIdx.x, k = blockIdx.y;
+    if (t >= vc->T) return;
     const int tx = t % tbx, ty = t / tbx;
     const int j = tx * TILE + tile_lx(), i = ty * TILE + tile_ly();
     const bool inside = i < H && j < W;
@@ -1237,22 +1437,25 @@ __global__ __launch_bounds__(RB) void k_raster_bwd_chunk(int W, int H, int tbx, 
     const float Tf = inside ? final_T[pid] : 1.f;
     const float vo0 = inside ? v_rend[3 * pid] : 0.f, vo1 = inside ? v_rend[3 * pid + 1] : 0.f,
                 vo2 = inside ? v_rend[3 * pid + 2] : 0.f;
-    const float bg[3] = {bg0, bg1, bg2};
+    const float bg[3] = {vc->bg[0], vc->bg[1], vc->bg[2]};
     backward_range(cs, ce, act ? bf : -1, act, (float)j, (float)i, Ts, b0, b1, b2, Tf, bg, vo0, vo1, vo2, pk, G, tmax);
 }
 
-void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, int W, int H, int tbx, int tby,
-                     const float bg[3], const float* v_rend, GradBufs& G) {
-    const int T = tbx * tby;
+void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, const ViewConsts* vc, const Geom& g,
+                     const float* v_rend, GradBufs& G, const RasterWork* work) {
     if (R.K <= 1) {
-        hipLaunchKernelGGL(k_raster_bwd, dim3(T), dim3(RB), 0, s, W, H, tbx, L.bins, L.view, bg[0], bg[1], bg[2], R.final_T,
-                           R.final_idx, v_rend, G);
+        hipLaunchKernelGGL(k_raster_bwd_persist, dim3(work->grid), dim3(RB), 0, s, work->views, work->vals_a,
+                           work->n_work, work->ctr);
         return;
     }
-    hipLaunchKernelGGL(k_chunk_prefix_suffix, dim3(cdiv((size_t)W * H, 256)), dim3(256), 0, s, W, H, (int)R.K,
+    hipLaunchKernelGGL(k_chunk_prefix_suffix, dim3(cdiv((size_t)g.W * g.H, 256)), dim3(256), 0, s, vc, (int)R.K,
                        R.chunk_T, R.chunk_C, R.prefix_T, R.after_C);
-    hipLaunchKernelGGL(k_raster_bwd_chunk, dim3(T, R.K), dim3(RB), 0, s, W, H, tbx, L.bins, L.view, bg[0], bg[1], bg[2],
-                       R.final_T, R.chunk_idx, R.prefix_T, R.chunk_T, R.after_C, v_rend, G);
+    hipLaunchKernelGGL(k_raster_bwd_chunk, dim3(g.T, R.K), dim3(RB), 0, s, vc, L.bins, L.view, R.final_T, R.chunk_idx,
+                       R.prefix_T, R.chunk_T, R.after_C, v_rend, G);
+}
+
+void raster_backward_views(hipStream_t s, const RasterWork& w) {
+    hipLaunchKernelGGL(k_raster_bwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1583,65 +1786,29 @@ __device__ __forceinline__ uint32_t pcg_hash32(uint32_t seed, uint32_t idx) {
 }
 __device__ __forceinline__ float pcg_unit(uint32_t h) { return ((float)(h >> 9) + 0.5f) * (1.0f / 8388608.0f); }
 
-template <bool BF>
-__global__ __launch_bounds__(256) void k_gaussian_update(
-    int N, float* __restrict__ means, float* __restrict__ scales, float* __restrict__ quats, float* __restrict__ f_dc,
-    float* __restrict__ opac, float* __restrict__ m0, float* __restrict__ v0, float* __restrict__ m1,
-    float* __restrict__ v1, float* __restrict__ m2, float* __restrict__ v2, typename Mom<BF>::T* __restrict__ m3,
-    typename Mom<BF>::T* __restrict__ v3, uint16_t* __restrict__ dc_shadow, float* __restrict__ m5,
-    float* __restrict__ v5, MsplatCamera cam,
-    const int32_t* __restrict__ radii, const float4* __restrict__ rA, const float4* __restrict__ rB, GradBufs G,
-    MsplatStepParams hp, uint32_t* __restrict__ nanc, float* __restrict__ dbg_vm, float* __restrict__ dbg_vs,
-    float* __restrict__ dbg_vq, float* __restrict__ dbg_vo) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    float p[3] = {means[3 * (size_t)i], means[3 * (size_t)i + 1], means[3 * (size_t)i + 2]};
-    float ls[3] = {scales[3 * (size_t)i], scales[3 * (size_t)i + 1], scales[3 * (size_t)i + 2]};
-    const float4 q4 = reinterpret_cast<const float4*>(quats)[i];
-    float q[4] = {q4.x, q4.y, q4.z, q4.w};
-    float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
+// regularizer gradients (mcmc_reg_grad_kernel, metal:3987-4026); vdc = sum of the DC exp_avg_sq
+__device__ __forceinline__ void reg_grads(int N, float opac_i, const float ls[3], float vdc, const MsplatStepParams& hp,
+                                          float& vo, float vs[3]) {
+    const float duty = (vdc == 0.0f) ? 1.0f : hp.reg_invis_scale;
+    const float invN = duty / (float)N;
+    const float sig = 1.0f / (1.0f + ms_expf(-opac_i));
+    vo += hp.op_reg * sig * (1.0f - sig) * invN;
+    const float inv3N = invN * (1.0f / 3.0f);
+#pragma unroll
+    for (int k = 0; k < 3; k++) vs[k] += hp.sc_reg * ms_expf_core(fminf(fmaxf(ls[k], -15.f), 15.f)) * inv3N;
+}
+
+// Adam on means / log-scales / quaternions / opacity with the non-finite gradient gate, then
+// SGLD on the updated parameters, then write-back of row i.
+__device__ __forceinline__ void adam_sgld_tail(int i, float p[3], float ls[3], float q[4], const float vm[3],
+                                               const float vs[3], const float vq[4], float vo,
+                                               float* __restrict__ means, float* __restrict__ scales,
+                                               float* __restrict__ quats, float* __restrict__ opac,
+                                               float* __restrict__ m0, float* __restrict__ v0, float* __restrict__ m1,
+                                               float* __restrict__ v1, float* __restrict__ m2, float* __restrict__ v2,
+                                               float* __restrict__ m5, float* __restrict__ v5,
+                                               const MsplatStepParams& hp, uint32_t* __restrict__ nanc) {
     const float b1 = hp.beta1, b2 = hp.beta2, eps = hp.eps;
-    if (radii[i] > 0) {
-        const float4 a = rA[i], b = rB[i];
-        const float conic[3] = {a.w, b.x, b.y};
-        const float2 vxy = G.v_xy[i];
-        const float vcn[3] = {G.v_conic[3 * (size_t)i], G.v_conic[3 * (size_t)i + 1], G.v_conic[3 * (size_t)i + 2]};
-        proj_vjp(p, ls, q, cam, conic, vxy.x, vxy.y, vcn, vm, vs, vq);
-        const float vc[3] = {G.v_rgb[3 * (size_t)i], G.v_rgb[3 * (size_t)i + 1], G.v_rgb[3 * (size_t)i + 2]};
-        if (ms_finite(vc[0]) && ms_finite(vc[1]) && ms_finite(vc[2])) {
-#pragma unroll
-            for (int c = 0; c < 3; c++) {
-                const size_t k = 3 * (size_t)i + c;
-                float pp = f_dc[k], mm_ = Mom<BF>::ld(m3, k), vv = Mom<BF>::ld(v3, k);
-                adam_el(pp, mm_, vv, SH_C0 * vc[c], hp.step_size[3], b1, b2, hp.bc2_sqrt[3], eps);
-                f_dc[k] = pp;
-                Mom<BF>::st(m3, k, mm_);
-                Mom<BF>::st(v3, k, vv);
-                if (BF) dc_shadow[k] = ms_bf16_bits(pp);
-            }
-        } else {
-            atomicAdd(&nanc[3], 1u);
-        }
-    }
-    float vo = G.v_opac[i];
-    if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
-        const float vdc = Mom<BF>::ld(v3, 3 * (size_t)i) + Mom<BF>::ld(v3, 3 * (size_t)i + 1) +
-                          Mom<BF>::ld(v3, 3 * (size_t)i + 2);
-        const float duty = (vdc == 0.0f) ? 1.0f : hp.reg_invis_scale;
-        const float invN = duty / (float)N;
-        const float sig = 1.0f / (1.0f + ms_expf(-opac[i]));
-        vo += hp.op_reg * sig * (1.0f - sig) * invN;
-        const float inv3N = invN * (1.0f / 3.0f);
-#pragma unroll
-        for (int k = 0; k < 3; k++) vs[k] += hp.sc_reg * ms_expf_core(fminf(fmaxf(ls[k], -15.f), 15.f)) * inv3N;
-    }
-    if (dbg_vm) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) { dbg_vm[3 * (size_t)i + k] = vm[k]; dbg_vs[3 * (size_t)i + k] = vs[k]; }
-#pragma unroll
-        for (int k = 0; k < 4; k++) dbg_vq[4 * (size_t)i + k] = vq[k];
-        dbg_vo[i] = vo;
-    }
     // fused_adam (metal:1680-1713) with the non-finite gradient gate, groups 0, 1, 2, 5
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -1716,6 +1883,62 @@ __global__ __launch_bounds__(256) void k_gaussian_update(
     opac[i] = o;
 }
 
+template <bool BF>
+__global__ __launch_bounds__(256) void k_gaussian_update(
+    int N, float* __restrict__ means, float* __restrict__ scales, float* __restrict__ quats, float* __restrict__ f_dc,
+    float* __restrict__ opac, float* __restrict__ m0, float* __restrict__ v0, float* __restrict__ m1,
+    float* __restrict__ v1, float* __restrict__ m2, float* __restrict__ v2, typename Mom<BF>::T* __restrict__ m3,
+    typename Mom<BF>::T* __restrict__ v3, uint16_t* __restrict__ dc_shadow, float* __restrict__ m5,
+    float* __restrict__ v5, MsplatCamera cam,
+    const int32_t* __restrict__ radii, const float4* __restrict__ rA, const float4* __restrict__ rB, GradBufs G,
+    MsplatStepParams hp, uint32_t* __restrict__ nanc, float* __restrict__ dbg_vm, float* __restrict__ dbg_vs,
+    float* __restrict__ dbg_vq, float* __restrict__ dbg_vo) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float p[3] = {means[3 * (size_t)i], means[3 * (size_t)i + 1], means[3 * (size_t)i + 2]};
+    float ls[3] = {scales[3 * (size_t)i], scales[3 * (size_t)i + 1], scales[3 * (size_t)i + 2]};
+    const float4 q4 = reinterpret_cast<const float4*>(quats)[i];
+    float q[4] = {q4.x, q4.y, q4.z, q4.w};
+    float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
+    const float b1 = hp.beta1, b2 = hp.beta2, eps = hp.eps;
+    if (radii[i] > 0) {
+        const float4 a = rA[i], b = rB[i];
+        const float conic[3] = {a.w, b.x, b.y};
+        const float2 vxy = G.v_xy[i];
+        const float vcn[3] = {G.v_conic[3 * (size_t)i], G.v_conic[3 * (size_t)i + 1], G.v_conic[3 * (size_t)i + 2]};
+        proj_vjp(p, ls, q, cam, conic, vxy.x, vxy.y, vcn, vm, vs, vq);
+        const float vc[3] = {G.v_rgb[3 * (size_t)i], G.v_rgb[3 * (size_t)i + 1], G.v_rgb[3 * (size_t)i + 2]};
+        if (ms_finite(vc[0]) && ms_finite(vc[1]) && ms_finite(vc[2])) {
+#pragma unroll
+            for (int c = 0; c < 3; c++) {
+                const size_t k = 3 * (size_t)i + c;
+                float pp = f_dc[k], mm_ = Mom<BF>::ld(m3, k), vv = Mom<BF>::ld(v3, k);
+                adam_el(pp, mm_, vv, SH_C0 * vc[c], hp.step_size[3], b1, b2, hp.bc2_sqrt[3], eps);
+                f_dc[k] = pp;
+                Mom<BF>::st(m3, k, mm_);
+                Mom<BF>::st(v3, k, vv);
+                if (BF) dc_shadow[k] = ms_bf16_bits(pp);
+            }
+        } else {
+            atomicAdd(&nanc[3], 1u);
+        }
+    }
+    float vo = G.v_opac[i];
+    if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
+        const float vdc = Mom<BF>::ld(v3, 3 * (size_t)i) + Mom<BF>::ld(v3, 3 * (size_t)i + 1) +
+                          Mom<BF>::ld(v3, 3 * (size_t)i + 2);
+        reg_grads(N, opac[i], ls, vdc, hp, vo, vs);
+    }
+    if (dbg_vm) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) { dbg_vm[3 * (size_t)i + k] = vm[k]; dbg_vs[3 * (size_t)i + k] = vs[k]; }
+#pragma unroll
+        for (int k = 0; k < 4; k++) dbg_vq[4 * (size_t)i + k] = vq[k];
+        dbg_vo[i] = vo;
+    }
+    adam_sgld_tail(i, p, ls, q, vm, vs, vq, vo, means, scales, quats, opac, m0, v0, m1, v1, m2, v2, m5, v5, hp, nanc);
+}
+
 void gaussian_update(hipStream_t s, const Params& P, const MsplatCamera& cam, const ProjOut& pj, const GradBufs& G,
                      const MsplatStepParams& hp, uint32_t* nan_counts, float* dbg_v_mean, float* dbg_v_scale,
                      float* dbg_v_quat, float* dbg_v_opac) {
@@ -1753,6 +1976,153 @@ void sh_refresh_shadows(hipStream_t s, const Params& P) {
     const int nb3 = (P.degree <= 0 ? 0 : (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15)) * 3;
     f32_to_bf16(s, P.f_dc, P.f_dc_h, (size_t)P.N * 3);
     f32_to_bf16(s, P.f_rest, P.f_rest_h, (size_t)P.N * nb3);
+}
+
+
+// ------------------------------------------------------------------------------------
+// Multi-view batched step (this build's addition; the reference trains one view per step).
+template <int NB3>
+__global__ __launch_bounds__(256) void k_view_accum(int N, int nuse3, const float* __restrict__ means,
+                                                    const float* __restrict__ scales, const float* __restrict__ quats,
+                                                    const ViewConsts* __restrict__ vc,
+                                                    const int32_t* __restrict__ radii,
+                                                    const float4* __restrict__ rA, const float4* __restrict__ rB,
+                                                    GradBufs G, BatchGrads A, uint32_t* __restrict__ nanc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N || radii[i] <= 0) return;
+    const MsplatCamera cam = vc->cam;
+    const float p[3] = {means[3 * (size_t)i], means[3 * (size_t)i + 1], means[3 * (size_t)i + 2]};
+    const float ls[3] = {scales[3 * (size_t)i], scales[3 * (size_t)i + 1], scales[3 * (size_t)i + 2]};
+    const float4 q4 = reinterpret_cast<const float4*>(quats)[i];
+    const float q[4] = {q4.x, q4.y, q4.z, q4.w};
+    const float4 a = rA[i], b = rB[i];
+    const float conic[3] = {a.w, b.x, b.y};
+    const float2 vxy = G.v_xy[i];
+    const float vcn[3] = {G.v_conic[3 * (size_t)i], G.v_conic[3 * (size_t)i + 1], G.v_conic[3 * (size_t)i + 2]};
+    float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
+    proj_vjp(p, ls, q, cam, conic, vxy.x, vxy.y, vcn, vm, vs, vq);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        A.v_mean[3 * (size_t)i + k] += vm[k];
+        A.v_scale[3 * (size_t)i + k] += vs[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) A.v_quat[4 * (size_t)i + k] += vq[k];
+    A.v_opac[i] += G.v_opac[i];
+    const float vcol[3] = {G.v_rgb[3 * (size_t)i], G.v_rgb[3 * (size_t)i + 1], G.v_rgb[3 * (size_t)i + 2]};
+    if (!(ms_finite(vcol[0]) && ms_finite(vcol[1]) && ms_finite(vcol[2]))) {
+        atomicAdd(&nanc[3], 1u);
+        return;
+    }
+    A.vis[i] = 1u;
+#pragma unroll
+    for (int c = 0; c < 3; c++) A.v_dc[3 * (size_t)i + c] += SH_C0 * vcol[c];
+    if (NB3 > 0 && nuse3 > 0) {
+        const float d0 = p[0] - cam.cam_pos[0], d1 = p[1] - cam.cam_pos[1], d2 = p[2] - cam.cam_pos[2];
+        const float inv = 1.f / sqrtf(d0 * d0 + d1 * d1 + d2 * d2);
+        float bs[16];
+        sh_basis_all(d0 * inv, d1 * inv, d2 * inv, bs);
+        float* row = A.v_rest + (size_t)NB3 * i;
+        for (int r = 0; r < nuse3; r++) row[r] += bs[r / 3] * vcol[r % 3];
+    }
+}
+
+template <bool BF, int NB3>
+__global__ __launch_bounds__(256) void k_batch_update(
+    int N, int nuse3, float* __restrict__ means, float* __restrict__ scales, float* __restrict__ quats,
+    float* __restrict__ f_dc, float* __restrict__ f_rest, float* __restrict__ opac, float* __restrict__ m0,
+    float* __restrict__ v0, float* __restrict__ m1, float* __restrict__ v1, float* __restrict__ m2,
+    float* __restrict__ v2, typename Mom<BF>::T* __restrict__ m3, typename Mom<BF>::T* __restrict__ v3,
+    typename Mom<BF>::T* __restrict__ m4, typename Mom<BF>::T* __restrict__ v4, uint16_t* __restrict__ dc_shadow,
+    uint16_t* __restrict__ rest_shadow, float* __restrict__ m5, float* __restrict__ v5, BatchGrads A,
+    const MsplatStepParams* __restrict__ hpd, uint32_t* __restrict__ nanc) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const MsplatStepParams hp = *hpd;
+    float p[3] = {means[3 * (size_t)i], means[3 * (size_t)i + 1], means[3 * (size_t)i + 2]};
+    float ls[3] = {scales[3 * (size_t)i], scales[3 * (size_t)i + 1], scales[3 * (size_t)i + 2]};
+    const float4 q4 = reinterpret_cast<const float4*>(quats)[i];
+    float q[4] = {q4.x, q4.y, q4.z, q4.w};
+    float vm[3], vs[3], vq[4];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        vm[k] = A.v_mean[3 * (size_t)i + k];
+        vs[k] = A.v_scale[3 * (size_t)i + k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) vq[k] = A.v_quat[4 * (size_t)i + k];
+    float vo = A.v_opac[i];
+    const float b1 = hp.beta1, b2 = hp.beta2, eps = hp.eps;
+    if (A.vis[i]) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const size_t k = 3 * (size_t)i + c;
+            float pp = f_dc[k], mm_ = Mom<BF>::ld(m3, k), vv = Mom<BF>::ld(v3, k);
+            adam_el(pp, mm_, vv, A.v_dc[k], hp.step_size[3], b1, b2, hp.bc2_sqrt[3], eps);
+            f_dc[k] = pp;
+            Mom<BF>::st(m3, k, mm_);
+            Mom<BF>::st(v3, k, vv);
+            if (BF) dc_shadow[k] = ms_bf16_bits(pp);
+        }
+        for (int r = 0; r < nuse3; r++) {
+            const size_t k = (size_t)NB3 * i + r;
+            float pp = f_rest[k], mm_ = Mom<BF>::ld(m4, k), vv = Mom<BF>::ld(v4, k);
+            adam_el(pp, mm_, vv, A.v_rest[k], hp.step_size[4], b1, b2, hp.bc2_sqrt[4], eps);
+            f_rest[k] = pp;
+            Mom<BF>::st(m4, k, mm_);
+            Mom<BF>::st(v4, k, vv);
+            if (BF) rest_shadow[k] = ms_bf16_bits(pp);
+        }
+    }
+    if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
+        const float vdc = Mom<BF>::ld(v3, 3 * (size_t)i) + Mom<BF>::ld(v3, 3 * (size_t)i + 1) +
+                          Mom<BF>::ld(v3, 3 * (size_t)i + 2);
+        reg_grads(N, opac[i], ls, vdc, hp, vo, vs);
+    }
+    adam_sgld_tail(i, p, ls, q, vm, vs, vq, vo, means, scales, quats, opac, m0, v0, m1, v1, m2, v2, m5, v5, hp, nanc);
+}
+
+static int nuse3_of(const Params& P, int degrees_to_use) {
+    const int du = degrees_to_use < P.degree ? degrees_to_use : P.degree;
+    return (du >= 3 ? 15 : du == 2 ? 8 : du == 1 ? 3 : 0) * 3;
+}
+
+void view_accumulate(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, const ProjOut& pj,
+                     const GradBufs& G, BatchGrads& A, uint32_t* nan_counts) {
+    if (P.N <= 0) return;
+    const int nuse3 = nuse3_of(P, degrees_to_use);
+    const dim3 grid(cdiv(P.N, 256));
+#define LAUNCH_ACC(NB) \
+    hipLaunchKernelGGL(k_view_accum<NB>, grid, dim3(256), 0, s, P.N, nuse3, P.means, P.scales, P.quats, vc, pj.radii, \
+                       pj.rec.A, pj.rec.B, G, A, nan_counts)
+    if (P.degree <= 0) LAUNCH_ACC(0);
+    else if (P.degree == 1) LAUNCH_ACC(9);
+    else if (P.degree == 2) LAUNCH_ACC(24);
+    else LAUNCH_ACC(45);
+#undef LAUNCH_ACC
+}
+
+void batch_update(hipStream_t s, const Params& P, const BatchGrads& A, const MsplatStepParams* hpd, int degrees_to_use,
+                  uint32_t* nan_counts) {
+    if (P.N <= 0) return;
+    const int nuse3 = nuse3_of(P, degrees_to_use);
+    const dim3 grid(cdiv(P.N, 256));
+#define LAUNCH_UPD(BF, MT, NB)                                                                                      \
+    hipLaunchKernelGGL((k_batch_update<BF, NB>), grid, dim3(256), 0, s, P.N, nuse3, P.means, P.scales, P.quats,     \
+                       P.f_dc, P.f_rest, P.opac, P.m[0], P.v[0], P.m[1], P.v[1], P.m[2], P.v[2], (MT*)P.m[3],       \
+                       (MT*)P.v[3], (MT*)P.m[4], (MT*)P.v[4], P.f_dc_h, P.f_rest_h, P.m[5], P.v[5], A, hpd, nan_counts)
+#define LAUNCH_UPD_NB(BF, MT)                        \
+    if (P.degree <= 0) LAUNCH_UPD(BF, MT, 0);        \
+    else if (P.degree == 1) LAUNCH_UPD(BF, MT, 9);   \
+    else if (P.degree == 2) LAUNCH_UPD(BF, MT, 24);  \
+    else LAUNCH_UPD(BF, MT, 45);
+    if (P.sh_bf16) {
+        LAUNCH_UPD_NB(true, uint16_t)
+    } else {
+        LAUNCH_UPD_NB(false, float)
+    }
+#undef LAUNCH_UPD_NB
+#undef LAUNCH_UPD
 }
 
 }  // namespace msk

@@ -108,6 +108,7 @@ Model::~Model() {
     }
     dc_h_.release();
     rest_h_.release();
+    for (auto& b : acc_) b.release();
     mcmc_.release();
 }
 
@@ -275,6 +276,41 @@ void Model::full_iteration(const TrainCamera& cam, int step, float ssim_weight) 
     const MsplatStepParams hp = step_params(cam, step, ssim_weight, os_.adam_step_count);
     const int rc = eng_.train_step(state(), c, cam.gt, hp, false);
     if (rc != 0) throw std::runtime_error("training step rejected its inputs (rc=" + std::to_string(rc) + ")");
+}
+
+void Model::full_iteration_batch(const TrainCamera* const* cams, int B, int step, float ssim_weight) {
+    if (B <= 0 || B > Engine::kLossSlots) throw std::invalid_argument("batch size must be in [1, 64]");
+    for (int v = 0; v < B; v++)
+        if (!cams[v] || !cams[v]->gt) throw std::runtime_error("camera has no ground-truth image");
+    const size_t rows = (size_t)cap_;
+    const size_t sizes[7] = {rows * 12, rows * 12, rows * 16, rows * 4, rows * 12,
+                             rows * std::max(row_len(4, degree_), 1) * 4, rows * 4};
+    for (int k = 0; k < 7; k++) acc_[k].ensure(sizes[k]);
+    msk::BatchGrads A{acc_[0].as<float>(), acc_[1].as<float>(), acc_[2].as<float>(), acc_[3].as<float>(),
+                      acc_[4].as<float>(), acc_[5].as<float>(), acc_[6].as<uint32_t>()};
+    os_.adam_step_count++;
+    std::vector<BatchView> views((size_t)B);
+    MsplatStepParams hp{};
+    for (int v = 0; v < B; v++) {
+        const TrainCamera& cam = *cams[v];
+        views[v].cam = prepare(cam);
+        views[v].gt = cam.gt;
+        views[v].inv_n = 1.0f / (float)(cam.height * cam.width * 3 * B);
+        hp = step_params(cam, step, ssim_weight, os_.adam_step_count);
+    }
+    if (!batch_) {
+        batch_.reset(new BatchRunner(eng_.stream()));
+        batch_->set_persist_grid(persist_grid_);
+    }
+    batch_->set_graph(batch_graph_);
+    batch_->run(state(), views.data(), B, hp, A, eng_.nan_counts());
+    batch_n_ = B;
+}
+
+std::vector<double> Model::batch_losses() {
+    std::vector<double> out;
+    for (int v = 0; v < batch_n_; v++) out.push_back(batch_->loss(v));
+    return out;
 }
 
 void Model::schedulers_step(int step) {

@@ -5,6 +5,9 @@
 #include <string>
 #include <vector>
 
+#include <memory>
+
+#include "batch.h"
 #include "engine.h"
 #include "mcmc.h"
 
@@ -60,6 +63,11 @@ public:
 
     // Model::fullIteration (model.cpp:901-980): one training step on `cam`.
     void full_iteration(const TrainCamera& cam, int step, float ssim_weight);
+    // One multi-view batched step over B cameras (one Adam update of the summed gradients;
+    // the objective is the mean of the per-view losses). Returns nothing; the per-view losses
+    // are in batch_losses().
+    void full_iteration_batch(const TrainCamera* const* cams, int B, int step, float ssim_weight);
+    std::vector<double> batch_losses();
     // Model::schedulersStep (model.cpp:256-261).
     void schedulers_step(int step);
     // Model::mcmcAfterTrain (model.cpp:497-565), on the device.
@@ -91,6 +99,15 @@ public:
     OptimState& optim() { return os_; }
     uint32_t relocation_count() const { return reloc_count_; }
     Engine& engine() { return eng_; }
+    // Batched steps replay one recorded HIP graph per launch configuration when on.
+    void set_batch_graph(bool on) { batch_graph_ = on; }
+    bool batch_graph() const { return batch_graph_; }
+    BatchRunner* batch_runner() { return batch_.get(); }
+    // Resident workgroups of the batched persistent raster launches (0 = default).
+    void set_persist_grid(int g) {
+        persist_grid_ = g;
+        if (batch_) batch_->set_persist_grid(g);
+    }
     int degrees_to_use(int step) const;
 
 private:
@@ -107,7 +124,12 @@ private:
     DBuf p_[6], m_[6], v_[6];
     bool sh_bf16_ = false;
     DBuf dc_h_, rest_h_;  // bf16 shadows of f_dc / f_rest (sh_bf16_ only)
+    DBuf acc_[7];         // batched-step gradient sums (allocated on first use)
     Engine eng_;
+    std::unique_ptr<BatchRunner> batch_;
+    bool batch_graph_ = false;
+    int batch_n_ = 0;
+    int persist_grid_ = 0;
     msk::McmcScratch mcmc_;
 };
 

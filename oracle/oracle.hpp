@@ -1081,6 +1081,126 @@ void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const M
     for (int k = 0; k < 6; k++) O.nan_adam[k] = nan_cnt[k];
 }
 
+// Multi-view batched step (a feature of this build; the reference trains one view per
+// step). Every view renders the same parameters; its raster, SSIM and projection gradients
+// are summed over the views in view order (hp[v].loss_inv_n carries 1 / (3 H_v W_v B), so
+// the objective is the mean of the per-view losses), and one update follows: Adam on the
+// DC and rest coefficients of Gaussians that received a finite colour gradient in at least
+// one view (bases up to degrees_to_use), the regulariser, Adam on means / scales / quats /
+// opacity over all N, then SGLD. With one view it is train_step.
+template <class T> struct BatchOut {
+    std::vector<StepOut<T>> views;
+    double loss_mean = 0.0;
+    uint32_t nan_adam[6] = {0, 0, 0, 0, 0, 0};
+};
+
+template <class T>
+void train_step_batch(State<T>& S, const MsplatCamera* cams, const uint8_t* const* gts, const MsplatStepParams* hps,
+                      int B, BatchOut<T>& O) {
+    const int N = S.N;
+    const int nb = num_sh_bases(S.degree) - 1;
+    const MsplatStepParams& hp = hps[0];
+    std::vector<T> dc_h, rest_h;
+    const T* f_dc_fwd = sh_view(S, S.f_dc, 3 * (size_t)N, dc_h);
+    const T* f_rest_fwd = sh_view(S, S.f_rest, (size_t)N * nb * 3, rest_h);
+    std::vector<T> vm((size_t)N * 3, T(0)), vs((size_t)N * 3, T(0)), vq((size_t)N * 4, T(0)), vo((size_t)N, T(0));
+    std::vector<T> vdc((size_t)N * 3, T(0)), vrest((size_t)N * nb * 3, T(0));
+    std::vector<uint8_t> vis((size_t)N, 0);
+    O.views.resize(B);
+    O.loss_mean = 0.0;
+    uint32_t nan3 = 0;
+    const int nuse = hp.degrees_to_use >= 3 ? 15 : hp.degrees_to_use == 2 ? 8 : hp.degrees_to_use == 1 ? 3 : 0;
+    for (int v = 0; v < B; v++) {
+        StepOut<T>& V = O.views[v];
+        const MsplatCamera& cam = cams[v];
+        const int W = cam.width, H = cam.height;
+        project_forward<T>(N, S.means, S.scales, S.quats, f_dc_fwd, f_rest_fwd, S.opac, S.degree, hp.degrees_to_use,
+                           cam, V.proj);
+        build_lists<T>(N, V.proj, W, H, V.lists);
+        const uint32_t K = k_max_rule(V.lists.capacity, V.lists.num_tiles);
+        raster_forward<T>(V.lists, W, H, hp.background, K, V.raster);
+        V.loss_sum = ssim_loss<T>(W, H, V.raster.img.data(), gts[v], T(hp.ssim_weight), T(hps[v].loss_inv_n), V.v_rend);
+        raster_backward<T>(V.lists, V.raster, hp.background, V.v_rend.data(), N, V.rg);
+        O.loss_mean += V.loss_sum / ((double)W * H) / B;
+#pragma omp parallel for schedule(static) reduction(+ : nan3)
+        for (int i = 0; i < N; i++) {
+            if (V.proj.radii[i] <= 0) continue;
+            const T* p = S.means + 3 * (size_t)i;
+            const T* ls = S.scales + 3 * (size_t)i;
+            const T* q = S.quats + 4 * (size_t)i;
+            T m3[3] = {0, 0, 0}, s3[3] = {0, 0, 0}, q4[4] = {0, 0, 0, 0};
+            proj_backward_one<T>(p, ls, q, cam, &V.proj.conics[3 * (size_t)i], &V.rg.v_xy[2 * (size_t)i],
+                                 &V.rg.v_conic[3 * (size_t)i], m3, s3, q4);
+            for (int k = 0; k < 3; k++) { vm[3 * (size_t)i + k] += m3[k]; vs[3 * (size_t)i + k] += s3[k]; }
+            for (int k = 0; k < 4; k++) vq[4 * (size_t)i + k] += q4[k];
+            vo[i] += V.rg.v_opac[i];
+            const T* vc = &V.rg.v_rgb[3 * (size_t)i];
+            if (!(t_finite(vc[0]) && t_finite(vc[1]) && t_finite(vc[2]))) { nan3++; continue; }
+            vis[i] = 1;
+            for (int c = 0; c < 3; c++) vdc[3 * (size_t)i + c] += T(SH_C0) * vc[c];
+            if (nuse > 0) {
+                T pc[3] = {p[0] - T(cam.cam_pos[0]), p[1] - T(cam.cam_pos[1]), p[2] - T(cam.cam_pos[2])};
+                T inv = T(1) / std::sqrt(pc[0] * pc[0] + pc[1] * pc[1] + pc[2] * pc[2]);
+                T vd[3] = {pc[0] * inv, pc[1] * inv, pc[2] * inv};
+                T bs[15];
+                sh_basis<T>(vd, bs);
+                for (int b = 0; b < nuse; b++)
+                    for (int c = 0; c < 3; c++) vrest[(size_t)nb * 3 * i + b * 3 + c] += bs[b] * vc[c];
+            }
+        }
+    }
+    const T b1 = hp.beta1, b2 = hp.beta2, eps = hp.eps;
+    uint32_t nan_cnt[6] = {0, 0, 0, nan3, 0, 0};
+#pragma omp parallel for schedule(static) reduction(+ : nan_cnt[:6])
+    for (int i = 0; i < N; i++) {
+        T* p = S.means + 3 * (size_t)i;
+        T* ls = S.scales + 3 * (size_t)i;
+        T* q = S.quats + 4 * (size_t)i;
+        if (vis[i]) {
+            for (int c = 0; c < 3; c++) {
+                const size_t k = 3 * (size_t)i + c;
+                adam<T>(S.f_dc[k], S.m[3][k], S.v[3][k], vdc[k], T(hp.step_size[3]), b1, b2, T(hp.bc2_sqrt[3]), eps);
+                S.m[3][k] = sh_store<T>(S.m[3][k], S.sh_bf16);
+                S.v[3][k] = sh_store<T>(S.v[3][k], S.sh_bf16);
+            }
+            for (int b = 0; b < nuse; b++)
+                for (int c = 0; c < 3; c++) {
+                    const size_t k = (size_t)nb * 3 * i + b * 3 + c;
+                    adam<T>(S.f_rest[k], S.m[4][k], S.v[4][k], vrest[k], T(hp.step_size[4]), b1, b2,
+                            T(hp.bc2_sqrt[4]), eps);
+                    S.m[4][k] = sh_store<T>(S.m[4][k], S.sh_bf16);
+                    S.v[4][k] = sh_store<T>(S.v[4][k], S.sh_bf16);
+                }
+        }
+        T gvo = vo[i];
+        T gvs[3] = {vs[3 * (size_t)i], vs[3 * (size_t)i + 1], vs[3 * (size_t)i + 2]};
+        if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
+            const T* dsq = S.v[3] + 3 * (size_t)i;
+            T vdcs = dsq[0] + dsq[1] + dsq[2];
+            T duty = (vdcs == T(0)) ? T(1) : T(hp.reg_invis_scale);
+            T invN = duty / T((float)N);
+            T sig = T(1) / (T(1) + t_exp<T>(-S.opac[i]));
+            gvo += T(hp.op_reg) * sig * (T(1) - sig) * invN;
+            T inv3N = invN * T(1.0f / 3.0f);
+            for (int k = 0; k < 3; k++) gvs[k] += T(hp.sc_reg) * t_exp<T>(t_clamp(ls[k], T(-15), T(15))) * inv3N;
+        }
+        auto adam_g = [&](int grp, T& pp, T& mm, T& vv, T g) {
+            if (!t_finite(g)) { nan_cnt[grp]++; return; }
+            adam<T>(pp, mm, vv, g, T(hp.step_size[grp]), b1, b2, T(hp.bc2_sqrt[grp]), eps);
+        };
+        for (int k = 0; k < 3; k++) adam_g(0, p[k], S.m[0][3 * (size_t)i + k], S.v[0][3 * (size_t)i + k], vm[3 * (size_t)i + k]);
+        for (int k = 0; k < 3; k++) adam_g(1, ls[k], S.m[1][3 * (size_t)i + k], S.v[1][3 * (size_t)i + k], gvs[k]);
+        for (int k = 0; k < 4; k++) adam_g(2, q[k], S.m[2][4 * (size_t)i + k], S.v[2][4 * (size_t)i + k], vq[4 * (size_t)i + k]);
+        adam_g(5, S.opac[i], S.m[5][i], S.v[5][i], gvo);
+        if (hp.sgld_enable) {
+            T n[3];
+            sgld_noise3<T>(hp.sgld_seed, (uint32_t)i, n);
+            sgld_apply<T>(p, ls, q, S.opac[i], n, T(hp.noise_lr), T(hp.xyz_lr), T(hp.max_disp));
+        }
+    }
+    for (int k = 0; k < 6; k++) O.nan_adam[k] = nan_cnt[k];
+}
+
 // Forward only (msplat_render, mm:1099-1116)
 template <class T>
 void render(const State<T>& S, const MsplatCamera& cam, int degrees_to_use, const float bg[3], StepOut<T>& O) {

@@ -4,6 +4,7 @@
 #include <cstdio>
 #include "oracle.hpp"
 #include "oracle_mcmc.hpp"
+#include "oracle_mlp.hpp"
 
 using namespace oracle;
 
@@ -145,6 +146,24 @@ uint32_t oracle_k_max(int64_t capacity, int num_tiles) { return k_max_rule(capac
 void oracle_eq9(float o_old, const float* s_old, int n, float* o_new, float* s_new) {
     relocation_eq9(o_old, s_old, n, o_new, s_new);
 }
+
+// Batched step (FP32 or FP64 state); outputs per view: image and raster gradients when the
+// pointers are set; returns the mean of the per-view losses.
+#define ORACLE_BATCH(SUF, NAME, FT)                                                                         \
+    double oracle_train_step_batch_##NAME(OracleState##SUF* s, const MsplatCamera* cams,                    \
+                                         const uint8_t* const* gts, const MsplatStepParams* hps, int B,     \
+                                         FT* const* imgs, FT* const* v_xy) {                                \
+        State<FT> S = make_state<FT>(s);                                                                    \
+        BatchOut<FT> O;                                                                                     \
+        train_step_batch<FT>(S, cams, gts, hps, B, O);                                                      \
+        for (int v = 0; v < B; v++) {                                                                       \
+            if (imgs && imgs[v]) cp(imgs[v], O.views[v].raster.img);                                        \
+            if (v_xy && v_xy[v]) cp(v_xy[v], O.views[v].rg.v_xy);                                          \
+        }                                                                                                   \
+        return O.loss_mean;                                                                                 \
+    }
+ORACLE_BATCH(F32, f32, float)
+ORACLE_BATCH(F64, f64, double)
 
 int oracle_mcmc_step(OracleStateF32* s, int32_t cap_max, float cull_radius, uint32_t reloc_count_in, int32_t* out_info) {
     State<float> S = make_state<float>(s);
@@ -317,4 +336,37 @@ extern "C" void oracle_eval_sums(int H, int W, const float* img, const uint8_t* 
     out3[0] = sq;
     out3[1] = ab;
     out3[2] = ss;
+}
+
+// ---- learned relocation proposal -------------------------------------------------------
+extern "C" {
+// out [N 4]; exact != 0: double arithmetic without bf16 rounding.
+void oracle_mlp_forward(const float* theta, const uint16_t* X, int N, int exact, double* out) {
+    for (int g = 0; g < N; g++) {
+        float x[oracle::MI];
+        for (int f = 0; f < oracle::MI; f++) x[f] = oracle::bf(X[(size_t)g * oracle::MI + f]);
+        if (exact) {
+            oracle::MlpActs<double> A;
+            oracle::mlp_forward_one<double>(theta, x, true, A);
+            for (int o = 0; o < oracle::MO; o++) out[(size_t)g * 4 + o] = A.z3[o];
+        } else {
+            oracle::MlpActs<float> A;
+            oracle::mlp_forward_one<float>(theta, x, false, A);
+            for (int o = 0; o < oracle::MO; o++) out[(size_t)g * 4 + o] = A.z3[o];
+        }
+    }
+}
+void oracle_mlp_backward(const float* theta, const uint16_t* X, const float* tgt, int N, int exact, double* grad,
+                         double* loss2) {
+    if (exact) {
+        oracle::mlp_backward<double>(theta, X, tgt, N, true, grad, loss2);
+    } else {
+        std::vector<float> g(oracle::NPAR);
+        oracle::mlp_backward<float>(theta, X, tgt, N, false, g.data(), loss2);
+        for (int k = 0; k < oracle::NPAR; k++) grad[k] = g[k];
+    }
+}
+double oracle_mlp_loss_exact(const double* theta, const uint16_t* X, const float* tgt, int N) {
+    return oracle::mlp_loss_exact(theta, X, tgt, N);
+}
 }

@@ -78,6 +78,11 @@ def lib():
         L.oracle_ms_expf.restype = C.c_float
         L.oracle_ms_logf.argtypes = [C.c_float]
         L.oracle_ms_logf.restype = C.c_float
+        L.oracle_mlp_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+        L.oracle_mlp_backward.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                          C.c_void_p]
+        L.oracle_mlp_loss_exact.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+        L.oracle_mlp_loss_exact.restype = C.c_double
         L.oracle_ms_sincosf.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         L.oracle_k_max.argtypes = [C.c_int64, C.c_int]
         L.oracle_k_max.restype = C.c_uint32
@@ -93,6 +98,10 @@ def lib():
         L.oracle_project_vjp_f64.argtypes = [C.c_int, vp, vp, vp, C.POINTER(MsplatCamera), vp, vp, vp, vp, vp, vp]
         L.oracle_sh_color_f64.argtypes = [C.c_int, vp, vp, vp, vp]
         L.oracle_ssim_window.argtypes = [vp]
+        L.oracle_train_step_batch_f32.argtypes = [C.POINTER(StateF32), vp, vp, vp, C.c_int, vp, vp]
+        L.oracle_train_step_batch_f32.restype = C.c_double
+        L.oracle_train_step_batch_f64.argtypes = [C.POINTER(StateF64), vp, vp, vp, C.c_int, vp, vp]
+        L.oracle_train_step_batch_f64.restype = C.c_double
         L.oracle_eval_sums.argtypes = [C.c_int, C.c_int, vp, vp, vp]
         _lib = L
     return _lib
@@ -210,6 +219,25 @@ def render(st: HostState, cam, degrees_to_use: int, bg, want_ids=True):
                 want_ids, render_args=(degrees_to_use, bg))
 
 
+def train_step_batch(st: HostState, cams, gts, hps, want_images=False):
+    """One multi-view batched step on the host state (updated in place): per-view cameras,
+    uint8 ground truths and step scalars. Returns (mean per-view loss, per-view images)."""
+    from msplat_amd._lib import MsplatCamera, MsplatStepParams
+    L = lib()
+    B = len(cams)
+    f64 = st.dtype == np.float64
+    cam_arr = (MsplatCamera * B)(*cams)
+    hp_arr = (MsplatStepParams * B)(*hps)
+    gts = [np.ascontiguousarray(g, np.uint8) for g in gts]
+    gt_ptrs = (C.c_void_p * B)(*[g.ctypes.data for g in gts])
+    imgs = [np.zeros((c.height, c.width, 3), st.dtype) for c in cams] if want_images else None
+    img_ptrs = (C.c_void_p * B)(*[a.ctypes.data for a in imgs]) if want_images else None
+    fn = L.oracle_train_step_batch_f64 if f64 else L.oracle_train_step_batch_f32
+    loss = fn(st.struct(), C.addressof(cam_arr), C.addressof(gt_ptrs), C.addressof(hp_arr), B,
+              C.addressof(img_ptrs) if want_images else None, None)
+    return loss, imgs
+
+
 def eval_sums(img: np.ndarray, gt: np.ndarray) -> np.ndarray:
     """{sum (r-g)^2, sum |r-g|, sum SSIM} of a float render [H, W, 3] against uint8 ground truth."""
     img = np.ascontiguousarray(img, np.float32)
@@ -229,3 +257,54 @@ def eval_metrics(img: np.ndarray, gt: np.ndarray) -> dict:
 
 def set_threads(n: int) -> int:
     return lib().oracle_set_threads(n)
+
+
+# ---- learned relocation proposal (restatement of this project's own network) ----------
+MLP_NPARAM = 5508
+MLP_BLOCKS = {"W1": (0, 1024), "b1": (1024, 1088), "W2": (1088, 5184), "b2": (5184, 5248), "W3": (5248, 5504),
+              "b3": (5504, 5508)}
+
+
+def mlp_init(seed: int = 0) -> np.ndarray:
+    """He-initialised parameter vector (biases zero) as the trainer seeds it."""
+    rng = np.random.default_rng(seed)
+    th = np.zeros(MLP_NPARAM, np.float32)
+    for name, fan_in in (("W1", 16), ("W2", 64), ("W3", 64)):
+        a, b = MLP_BLOCKS[name]
+        th[a:b] = (rng.standard_normal(b - a) * np.sqrt(2.0 / fan_in)).astype(np.float32)
+    return th
+
+
+def mlp_forward(theta: np.ndarray, X: np.ndarray, exact: bool = False) -> np.ndarray:
+    """[N 4] outputs; X is [N 16] uint16 (bf16 bit patterns)."""
+    theta = np.ascontiguousarray(theta, np.float32)
+    X = np.ascontiguousarray(X, np.uint16)
+    out = np.zeros((X.shape[0], 4), np.float64)
+    lib().oracle_mlp_forward(theta.ctypes.data, X.ctypes.data, X.shape[0], int(exact), out.ctypes.data)
+    return out
+
+
+def mlp_backward(theta: np.ndarray, X: np.ndarray, tgt: np.ndarray, exact: bool = False):
+    """(d loss / d theta [5508], (logit loss, offset loss))."""
+    theta = np.ascontiguousarray(theta, np.float32)
+    X = np.ascontiguousarray(X, np.uint16)
+    tgt = np.ascontiguousarray(tgt, np.float32)
+    grad = np.zeros(MLP_NPARAM, np.float64)
+    loss = np.zeros(2, np.float64)
+    lib().oracle_mlp_backward(theta.ctypes.data, X.ctypes.data, tgt.ctypes.data, X.shape[0], int(exact),
+                              grad.ctypes.data, loss.ctypes.data)
+    return grad, (float(loss[0]), float(loss[1]))
+
+
+def mlp_loss_exact(theta64: np.ndarray, X: np.ndarray, tgt: np.ndarray) -> float:
+    theta64 = np.ascontiguousarray(theta64, np.float64)
+    X = np.ascontiguousarray(X, np.uint16)
+    tgt = np.ascontiguousarray(tgt, np.float32)
+    return float(lib().oracle_mlp_loss_exact(theta64.ctypes.data, X.ctypes.data, tgt.ctypes.data, X.shape[0]))
+
+
+def bf16_bits(x: np.ndarray) -> np.ndarray:
+    """float32 -> bfloat16 bit patterns, round to nearest even (the device convention)."""
+    b = np.ascontiguousarray(x, np.float32).view(np.uint32)
+    r = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16).astype(np.uint16)
+    return r

@@ -11,11 +11,13 @@
 // wave through a double-buffered exchange area; a barrier that can never open (divergent
 // barrier) aborts with the kernel and workgroup named. Static __shared__ arrays become function statics,
 // which is sound because workgroups run one at a time. Streams are synchronous. Device
-// allocations are exact-size malloc blocks filled with 0xFF bytes, so reads of unwritten
-// memory show up as NaN / 0xFFFFFFFF rather than as zeros.
+// allocations are guarded blocks in a reserved address range filled with 0xFF bytes, so
+// reads of unwritten memory show up as NaN / 0xFFFFFFFF rather than as zeros, and host
+// pointers handed to copies or kernels in place of device pointers are reported.
 #pragma once
 
 #include <setjmp.h>
+#include <sys/mman.h>
 #include <ucontext.h>
 
 #include <algorithm>
@@ -28,7 +30,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <tuple>
 #include <type_traits>
 #include <utility>
@@ -151,6 +155,45 @@ inline uint64_t emu_exchange(uint64_t v, unsigned src_lane) {
     return x[base + (src_lane & 63u)];
 }
 
+// every lane's value, in lane order
+inline void emu_gather(uint64_t v, uint64_t* out) {
+    const unsigned t = emu_tid();
+    std::vector<uint64_t>& x = emu_blk->xch[emu_blk->wave_op[t]++ & 1u];
+    const size_t base = (size_t)emu_wave() * 64;
+    x[base + emu_lane()] = v;
+    emu_wave_sync();
+    for (unsigned l = 0; l < 64; l++) out[l] = x[base + l];
+}
+
+// v_mfma_f32_16x16x16_bf16, as the CDNA3/CDNA4 instruction set documents its operand
+// layout: lane l holds A[l % 16][4 (l / 16) + q], B[4 (l / 16) + q][l % 16] and
+// C/D[4 (l / 16) + q][l % 16] for q = 0..3. Products and sums in fp32.
+typedef short emu_bf16x4 __attribute__((vector_size(8)));
+typedef float emu_f32x4 __attribute__((vector_size(16)));
+inline emu_f32x4 __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(emu_bf16x4 a, emu_bf16x4 b, emu_f32x4 c, int, int, int) {
+    uint64_t av, bv, A[64], B[64];
+    std::memcpy(&av, &a, 8);
+    std::memcpy(&bv, &b, 8);
+    emu_gather(av, A);
+    emu_gather(bv, B);
+    auto el = [](uint64_t w, int q) {
+        const uint32_t bits = (uint32_t)((w >> (16 * q)) & 0xFFFFu) << 16;
+        float f;
+        std::memcpy(&f, &bits, 4);
+        return f;
+    };
+    const unsigned l = emu_lane();
+    const int col = (int)(l % 16), kb = (int)(l / 16);
+    emu_f32x4 d;
+    for (int q = 0; q < 4; q++) {
+        const int row = 4 * kb + q;
+        float acc = c[q];
+        for (int k = 0; k < 16; k++) acc += el(A[row + 16 * (k / 4)], k % 4) * el(B[col + 16 * (k / 4)], k % 4);
+        d[q] = acc;
+    }
+    return d;
+}
+
 inline void __builtin_amdgcn_wave_barrier() { emu_wave_sync(); }
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 
@@ -236,6 +279,11 @@ inline float atomicAdd(float* p, float v) {
         }
     }
 }
+inline double atomicAdd(double* p, double v) {
+    const double old = *p;
+    *p = old + v;
+    return old;
+}
 inline unsigned atomicMax(unsigned* p, unsigned v) {
     unsigned cur = __atomic_load_n(p, __ATOMIC_SEQ_CST);
     while (cur < v && !__atomic_compare_exchange_n(p, &cur, v, false, __ATOMIC_SEQ_CST, __ATOMIC_SEQ_CST)) {}
@@ -269,29 +317,218 @@ typedef struct EmuStream { int unused; }* hipStream_t;
 typedef struct EmuEvent { std::chrono::steady_clock::time_point t; }* hipEvent_t;
 struct hipDeviceProp_t { size_t sharedMemPerBlock = 160 * 1024; int multiProcessorCount = 256; char name[256] = "emu"; };
 
+// ---- stream capture (HIP graphs) ------------------------------------------------------
+// A capture records launches (and copies) as closures holding their arguments by value; a
+// graph launch replays them in order. Operations a capture must not contain here abort:
+// memset nodes, synchronisation, allocation.
+struct EmuGraph { std::vector<std::function<void()>> nodes; };
+typedef EmuGraph* hipGraph_t;
+typedef EmuGraph* hipGraphExec_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
+                            hipStreamCaptureModeRelaxed = 2 };
+inline EmuGraph*& emu_capture() {
+    static EmuGraph* g = nullptr;
+    return g;
+}
+inline void emu_no_capture(const char* what) {
+    if (emu_capture()) {
+        std::fprintf(stderr, "emu: %s during stream capture\n", what);
+        std::abort();
+    }
+}
+
+// ---- device memory -------------------------------------------------------------------
+// Device allocations live in their own reserved address range, each right-aligned (256 B)
+// against a PROT_NONE guard page and preceded by one, its tail past the requested size
+// poisoned for AddressSanitizer; freed ranges become PROT_NONE and are never reused. Copies
+// and fills check that their device side lies inside one live allocation and their host
+// side outside every one, and every launch scans its arguments for host addresses: the
+// pointer-kind mistakes a CPU run would otherwise accept and a GPU turns into a fault.
+extern "C" void __asan_poison_memory_region(void const volatile* addr, size_t size) __attribute__((weak));
+
+struct EmuDevMem {
+    static constexpr uintptr_t kBase = 0x300000000000ull;
+    static constexpr size_t kSpan = (size_t)1 << 40;
+    static constexpr size_t kPage = 4096;
+    uintptr_t base = 0, next = 0;
+    std::map<uintptr_t, size_t> live;  // start -> requested bytes
+    std::mutex mu;
+    EmuDevMem() {
+        void* r = mmap((void*)kBase, kSpan, PROT_NONE, MAP_PRIVATE | MAP_ANONYMOUS | MAP_NORESERVE | MAP_FIXED_NOREPLACE,
+                       -1, 0);
+        if (r == MAP_FAILED || (uintptr_t)r != kBase) {
+            std::fprintf(stderr, "emu: cannot reserve the device address range\n");
+            std::abort();
+        }
+        base = next = kBase + kPage;
+    }
+    bool inside(uintptr_t a) const { return a >= kBase && a < kBase + kSpan; }
+    // the live allocation holding [a, a + n), or 0
+    uintptr_t find(uintptr_t a, size_t n) {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = live.upper_bound(a);
+        if (it == live.begin()) return 0;
+        --it;
+        if (a < it->first || a + n > it->first + (it->second ? it->second : 1)) return 0;
+        return it->first;
+    }
+};
+inline EmuDevMem& emu_dev() {
+    static EmuDevMem* m = new EmuDevMem();
+    return *m;
+}
+
+[[noreturn]] inline void emu_fault(const char* what, const void* p, size_t n) {
+    std::fprintf(stderr, "emu: GPU memory fault: %s (address %p, %zu bytes)\n", what, p, n);
+    std::abort();
+}
+
+inline void emu_check_dev(const void* p, size_t n, const char* what) {
+    if (n == 0) return;
+    if (!emu_dev().find((uintptr_t)p, n)) emu_fault(what, p, n);
+}
+inline void emu_check_host(const void* p, size_t n, const char* what) {
+    if (n == 0) return;
+    if (emu_dev().inside((uintptr_t)p)) emu_fault(what, p, n);
+}
+
 inline hipError_t hipMalloc(void** p, size_t bytes) {
-    *p = std::malloc(bytes ? bytes : 1);
-    if (!*p) return hipErrorOutOfMemory;
-    std::memset(*p, 0xFF, bytes);
+    emu_no_capture("hipMalloc");
+    EmuDevMem& m = emu_dev();
+    const size_t n = bytes ? bytes : 1;
+    const size_t body = (n + 255) & ~(size_t)255;
+    const size_t pages = (body + EmuDevMem::kPage - 1) / EmuDevMem::kPage;
+    uintptr_t start;
+    {
+        std::lock_guard<std::mutex> g(m.mu);
+        start = m.next;
+        if (start + (pages + 1) * EmuDevMem::kPage > EmuDevMem::kBase + EmuDevMem::kSpan) return hipErrorOutOfMemory;
+        m.next = start + (pages + 1) * EmuDevMem::kPage;  // + the guard page after it
+    }
+    if (mprotect((void*)start, pages * EmuDevMem::kPage, PROT_READ | PROT_WRITE) != 0) return hipErrorOutOfMemory;
+    const uintptr_t end = start + pages * EmuDevMem::kPage;
+    const uintptr_t q = end - body;
+    std::memset((void*)q, 0xFF, body);
+    if (__asan_poison_memory_region) {
+        if (q > start) __asan_poison_memory_region((void*)start, q - start);
+        if (body > n) __asan_poison_memory_region((void*)(q + n), body - n);
+    }
+    {
+        std::lock_guard<std::mutex> g(m.mu);
+        m.live[q] = n;
+    }
+    *p = (void*)q;
     return hipSuccess;
 }
 template <class T> inline hipError_t hipMalloc(T** p, size_t bytes) { return hipMalloc(reinterpret_cast<void**>(p), bytes); }
-inline hipError_t hipFree(void* p) { std::free(p); return hipSuccess; }
-inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind) { if (n) std::memmove(d, s, n); return hipSuccess; }
+inline hipError_t hipFree(void* p) {
+    emu_no_capture("hipFree");
+    if (!p) return hipSuccess;
+    EmuDevMem& m = emu_dev();
+    size_t n;
+    {
+        std::lock_guard<std::mutex> g(m.mu);
+        auto it = m.live.find((uintptr_t)p);
+        if (it == m.live.end()) emu_fault("hipFree of an address that is not a live allocation", p, 0);
+        n = it->second;
+        m.live.erase(it);
+    }
+    const uintptr_t lo = (uintptr_t)p & ~(uintptr_t)(EmuDevMem::kPage - 1);
+    const uintptr_t hi = ((uintptr_t)p + ((n + 255) & ~(size_t)255) + EmuDevMem::kPage - 1) & ~(uintptr_t)(EmuDevMem::kPage - 1);
+    madvise((void*)lo, hi - lo, MADV_DONTNEED);
+    mprotect((void*)lo, hi - lo, PROT_NONE);
+    return hipSuccess;
+}
+inline hipError_t hipMemcpy(void* d, const void* s, size_t n, hipMemcpyKind k) {
+    emu_no_capture("synchronous hipMemcpy");
+    if (n == 0) return hipSuccess;
+    switch (k) {
+        case hipMemcpyHostToDevice: emu_check_dev(d, n, "host-to-device copy into non-device memory");
+                                    emu_check_host(s, n, "host-to-device copy from device memory"); break;
+        case hipMemcpyDeviceToHost: emu_check_dev(s, n, "device-to-host copy from non-device memory");
+                                    emu_check_host(d, n, "device-to-host copy into device memory"); break;
+        case hipMemcpyDeviceToDevice: emu_check_dev(d, n, "device-to-device copy into non-device memory");
+                                      emu_check_dev(s, n, "device-to-device copy from non-device memory"); break;
+        default: emu_check_host(d, n, "host copy into device memory"); emu_check_host(s, n, "host copy from device memory");
+    }
+    std::memmove(d, s, n);
+    return hipSuccess;
+}
 inline hipError_t hipMemcpyAsync(void* d, const void* s, size_t n, hipMemcpyKind k, hipStream_t = nullptr) {
+    if (EmuGraph* g = emu_capture()) {
+        if (k == hipMemcpyHostToDevice || k == hipMemcpyDeviceToHost) {
+            std::fprintf(stderr, "emu: host copy recorded into a graph\n");
+            std::abort();
+        }
+        g->nodes.push_back([=]() {
+            EmuGraph* saved = emu_capture();
+            emu_capture() = nullptr;
+            hipMemcpy(d, s, n, k);
+            emu_capture() = saved;
+        });
+        return hipSuccess;
+    }
     return hipMemcpy(d, s, n, k);
 }
-inline hipError_t hipMemset(void* d, int v, size_t n) { std::memset(d, v, n); return hipSuccess; }
+inline hipError_t hipMemset(void* d, int v, size_t n) {
+    emu_no_capture("hipMemset (a memset node)");
+    emu_check_dev(d, n, "memset of non-device memory");
+    std::memset(d, v, n);
+    return hipSuccess;
+}
 inline hipError_t hipMemsetAsync(void* d, int v, size_t n, hipStream_t = nullptr) { return hipMemset(d, v, n); }
+
+// A kernel argument word that is a mapped host address outside the device range. Words are
+// read at 8-byte alignment from the argument tuple; ints and floats that happen to look like
+// a canonical address are only reported when that address is mapped, which in practice means
+// a real host pointer.
+inline bool emu_host_address(uint64_t w) {
+    if (w < 0x10007fff8000ull || w >= 0x800000000000ull) return false;  // below ASan's high memory
+    if (emu_dev().inside((uintptr_t)w)) return false;
+    const uintptr_t pg = (uintptr_t)w & ~(uintptr_t)4095;
+    return msync((void*)pg, 4096, MS_ASYNC) == 0;
+}
+
 inline hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) { *s = new EmuStream(); return hipSuccess; }
 inline hipError_t hipStreamCreate(hipStream_t* s) { return hipStreamCreateWithFlags(s, 0); }
 inline hipError_t hipStreamDestroy(hipStream_t s) { delete s; return hipSuccess; }
-inline hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
-inline hipError_t hipDeviceSynchronize() { return hipSuccess; }
+inline hipError_t hipStreamSynchronize(hipStream_t) { emu_no_capture("hipStreamSynchronize"); return hipSuccess; }
+inline hipError_t hipDeviceSynchronize() { emu_no_capture("hipDeviceSynchronize"); return hipSuccess; }
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) {
+    emu_no_capture("hipStreamBeginCapture");
+    emu_capture() = new EmuGraph();
+    return hipSuccess;
+}
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t* g) {
+    *g = emu_capture();
+    emu_capture() = nullptr;
+    return *g ? hipSuccess : hipErrorInvalidValue;
+}
+inline hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, void*, char*, size_t) {
+    *e = new EmuGraph(*g);
+    return hipSuccess;
+}
+inline hipError_t hipGraphDestroy(hipGraph_t g) { delete g; return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t e) { delete e; return hipSuccess; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t) {
+    emu_no_capture("hipGraphLaunch");
+    for (auto& f : e->nodes) f();
+    return hipSuccess;
+}
 inline hipError_t hipEventCreate(hipEvent_t* e) { *e = new EmuEvent(); return hipSuccess; }
+enum { hipEventDefault = 0, hipEventDisableTiming = 2 };
+inline hipError_t hipEventCreateWithFlags(hipEvent_t* e, unsigned) { return hipEventCreate(e); }
+enum { hipHostMallocDefault = 0 };
+inline hipError_t hipHostMalloc(void** p, size_t bytes, unsigned = 0) {
+    *p = std::malloc(bytes ? bytes : 1);
+    return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+inline hipError_t hipHostFree(void* p) { std::free(p); return hipSuccess; }
+enum hipDeviceAttribute_t { hipDeviceAttributeMultiprocessorCount = 63 };
+inline hipError_t hipDeviceGetAttribute(int* v, hipDeviceAttribute_t, int) { *v = 256; return hipSuccess; }
 inline hipError_t hipEventDestroy(hipEvent_t e) { delete e; return hipSuccess; }
 inline hipError_t hipEventRecord(hipEvent_t e, hipStream_t = nullptr) { e->t = std::chrono::steady_clock::now(); return hipSuccess; }
-inline hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+inline hipError_t hipEventSynchronize(hipEvent_t) { emu_no_capture("hipEventSynchronize"); return hipSuccess; }
 inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t a, hipEvent_t b) {
     *ms = std::chrono::duration<float, std::milli>(b->t - a->t).count();
     return hipSuccess;
@@ -401,9 +638,44 @@ inline void emu_run_block(EmuBlock& b, const char* name) {
     }
 }
 
+// Pointer arguments are checked directly; struct arguments word by word at 8-byte
+// alignment (their padding is copied from initialised objects in this code base).
+template <class T> void emu_check_arg(const char* name, int k, const T& a) {
+    if constexpr (std::is_pointer_v<T>) {
+        if (a && emu_host_address((uint64_t)(uintptr_t)a)) {
+            std::fprintf(stderr, "emu: kernel %s argument %d is host address %p\n", name, k, (const void*)a);
+            emu_fault("host pointer passed to a kernel", (const void*)a, 8);
+        }
+    } else if constexpr (std::is_class_v<T> && sizeof(T) >= 8 && alignof(T) >= 8) {
+        uint64_t w[sizeof(T) / 8];
+        std::memcpy(w, &a, sizeof(w));
+        for (size_t i = 0; i < sizeof(T) / 8; i++)
+            if (emu_host_address(w[i])) {
+                std::fprintf(stderr, "emu: kernel %s argument %d bytes [%zu, %zu) hold host address %#llx\n", name, k,
+                             8 * i, 8 * i + 8, (unsigned long long)w[i]);
+                emu_fault("host pointer passed to a kernel", (void*)w[i], 8);
+            }
+    }
+}
+
+template <class... KArgs>
+void emu_launch_impl(const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t shmem,
+                     const std::tuple<std::decay_t<KArgs>...>& params);
+
 template <class... KArgs, class... Args>
 void emu_launch(const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t shmem, hipStream_t,
                 Args&&... args) {
+    std::tuple<std::decay_t<KArgs>...> params(static_cast<std::decay_t<KArgs>>(args)...);
+    if (EmuGraph* g = emu_capture()) {
+        g->nodes.push_back([=]() { emu_launch_impl(name, kernel, grid, block, shmem, params); });
+        return;
+    }
+    emu_launch_impl(name, kernel, grid, block, shmem, params);
+}
+
+template <class... KArgs>
+void emu_launch_impl(const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 block, size_t shmem,
+                     const std::tuple<std::decay_t<KArgs>...>& params) {
     emu_launch_count()++;
     static const bool trace = std::getenv("EMU_TRACE") != nullptr;
     const auto t_start = std::chrono::steady_clock::now();
@@ -414,7 +686,10 @@ void emu_launch(const char* name, void (*kernel)(KArgs...), dim3 grid, dim3 bloc
     }
     const unsigned long long nblocks = (unsigned long long)grid.x * grid.y * grid.z;
     if (nblocks == 0) return;
-    std::tuple<std::decay_t<KArgs>...> params(static_cast<std::decay_t<KArgs>>(args)...);
+    {
+        int k = 0;
+        std::apply([&](const auto&... a) { (emu_check_arg(name, k++, a), ...); }, params);
+    }
     std::vector<unsigned char> dyn(shmem ? shmem : 1);
     std::vector<char*>& stacks = emu_stacks().v;
     while (stacks.size() < nthreads) stacks.push_back(static_cast<char*>(std::malloc(kEmuStack)));

@@ -489,3 +489,32 @@ def test_eval_metrics_pins():
     ref = _ssim_fp64_scipy(img, gt)
     assert abs(s[2] - ref) / abs(ref) < 2e-5, (s[2], ref)
     assert abs(s[0] - ((img.astype(np.float64) - gt / 255.0) ** 2).sum()) / s[0] < 1e-6
+
+
+def test_batch_step_restatement_pins():
+    """The oracle's multi-view batched step: a one-view batch is the plain step, and a batch
+    of the same view twice (each weighted 1/2, an exact halving) is the plain step too, bit
+    for bit; a two-view batch differs from either single view."""
+    g = scene.synth_gaussians(0, 4, 600, 3)
+    hidden = scene.synth_gaussians(0, 5, 400, 3)
+    c2w = scene.synth_cameras(9, 2, 3.0, 0.0)
+    W, H = 64, 48
+    cams = [scene.prepare_camera(c, 64.0, 64.0, W / 2.0, H / 2.0, W, H) for c in c2w]
+    gts = [scene.quantize(O.render(O.HostState(hidden), c, 3, scene.DEFAULT_BG, want_ids=False)["img"])
+           for c in cams]
+    hp = scene.step_params(1, scene.LR_INIT, W, H, 1, 3, 3)
+    plain = O.HostState(g)
+    O.train_step(plain, cams[0], gts[0], hp, want_ids=False)
+    one = O.HostState(g)
+    O.train_step_batch(one, [cams[0]], [gts[0]], [hp])
+    hp2 = scene.step_params(1, scene.LR_INIT, W, H, 1, 3, 3)
+    hp2.loss_inv_n = float(np.float32(1.0) / np.float32(W * H * 3 * 2))
+    twice = O.HostState(g)
+    O.train_step_batch(twice, [cams[0], cams[0]], [gts[0], gts[0]], [hp2, hp2])
+    both = O.HostState(g)
+    O.train_step_batch(both, cams, gts, [hp2, hp2])
+    for k in O.GROUP_SHAPES:
+        assert np.array_equal(one.p[k], plain.p[k]), k
+        assert np.array_equal(one.m[k], plain.m[k]) and np.array_equal(one.v[k], plain.v[k]), k
+        assert np.array_equal(twice.p[k], plain.p[k]), k
+    assert not np.array_equal(both.p["means"], plain.p["means"])
