@@ -71,6 +71,26 @@ MsplatStats msplat_trainer_step_batch(MsplatTrainer t, int views);
 /* Cameras (train indices) and per-view losses of the most recent batched step; returns the
  * number of views, or -1. Either output may be NULL. Waits for the device when losses is set. */
 int msplat_trainer_last_batch(MsplatTrainer t, int32_t* cameras, double* losses, int max_views);
+/* ---- Learned relocation proposal (no counterpart in the reference) -------------------
+ * msplat_trainer_set_option(t, "learned_relocation", 1) starts gathering per-Gaussian
+ * statistics and training the proposal network on the outcomes of every relocation event
+ * (plain ones included); "relocation_mode" 0 lets the held-out-loss judge pick plain or
+ * learned per event (learned only while its held-out loss change is no worse), 1 forces
+ * plain, 2 forces learned once the network has trained. out = {events, learned events,
+ * plain events, training samples, network logistic loss on the newest unseen outcomes,
+ * best constant predictor's, network offset loss on them, best constant offset's, training
+ * rounds, last plain held-out change, last learned held-out change, last held-out loss}.
+ * Returns 0, or -1 when the proposal is off. */
+int msplat_trainer_proposal_info(MsplatTrainer t, double out[12]);
+/* Reads (set = 0) or replaces (set = 1) the network's 5508 fp32 parameters (layout in
+ * msplat_kernels.h). Returns 0 / -1. */
+int msplat_trainer_proposal_params(MsplatTrainer t, float* theta, int set);
+/* Copies a proposal buffer to host memory: "out" ([N 4] network outputs of the last learned
+ * event), "features" ([N 16] bfloat16 of the last event), "stats" ([5 N]: visible views,
+ * sum |grad2d|, largest radius, depth sum, depth square sum since the last event). Copies
+ * min(size, bytes); returns the buffer's size in bytes, or -1. host may be NULL. */
+int64_t msplat_trainer_proposal_read(MsplatTrainer t, const char* name, void* host, int64_t bytes);
+
 /* Recorded-graph counters of the batched steps: out = {graphs recorded, steps replayed from
  * a recorded graph, launch width, launch height}. Batched steps are recorded once per launch
  * configuration and replayed when msplat_trainer_set_option(t, "batch_graph", 1) is set.

@@ -172,6 +172,13 @@ def bind(path: str) -> C.CDLL:
         "msplat_trainer_step_batch": ([vp, C.c_int], MsplatStats),
         "msplat_trainer_last_batch": ([vp, C.POINTER(i32), C.POINTER(f64), C.c_int], C.c_int),
         "msplat_trainer_batch_info": ([vp, C.POINTER(C.c_int64)], C.c_int),
+        "msplat_trainer_proposal_info": ([vp, C.POINTER(f64)], C.c_int),
+        "msplat_trainer_proposal_params": ([vp, fp, C.c_int], C.c_int),
+        "msplat_trainer_proposal_read": ([vp, C.c_char_p, vp, C.c_int64], C.c_int64),
+        "msplat_mlp_packed_bytes": ([], C.c_size_t),
+        "msplat_mlp_pack": ([vp, vp], C.c_int),
+        "msplat_mlp_forward": ([C.c_int, vp, vp, vp], C.c_int),
+        "msplat_mlp_backward": ([C.c_int, vp, vp, vp, vp, vp], C.c_int),
         "msplat_trainer_last_loss": ([vp], f64),
         "msplat_trainer_read_group": ([vp, C.c_int, C.c_int, fp, i64], i64),
         "msplat_trainer_run_mcmc": ([vp, C.c_int, C.POINTER(i32)], C.c_int),

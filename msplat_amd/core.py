@@ -220,6 +220,38 @@ class GaussianTrainer:
         L.msplat_trainer_last_batch(self._h, cams, ls if losses else None, n)
         return [int(c) for c in cams[:n]], ([float(x) for x in ls[:n]] if losses else None)
 
+    def proposal_info(self) -> dict:
+        """Counters of the learned relocation proposal (msplat_trainer_proposal_info)."""
+        out = (C.c_double * 12)()
+        if _lib.lib().msplat_trainer_proposal_info(self._h, out) != 0:
+            raise RuntimeError("learned relocation is not enabled")
+        keys = ("events", "learned_events", "plain_events", "samples", "val_bce", "base_bce", "val_off", "base_off",
+                "trained", "delta_plain", "delta_learned", "heldout_loss")
+        return {k: float(v) for k, v in zip(keys, out)}
+
+    def proposal_params(self, theta=None):
+        """The proposal network's parameters; replaces them when `theta` is given."""
+        L = _lib.lib()
+        if theta is not None:
+            th = np.ascontiguousarray(theta, np.float32)
+            if L.msplat_trainer_proposal_params(self._h, fptr(th), 1) != 0:
+                raise RuntimeError("could not set the proposal parameters")
+            return th
+        th = np.zeros(5508, np.float32)
+        if L.msplat_trainer_proposal_params(self._h, fptr(th), 0) != 0:
+            raise RuntimeError("learned relocation is not enabled")
+        return th
+
+    def proposal_read(self, name: str, dtype=np.float32) -> np.ndarray:
+        """A proposal buffer ("out", "features" as uint16, "stats") as a flat array."""
+        L = _lib.lib()
+        n = L.msplat_trainer_proposal_read(self._h, name.encode(), None, 0)
+        if n < 0:
+            raise RuntimeError(f"proposal buffer {name!r} unavailable")
+        out = np.zeros(n // np.dtype(dtype).itemsize, dtype)
+        L.msplat_trainer_proposal_read(self._h, name.encode(), out.ctypes.data, n)
+        return out
+
     def batch_info(self) -> dict:
         """Recorded-graph counters of the batched steps (msplat_trainer_batch_info)."""
         out = (C.c_int64 * 4)()
@@ -312,7 +344,9 @@ class GaussianTrainer:
     def set_option(self, name: str, value) -> None:
         """Options beyond TrainingConfig: "sh_bf16" (bfloat16 SH coefficients and moments),
         "batch_graph" (record batched steps as HIP graphs and replay them), "persist_grid"
-        (resident workgroups of the persistent raster kernels, 0 = default)."""
+        (resident workgroups of the persistent raster kernels, 0 = default),
+        "learned_relocation" (train and use the relocation proposal network),
+        "relocation_mode" (0 judged by held-out loss, 1 plain, 2 learned)."""
         if _lib.lib().msplat_trainer_set_option(self._h, name.encode(), float(value)) != 0:
             raise RuntimeError(f"option {name!r} rejected (reason on stderr)")
 

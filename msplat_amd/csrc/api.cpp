@@ -168,6 +168,16 @@ struct TrainerImpl {
     std::vector<int> last_batch;
     MsplatEngine handle{nullptr, false};
     DBuf rgba, partials, sums;
+    // judge of the learned relocation proposal: held-out loss change over the interval that
+    // follows an event, per mode (0 plain, 1 learned); the learned mode is used only while
+    // its change is no worse than the plain one's, re-probed after an exponential back-off
+    struct Judge {
+        int force = 0;          // 0 judge, 1 always plain, 2 always learned
+        int prev_mode = -1;
+        double prev_loss = 0.0;
+        double delta[2] = {NAN, NAN};
+        int wait = 0, backoff = 1, plain_run = 0;
+    } judge;
 
     void shuffle() {
         for (int i = (int)order.size() - 1; i > 0; i--) {
@@ -202,6 +212,59 @@ static void metrics_for(TrainerImpl* T, const TrainCamera& cam, const float* img
     msk::eval_sums(s, img, cam.gt, cam.width, cam.height, window_1d, T->partials.as<double>(), T->sums.as<double>());
     hip_check(hipMemcpyAsync(out3, T->sums.p, 24, hipMemcpyDeviceToHost, s), "metrics");
     hip_check(hipStreamSynchronize(s), "metrics");
+}
+
+// Held-out objective: mean over the test cameras (or, without a test split, 8 evenly spaced
+// training cameras) of (1 - w) L1 + w (1 - SSIM) of the current state.
+static double heldout_loss(TrainerImpl* T) {
+    std::vector<const TrainCamera*> cams;
+    if (!T->ds->test.empty()) {
+        for (const TrainCamera& c : T->ds->test) cams.push_back(&c);
+    } else {
+        const int n = (int)T->ds->train.size(), k = std::min(n, 8);
+        for (int i = 0; i < k; i++) cams.push_back(&T->ds->train[(size_t)i * n / k]);
+    }
+    double acc = 0.0;
+    for (const TrainCamera* c : cams) {
+        const float* img = T->model->render(*c, T->step);
+        double s3[3];
+        metrics_for(T, *c, img, s3);
+        const double n = (double)c->width * c->height * 3;
+        const double w = T->cfg.ssimWeight;
+        acc += (1.0 - w) * (s3[1] / n) + w * (1.0 - s3[2] / n);
+    }
+    return cams.empty() ? 0.0 : acc / (double)cams.size();
+}
+
+// Chooses the relocation mode of the event at T->step.
+static void judge_event(TrainerImpl* T) {
+    TrainerImpl::Judge& J = T->judge;
+    const double L = heldout_loss(T);
+    if (J.prev_mode >= 0) J.delta[J.prev_mode] = L - J.prev_loss;
+    bool learned = false;
+    if (J.force == 2) {
+        learned = true;
+    } else if (J.force == 0 && T->model->proposal_ready()) {
+        if (J.wait > 0) {
+            J.wait--;
+        } else if (std::isnan(J.delta[1])) {
+            learned = true;                          // untried (or due for a re-probe)
+        } else if (std::isnan(J.delta[0]) || J.delta[1] <= J.delta[0]) {
+            learned = true;                          // no worse than plain
+            J.backoff = 1;
+            if (++J.plain_run >= 4) {                // refresh the plain figure now and then
+                learned = false;
+                J.plain_run = 0;
+            }
+        } else {                                     // worse: plain, re-probe after a back-off
+            J.wait = J.backoff;
+            J.backoff = std::min(J.backoff * 2, 16);
+            J.delta[1] = NAN;
+        }
+    }
+    T->model->set_proposal_mode(learned);
+    J.prev_mode = learned ? 1 : 0;
+    J.prev_loss = L;
 }
 
 static MsplatPixelBuffer host_image(TrainerImpl* T, const float* dimg, int W, int H) {
@@ -429,6 +492,7 @@ MsplatStats msplat_trainer_step(MsplatTrainer t) {
     const auto t0 = std::chrono::high_resolution_clock::now();
     T->model->full_iteration(cam, T->step, T->cfg.ssimWeight);
     T->model->schedulers_step(T->step);
+    if (T->model->proposal_enabled() && T->model->is_mcmc_step(T->step)) judge_event(T);
     T->model->mcmc_after_train(T->step);
     const auto t1 = std::chrono::high_resolution_clock::now();
     s.iteration = T->step;
@@ -455,6 +519,7 @@ MsplatStats msplat_trainer_step_batch(MsplatTrainer t, int views) {
     T->last_cam = T->last_batch.back();
     T->model->full_iteration_batch(cams.data(), views, T->step, T->cfg.ssimWeight);
     T->model->schedulers_step(T->step);
+    if (T->model->proposal_enabled() && T->model->is_mcmc_step(T->step)) judge_event(T);
     T->model->mcmc_after_train(T->step);
     const auto t1 = std::chrono::high_resolution_clock::now();
     s.iteration = T->step;
@@ -476,6 +541,33 @@ int msplat_trainer_last_batch(MsplatTrainer t, int32_t* cameras, double* losses,
         if (losses && v < (int)l.size()) losses[v] = l[v];
     }
     return n;
+    MS_GUARD_END(-1)
+}
+
+int msplat_trainer_proposal_info(MsplatTrainer t, double out[12]) {
+    if (!t || !out) return -1;
+    TrainerImpl* T = TR(t);
+    const msplat_hip::ProposalInfo& p = T->model->proposal_info();
+    const double v[12] = {(double)p.events, (double)p.learned_events, (double)p.plain_events, (double)p.samples,
+                          p.val_bce, p.base_bce, p.val_off, p.base_off, (double)p.trained, T->judge.delta[0],
+                          T->judge.delta[1], T->judge.prev_loss};
+    for (int k = 0; k < 12; k++) out[k] = v[k];
+    return T->model->proposal_enabled() ? 0 : -1;
+}
+
+int msplat_trainer_proposal_params(MsplatTrainer t, float* theta, int set) {
+    if (!t || !theta) return -1;
+    MS_GUARD_BEGIN
+    if (set) TR(t)->model->set_proposal_params(theta);
+    else TR(t)->model->proposal_params(theta);
+    return 0;
+    MS_GUARD_END(-1)
+}
+
+int64_t msplat_trainer_proposal_read(MsplatTrainer t, const char* name, void* host, int64_t bytes) {
+    if (!t || !name) return -1;
+    MS_GUARD_BEGIN
+    return TR(t)->model->proposal_read(name, host, bytes);
     MS_GUARD_END(-1)
 }
 
@@ -639,6 +731,16 @@ int msplat_trainer_set_option(MsplatTrainer t, const char* name, double value) {
     const std::string n(name);
     if (n == "sh_bf16") {
         TR(t)->model->set_sh_bf16(value != 0.0);
+        return 0;
+    }
+    if (n == "learned_relocation") {
+        TR(t)->model->enable_proposal(value != 0.0, 0x51A7C0DEull);
+        return 0;
+    }
+    if (n == "relocation_mode") {
+        if (value != 0.0 && value != 1.0 && value != 2.0) throw std::invalid_argument("relocation_mode is 0, 1 or 2");
+        TR(t)->judge.force = (int)value;
+        TR(t)->model->set_proposal_mode(value == 2.0);
         return 0;
     }
     if (n == "batch_graph") {

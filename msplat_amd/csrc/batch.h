@@ -40,6 +40,8 @@ public:
     int64_t captures() const { return captures_; }
     // Tile-count geometry the recorded graph is launched with (grows monotonically).
     msk::Geom launch_geom() const { return hw_; }
+    // Engine of view slot v of the last step.
+    Engine* slot(int v) { return v >= 0 && v < (int)slots_.size() ? slots_[v].get() : nullptr; }
 
 private:
     struct Key {

@@ -93,6 +93,10 @@ public:
     void raster_bwd_own();
     msk::RasterView raster_view(bool train) const;
     int64_t generation() const { return gen_; }
+    // This view's projection outputs and raster gradients (valid after its backward).
+    const int32_t* radii_dev() const { return radii_.as<int32_t>(); }
+    const float* depths_dev() const { return depths_.as<float>(); }
+    const float2* v_xy_dev() const { return v_xy_.as<float2>(); }
     void set_persist_grid(int g) { persist_grid_ = g; }
     int persist_grid() const { return persist_grid_; }
 

@@ -637,7 +637,7 @@ static int hist_blocks(int N) {
 // digit histograms, one scan of the digit-major (digit, workgroup) count matrix, and a
 // scatter that ranks each item among equal digits of earlier items with wave ballots. Only
 // kernels: no memset or host synchronisation, so a step that uses it can be captured.
-constexpr int RS_THREADS = 256, RS_ITEMS = 16, RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_THREADS = 256, RS_ITEMS = 4, RS_TILE = RS_THREADS * RS_ITEMS;
 
 static int rs_blocks(int N) { return (N + RS_TILE - 1) / RS_TILE; }
 
@@ -647,7 +647,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(int N, const uint32_t* _
     h[threadIdx.x] = 0u;
     __syncthreads();
     const int base = blockIdx.x * RS_TILE + threadIdx.x;
-#pragma unroll 4
+#pragma unroll
     for (int k = 0; k < RS_ITEMS; k++) {
         const int i = base + k * RS_THREADS;
         if (i < N) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
@@ -656,30 +656,38 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(int N, const uint32_t* _
     counts[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// In-place exclusive scan of n counts by one workgroup of 1024.
-__global__ __launch_bounds__(1024) void k_rs_scan(uint32_t* __restrict__ counts, int n) {
-    __shared__ uint32_t lw[17];
-    const int per = (n + 1023) / 1024;
-    const int b0 = threadIdx.x * per, b1 = min(n, b0 + per);
+// Per digit d (one workgroup each): exclusive scan of its row of per-workgroup counts, in
+// place, and the digit's total.
+__global__ __launch_bounds__(RS_THREADS) void k_rs_rowscan(uint32_t* __restrict__ counts, int nblk,
+                                                           uint32_t* __restrict__ rowtot) {
+    __shared__ uint32_t lw[RS_THREADS / 64 + 1];
+    uint32_t* row = counts + (size_t)blockIdx.x * nblk;
+    const int per = (nblk + RS_THREADS - 1) / RS_THREADS;
+    const int b0 = threadIdx.x * per, b1 = min(nblk, b0 + per);
     uint32_t sum = 0;
-    for (int i = b0; i < b1; i++) sum += counts[i];
+    for (int i = b0; i < b1; i++) sum += row[i];
     uint32_t tot;
     uint32_t run = block_exclusive_scan(sum, lw, &tot);
     for (int i = b0; i < b1; i++) {
-        const uint32_t v = counts[i];
-        counts[i] = run;
+        const uint32_t v = row[i];
+        row[i] = run;
         run += v;
     }
+    if (threadIdx.x == 0) rowtot[blockIdx.x] = tot;
 }
 
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
                                                            uint32_t* __restrict__ vout, int shift,
-                                                           const uint32_t* __restrict__ counts, int nblk) {
+                                                           const uint32_t* __restrict__ counts,
+                                                           const uint32_t* __restrict__ rowtot, int nblk) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
+    __shared__ uint32_t lw[RS_THREADS / 64 + 1];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    run[threadIdx.x] = counts[(size_t)threadIdx.x * nblk + blockIdx.x];
+    uint32_t tot;
+    const uint32_t dbase = block_exclusive_scan(rowtot[threadIdx.x], lw, &tot);
+    run[threadIdx.x] = dbase + counts[(size_t)threadIdx.x * nblk + blockIdx.x];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int r = 0; r < RS_ITEMS; r++) {
 #pragma unroll
@@ -713,7 +721,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
     }
 }
 
-size_t lists_temp_bytes(int N) { return ((size_t)256 * (size_t)rs_blocks(N > 0 ? N : 1) + 64) * 4; }
+size_t lists_temp_bytes(int N) { return ((size_t)256 * (size_t)rs_blocks(N > 0 ? N : 1) + 256 + 64) * 4; }
 
 size_t lists_hist_bytes(int N, int T) { return (size_t)hist_blocks(N) * (size_t)T * 4; }
 
@@ -739,11 +747,13 @@ void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t
     if (n <= 0) return;
     const int nblk = rs_blocks(n);
     uint32_t* counts = reinterpret_cast<uint32_t*>(temp);
+    uint32_t* rowtot = counts + (size_t)256 * nblk;
     for (int pass = 0; pass < passes; pass++) {
         const int shift = 8 * pass;
         hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, s, n, ka, shift, counts, nblk);
-        hipLaunchKernelGGL(k_rs_scan, dim3(1), dim3(1024), 0, s, counts, 256 * nblk);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, s, n, ka, va, kb, vb, shift, counts, nblk);
+        hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(RS_THREADS), 0, s, counts, nblk, rowtot);
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, s, n, ka, va, kb, vb, shift, counts, rowtot,
+                           nblk);
         std::swap(ka, kb);
         std::swap(va, vb);
     }

@@ -33,12 +33,21 @@ __device__ __forceinline__ uint64_t reloc_weight(float logit) {
     return (uint64_t)((double)p * 4294967296.0);
 }
 
+// Learned proposal weight: sigmoid(o) exp(clamp(z, -8, 8)) in 2^-20 units (fits u64 sums
+// over 2^31 rows).
+__device__ __forceinline__ uint64_t learned_weight(float logit, float z) {
+    const float p = 1.0f / (1.0f + ms_expf(-logit));
+    if (!ms_finite(p) || !(p > 0.0f) || !ms_finite(z)) return 0;
+    const float e = ms_expf(fminf(fmaxf(z, -8.0f), 8.0f));
+    return (uint64_t)((double)p * (double)e * 1048576.0);
+}
+
 // dead = non-finite parameter, sigmoid(opacity) <= 0.005, or |mean|^2 > r^2 (model.cpp:521-558)
 __global__ void k_classify(int N, int nr, const float* __restrict__ means, const float* __restrict__ scales,
                            const float* __restrict__ quats, const float* __restrict__ f_dc,
                            const float* __restrict__ f_rest, const float* __restrict__ opac, float r2,
-                           uint32_t* __restrict__ dead, uint32_t* __restrict__ alive, uint64_t* __restrict__ w,
-                           uint32_t* __restrict__ counters) {
+                           const float* __restrict__ mlp_out, uint32_t* __restrict__ dead,
+                           uint32_t* __restrict__ alive, uint64_t* __restrict__ w, uint32_t* __restrict__ counters) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     bool fin = ms_finite(opac[i]);
@@ -58,7 +67,7 @@ __global__ void k_classify(int N, int nr, const float* __restrict__ means, const
     }
     dead[i] = is_dead ? 1u : 0u;
     alive[i] = is_dead ? 0u : 1u;
-    w[i] = is_dead ? 0ull : reloc_weight(opac[i]);
+    w[i] = is_dead ? 0ull : (mlp_out ? learned_weight(opac[i], mlp_out[4 * (size_t)i]) : reloc_weight(opac[i]));
 }
 
 __global__ void k_compact(int N, const uint32_t* __restrict__ dead, const uint32_t* __restrict__ dpos,
@@ -160,7 +169,9 @@ __global__ void k_eq9_sources(int ncand, const uint32_t* __restrict__ cand_list,
 }
 
 __global__ void k_copy_rows(int num, const uint32_t* __restrict__ dst_list, int dst_base,
-                            const uint32_t* __restrict__ cand_list, const uint32_t* __restrict__ chosen, Rows R) {
+                            const uint32_t* __restrict__ cand_list, const uint32_t* __restrict__ chosen, Rows R,
+                            const float* __restrict__ mlp_out, uint32_t* __restrict__ pair_src,
+                            uint32_t* __restrict__ pair_dst) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= num) return;
     const int dst = dst_list ? (int)dst_list[j] : dst_base + j;
@@ -168,6 +179,19 @@ __global__ void k_copy_rows(int num, const uint32_t* __restrict__ dst_list, int 
     for (int g = 0; g < 8; g++)
         if (R.pb[g]) copy_row(R.p[g], R.pb[g], dst, src);
     zero_moments(R, dst);
+    if (mlp_out) {  // learned placement: offset in units of the copy's largest scale
+        float* mu = reinterpret_cast<float*>(R.p[0]) + 3 * (size_t)dst;
+        const float* ls = reinterpret_cast<const float*>(R.p[1]) + 3 * (size_t)dst;
+        const float s = ms_expf(fmaxf(fmaxf(ls[0], ls[1]), ls[2]));
+        for (int k = 0; k < 3; k++) {
+            const float o = mlp_out[4 * (size_t)src + 1 + k];
+            mu[k] = fmaf(s, ms_finite(o) ? fminf(fmaxf(o, -3.0f), 3.0f) : 0.0f, mu[k]);
+        }
+    }
+    if (pair_src) {
+        pair_src[j] = (uint32_t)src;
+        pair_dst[j] = (uint32_t)dst;
+    }
 }
 
 // Eq. 9 on explicit (opacity, count) pairs, for the known-answer tests.
@@ -231,7 +255,8 @@ static Rows make_rows(const Params& P) {
 // Samples `num` candidates from weights w[0..ncand) and applies Eq. 9 + copies.
 static bool draw_and_apply(hipStream_t s, McmcScratch& S, const Rows& R, uint64_t* w, int ncand, int num,
                            const uint32_t* cand_list, const uint32_t* dst_list, int dst_base, uint32_t seed_base,
-                           uint32_t& count) {
+                           uint32_t& count, const float* mlp_out = nullptr, uint32_t* pair_src = nullptr,
+                           uint32_t* pair_dst = nullptr) {
     uint64_t* cdf = grow<uint64_t>(S, 8, (size_t)ncand);
     size_t tb = 0;
     check(hipcub::DeviceScan::InclusiveSum(nullptr, tb, w, cdf, ncand, s), "scan");
@@ -247,13 +272,16 @@ static bool draw_and_apply(hipStream_t s, McmcScratch& S, const Rows& R, uint64_
     fill_zero(s, cnt, (size_t)ncand * 4);
     hipLaunchKernelGGL(k_sample, dim3(cdiv2(num, 256)), dim3(256), 0, s, num, seed, ncand, cdf, chosen, cnt);
     hipLaunchKernelGGL(k_eq9_sources, dim3(cdiv2(ncand, 256)), dim3(256), 0, s, ncand, cand_list, cnt, R);
-    hipLaunchKernelGGL(k_copy_rows, dim3(cdiv2(num, 256)), dim3(256), 0, s, num, dst_list, dst_base, cand_list, chosen, R);
+    hipLaunchKernelGGL(k_copy_rows, dim3(cdiv2(num, 256)), dim3(256), 0, s, num, dst_list, dst_base, cand_list, chosen, R,
+                       mlp_out, pair_src, pair_dst);
     check(hipGetLastError(), "launch");
     return true;
 }
 
 McmcInfo mcmc_relocate_and_grow(hipStream_t s, const Params& P, int cap_max, float cull_radius, uint32_t& count,
-                                McmcScratch& S) {
+                                McmcScratch& S, ReloHooks* hooks) {
+    const float* mlp_out = hooks ? hooks->mlp_out : nullptr;
+    if (hooks) hooks->pairs = 0;
     McmcInfo info;
     const int N = P.N;
     info.num_active = N;
@@ -269,7 +297,7 @@ McmcInfo mcmc_relocate_and_grow(hipStream_t s, const Params& P, int cap_max, flo
     fill_zero(s, ctr, 16);
     const float r2 = cull_radius > 0.0f ? cull_radius * cull_radius : -1.0f;
     hipLaunchKernelGGL(k_classify, dim3(cdiv2(N, 256)), dim3(256), 0, s, N, nr, P.means, P.scales, P.quats, P.f_dc,
-                       P.f_rest, P.opac, r2, dead, alive, w, ctr);
+                       P.f_rest, P.opac, r2, mlp_out, dead, alive, w, ctr);
     size_t tb = 0;
     check(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, dead, dpos, N, s), "scan");
     void* tmp = grow<uint8_t>(S, 6, tb);
@@ -289,7 +317,11 @@ McmcInfo mcmc_relocate_and_grow(hipStream_t s, const Params& P, int cap_max, flo
     info.dead = D;
     info.nonfinite = (int)h[0];
     info.culled = (int)h[1];
-    if (D > 0 && A > 0) draw_and_apply(s, S, R, alive_w, A, D, alive_list, dead_list, 0, 0x9E3779B9u, count);
+    if (D > 0 && A > 0 &&
+        draw_and_apply(s, S, R, alive_w, A, D, alive_list, dead_list, 0, 0x9E3779B9u, count, mlp_out,
+                       hooks ? hooks->pair_src : nullptr, hooks ? hooks->pair_dst : nullptr) &&
+        hooks && hooks->pair_src)
+        hooks->pairs = D;
     // growth toward cap_max by ~5% (model.cpp:420-421)
     const int target = std::min(cap_max, (int)(1.05f * (float)N));
     const int num_new = target - N;

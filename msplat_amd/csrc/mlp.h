@@ -53,4 +53,24 @@ void mlp_backward(hipStream_t s, int N, const uint16_t* X, const float* targets,
 void mlp_adam(hipStream_t s, float* theta, float* m, float* v, const float* grad, float lr, float beta1, float beta2,
               float eps, float bc1, float bc2_sqrt, MlpPacked* w);
 
+// Training records of one relocation event: for each relocated row j, the features of its
+// source (before the event), its destination row, and the destination's state right after
+// the event (opacity logit, source mean, largest scale).
+struct ReloRecords {
+    uint16_t* X;      // [n 16]
+    uint32_t* dst;    // [n]
+    float* o;         // [n]
+    float* mu;        // [n 3]
+    float* s;         // [n]
+};
+void relo_record(hipStream_t s, int n, const uint32_t* pair_src, const uint32_t* pair_dst, const uint16_t* X,
+                 const Params& P, ReloRecords R);
+// Outcome of recorded relocations, n samples written to X_out [n 16] / T_out [n 8]:
+// y = 1 if the copy's opacity logit grew since the event; offset target = (mean now -
+// source mean) / scale, clamped to [-3, 3]; weights wl, wo.
+void relo_label(hipStream_t s, int n, const ReloRecords R, const Params& P, float wl, float wo, uint16_t* X_out,
+                float* T_out);
+// Sets the loss weights of samples [0, n).
+void relo_weights(hipStream_t s, int n, float* T, float wl, float wo);
+
 }  // namespace msk

@@ -108,6 +108,78 @@ void relo_features(hipStream_t s, const Params& P, const ReloStats& st, int view
                        P.f_dc, st, views > 0 ? 1.f / (float)views : 0.f, X);
 }
 
+__global__ void k_relo_record(int n, const uint32_t* __restrict__ pair_src, const uint32_t* __restrict__ pair_dst,
+                              const uint16_t* __restrict__ X, const float* __restrict__ means,
+                              const float* __restrict__ scales, const float* __restrict__ opac, ReloRecords R) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t src = pair_src[j], dst = pair_dst[j];
+    const uint4* xs = reinterpret_cast<const uint4*>(X) + 2 * (size_t)src;
+    uint4* xd = reinterpret_cast<uint4*>(R.X) + 2 * (size_t)j;
+    xd[0] = xs[0];
+    xd[1] = xs[1];
+    R.dst[j] = dst;
+    R.o[j] = opac[dst];
+    for (int k = 0; k < 3; k++) R.mu[3 * (size_t)j + k] = means[3 * (size_t)src + k];
+    const float* ls = scales + 3 * (size_t)dst;
+    R.s[j] = ms_expf(fmaxf(fmaxf(ls[0], ls[1]), ls[2]));
+}
+
+void relo_record(hipStream_t s, int n, const uint32_t* pair_src, const uint32_t* pair_dst, const uint16_t* X,
+                 const Params& P, ReloRecords R) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_relo_record, dim3(cdiv_i(n, 256)), dim3(256), 0, s, n, pair_src, pair_dst, X, P.means,
+                       P.scales, P.opac, R);
+}
+
+__global__ void k_relo_label(int n, const ReloRecords R, const float* __restrict__ means,
+                             const float* __restrict__ opac, float wl, float wo, uint16_t* __restrict__ X_out,
+                             float* __restrict__ T_out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const uint32_t d = R.dst[j];
+    const uint4* xs = reinterpret_cast<const uint4*>(R.X) + 2 * (size_t)j;
+    uint4* xd = reinterpret_cast<uint4*>(X_out) + 2 * (size_t)j;
+    xd[0] = xs[0];
+    xd[1] = xs[1];
+    const float on = opac[d];
+    float t[8];
+    t[0] = (ms_finite(on) && on > R.o[j]) ? 1.f : 0.f;
+    t[1] = wl;
+    const float inv = R.s[j] > 0.f ? 1.f / R.s[j] : 0.f;
+    bool ok = ms_finite(inv);
+    for (int k = 0; k < 3; k++) {
+        const float v = (means[3 * (size_t)d + k] - R.mu[3 * (size_t)j + k]) * inv;
+        ok = ok && ms_finite(v);
+        t[2 + k] = fminf(fmaxf(v, -3.f), 3.f);
+    }
+    if (!ok) t[2] = t[3] = t[4] = 0.f;
+    t[5] = ok ? wo : 0.f;
+    t[6] = t[7] = 0.f;
+    float4* to = reinterpret_cast<float4*>(T_out) + 2 * (size_t)j;
+    to[0] = make_float4(t[0], t[1], t[2], t[3]);
+    to[1] = make_float4(t[4], t[5], t[6], t[7]);
+}
+
+void relo_label(hipStream_t s, int n, const ReloRecords R, const Params& P, float wl, float wo, uint16_t* X_out,
+                float* T_out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_relo_label, dim3(cdiv_i(n, 256)), dim3(256), 0, s, n, R, P.means, P.opac, wl, wo, X_out,
+                       T_out);
+}
+
+__global__ void k_relo_weights(int n, float* __restrict__ T, float wl, float wo) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    T[8 * (size_t)j + 1] = wl;
+    if (T[8 * (size_t)j + 5] != 0.f) T[8 * (size_t)j + 5] = wo;
+}
+
+void relo_weights(hipStream_t s, int n, float* T, float wl, float wo) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_relo_weights, dim3(cdiv_i(n, 256)), dim3(256), 0, s, n, T, wl, wo);
+}
+
 // ------------------------------------------------------------------------------------
 // packing and Adam
 

@@ -110,6 +110,7 @@ Model::~Model() {
     rest_h_.release();
     for (auto& b : acc_) b.release();
     mcmc_.release();
+    prop_.release();
 }
 
 void Model::allocate(int degree) {
@@ -276,6 +277,7 @@ void Model::full_iteration(const TrainCamera& cam, int step, float ssim_weight) 
     const MsplatStepParams hp = step_params(cam, step, ssim_weight, os_.adam_step_count);
     const int rc = eng_.train_step(state(), c, cam.gt, hp, false);
     if (rc != 0) throw std::runtime_error("training step rejected its inputs (rc=" + std::to_string(rc) + ")");
+    if (prop_.enabled) proposal_accumulate(eng_);
 }
 
 void Model::full_iteration_batch(const TrainCamera* const* cams, int B, int step, float ssim_weight) {
@@ -305,6 +307,8 @@ void Model::full_iteration_batch(const TrainCamera* const* cams, int B, int step
     batch_->set_graph(batch_graph_);
     batch_->run(state(), views.data(), B, hp, A, eng_.nan_counts());
     batch_n_ = B;
+    if (prop_.enabled)
+        for (int v = 0; v < B; v++) proposal_accumulate(*batch_->slot(v));
 }
 
 std::vector<double> Model::batch_losses() {
@@ -318,11 +322,14 @@ void Model::schedulers_step(int step) {
     os_.adam_lr[0] = std::exp(std::log(os_.means_lr_init) * (1.f - t) + std::log(os_.means_lr_final) * t);
 }
 
+bool Model::is_mcmc_step(int step) const {
+    return step >= 500 && step % 100 == 0 && step < (int)((float)opt_.max_steps * 0.833f);
+}
+
 msk::McmcInfo Model::mcmc_after_train(int step) {
     msk::McmcInfo info;
     info.num_active = num_active_;
-    if (step < 500 || step % 100 != 0) return info;
-    if (step >= (int)((float)opt_.max_steps * 0.833f)) return info;
+    if (!is_mcmc_step(step)) return info;
     msk::Params P;
     const MsplatDeviceState st = state();
     P.means = st.means; P.scales = st.scales; P.quats = st.quats; P.f_dc = st.f_dc; P.f_rest = st.f_rest;
@@ -333,7 +340,11 @@ msk::McmcInfo Model::mcmc_after_train(int step) {
     P.sh_bf16 = st.sh_bf16;
     P.f_dc_h = st.f_dc_bf16;
     P.f_rest_h = st.f_rest_bf16;
-    info = msk::mcmc_relocate_and_grow(eng_.stream(), P, cap_, opt_.cull_radius, reloc_count_, mcmc_);
+    msk::ReloHooks hooks;
+    if (prop_.enabled) proposal_before_event(P, hooks);
+    info = msk::mcmc_relocate_and_grow(eng_.stream(), P, cap_, opt_.cull_radius, reloc_count_, mcmc_,
+                                       prop_.enabled ? &hooks : nullptr);
+    if (prop_.enabled) proposal_after_event(P, hooks);
     num_active_ = info.num_active;
     return info;
 }

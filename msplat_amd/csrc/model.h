@@ -10,6 +10,7 @@
 #include "batch.h"
 #include "engine.h"
 #include "mcmc.h"
+#include "proposal.h"
 
 namespace msplat_hip {
 
@@ -103,6 +104,23 @@ public:
     void set_batch_graph(bool on) { batch_graph_ = on; }
     bool batch_graph() const { return batch_graph_; }
     BatchRunner* batch_runner() { return batch_.get(); }
+    // ---- learned relocation proposal (no counterpart in the reference) ----
+    // Allocates its state and starts gathering statistics; relocation stays opacity-weighted
+    // until the network has trained on the outcomes of at least one event and the judge
+    // (the trainer) selects the learned mode.
+    void enable_proposal(bool on, uint64_t seed);
+    bool proposal_enabled() const { return prop_.enabled; }
+    void set_proposal_mode(bool learned) { prop_.use_learned = learned; }
+    bool proposal_ready() const { return prop_.ready(); }
+    const ProposalInfo& proposal_info() const { return prop_.info; }
+    void proposal_params(float* theta) const;
+    void set_proposal_params(const float* theta);
+    // Copies a proposal buffer to the host: "out" [N 4] network outputs of the last learned
+    // event, "features" [N 16] bf16 of the last event, "stats" [5 N] statistics gathered
+    // since the last event. Returns the buffer's size in bytes (copies min(size, bytes)).
+    int64_t proposal_read(const std::string& name, void* host, int64_t bytes) const;
+    // Model::mcmcAfterTrain's schedule (model.cpp:498-501): is `step` a relocation step?
+    bool is_mcmc_step(int step) const;
     // Resident workgroups of the batched persistent raster launches (0 = default).
     void set_persist_grid(int g) {
         persist_grid_ = g;
@@ -131,6 +149,10 @@ private:
     int batch_n_ = 0;
     int persist_grid_ = 0;
     msk::McmcScratch mcmc_;
+    Proposal prop_;
+    void proposal_accumulate(const Engine& e);
+    void proposal_before_event(const msk::Params& P, msk::ReloHooks& hooks);
+    void proposal_after_event(const msk::Params& P, const msk::ReloHooks& hooks);
 };
 
 // On-disk formats (reference save_gaussians.cpp:10-231, model.cpp:698-843).

@@ -165,6 +165,20 @@ void oracle_eq9(float o_old, const float* s_old, int n, float* o_new, float* s_n
 ORACLE_BATCH(F32, f32, float)
 ORACLE_BATCH(F64, f64, double)
 
+int oracle_mcmc_step_learned(OracleStateF32* s, int32_t cap_max, float cull_radius, uint32_t reloc_count_in,
+                             const float* mlp_out, int32_t* out_info) {
+    State<float> S = make_state<float>(s);
+    McmcInfo info;
+    mcmc_relocate_and_grow(S, cap_max, cull_radius, reloc_count_in, info, mlp_out);
+    out_info[0] = info.num_active;
+    out_info[1] = info.dead;
+    out_info[2] = info.grown;
+    out_info[3] = info.nonfinite;
+    out_info[4] = info.culled;
+    out_info[5] = (int32_t)info.reloc_count_out;
+    return 0;
+}
+
 int oracle_mcmc_step(OracleStateF32* s, int32_t cap_max, float cull_radius, uint32_t reloc_count_in, int32_t* out_info) {
     State<float> S = make_state<float>(s);
     McmcInfo info;

@@ -33,6 +33,15 @@ inline uint64_t reloc_weight(float logit) {
     return (uint64_t)((double)p * 4294967296.0);
 }
 
+// Learned proposal weight (this build's own design): sigmoid(o) exp(clamp(z, -8, 8)) in
+// 2^-20 units.
+inline uint64_t learned_weight(float logit, float z) {
+    float p = 1.0f / (1.0f + ms_expf(-logit));
+    if (!std::isfinite(p) || !(p > 0.0f) || !std::isfinite(z)) return 0;
+    const float e = ms_expf(std::min(std::max(z, -8.0f), 8.0f));
+    return (uint64_t)((double)p * (double)e * 1048576.0);
+}
+
 inline uint64_t draw_target(uint32_t seed, uint32_t j, uint64_t total) {
     uint64_t u = splitmix64(((uint64_t)seed << 32) | (uint64_t)j);
     return (uint64_t)(((unsigned __int128)u * (unsigned __int128)total) >> 64);
@@ -118,7 +127,10 @@ inline bool sample_sources(const std::vector<uint64_t>& w, uint32_t seed_base, u
     return true;
 }
 
-inline void mcmc_relocate_and_grow(State<float>& S, int cap_max, float cull_radius, uint32_t count, McmcInfo& info) {
+// mlp_out (optional, [N 4]): the learned proposal's weights and placement offsets for the
+// relocation draw; growth stays opacity-weighted.
+inline void mcmc_relocate_and_grow(State<float>& S, int cap_max, float cull_radius, uint32_t count, McmcInfo& info,
+                                   const float* mlp_out = nullptr) {
     const int N = S.N;
     const int nr = (num_sh_bases(S.degree) - 1) * 3;
     const float r2 = cull_radius > 0.0f ? cull_radius * cull_radius : -1.0f;
@@ -141,14 +153,26 @@ inline void mcmc_relocate_and_grow(State<float>& S, int cap_max, float cull_radi
     info.dead = (int)dead.size();
     if (!dead.empty() && !alive.empty()) {
         std::vector<uint64_t> w(alive.size());
-        for (size_t k = 0; k < alive.size(); k++) w[k] = reloc_weight(S.opac[alive[k]]);
+        for (size_t k = 0; k < alive.size(); k++)
+            w[k] = mlp_out ? learned_weight(S.opac[alive[k]], mlp_out[4 * (size_t)alive[k]])
+                           : reloc_weight(S.opac[alive[k]]);
         std::vector<int> chosen, counts;
         if (sample_sources(w, 0x9E3779B9u, count, (int)dead.size(), chosen, counts)) {
             for (size_t k = 0; k < alive.size(); k++)
                 if (counts[k] > 0) apply_eq9(S, alive[k], counts[k]);
             for (size_t j = 0; j < dead.size(); j++) {
-                copy_params(S, dead[j], alive[chosen[j]]);
-                zero_moments(S, dead[j]);
+                const int d = dead[j], src = alive[chosen[j]];
+                copy_params(S, d, src);
+                zero_moments(S, d);
+                if (mlp_out) {
+                    const float* ls = S.scales + 3 * (size_t)d;
+                    const float sc = ms_expf(std::max(std::max(ls[0], ls[1]), ls[2]));
+                    for (int k = 0; k < 3; k++) {
+                        const float o = mlp_out[4 * (size_t)src + 1 + k];
+                        const float oc = std::isfinite(o) ? std::min(std::max(o, -3.0f), 3.0f) : 0.0f;
+                        S.means[3 * (size_t)d + k] = std::fma(sc, oc, S.means[3 * (size_t)d + k]);
+                    }
+                }
             }
         }
     }

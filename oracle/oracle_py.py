@@ -88,6 +88,8 @@ def lib():
         L.oracle_k_max.restype = C.c_uint32
         L.oracle_eq9.argtypes = [C.c_float, C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_void_p]
         L.oracle_mcmc_step.argtypes = [C.POINTER(StateF32), C.c_int32, C.c_float, C.c_uint32, C.c_void_p]
+        L.oracle_mcmc_step_learned.argtypes = [C.POINTER(StateF32), C.c_int32, C.c_float, C.c_uint32, C.c_void_p,
+                                               C.c_void_p]
         L.oracle_set_threads.argtypes = [C.c_int]
         L.oracle_set_threads.restype = C.c_int
         vp = C.c_void_p
