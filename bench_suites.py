@@ -8,6 +8,17 @@ c1_full     C1 for 1000 steps with the plain MCMC relocation / growth at steps 5
             (cap 3M, 8 of the 64 views held out): wall time, Gaussian count, held-out PSNR.
 c3_bf16     3M clustered Gaussians at 1920x1080: it/s with all-fp32 storage and with
             bfloat16 SH coefficients + moments, per-stage times and HBM bytes saved.
+c2          batched multi-view: 2M clustered Gaussians, 64 cameras whose sizes are drawn from
+            640x480 / 1024x768 / 1600x1040 / 1920x1080, 8 views per step, 500 steps from the
+            same initial state in three modes: the views one after another through the
+            single-view kernels ("sequential", the baseline), one persistent raster launch
+            per direction over all views ("persistent"), and that recorded once as a HIP graph
+            and replayed, with the residency autotuned ("graph"); views/s of each and the
+            batch loss / held-out PSNR after 500 steps (equal up to float summation order).
+c4          learned relocation proposal: 2M clustered Gaussians, 8-view batches at
+            1600x1040, 8 withheld views; a plain run and a learned-proposal run (judged by
+            held-out loss) from the same state; steps and seconds for the learned run to reach
+            the plain run's final held-out PSNR (within 0.1 dB), and both trajectories.
 bf16_error  error against the FP64 oracle of the fp32 and the bf16-SH GPU paths, after 1
             step (image, loss, raster gradients, updated parameters) and after 2000 steps
             (parameters, render, loss), on a small scene (20k Gaussians, 128x128, 8 views).
@@ -213,3 +224,88 @@ def bf16_error(a):
     out["note"] = "max-norm relative error per array against the FP64 oracle from the same initial state, " \
                   "camera order and step scalars"
     print(json.dumps(out), flush=True)
+
+
+def c2(a):
+    from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, _lib
+    L = _lib.lib()
+    n, views, steps = 2_000_000, 8, int(os.environ.get("MSPLAT_C2_STEPS", "500"))
+    ds = Dataset(f"synthetic:c1:n={n},cams=64,mixed=1", eval_mode=True, test_every=8)
+    sizes = sorted({(c["width"], c["height"]) for c in (ds.camera(i, image=False) for i in range(ds.num_train))})
+    res = {}
+    for mode in ("sequential", "persistent", "graph"):
+        tr = GaussianTrainer(ds, TrainingConfig(iterations=30000, cap_max=n))
+        tr.set_option("mcmc", 0)
+        tr.set_option("batch_sequential", 1 if mode == "sequential" else 0)
+        tr.set_option("batch_graph", 1 if mode == "graph" else 0)
+        if mode == "graph":
+            tr.set_option("autotune", 1)
+        L.msplat_sync()
+        t0 = time.perf_counter()
+        t10 = None
+        for s in range(1, steps + 1):
+            tr.step_batch(views)
+            if s == 10:
+                L.msplat_sync()
+                t10 = time.perf_counter()
+        L.msplat_sync()
+        t1 = time.perf_counter()
+        _, losses = tr.last_batch(losses=True)
+        m = tr.evaluate()
+        r = {"views_per_s": round(steps * views / (t1 - t0), 2),
+             "views_per_s_after_10": round((steps - 10) * views / (t1 - t10), 2),
+             "ms_per_step": round(1000 * (t1 - t10) / (steps - 10), 3),
+             "final_batch_loss": float(np.mean(losses)), "heldout_psnr": m["psnr"]}
+        if mode == "graph":
+            r["graph"] = tr.batch_info()
+            r["autotune"] = tr.autotune_info()
+        res[mode] = r
+        _log(f"c2 {mode}: {r}")
+        del tr
+    base = res["sequential"]["views_per_s_after_10"]
+    print(json.dumps({"suite": "c2", "gaussians": n, "views_per_step": views, "steps": steps,
+                      "view_sizes": [f"{w}x{h}" for w, h in sizes], "modes": res,
+                      "speedup_persistent": round(res["persistent"]["views_per_s_after_10"] / base, 3),
+                      "speedup_graph": round(res["graph"]["views_per_s_after_10"] / base, 3),
+                      "note": "same initial state and camera order in every mode; relocation off "
+                              "(fixed N)"}), flush=True)
+
+
+def c4(a):
+    from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, _lib
+    L = _lib.lib()
+    n, views = 2_000_000, 8
+    steps = int(os.environ.get("MSPLAT_C4_STEPS", "1200"))
+    every = 100
+    ds = Dataset(f"synthetic:c1:n={n},cams=72", eval_mode=True, test_every=9)
+    runs = {}
+    for mode in ("plain", "learned"):
+        tr = GaussianTrainer(ds, TrainingConfig(iterations=steps, cap_max=int(1.5 * n)))
+        tr.set_option("batch_graph", 1)
+        if mode == "learned":
+            tr.set_option("learned_relocation", 1)
+        traj = []
+        L.msplat_sync()
+        t0 = time.perf_counter()
+        busy = 0.0
+        for s in range(1, steps + 1):
+            tr.step_batch(views)
+            if s % every == 0:
+                L.msplat_sync()
+                busy += time.perf_counter() - t0
+                m = tr.evaluate()
+                traj.append({"step": s, "s": round(busy, 2), "psnr": m["psnr"], "gaussians": m["num_gaussians"]})
+                _log(f"c4 {mode} {traj[-1]}")
+                t0 = time.perf_counter()
+        run = {"trajectory": traj}
+        if mode == "learned":
+            run["proposal"] = tr.proposal_info()
+        runs[mode] = run
+        del tr
+    target = runs["plain"]["trajectory"][-1]["psnr"] - 0.1
+    hit = next((p for p in runs["learned"]["trajectory"] if p["psnr"] >= target), None)
+    plain_hit = next((p for p in runs["plain"]["trajectory"] if p["psnr"] >= target), None)
+    print(json.dumps({"suite": "c4", "gaussians": n, "views_per_step": views, "view": "1600x1040",
+                      "heldout_views": ds.num_test, "steps": steps,
+                      "target_psnr": round(target, 3), "plain_reaches_at": plain_hit, "learned_reaches_at": hit,
+                      "runs": runs}), flush=True)

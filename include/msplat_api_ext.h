@@ -96,6 +96,13 @@ int64_t msplat_trainer_proposal_read(MsplatTrainer t, const char* name, void* ho
  * configuration and replayed when msplat_trainer_set_option(t, "batch_graph", 1) is set.
  * Returns 0 / -1. */
 int msplat_trainer_batch_info(MsplatTrainer t, int64_t out[4]);
+/* Launch-geometry autotuner of the batched steps (msplat_trainer_set_option(t, "autotune",
+ * 1)): the candidates timed so far (pixels per lane of the persistent raster kernels, their
+ * resident workgroups, ms per megapixel of the batch) and, once tuning has finished, the chosen pair
+ * in chosen[0..1] (zeros before). Returns the number of timed candidates (at most max
+ * written), or -1. */
+int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* ppl, int32_t* grids, float* ms, int max,
+                                 int32_t chosen[2]);
 
 /* ---- Trainer introspection (tests, benchmarks) --------------------------------------- */
 /* The scalars Model::fullIteration would hand to the kernels for (camera, step, adam step). */

@@ -252,6 +252,17 @@ class GaussianTrainer:
         L.msplat_trainer_proposal_read(self._h, name.encode(), out.ctypes.data, n)
         return out
 
+    def autotune_info(self) -> dict:
+        """Timed (pixels per lane, workgroups, ms) candidates and the chosen pair of the
+        batched-step autotuner."""
+        p = (C.c_int32 * 16)()
+        g = (C.c_int32 * 16)()
+        ms = np.zeros(16, np.float32)
+        ch = (C.c_int32 * 2)()
+        n = _lib.lib().msplat_trainer_autotune_info(self._h, p, g, fptr(ms), 16, ch)
+        return {"timed": [(int(p[k]), int(g[k]), float(ms[k])) for k in range(max(n, 0))],
+                "chosen": (int(ch[0]), int(ch[1]))}
+
     def batch_info(self) -> dict:
         """Recorded-graph counters of the batched steps (msplat_trainer_batch_info)."""
         out = (C.c_int64 * 4)()
@@ -346,7 +357,11 @@ class GaussianTrainer:
         "batch_graph" (record batched steps as HIP graphs and replay them), "persist_grid"
         (resident workgroups of the persistent raster kernels, 0 = default),
         "learned_relocation" (train and use the relocation proposal network),
-        "relocation_mode" (0 judged by held-out loss, 1 plain, 2 learned)."""
+        "relocation_mode" (0 judged by held-out loss, 1 plain, 2 learned), "raster_ppl"
+        (pixels per lane of the persistent raster kernels, 1 or 4), "autotune"
+        (time the persistent raster residencies over the first batched steps and keep the
+        fastest), "batch_sequential" (batched steps view by view through the single-view
+        path), "mcmc" (relocation / growth on their schedule; default 1)."""
         if _lib.lib().msplat_trainer_set_option(self._h, name.encode(), float(value)) != 0:
             raise RuntimeError(f"option {name!r} rejected (reason on stderr)")
 

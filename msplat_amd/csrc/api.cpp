@@ -81,6 +81,7 @@ static DatasetImpl* synthetic_dataset(const std::string& spec, float downscale, 
         throw std::invalid_argument("unknown synthetic preset '" + preset + "' (use c0 or c1)");
     }
     kv["seed"] = 1; kv["hidden_seed"] = 2; kv["cam_seed"] = 5; kv["radius"] = 3.0; kv["degree"] = 3;
+    kv["mixed"] = 0;  // 1: each camera's size drawn from 640x480 / 1024x768 / 1600x1040 / 1920x1080
     std::stringstream ss(opts);
     std::string item;
     while (std::getline(ss, item, ',')) {
@@ -118,16 +119,34 @@ static DatasetImpl* synthetic_dataset(const std::string& spec, float downscale, 
         Model hid(go, synth_set(kind, (uint64_t)kv["hidden_seed"], hidden, 3), 1.0f, zero);
         std::vector<float> img((size_t)W * H * 3);
         std::vector<uint8_t> q(img.size());
+        static const int kSizes[4][2] = {{640, 480}, {1024, 768}, {1600, 1040}, {1920, 1080}};
+        uint64_t size_rng = (uint64_t)kv["cam_seed"] * 0x2545F4914F6CDD1Dull + 17;
         for (int i = 0; i < cams; i++) {
             TrainCamera c;
             std::memcpy(c.c2w, &c2w[(size_t)i * 16], sizeof(c.c2w));
-            c.fx = f; c.fy = f; c.cx = cx; c.cy = cy; c.width = W; c.height = H;
+            int w = W, h = H;
+            float fi = f, cxi = cx, cyi = cy;
+            if (kv["mixed"] != 0.0) {
+                uint64_t x = (size_rng += 0x9E3779B97F4A7C15ull);
+                x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+                x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+                x ^= x >> 31;
+                const int k = (int)(x % 4);
+                w = (int)((float)kSizes[k][0] / std::max(downscale, 1.0f));
+                h = (int)((float)kSizes[k][1] / std::max(downscale, 1.0f));
+                fi = f * (float)w / (float)W;  // same horizontal field of view as the base size
+                cxi = (float)w / 2.0f;
+                cyi = (float)h / 2.0f;
+            }
+            c.fx = fi; c.fy = fi; c.cx = cxi; c.cy = cyi; c.width = w; c.height = h;
+            img.resize((size_t)w * h * 3);
+            q.resize(img.size());
             const float* d = hid.render(c, 1 << 20);
             hip_check(hipStreamSynchronize(hid.engine().stream()), "render");
             hip_check(hipMemcpy(img.data(), d, img.size() * 4, hipMemcpyDeviceToHost), "download");
             msplat_quantize_rgb(img.data(), (int64_t)img.size(), q.data());
             const bool is_test = eval_mode && test_every > 0 && (i % test_every == 0);
-            ds->add_camera(c.c2w, f, f, cx, cy, W, H, q.data(), nullptr, is_test);
+            ds->add_camera(c.c2w, fi, fi, cxi, cyi, w, h, q.data(), nullptr, is_test);
         }
     }
     ds->init = synth_set(kind, (uint64_t)kv["seed"], n, degree);
@@ -581,6 +600,23 @@ int msplat_trainer_batch_info(MsplatTrainer t, int64_t out[4]) {
     return 0;
 }
 
+int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* ppl, int32_t* grids, float* ms, int max, int32_t chosen[2]) {
+    if (!t) return -1;
+    msplat_hip::BatchRunner* b = TR(t)->model->batch_runner();
+    if (!b) return 0;
+    const auto& r = b->tune_results();
+    for (int k = 0; k < (int)r.size() && k < max; k++) {
+        if (ppl) ppl[k] = r[k].ppl;
+        if (grids) grids[k] = r[k].grid;
+        if (ms) ms[k] = r[k].ms;
+    }
+    if (chosen) {
+        chosen[0] = b->tuned() ? b->raster_ppl() : 0;
+        chosen[1] = b->tuned() ? b->persist_grid() : 0;
+    }
+    return (int)r.size();
+}
+
 void msplat_trainer_train(MsplatTrainer t) {
     if (!t) return;
     while (TR(t)->step < TR(t)->cfg.iterations) {
@@ -743,8 +779,26 @@ int msplat_trainer_set_option(MsplatTrainer t, const char* name, double value) {
         TR(t)->model->set_proposal_mode(value == 2.0);
         return 0;
     }
+    if (n == "autotune") {
+        TR(t)->model->set_batch_autotune(value != 0.0);
+        return 0;
+    }
+    if (n == "batch_sequential") {
+        TR(t)->model->set_batch_sequential(value != 0.0);
+        return 0;
+    }
+    if (n == "mcmc") {
+        TR(t)->model->set_mcmc_enabled(value != 0.0);
+        return 0;
+    }
     if (n == "batch_graph") {
         TR(t)->model->set_batch_graph(value != 0.0);
+        return 0;
+    }
+    if (n == "raster_ppl") {
+        if (value != 1.0 && value != 4.0) throw std::invalid_argument("raster_ppl is 1 or 4");
+        TR(t)->model->engine().set_raster_ppl((int)value);
+        TR(t)->model->set_raster_ppl((int)value);
         return 0;
     }
     if (n == "persist_grid") {

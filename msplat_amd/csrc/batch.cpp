@@ -12,7 +12,8 @@
 namespace msplat_hip {
 
 bool BatchRunner::Key::operator==(const Key& o) const {
-    if (N != o.N || B != o.B || du != o.du || degree != o.degree || bf16 != o.bf16 || grid != o.grid) return false;
+    if (N != o.N || B != o.B || du != o.du || degree != o.degree || bf16 != o.bf16 || grid != o.grid || ppl != o.ppl)
+        return false;
     if (g.W != o.g.W || g.H != o.g.H || g.T != o.g.T || gens != o.gens || nanc != o.nanc) return false;
     for (int k = 0; k < 8; k++)
         if (state[k] != o.state[k]) return false;
@@ -30,6 +31,8 @@ BatchRunner::BatchRunner(hipStream_t s) : s_(s) { persist_grid_ = default_persis
 BatchRunner::~BatchRunner() {
     (void)hipStreamSynchronize(s_);
     release_graph();
+    for (auto& e : tev_)
+        if (e) (void)hipEventDestroy(e);
     DBuf* all[] = {&block_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_};
     for (DBuf* b : all) b->release();
 }
@@ -43,6 +46,11 @@ void BatchRunner::release_graph() {
 void BatchRunner::set_persist_grid(int g) {
     persist_grid_ = g > 0 ? g : default_persist_grid();
     for (auto& e : slots_) e->set_persist_grid(persist_grid_);
+}
+
+void BatchRunner::set_raster_ppl(int p) {
+    ppl_ = p == 4 ? 4 : 1;
+    for (auto& e : slots_) e->set_raster_ppl(ppl_);
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -72,6 +80,7 @@ void BatchRunner::enqueue(const msk::Params& P, int B, int du, int Tmax, msk::Ba
     w.n_work = wmisc_.as<uint32_t>();
     w.ctr = wmisc_.as<uint32_t>() + 1;
     w.grid = persist_grid_;
+    w.ppl = ppl_;
     for (int v = 0; v < B; v++)
         if (slots_[v]->chunked()) slots_[v]->raster_fwd_own();
     msk::raster_work_build(s_, w);
@@ -94,17 +103,75 @@ void BatchRunner::enqueue(const msk::Params& P, int B, int du, int Tmax, msk::Ba
     msk::batch_update(s_, P, A, block_.as<MsplatStepParams>(), du, nan_counts);
 }
 
+void BatchRunner::set_autotune(bool on) {
+    autotune_ = on;
+    tune_done_ = !on;
+    tune_step_ = 0;
+    tune_ms_.clear();
+    tune_cands_.clear();
+    if (on) {
+        const int cus = default_persist_grid() / 8;  // the default is 8 workgroups per CU
+        for (int k : {2, 4, 8, 16}) tune_cands_.push_back({1, cus * k});
+        for (int k : {8, 16, 32}) tune_cands_.push_back({4, cus * k});
+    }
+}
+
+// Each candidate runs kTuneSteps steps: the first records its graph (untimed), the others
+// are timed and normalised by the pixels of their views, since the views (and their sizes)
+// change from step to step.
+constexpr int kTuneSteps = 4;
+
 void BatchRunner::run(const MsplatDeviceState& st, const BatchView* views, int B, const MsplatStepParams& hp,
                       msk::BatchGrads A, uint32_t* nan_counts) {
+    if (!autotune_ || tune_done_) {
+        run_inner(st, views, B, hp, A, nan_counts);
+        return;
+    }
+    const int c = tune_step_ / kTuneSteps;
+    set_raster_ppl(tune_cands_[c].first);
+    set_persist_grid(tune_cands_[c].second);
+    const bool timed = (tune_step_ % kTuneSteps) != 0;
+    if (timed) {
+        if (!tev_[0]) {
+            hip_check(hipEventCreate(&tev_[0]), "hipEventCreate");
+            hip_check(hipEventCreate(&tev_[1]), "hipEventCreate");
+        }
+        hip_check(hipEventRecord(tev_[0], s_), "hipEventRecord");
+    }
+    run_inner(st, views, B, hp, A, nan_counts);
+    if (timed) {
+        hip_check(hipEventRecord(tev_[1], s_), "hipEventRecord");
+        hip_check(hipEventSynchronize(tev_[1]), "hipEventSynchronize");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, tev_[0], tev_[1]), "hipEventElapsedTime");
+        double px = 0.0;
+        for (auto& wh : sizes_) px += (double)wh.first * wh.second;
+        if ((int)tune_ms_.size() <= c) tune_ms_.push_back({tune_cands_[c].first, tune_cands_[c].second, 0.f});
+        tune_ms_[c].ms += (float)(ms / (px * 1e-6));  // ms per megapixel, summed over the timed steps
+    }
+    if (++tune_step_ >= kTuneSteps * (int)tune_cands_.size()) {
+        for (auto& r : tune_ms_) r.ms /= (float)(kTuneSteps - 1);
+        TuneResult best = tune_ms_.front();
+        for (auto& r : tune_ms_)
+            if (r.ms < best.ms) best = r;
+        set_raster_ppl(best.ppl);
+        set_persist_grid(best.grid);
+        tune_done_ = true;
+    }
+}
+
+void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views, int B, const MsplatStepParams& hp,
+                            msk::BatchGrads A, uint32_t* nan_counts) {
     if (B <= 0 || B > Engine::kLossSlots) throw std::invalid_argument("batch size must be in [1, 64]");
     const msk::Params P = params_of(st);
     const int du = hp.degrees_to_use < st.degree ? hp.degrees_to_use : st.degree;
     while ((int)slots_.size() < B) {
         slots_.emplace_back(new Engine(s_));
         slots_.back()->set_persist_grid(persist_grid_);
+        slots_.back()->set_raster_ppl(ppl_);
     }
     sizes_.assign(B, {0, 0});
-    bool graphable = graph_on_;
+    bool graphable = graph_on_ && !sequential_;
     msk::Geom gmax{};
     for (int v = 0; v < B; v++) {
         const MsplatCamera& c = views[v].cam;
@@ -150,6 +217,25 @@ void BatchRunner::run(const MsplatDeviceState& st, const BatchView* views, int B
         std::memcpy(h.data() + off_rv + sizeof(rv) * v, &rv, sizeof(rv));
     }
     ring_.upload(s_, block_.p, h.data(), bytes);
+    if (sequential_) {
+        const size_t n = (size_t)(P.N > 0 ? P.N : 1);
+        const int nb3 = P.degree <= 0 ? 0 : (P.degree == 1 ? 9 : P.degree == 2 ? 24 : 45);
+        void* zs[7] = {A.v_mean, A.v_scale, A.v_quat, A.v_opac, A.v_dc, A.v_rest, A.vis};
+        const size_t zb[7] = {n * 12, n * 12, n * 16, n * 4, n * 12, n * (size_t)std::max(nb3, 1) * 4, n * 4};
+        for (int k = 0; k < 7; k++) msk::fill_zero(s_, zs[k], zb[k]);
+        for (int v = 0; v < B; v++) {
+            Engine& e = *slots_[v];
+            e.own_view(Engine::make_view(views[v].cam, views[v].gt, hp.background, hp.ssim_weight, views[v].inv_n));
+            e.front(P, du, true);
+            e.raster_fwd_own();
+            e.loss_to(0);
+            e.raster_bwd_own();
+            e.accumulate(st, du, A);
+        }
+        msk::batch_update(s_, P, A, block_.as<MsplatStepParams>(), du, nan_counts);
+        hip_check(hipGetLastError(), "batched step");
+        return;
+    }
     if (!graphable) {
         enqueue(P, B, du, Tmax, A, nan_counts);
         hip_check(hipGetLastError(), "batched step");
@@ -162,6 +248,7 @@ void BatchRunner::run(const MsplatDeviceState& st, const BatchView* views, int B
     k.degree = st.degree;
     k.bf16 = st.sh_bf16;
     k.grid = persist_grid_;
+    k.ppl = ppl_;
     k.g = hw_;
     int64_t gens = 0;
     // per slot: buffer generation and depth-chunk count (chunked views record their own path)

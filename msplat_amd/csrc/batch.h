@@ -31,10 +31,27 @@ public:
     // Mean loss of view v of the last step (waits for the stream).
     double loss(int v);
     void set_graph(bool on) { graph_on_ = on; }
+    // Views one after another through the single-view path (lists, raster forward, loss,
+    // raster backward per view) instead of one persistent launch over all of them; never
+    // recorded. The baseline the multi-view path is measured against.
+    void set_sequential(bool on) { sequential_ = on; }
     bool graph() const { return graph_on_; }
     // Work-queue residency of the persistent raster kernels (0 = default).
     void set_persist_grid(int g);
     int persist_grid() const { return persist_grid_; }
+    // Autotuner of the launch geometry the results do not depend on (pixels per lane and
+    // residency of the persistent raster kernels): the first steps cycle through the
+    // candidates, each timed per megapixel over three steps after the one that records its
+    // graph, and the fastest is kept for the remaining steps.
+    void set_autotune(bool on);
+    bool tuned() const { return tune_done_; }
+    struct TuneResult {
+        int ppl, grid;
+        float ms;
+    };
+    const std::vector<TuneResult>& tune_results() const { return tune_ms_; }
+    void set_raster_ppl(int p);
+    int raster_ppl() const { return ppl_; }
     // Counters: steps replayed from a recorded graph, graphs recorded.
     int64_t replays() const { return replays_; }
     int64_t captures() const { return captures_; }
@@ -45,7 +62,7 @@ public:
 
 private:
     struct Key {
-        int N = -1, B = 0, du = -1, degree = -1, bf16 = 0, grid = 0;
+        int N = -1, B = 0, du = -1, degree = -1, bf16 = 0, grid = 0, ppl = 1;
         msk::Geom g{};
         int64_t gens = 0;
         const void* state[8] = {};
@@ -65,9 +82,18 @@ private:
     DBuf wka_, wva_, wkb_, wvb_, wtemp_, wmisc_;     // batch work queue
     ConstRing ring_;
     bool graph_on_ = false;
+    bool sequential_ = false;
     int persist_grid_ = 0;
     msk::Geom hw_{};                                 // high-water launch geometry (graph mode)
     hipGraphExec_t exec_ = nullptr;
+    bool autotune_ = false, tune_done_ = false;
+    int tune_step_ = 0;
+    std::vector<std::pair<int, int>> tune_cands_;  // (pixels per lane, grid)
+    std::vector<TuneResult> tune_ms_;
+    int ppl_ = 1;
+    hipEvent_t tev_[2] = {nullptr, nullptr};
+    void run_inner(const MsplatDeviceState& st, const BatchView* views, int B, const MsplatStepParams& hp,
+                   msk::BatchGrads A, uint32_t* nan_counts);
     Key key_;
     int64_t replays_ = 0, captures_ = 0;
 };

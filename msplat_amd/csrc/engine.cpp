@@ -266,6 +266,7 @@ msk::RasterWork Engine::own_work() {
     w.n_work = wmisc_.as<uint32_t>();
     w.ctr = wmisc_.as<uint32_t>() + 1;
     w.grid = persist_grid_;
+    w.ppl = ppl_;
     return w;
 }
 

@@ -83,6 +83,9 @@ public:
                                      float inv_n);
     // The view's constants live at `d` (already on the device, written by the caller).
     void bind_view(const msk::ViewConsts& h, const msk::ViewConsts* d);
+    // The view's constants in the engine's own block (with its raster view table), so that
+    // raster_fwd_own / raster_bwd_own run this view alone.
+    void own_view(const msk::ViewConsts& h) { upload_view(h, true); }
     // Gradient accumulators cleared (train), projection, tile lists.
     void front(const msk::Params& P, int degrees_to_use, bool train);
     void loss_to(int slot);
@@ -99,6 +102,9 @@ public:
     const float2* v_xy_dev() const { return v_xy_.as<float2>(); }
     void set_persist_grid(int g) { persist_grid_ = g; }
     int persist_grid() const { return persist_grid_; }
+    // Pixels per lane of the persistent raster kernels (1 or 4; the image is the same).
+    void set_raster_ppl(int p) { ppl_ = p == 4 ? 4 : 1; }
+    int raster_ppl() const { return ppl_; }
 
 private:
     void upload_view(const msk::ViewConsts& h, bool train);
@@ -116,6 +122,7 @@ private:
     uint32_t K_ = 1;
     int64_t gen_ = 0;   // bumps whenever a buffer is re-allocated
     int persist_grid_ = 0;
+    int ppl_ = 1;
     // view constants: host copy, device copy in use, own device block [ViewConsts | RasterView]
     msk::ViewConsts vch_{};
     const msk::ViewConsts* vcd_ = nullptr;

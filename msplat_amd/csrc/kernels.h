@@ -120,6 +120,7 @@ struct RasterWork {
     uint32_t* n_work;          // [1]
     uint32_t* ctr;             // [2] forward / backward queue heads
     int grid;                  // resident workgroups
+    int ppl = 1;               // pixels per lane: 1 (256-lane workgroup per tile) or 4 (one wave)
 };
 
 void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, ProjOut& o,

@@ -417,12 +417,15 @@ __global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const 
 }
 
 // Exclusive scan of each tile's column over the chunks; the column total is the tile count.
+// The offsets are also written tile-major (matT [T][nblk]) for the per-tile sort, which
+// reads one tile's offsets contiguously.
 __global__ void k_col_scan(const ViewConsts* __restrict__ vc, int nblk, uint32_t* __restrict__ mat,
-                           uint32_t* __restrict__ counts) {
+                           uint32_t* __restrict__ matT, uint32_t* __restrict__ counts) {
     const int T = vc->T;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     uint32_t run = 0;
+    uint32_t* rowT = matT + (size_t)t * nblk;
     int b = 0;
     for (; b + 8 <= nblk; b += 8) {
         uint32_t v[8];
@@ -431,6 +434,7 @@ __global__ void k_col_scan(const ViewConsts* __restrict__ vc, int nblk, uint32_t
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             mat[(size_t)(b + u) * T + t] = run;
+            rowT[b + u] = run;
             run += v[u];
         }
     }
@@ -438,6 +442,7 @@ __global__ void k_col_scan(const ViewConsts* __restrict__ vc, int nblk, uint32_t
         const size_t o = (size_t)b * T + t;
         const uint32_t v = mat[o];
         mat[o] = run;
+        rowT[b] = run;
         run += v;
     }
     counts[t] = run;
@@ -559,7 +564,7 @@ constexpr int WAVE_LDS_KEYS = 2048;
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(const ViewConsts* __restrict__ vc, int nblk,
                                                             const uint32_t* __restrict__ counts,
                                                             const uint32_t* __restrict__ exact_off,
-                                                            const uint32_t* __restrict__ mat,
+                                                            const uint32_t* __restrict__ matT,
                                                             const uint32_t* __restrict__ keys,
                                                             const uint32_t* __restrict__ sorted_ids,
                                                             uint32_t* __restrict__ ids, int N, uint32_t cap) {
@@ -577,7 +582,8 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(const ViewConsts* __
             if (base + i < cap) ids[base + i] = (uint32_t)N;
         return;
     }
-    for (int b = threadIdx.x; b < nblk; b += SORT_THREADS) off[b] = mat[(size_t)b * T + t];
+    const uint32_t* rowT = matT + (size_t)t * nblk;
+    for (int b = threadIdx.x; b < nblk; b += SORT_THREADS) off[b] = rowT[b];
     if (threadIdx.x == 0) off[nblk] = count;
     __syncthreads();
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -723,7 +729,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
 
 size_t lists_temp_bytes(int N) { return ((size_t)256 * (size_t)rs_blocks(N > 0 ? N : 1) + 256 + 64) * 4; }
 
-size_t lists_hist_bytes(int N, int T) { return (size_t)hist_blocks(N) * (size_t)T * 4; }
+// [chunks x T] histogram / bucket offsets, then the same offsets tile-major [T x chunks]
+size_t lists_hist_bytes(int N, int T) { return 2 * (size_t)hist_blocks(N) * (size_t)T * 4; }
 
 static size_t g_lds_limit = 0;
 static size_t lds_limit() {
@@ -774,7 +781,8 @@ void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, 
         if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_hist, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, pj.radii, pj.rect,
                            vc, L.hist);
-        hipLaunchKernelGGL(k_col_scan, dim3(cdiv(g.T, 64)), dim3(64), 0, s, vc, nblk, L.hist, L.tile_counts);
+        hipLaunchKernelGGL(k_col_scan, dim3(cdiv(g.T, 64)), dim3(64), 0, s, vc, nblk, L.hist,
+                           L.hist + (size_t)nblk * g.T, L.tile_counts);
     } else {
         fill_zero(s, L.tile_counts, (size_t)g.T * 4);
     }
@@ -796,7 +804,8 @@ void lists_sort_pack(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, 
     const int nblk = hist_blocks(N);
     const size_t lds = ((size_t)(nblk + 4) & ~(size_t)3) * 4 + 4 * WAVE_LDS_KEYS * 4;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_sort_pack, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_sort_pack, dim3(g.T), dim3(SORT_THREADS), lds, s, vc, nblk, L.tile_counts, L.exact_off, L.hist,
+    hipLaunchKernelGGL(k_sort_pack, dim3(g.T), dim3(SORT_THREADS), lds, s, vc, nblk, L.tile_counts, L.exact_off,
+                       L.hist + (size_t)nblk * g.T,
                        L.keys, L.sorted_ids, L.ids, N, (uint32_t)L.capacity);
 }
 
@@ -1000,18 +1009,13 @@ void raster_work_build(hipStream_t s, const RasterWork& w) {
 void raster_forward(hipStream_t s, const ListBufs& L, const ViewConsts* vc, const Geom& g, RasterBufs& R,
                     const RasterWork* work) {
     if (R.K <= 1) {
-        hipLaunchKernelGGL(k_raster_fwd_persist, dim3(work->grid), dim3(RB), 0, s, work->views, work->vals_a,
-                           work->n_work, work->ctr);
+        raster_forward_views(s, *work);
         return;
     }
     hipLaunchKernelGGL(k_raster_fwd_chunk, dim3(g.T, R.K), dim3(RB), 0, s, vc, L.bins, L.view, R.chunk_T, R.chunk_C,
                        R.chunk_idx);
     hipLaunchKernelGGL(k_raster_merge, dim3(cdiv((size_t)g.W * g.H, 256)), dim3(256), 0, s, vc, (int)R.K, R.chunk_T,
                        R.chunk_C, R.chunk_idx, R.img, R.final_T, R.final_idx);
-}
-
-void raster_forward_views(hipStream_t s, const RasterWork& w) {
-    hipLaunchKernelGGL(k_raster_fwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
 }
 
 // ------------------------------------------------------------------------------------
@@ -1454,8 +1458,7 @@ __global__ __launch_bounds__(RB) void k_raster_bwd_chunk(const ViewConsts* __res
 void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, const ViewConsts* vc, const Geom& g,
                      const float* v_rend, GradBufs& G, const RasterWork* work) {
     if (R.K <= 1) {
-        hipLaunchKernelGGL(k_raster_bwd_persist, dim3(work->grid), dim3(RB), 0, s, work->views, work->vals_a,
-                           work->n_work, work->ctr);
+        raster_backward_views(s, *work);
         return;
     }
     hipLaunchKernelGGL(k_chunk_prefix_suffix, dim3(cdiv((size_t)g.W * g.H, 256)), dim3(256), 0, s, vc, (int)R.K,
@@ -1464,8 +1467,244 @@ void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, cons
                        R.prefix_T, R.chunk_T, R.after_C, v_rend, G);
 }
 
+// ---- four pixels per lane: one 64-lane wave per 16x16 tile --------------------------
+// Lane l covers the 2x2 pixel block at (2 (l % 8), 2 (l / 8)) of its tile. Every pixel runs
+// the same operation sequence as in the one-pixel-per-thread kernels (identical images); in
+// the backward each lane first sums its four pixels' terms, so the wave reduction per entry
+// is shared by 256 pixels instead of 64, and the one wave issues the entry's atomics itself.
+constexpr int RB4 = 64;
+
+__device__ __forceinline__ void px4_coords(int t, int tbx, float px[4], float py[4], int ix[4], int iy[4]) {
+    const int lane = threadIdx.x & 63;
+    const int x0 = (t % tbx) * TILE + 2 * (lane & 7), y0 = (t / tbx) * TILE + 2 * (lane >> 3);
+    for (int q = 0; q < 4; q++) {
+        ix[q] = x0 + (q & 1);
+        iy[q] = y0 + (q >> 1);
+        px[q] = (float)ix[q];
+        py[q] = (float)iy[q];
+    }
+}
+
+__global__ __launch_bounds__(RB4) void k_raster_fwd_persist4(const RasterView* __restrict__ views,
+                                                             const uint32_t* __restrict__ work,
+                                                             const uint32_t* __restrict__ n_work,
+                                                             uint32_t* __restrict__ ctr) {
+    __shared__ float4 sA[RB4];
+    __shared__ float4 sB[RB4];
+    __shared__ float sCx[RB4];
+    const uint32_t total = *n_work;
+    const int lane = threadIdx.x;
+    for (;;) {
+        const uint32_t it = next_item(ctr);
+        if (it >= total) break;
+        const uint32_t code = work[it];
+        const RasterView& rv = views[code >> WORK_TILE_BITS];
+        const ViewConsts& vc = *rv.vc;
+        const int t = (int)(code & ((1u << WORK_TILE_BITS) - 1u));
+        const int W = vc.W, H = vc.H;
+        float px[4], py[4];
+        int ix[4], iy[4];
+        px4_coords(t, vc.tbx, px, py, ix, iy);
+        const int2 r = rv.bins[t];
+        float T[4], C[4][3];
+        int last[4];
+        bool done[4];
+        for (int q = 0; q < 4; q++) {
+            T[q] = 1.f;
+            C[q][0] = C[q][1] = C[q][2] = 0.f;
+            last[q] = r.x - 1;
+            done[q] = !(ix[q] < W && iy[q] < H);
+        }
+        for (int b0 = r.x; b0 < r.y; b0 += RB4) {
+            if (__syncthreads_and(done[0] && done[1] && done[2] && done[3])) break;
+            const int idx = b0 + lane;
+            if (idx < r.y) {
+                const uint32_t g = rv.pk.ids[idx];
+                sA[lane] = rv.pk.rec.A[g];
+                sB[lane] = rv.pk.rec.B[g];
+                sCx[lane] = rv.pk.rec.C[g].x;
+            }
+            __syncthreads();
+            const int n = min(RB4, r.y - b0);
+            for (int e = 0; e < n; e++) {
+                if (done[0] && done[1] && done[2] && done[3]) break;
+                const float4 a = sA[e];
+                const float4 b = sB[e];
+                const float cz = sCx[e];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (done[q]) continue;
+                    const float dx = a.x - px[q], dy = a.y - py[q];
+                    const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
+                    if (!ms_finite(sigma) || sigma < 0.f || sigma >= 5.55f) continue;
+                    const float alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
+                    if (alpha < 1.f / 255.f) continue;
+                    const float nT = T[q] * (1.f - alpha);
+                    if (nT <= 1e-4f) {
+                        last[q] = b0 + e - 1;
+                        done[q] = true;
+                        continue;
+                    }
+                    const float vis = alpha * T[q];
+                    C[q][0] = fmaf(fmaxf(b.z + 0.5f, 0.f), vis, C[q][0]);
+                    C[q][1] = fmaf(fmaxf(b.w + 0.5f, 0.f), vis, C[q][1]);
+                    C[q][2] = fmaf(fmaxf(cz + 0.5f, 0.f), vis, C[q][2]);
+                    T[q] = nT;
+                    last[q] = b0 + e;
+                }
+            }
+            __syncthreads();
+        }
+        for (int q = 0; q < 4; q++) {
+            if (!(ix[q] < W && iy[q] < H)) continue;
+            const size_t pid = (size_t)iy[q] * W + ix[q];
+            rv.final_T[pid] = T[q];
+            rv.final_idx[pid] = last[q];
+            rv.img[3 * pid + 0] = fminf(fmaxf(fmaf(vc.bg[0], T[q], C[q][0]), 0.f), 1.f);
+            rv.img[3 * pid + 1] = fminf(fmaxf(fmaf(vc.bg[1], T[q], C[q][1]), 0.f), 1.f);
+            rv.img[3 * pid + 2] = fminf(fmaxf(fmaf(vc.bg[2], T[q], C[q][2]), 0.f), 1.f);
+        }
+    }
+}
+
+__global__ __launch_bounds__(RB4) void k_raster_bwd_persist4(const RasterView* __restrict__ views,
+                                                             const uint32_t* __restrict__ work,
+                                                             const uint32_t* __restrict__ n_work,
+                                                             uint32_t* __restrict__ ctr) {
+    __shared__ float4 sA[RB4];
+    __shared__ float4 sB[RB4];
+    __shared__ float2 sC[RB4];
+    const uint32_t total = *n_work;
+    const int lane = threadIdx.x;
+    for (;;) {
+        const uint32_t it = next_item(ctr + 1);
+        if (it >= total) break;
+        const uint32_t code = work[it];
+        const RasterView& rv = views[code >> WORK_TILE_BITS];
+        const ViewConsts& vc = *rv.vc;
+        const int t = (int)(code & ((1u << WORK_TILE_BITS) - 1u));
+        const int2 r = rv.bins[t];
+        if (r.y <= r.x) continue;  // uniform over the wave
+        const int W = vc.W, H = vc.H;
+        float px[4], py[4];
+        int ix[4], iy[4];
+        px4_coords(t, vc.tbx, px, py, ix, iy);
+        float T[4], buf[4][3], Tf[4], vo[4][3];
+        int bf[4];
+        bool inside[4];
+        int mx = INT_MIN;
+        for (int q = 0; q < 4; q++) {
+            inside[q] = ix[q] < W && iy[q] < H;
+            const size_t pid = inside[q] ? (size_t)iy[q] * W + ix[q] : 0;
+            Tf[q] = inside[q] ? rv.final_T[pid] : 1.f;
+            bf[q] = inside[q] ? rv.final_idx[pid] : -1;
+            for (int c = 0; c < 3; c++) {
+                vo[q][c] = inside[q] ? rv.v_rend[3 * pid + c] : 0.f;
+                buf[q][c] = 0.f;
+            }
+            T[q] = Tf[q];
+            if (inside[q]) mx = max(mx, bf[q]);
+        }
+        const int tmax = block_max_int(mx);
+        const float bgc[3] = {vc.bg[0], vc.bg[1], vc.bg[2]};
+        const int top = min(r.y, tmax + 1);
+        for (int bhi = top; bhi > r.x; bhi -= RB4) {
+            const int blo = max(r.x, bhi - RB4);
+            const int n = bhi - blo;
+            __syncthreads();
+            if (lane < n) {
+                const int idx = bhi - 1 - lane;  // slot 0 = furthest back
+                const uint32_t g = rv.pk.ids[idx];
+                sA[lane] = rv.pk.rec.A[g];
+                sB[lane] = rv.pk.rec.B[g];
+                sC[lane] = make_float2(rv.pk.rec.C[g].x, __builtin_bit_cast(float, g));
+            }
+            __syncthreads();
+            for (int tt = 0; tt < n; tt++) {
+                const int e = bhi - 1 - tt;
+                const float4 a = sA[tt];
+                const float4 b = sB[tt];
+                const float cz = sC[tt].x;
+                const float r0 = fmaxf(b.z + 0.5f, 0.f), r1 = fmaxf(b.w + 0.5f, 0.f), r2 = fmaxf(cz + 0.5f, 0.f);
+                float g[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                bool any = false;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    bool valid = inside[q] && e <= bf[q];
+                    float alpha = 0.f, dx = 0.f, dy = 0.f;
+                    if (valid) {
+                        dx = a.x - px[q];
+                        dy = a.y - py[q];
+                        const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
+                        if (sigma < 0.f || sigma >= 5.55f) {
+                            valid = false;
+                        } else {
+                            alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
+                            if (alpha < 1.f / 255.f) valid = false;
+                        }
+                    }
+                    if (!(valid && alpha < 0.999f)) continue;
+                    any = true;
+                    const float ra = 1.f / (1.f - alpha);
+                    T[q] *= ra;
+                    const float fac = alpha * T[q];
+                    const float tfb0 = Tf[q] * bgc[0], tfb1 = Tf[q] * bgc[1], tfb2 = Tf[q] * bgc[2];
+                    const float q0 = fmaf(r0, T[q], fmaf(-buf[q][0], ra, -ra * tfb0));
+                    const float q1 = fmaf(r1, T[q], fmaf(-buf[q][1], ra, -ra * tfb1));
+                    const float q2 = fmaf(r2, T[q], fmaf(-buf[q][2], ra, -ra * tfb2));
+                    const float v_alpha = q0 * vo[q][0] + q1 * vo[q][1] + q2 * vo[q][2];
+                    buf[q][0] = fmaf(r0, fac, buf[q][0]);
+                    buf[q][1] = fmaf(r1, fac, buf[q][1]);
+                    buf[q][2] = fmaf(r2, fac, buf[q][2]);
+                    const float v_sigma = -alpha * v_alpha;
+                    g[0] += fac * vo[q][0];
+                    g[1] += fac * vo[q][1];
+                    g[2] += fac * vo[q][2];
+                    const float hs = 0.5f * v_sigma;
+                    g[3] += hs * (dx * dx);
+                    g[4] += hs * (dx * dy);
+                    g[5] += hs * (dy * dy);
+                    g[6] += v_sigma * fmaf(a.w, dx, b.x * dy);
+                    g[7] += v_sigma * fmaf(b.x, dx, b.y * dy);
+                    g[8] += -v_sigma * (1.f - a.z);
+                }
+                if (__ballot(any) == 0ull) continue;
+#pragma unroll
+                for (int k = 0; k < 9; k++) g[k] = wave_sum_to_lane63(g[k]);
+                if (lane == 63) {
+                    bool nz = false;
+#pragma unroll
+                    for (int k = 0; k < 9; k++) nz |= (g[k] != 0.f);
+                    if (nz) {
+                        const uint32_t gid = __builtin_bit_cast(uint32_t, sC[tt].y);
+                        if (b.z + 0.5f >= 0.f) atomicAdd(&rv.G.v_rgb[3 * (size_t)gid + 0], g[0]);
+                        if (b.w + 0.5f >= 0.f) atomicAdd(&rv.G.v_rgb[3 * (size_t)gid + 1], g[1]);
+                        if (cz + 0.5f >= 0.f) atomicAdd(&rv.G.v_rgb[3 * (size_t)gid + 2], g[2]);
+                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 0], g[3]);
+                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 1], g[4]);
+                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 2], g[5]);
+                        atomicAdd(&rv.G.v_xy[gid].x, g[6]);
+                        atomicAdd(&rv.G.v_xy[gid].y, g[7]);
+                        atomicAdd(&rv.G.v_opac[gid], g[8]);
+                    }
+                }
+            }
+        }
+    }
+}
+
 void raster_backward_views(hipStream_t s, const RasterWork& w) {
-    hipLaunchKernelGGL(k_raster_bwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
+    if (w.ppl == 4)
+        hipLaunchKernelGGL(k_raster_bwd_persist4, dim3(w.grid), dim3(RB4), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
+    else
+        hipLaunchKernelGGL(k_raster_bwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
+}
+
+void raster_forward_views(hipStream_t s, const RasterWork& w) {
+    if (w.ppl == 4)
+        hipLaunchKernelGGL(k_raster_fwd_persist4, dim3(w.grid), dim3(RB4), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
+    else
+        hipLaunchKernelGGL(k_raster_fwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
 }
 
 // ------------------------------------------------------------------------------------

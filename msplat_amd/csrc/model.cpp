@@ -303,8 +303,11 @@ void Model::full_iteration_batch(const TrainCamera* const* cams, int B, int step
     if (!batch_) {
         batch_.reset(new BatchRunner(eng_.stream()));
         batch_->set_persist_grid(persist_grid_);
+        batch_->set_raster_ppl(raster_ppl_);
+        batch_->set_autotune(batch_autotune_);
     }
     batch_->set_graph(batch_graph_);
+    batch_->set_sequential(batch_sequential_);
     batch_->run(state(), views.data(), B, hp, A, eng_.nan_counts());
     batch_n_ = B;
     if (prop_.enabled)
@@ -323,7 +326,7 @@ void Model::schedulers_step(int step) {
 }
 
 bool Model::is_mcmc_step(int step) const {
-    return step >= 500 && step % 100 == 0 && step < (int)((float)opt_.max_steps * 0.833f);
+    return mcmc_on_ && step >= 500 && step % 100 == 0 && step < (int)((float)opt_.max_steps * 0.833f);
 }
 
 msk::McmcInfo Model::mcmc_after_train(int step) {

@@ -102,6 +102,14 @@ public:
     Engine& engine() { return eng_; }
     // Batched steps replay one recorded HIP graph per launch configuration when on.
     void set_batch_graph(bool on) { batch_graph_ = on; }
+    void set_batch_sequential(bool on) { batch_sequential_ = on; }
+    // Relocation / growth on their schedule (on by default; off only for benchmarks that
+    // compare step throughput at a fixed Gaussian count).
+    void set_mcmc_enabled(bool on) { mcmc_on_ = on; }
+    void set_batch_autotune(bool on) {
+        batch_autotune_ = on;
+        if (batch_) batch_->set_autotune(on);
+    }
     bool batch_graph() const { return batch_graph_; }
     BatchRunner* batch_runner() { return batch_.get(); }
     // ---- learned relocation proposal (no counterpart in the reference) ----
@@ -126,6 +134,10 @@ public:
         persist_grid_ = g;
         if (batch_) batch_->set_persist_grid(g);
     }
+    void set_raster_ppl(int p) {
+        raster_ppl_ = p;
+        if (batch_) batch_->set_raster_ppl(p);
+    }
     int degrees_to_use(int step) const;
 
 private:
@@ -146,8 +158,12 @@ private:
     Engine eng_;
     std::unique_ptr<BatchRunner> batch_;
     bool batch_graph_ = false;
+    bool batch_sequential_ = false;
+    bool mcmc_on_ = true;
+    bool batch_autotune_ = false;
     int batch_n_ = 0;
     int persist_grid_ = 0;
+    int raster_ppl_ = 1;
     msk::McmcScratch mcmc_;
     Proposal prop_;
     void proposal_accumulate(const Engine& e);

@@ -91,6 +91,39 @@ def test_graph_batched_steps_match_oracle(case):
     _check_batched_steps(tr, cams, views, (6,))
 
 
+def test_sequential_batch_matches_oracle():
+    """The baseline mode: views one after another through the single-view kernels."""
+    g, hidden, _ = small_scene(n=2500, n_hidden=1200)
+    ds, cams = _uneven_dataset(g, hidden, n_cams=6, sizes=[(320, 320), (96, 80), (128, 96)])
+    tr = GaussianTrainer(ds, TrainingConfig(iterations=1000, sh_degree_interval=1, cap_max=2 * 2500))
+    tr.set_option("batch_sequential", 1)
+    _check_batched_steps(tr, cams, 3, (1, 2, 3))
+
+
+def test_autotuned_graph_steps_match_oracle():
+    """Every autotuner candidate (persistent raster residency) trains identically."""
+    g, hidden, _ = small_scene(n=2500, n_hidden=1200)
+    ds, cams = _uneven_dataset(g, hidden, n_cams=6, sizes=[(320, 320), (96, 80)])
+    tr = GaussianTrainer(ds, TrainingConfig(iterations=1000, sh_degree_interval=1, cap_max=2 * 2500))
+    tr.set_option("batch_graph", 1)
+    tr.set_option("autotune", 1)
+    _check_batched_steps(tr, cams, 2, tuple(range(1, 30)))
+    info = tr.autotune_info()
+    cands = [(p, g) for p, g, _ in info["timed"]]
+    assert len(cands) == 7 and len(set(cands)) == 7 and {p for p, _ in cands} == {1, 4}
+    assert info["chosen"] in cands
+    assert all(ms > 0 for _, _, ms in info["timed"])
+
+
+def test_four_pixels_per_lane_match_oracle():
+    """The one-wave-per-tile raster kernels (4 pixels per lane) in plain and batched steps."""
+    g, hidden, _ = small_scene(n=2500, n_hidden=1200)
+    ds, cams = _uneven_dataset(g, hidden, n_cams=4, sizes=[(320, 320), (336, 320)])
+    tr = GaussianTrainer(ds, TrainingConfig(iterations=1000, sh_degree_interval=1, cap_max=2 * 2500))
+    tr.set_option("raster_ppl", 4)
+    _check_batched_steps(tr, cams, 2, (1, 2))
+
+
 def test_one_view_batch_equals_plain_step():
     g, hidden, _ = small_scene(n=2000, n_hidden=1000)
     ds, cams = _uneven_dataset(g, hidden, n_cams=4)
