@@ -122,7 +122,7 @@ void Engine::trim() {
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &keys_, &ids_, &hist_, &img_,
                    &finalT_, &finalIdx_, &vrend_, &hbuf_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
-                   &prefixT_, &afterC_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_};
+                   &prefixT_, &afterC_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_, &items_, &srect_};
     for (DBuf* b : all) b->release();
     debug_valid_ = false;
     gen_++;
@@ -141,7 +141,7 @@ Engine::~Engine() {
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &keys_, &ids_, &hist_, &img_, &finalT_, &finalIdx_,
                    &vrend_, &hbuf_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
-                   &afterC_, &nanc_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_};
+                   &afterC_, &nanc_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_, &items_, &srect_};
     for (DBuf* b : all) b->release();
     if (stream_ && owns_stream_) hipStreamDestroy(stream_);
 }
@@ -180,7 +180,9 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
     ensure(radii_, n * 4); ensure(tiles_, n * 4); ensure(depths_, n * 4); ensure(rect_, n * 8); ensure(aabb_, n * 8);
     ensure(recA_, (n + 1) * 16); ensure(recB_, (n + 1) * 16); ensure(recC_, (n + 1) * 8);  // row N: zero record
     ensure(counts_, (size_t)T * 4); ensure(exact_, (size_t)T * 4); ensure(bins_, (size_t)T * 8);
-    ensure(keys_, (size_t)capacity_ * 4);
+    ensure(items_, (size_t)T * 4);
+    ensure(keys_, (size_t)capacity_ * 8);
+    ensure(srect_, n * 8);
     ensure(dkeys_, n * 4); ensure(dkeys_out_, n * 4); ensure(dvals_, n * 4); ensure(sorted_ids_, n * 4);
     ensure(temp_, msk::lists_temp_bytes(N));
     ensure(ids_, (size_t)capacity_ * 4);
@@ -360,11 +362,14 @@ msk::ListBufs Engine::list_bufs() {
     L.tile_counts = counts_.as<uint32_t>();
     L.exact_off = exact_.as<uint32_t>();
     L.bins = bins_.as<int2>();
-    L.keys = keys_.as<uint32_t>();
+    L.keys = keys_.as<uint64_t>();
+    L.srect = srect_.as<ushort4>();
     L.ids = ids_.as<uint32_t>();
     L.view = {ids_.as<uint32_t>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
     L.hist = hist_.as<uint32_t>();
     L.totals = totals_.as<uint32_t>();
+    L.item_off = items_.as<uint32_t>();
+    L.sort_grid = std::max(persist_grid_, 1);
     L.capacity = capacity_;
     L.num_tiles = tbx_ * tby_;
     L.dkeys = dkeys_.as<uint32_t>();

@@ -133,8 +133,8 @@ private:
     DBuf v_xy_, v_conic_, v_rgb_, v_opac_;
     DBuf dbg_vm_, dbg_vs_, dbg_vq_, dbg_vo_;
     // per-tile
-    DBuf counts_, exact_, bins_, totals_;
-    DBuf dkeys_, dkeys_out_, dvals_, sorted_ids_, temp_;
+    DBuf counts_, exact_, bins_, totals_, items_;
+    DBuf dkeys_, dkeys_out_, dvals_, sorted_ids_, temp_, srect_;
     DBuf wka_, wva_, wkb_, wvb_, wtemp_, wmisc_;   // own persistent-raster work queue
     // per-intersection
     DBuf keys_, ids_, hist_;

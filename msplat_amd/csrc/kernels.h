@@ -41,15 +41,19 @@ struct ListBufs {
     uint32_t* tile_counts;     // [T]
     uint32_t* exact_off;       // [T]
     int2* bins;                // [T] (start, end) of each tile's list
-    uint32_t* keys;            // [capacity] depth ranks, bucketed per tile
+    uint64_t* keys;            // [capacity] (depth rank << 32 | Gaussian id), bucketed per tile
     uint32_t* ids;             // [capacity] sorted per-tile Gaussian ids (the packed lists)
     EntryView view;            // ids + the projection's records
     uint32_t* hist;            // [chunks x T] per-chunk tile histograms -> bucket offsets
-    uint32_t* totals;          // [8]: total, padded total (reference layout), overflow flag, max count
+    uint32_t* totals;          // [8]: total, padded total (reference layout), overflow flag, max count,
+                               //      sort work items, their queue head
+    uint32_t* item_off;        // [T] first sort work item of each tile
+    int sort_grid;             // resident workgroups of the per-tile sort
     int64_t capacity;
     int num_tiles;
     // global depth order
     uint32_t *dkeys, *dkeys_out, *dvals, *sorted_ids;  // [N]
+    ushort4* srect;            // [N] tile rectangles in depth-rank order
     void* temp;                // radix-sort scratch
     size_t temp_bytes;
 };

@@ -92,6 +92,17 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* l
     return r;
 }
 
+// Next work item of the calling workgroup from a queue head in global memory (all threads
+// of the workgroup call it; the value is the same for all).
+__device__ __forceinline__ uint32_t next_item(uint32_t* ctr) {
+    __shared__ uint32_t s_item;
+    if (threadIdx.x == 0) s_item = atomicAdd(ctr, 1u);
+    __syncthreads();
+    const uint32_t it = s_item;
+    __syncthreads();
+    return it;
+}
+
 __global__ void k_fill_zero(uint32_t* p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -394,9 +405,9 @@ __device__ __forceinline__ uint32_t pad_count(uint32_t c) {  // metal:2248-2256
 constexpr int HIST_THREADS = 512;
 constexpr int MAX_CHUNK = 1984;  // Gaussians per chunk: every window fits 2048 keys
 
-__global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const uint32_t* __restrict__ sorted_ids,
-                                                            const int32_t* __restrict__ radii,
-                                                            const ushort4* __restrict__ rect,
+// srect [N]: tile rectangles in depth-rank order (empty for invisible Gaussians), so the
+// chunk kernels read them contiguously.
+__global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const ushort4* __restrict__ srect,
                                                             const ViewConsts* __restrict__ vc,
                                                             uint32_t* __restrict__ mat) {
     extern __shared__ uint32_t hist[];
@@ -405,9 +416,7 @@ __global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const 
     __syncthreads();
     const int r0 = blockIdx.x * G, r1 = min(N, r0 + G);
     for (int r = r0 + threadIdx.x; r < r1; r += HIST_THREADS) {
-        const uint32_t g = sorted_ids[r];
-        if (radii[g] <= 0) continue;
-        const ushort4 q = rect[g];
+        const ushort4 q = srect[r];
         for (int y = q.y; y < q.w; y++)
             for (int x = q.x; x < q.z; x++) atomicAdd(&hist[y * tbx + x], 1u);
     }
@@ -448,27 +457,36 @@ __global__ void k_col_scan(const ViewConsts* __restrict__ vc, int nblk, uint32_t
     counts[t] = run;
 }
 
+// Windows of the per-tile sort grouped into work items of this many (heavy tiles spread
+// over many workgroups).
+constexpr uint32_t SORT_ITEM_WINDOWS = 16;
+
 __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ vc, const uint32_t* __restrict__ counts,
                                                  uint32_t* __restrict__ exact_off, int2* __restrict__ bins,
-                                                 uint32_t* __restrict__ totals, uint32_t cap) {
+                                                 uint32_t* __restrict__ item_off, uint32_t* __restrict__ totals,
+                                                 uint32_t cap) {
     __shared__ uint32_t lw[17];
     const int T = vc->T;
-    uint32_t run_ex = 0, run_pd = 0, mx = 0;
+    uint32_t run_ex = 0, run_pd = 0, run_it = 0, mx = 0;
     for (int c0 = 0; c0 < T; c0 += 1024) {
         const int t = c0 + threadIdx.x;
         const uint32_t val = t < T ? counts[t] : 0u;
         const uint32_t pv = pad_count(val);
         mx = max(mx, val);
-        uint32_t tot_e, tot_p;
+        const uint32_t nit = (val + 64u * SORT_ITEM_WINDOWS - 1u) / (64u * SORT_ITEM_WINDOWS);
+        uint32_t tot_e, tot_p, tot_i;
         const uint32_t ex = block_exclusive_scan(val, lw, &tot_e);
         const uint32_t pd = block_exclusive_scan(pv, lw, &tot_p);
+        const uint32_t io = block_exclusive_scan(nit, lw, &tot_i);
         if (t < T) {
+            item_off[t] = run_it + io;
             exact_off[t] = run_ex + ex;
             // never past the list buffers (the reference scatter drops overflow, metal:2474-2477)
             bins[t] = make_int2((int)min(run_ex + ex, cap), (int)min(run_ex + ex + val, cap));
         }
         run_ex += tot_e;
         run_pd += tot_p;
+        run_it += tot_i;
     }
     __shared__ uint32_t smax;
     if (threadIdx.x == 0) smax = 0;
@@ -480,16 +498,19 @@ __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ 
         totals[1] = run_pd;   // what the reference's padded buckets would occupy
         totals[2] = run_ex > cap ? 1u : 0u;
         totals[3] = smax;
+        totals[4] = run_it;   // per-tile sort work items
+        totals[5] = 0u;       // their queue head
     }
 }
 
+// Each hit's key is (rank << 32) | Gaussian id: sorting by key orders by depth rank and the
+// id comes along, so the per-tile sort needs no gather.
 __global__ __launch_bounds__(HIST_THREADS) void k_tile_scatter(int N, int G, const uint32_t* __restrict__ sorted_ids,
-                                                               const int32_t* __restrict__ radii,
-                                                               const ushort4* __restrict__ rect,
+                                                               const ushort4* __restrict__ srect,
                                                                const ViewConsts* __restrict__ vc,
                                                                const uint32_t* __restrict__ mat,
                                                                const uint32_t* __restrict__ exact_off,
-                                                               uint32_t* __restrict__ keys, uint32_t cap) {
+                                                               uint64_t* __restrict__ keys, uint32_t cap) {
     extern __shared__ uint32_t cur[];
     const int tbx = vc->tbx, T = vc->T;
     const uint32_t* row = mat + (size_t)blockIdx.x * T;
@@ -497,43 +518,47 @@ __global__ __launch_bounds__(HIST_THREADS) void k_tile_scatter(int N, int G, con
     __syncthreads();
     const int r0 = blockIdx.x * G, r1 = min(N, r0 + G);
     for (int r = r0 + threadIdx.x; r < r1; r += HIST_THREADS) {
-        const uint32_t g = sorted_ids[r];
-        if (radii[g] <= 0) continue;
-        const ushort4 q = rect[g];
+        const ushort4 q = srect[r];
+        if (q.x >= q.z || q.y >= q.w) continue;
+        const uint64_t key = ((uint64_t)(uint32_t)r << 32) | (uint64_t)sorted_ids[r];
         for (int y = q.y; y < q.w; y++)
             for (int x = q.x; x < q.z; x++) {
                 const uint32_t dst = atomicAdd(&cur[y * tbx + x], 1u);
-                if (dst < cap) keys[dst] = (uint32_t)r;
+                if (dst < cap) keys[dst] = key;
             }
     }
 }
 
+template <class K> __device__ __forceinline__ K kmin(K a, K b) { return a < b ? a : b; }
+template <class K> __device__ __forceinline__ K kmax(K a, K b) { return a < b ? b : a; }
+
 // One compare-exchange stage with partner distance j < 128 on the two consecutive
 // elements e0, e0 + 1 a lane holds (e0 = 2 lane).
-__device__ __forceinline__ void reg_stage(uint32_t& v0, uint32_t& v1, uint32_t e0, uint32_t k, uint32_t j, int lane) {
+template <class K>
+__device__ __forceinline__ void reg_stage(K& v0, K& v1, uint32_t e0, uint32_t k, uint32_t j, int lane) {
     const bool asc = (e0 & k) == 0;
     if (j == 1) {
-        const uint32_t lo = min(v0, v1), hi = max(v0, v1);
+        const K lo = kmin(v0, v1), hi = kmax(v0, v1);
         v0 = asc ? lo : hi;
         v1 = asc ? hi : lo;
     } else {
         const int m = (int)(j >> 1);
-        const uint32_t p0 = __shfl_xor(v0, m, 64), p1 = __shfl_xor(v1, m, 64);
+        const K p0 = __shfl_xor(v0, m, 64), p1 = __shfl_xor(v1, m, 64);
         const bool keep_min = (((lane & m) == 0) == asc);
-        v0 = keep_min ? min(v0, p0) : max(v0, p0);
-        v1 = keep_min ? min(v1, p1) : max(v1, p1);
+        v0 = keep_min ? kmin(v0, p0) : kmax(v0, p0);
+        v1 = keep_min ? kmin(v1, p1) : kmax(v1, p1);
     }
 }
 
 // Bitonic sort of 64 keys, one per lane, ascending by lane.
-__device__ __forceinline__ uint32_t wave_sort64(uint32_t v, int lane) {
+template <class K> __device__ __forceinline__ K wave_sort64(K v, int lane) {
 #pragma unroll
     for (int k = 2; k <= 64; k <<= 1)
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            const uint32_t p = __shfl_xor(v, j, 64);
+            const K p = __shfl_xor(v, j, 64);
             const bool keep_min = (((lane & j) == 0) == ((lane & k) == 0));
-            v = keep_min ? min(v, p) : max(v, p);
+            v = keep_min ? kmin(v, p) : kmax(v, p);
         }
     return v;
 }
@@ -541,14 +566,14 @@ __device__ __forceinline__ uint32_t wave_sort64(uint32_t v, int lane) {
 // Bitonic sort of p (power of two, 256..2048) keys in a wave-private LDS region, by one
 // wave: LDS accesses of one wave are performed in issue order, so a wave-level fence
 // between stages is enough.
-__device__ void wave_sort_lds(uint32_t* a, uint32_t p, int lane) {
+template <class K> __device__ void wave_sort_lds(K* a, uint32_t p, int lane) {
     for (uint32_t k = 2; k <= p; k <<= 1)
         for (uint32_t j = k >> 1; j > 0; j >>= 1) {
             for (uint32_t q = lane; q < (p >> 1); q += 64) {
                 const uint32_t lo = ((q & ~(j - 1)) << 1) | (q & (j - 1));
                 const uint32_t hi = lo + j;
                 const bool asc = (lo & k) == 0;
-                const uint32_t x = a[lo], y = a[hi];
+                const K x = a[lo], y = a[hi];
                 if ((x > y) == asc) { a[lo] = y; a[hi] = x; }
             }
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -557,81 +582,108 @@ __device__ void wave_sort_lds(uint32_t* a, uint32_t p, int lane) {
         }
 }
 
-constexpr int SORT_THREADS = 256;
+constexpr int SORT_THREADS = 128;
 constexpr int WIN = 64;
 constexpr int WAVE_LDS_KEYS = 2048;
 
+// Work item = up to SORT_ITEM_WINDOWS windows of one tile; resident workgroups take items
+// from the queue (totals[5]) so that a tile of 100k entries is sorted by many workgroups.
 __global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(const ViewConsts* __restrict__ vc, int nblk,
                                                             const uint32_t* __restrict__ counts,
                                                             const uint32_t* __restrict__ exact_off,
+                                                            const uint32_t* __restrict__ item_off,
+                                                            uint32_t* __restrict__ totals,
                                                             const uint32_t* __restrict__ matT,
-                                                            const uint32_t* __restrict__ keys,
-                                                            const uint32_t* __restrict__ sorted_ids,
+                                                            const uint64_t* __restrict__ keys,
                                                             uint32_t* __restrict__ ids, int N, uint32_t cap) {
     extern __shared__ uint32_t sm[];
     uint32_t* off = sm;                          // [nblk + 1] segment starts of this tile
-    uint32_t* wl = sm + ((nblk + 4) & ~3);       // [4][WAVE_LDS_KEYS]
+    uint64_t* wl = reinterpret_cast<uint64_t*>(sm + ((nblk + 4) & ~3));  // [waves][WAVE_LDS_KEYS]
+    __shared__ int s_tile;
     const int T = vc->T;
-    const int t = blockIdx.x;
-    if (t >= T) return;
-    const uint32_t count = counts[t];
-    if (count == 0) return;
-    const uint32_t base = exact_off[t];
-    if (base + count > cap) {
-        for (uint32_t i = threadIdx.x; i < count; i += SORT_THREADS)
-            if (base + i < cap) ids[base + i] = (uint32_t)N;
-        return;
-    }
-    const uint32_t* rowT = matT + (size_t)t * nblk;
-    for (int b = threadIdx.x; b < nblk; b += SORT_THREADS) off[b] = rowT[b];
-    if (threadIdx.x == 0) off[nblk] = count;
-    __syncthreads();
+    const uint32_t total = totals[4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint32_t* wlds = wl + wid * WAVE_LDS_KEYS;
-    const uint32_t nwin = (count + WIN - 1) / WIN;
-    // window w = entries [first segment start >= 64 w, first segment start >= 64 (w + 1))
-    auto seg_start_at_or_after = [&](uint32_t x) -> uint32_t {
-        int lo = 0, hi = nblk;  // off[nblk] = count >= x for x <= count
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (off[mid] >= x) hi = mid; else lo = mid + 1;
-        }
-        return off[lo];
+    uint64_t* wlds = wl + wid * WAVE_LDS_KEYS;
+    const uint64_t pad = ~0ull;
+    // a key's low half is the Gaussian id (< N for every written slot)
+    auto id_of = [&](uint64_t v) -> uint32_t {
+        const uint32_t g = (uint32_t)v;
+        return g < (uint32_t)N ? g : (uint32_t)N;
     };
-    for (uint32_t w = wid; w < nwin; w += SORT_THREADS / 64) {
-        const uint32_t lo = seg_start_at_or_after(w * WIN);
-        const uint32_t hi = w + 1 < nwin ? seg_start_at_or_after((w + 1) * WIN) : count;
-        if (hi <= lo) continue;
-        const uint32_t n = hi - lo;
-        const uint32_t* ks = keys + base + lo;
-        uint32_t* out = ids + base + lo;
-        if (n <= 64) {
-            uint32_t v = (uint32_t)lane < n ? ks[lane] : 0xFFFFFFFFu;
-            v = wave_sort64(v, lane);
-            if ((uint32_t)lane < n) out[lane] = v < (uint32_t)N ? sorted_ids[v] : (uint32_t)N;
-        } else if (n <= 128) {
-            const uint32_t e0 = 2 * lane;
-            uint32_t v0 = e0 < n ? ks[e0] : 0xFFFFFFFFu, v1 = e0 + 1 < n ? ks[e0 + 1] : 0xFFFFFFFFu;
-            for (uint32_t k = 2; k <= 128; k <<= 1)
-                for (uint32_t j = k >> 1; j > 0; j >>= 1) reg_stage(v0, v1, e0, k, j, lane);
-            if (e0 < n) out[e0] = v0 < (uint32_t)N ? sorted_ids[v0] : (uint32_t)N;
-            if (e0 + 1 < n) out[e0 + 1] = v1 < (uint32_t)N ? sorted_ids[v1] : (uint32_t)N;
-        } else {
-            uint32_t p = 256;
-            while (p < n) p <<= 1;  // n <= MAX_CHUNK + WIN - 1 < WAVE_LDS_KEYS
-            for (uint32_t i = lane; i < p; i += 64) wlds[i] = i < n ? ks[i] : 0xFFFFFFFFu;
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            wave_sort_lds(wlds, p, lane);
-            for (uint32_t i = lane; i < n; i += 64) {
-                const uint32_t v = wlds[i];
-                out[i] = v < (uint32_t)N ? sorted_ids[v] : (uint32_t)N;
+    for (;;) {
+        const uint32_t it = next_item(totals + 5);
+        if (it >= total) break;
+        if (threadIdx.x == 0) {  // last tile whose first item is <= it (tiles without items share offsets)
+            int lo = 0, hi = T;
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (item_off[mid] <= it) lo = mid; else hi = mid;
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
+            s_tile = lo;
+        }
+        __syncthreads();
+        const int t = s_tile;
+        const uint32_t sub = it - item_off[t];
+        const uint32_t count = counts[t];
+        const uint32_t base = exact_off[t];
+        const uint32_t nwin = (count + WIN - 1) / WIN;
+        const uint32_t w0 = sub * SORT_ITEM_WINDOWS, w1 = min(nwin, w0 + SORT_ITEM_WINDOWS);
+        if (base + count > cap) {  // the lists overflowed: this item's entries become the empty record
+            const uint32_t e1 = min(count, w1 * WIN);
+            for (uint32_t i = w0 * WIN + threadIdx.x; i < e1; i += SORT_THREADS)
+                if (base + i < cap) ids[base + i] = (uint32_t)N;
+            continue;
+        }
+        const uint32_t* rowT = matT + (size_t)t * nblk;
+        for (int b = threadIdx.x; b < nblk; b += SORT_THREADS) off[b] = rowT[b];
+        if (threadIdx.x == 0) off[nblk] = count;
+        __syncthreads();
+        // window w = entries [first segment start >= 64 w, first segment start >= 64 (w + 1))
+        auto seg_start_at_or_after = [&](uint32_t x) -> uint32_t {
+            int lo = 0, hi = nblk;  // off[nblk] = count >= x for x <= count
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (off[mid] >= x) hi = mid; else lo = mid + 1;
+            }
+            return off[lo];
+        };
+        for (uint32_t w = w0 + wid; w < w1; w += SORT_THREADS / 64) {
+            const uint32_t lo = seg_start_at_or_after(w * WIN);
+            const uint32_t hi = w + 1 < nwin ? seg_start_at_or_after((w + 1) * WIN) : count;
+            if (hi <= lo) continue;
+            const uint32_t n = hi - lo;
+            const uint64_t* ks = keys + base + lo;
+            uint32_t* out = ids + base + lo;
+            if (n <= 64) {
+                uint64_t v = (uint32_t)lane < n ? ks[lane] : pad;
+                v = wave_sort64(v, lane);
+                if ((uint32_t)lane < n) out[lane] = id_of(v);
+            } else if (n <= 128) {
+                const uint32_t e0 = 2 * lane;
+                uint64_t v0 = e0 < n ? ks[e0] : pad, v1 = e0 + 1 < n ? ks[e0 + 1] : pad;
+                for (uint32_t k = 2; k <= 128; k <<= 1)
+                    for (uint32_t j = k >> 1; j > 0; j >>= 1) reg_stage(v0, v1, e0, k, j, lane);
+                if (e0 < n) out[e0] = id_of(v0);
+                if (e0 + 1 < n) out[e0 + 1] = id_of(v1);
+            } else {
+                uint32_t p = 256;
+                while (p < n) p <<= 1;  // n <= MAX_CHUNK + WIN - 1 < WAVE_LDS_KEYS
+                for (uint32_t i = lane; i < p; i += 64) wlds[i] = i < n ? ks[i] : pad;
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                wave_sort_lds(wlds, p, lane);
+                for (uint32_t i = lane; i < n; i += 64) out[i] = id_of(wlds[i]);
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+            }
         }
     }
+}
+
+// LDS of the per-tile sort: segment offsets + one u64 window buffer per wave
+static size_t sort_lds_bytes(int nblk) {
+    return ((size_t)(nblk + 4) & ~(size_t)3) * 4 + (size_t)(SORT_THREADS / 64) * WAVE_LDS_KEYS * 8;
 }
 
 static int hist_blocks(int N) {
@@ -745,7 +797,7 @@ static size_t lds_limit() {
 }
 
 bool lists_supported(int N, int T) {
-    const size_t sort_lds = ((size_t)(hist_blocks(N) + 4) & ~(size_t)3) * 4 + 4 * WAVE_LDS_KEYS * 4;
+    const size_t sort_lds = sort_lds_bytes(hist_blocks(N));
     return (size_t)T * 4 <= lds_limit() && sort_lds <= lds_limit();
 }
 
@@ -766,12 +818,22 @@ void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t
     }
 }
 
+__global__ void k_gather_rect(int N, const uint32_t* __restrict__ sorted_ids, const int32_t* __restrict__ radii,
+                              const ushort4* __restrict__ rect, ushort4* __restrict__ srect) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const uint32_t g = sorted_ids[r];
+    srect[r] = radii[g] > 0 ? rect[g] : make_ushort4(0, 0, 0, 0);
+}
+
 void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L) {
     if (N <= 0) return;
     // keys start in (dkeys_out, sorted_ids); four passes ping-pong and end there
     hipLaunchKernelGGL(k_depth_keys, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.depths, L.dkeys_out,
                        L.sorted_ids);
     radix_sort_pairs(s, N, L.dkeys_out, L.sorted_ids, L.dkeys, L.dvals, L.temp, 4);
+    hipLaunchKernelGGL(k_gather_rect, dim3(cdiv(N, 256)), dim3(256), 0, s, N, L.sorted_ids, pj.radii, pj.rect,
+                       L.srect);
 }
 
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L) {
@@ -779,14 +841,13 @@ void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, 
         const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
         const size_t lds = (size_t)g.T * 4;
         if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_hist, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, pj.radii, pj.rect,
-                           vc, L.hist);
+        hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.srect, vc, L.hist);
         hipLaunchKernelGGL(k_col_scan, dim3(cdiv(g.T, 64)), dim3(64), 0, s, vc, nblk, L.hist,
                            L.hist + (size_t)nblk * g.T, L.tile_counts);
     } else {
         fill_zero(s, L.tile_counts, (size_t)g.T * 4);
     }
-    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, vc, L.tile_counts, L.exact_off, L.bins, L.totals,
+    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, vc, L.tile_counts, L.exact_off, L.bins, L.item_off, L.totals,
                        (uint32_t)L.capacity);
 }
 
@@ -795,18 +856,18 @@ void lists_scatter(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc
     const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
     const size_t lds = (size_t)g.T * 4;
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_tile_scatter, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, pj.radii, pj.rect, vc,
-                       L.hist, L.exact_off, L.keys, (uint32_t)L.capacity);
+    hipLaunchKernelGGL(k_tile_scatter, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, L.srect, vc, L.hist,
+                       L.exact_off, L.keys, (uint32_t)L.capacity);
 }
 
 void lists_sort_pack(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N <= 0) return;
     const int nblk = hist_blocks(N);
-    const size_t lds = ((size_t)(nblk + 4) & ~(size_t)3) * 4 + 4 * WAVE_LDS_KEYS * 4;
+    const size_t lds = sort_lds_bytes(nblk);
     if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_sort_pack, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_sort_pack, dim3(g.T), dim3(SORT_THREADS), lds, s, vc, nblk, L.tile_counts, L.exact_off,
-                       L.hist + (size_t)nblk * g.T,
-                       L.keys, L.sorted_ids, L.ids, N, (uint32_t)L.capacity);
+    hipLaunchKernelGGL(k_sort_pack, dim3(L.sort_grid), dim3(SORT_THREADS), lds, s, vc, nblk, L.tile_counts,
+                       L.exact_off, L.item_off, L.totals, L.hist + (size_t)nblk * g.T, L.keys, L.ids, N,
+                       (uint32_t)L.capacity);
 }
 
 // ------------------------------------------------------------------------------------
@@ -891,16 +952,6 @@ __global__ void k_work_keys(const RasterView* __restrict__ views, int B, int Tma
     const uint32_t len = (uint32_t)max(0, r.y - r.x);
     keys[k] = 0xFFFEu - min(len, 0xFFFEu);
     vals[k] = ((uint32_t)v << WORK_TILE_BITS) | (uint32_t)t;
-}
-
-// Next work item of the calling workgroup, or >= n_work when the queue is drained.
-__device__ __forceinline__ uint32_t next_item(uint32_t* ctr) {
-    __shared__ uint32_t s_item;
-    if (threadIdx.x == 0) s_item = atomicAdd(ctr, 1u);
-    __syncthreads();
-    const uint32_t it = s_item;
-    __syncthreads();
-    return it;
 }
 
 __global__ __launch_bounds__(RB) void k_raster_fwd_persist(const RasterView* __restrict__ views,
