@@ -57,6 +57,8 @@ def parse():
     p.add_argument("--sh-bf16", action="store_true", help="bfloat16 SH storage for the c1 suite")
     p.add_argument("--raster-ppl", type=int, default=0, choices=[0, 1, 4],
                    help="pixels per lane of the persistent raster kernels (0: library default)")
+    p.add_argument("--list-key-bytes", type=int, default=0, choices=[0, 4, 8],
+                   help="tile-list key width (0: library default)")
     return p.parse_args()
 
 
@@ -73,6 +75,19 @@ def pair_counts(final_idx, bins, W, H):
     start = bins[tile, 0].astype(np.int64)
     fi = final_idx.reshape(-1).astype(np.int64)
     return int(np.maximum(fi - start + 1, 0).sum())
+
+
+def tile_work(final_idx, bins, W, H):
+    """Entries each tile's raster workgroup walks (through its last contributor), as
+    (largest, total): the longest tile bounds a persistent launch from below."""
+    tbx = (W + 15) // 16
+    T = bins.shape[0]
+    ys, xs = np.divmod(np.arange(W * H), W)
+    tile = (ys // 16) * tbx + xs // 16
+    last = np.full(T, -1, np.int64)
+    np.maximum.at(last, tile, final_idx.reshape(-1).astype(np.int64))
+    work = np.maximum(last - bins[:, 0].astype(np.int64) + 1, 0)
+    return int(work.max()), int(work.sum())
 
 
 def load_traffic(kernel, workload):
@@ -114,6 +129,8 @@ def main():
         tr.set_option("sh_bf16", 1)
     if a.raster_ppl:
         tr.set_option("raster_ppl", a.raster_ppl)
+    if a.list_key_bytes:
+        tr.set_option("list_key_bytes", a.list_key_bytes)
     eng = tr.engine_handle
     log(f"setup {time.time() - t_setup:.1f}s ({ds.num_train} cameras)")
 
@@ -167,6 +184,7 @@ def main():
     bins = read("tile_bins", np.int32, 2 * T).reshape(T, 2)
     final_idx = read("final_idx", np.int32, W * H)
     pairs = pair_counts(final_idx, bins, W, H)
+    tw_max, tw_total = tile_work(final_idx, bins, W, H)
     L.msplat_engine_stats(eng, stats)
     I = int(stats[0])
 
@@ -243,6 +261,7 @@ def main():
                    "gaussians": N, "width": W, "height": H, "fx": a.fx, "cameras": ds.num_train,
                    "hidden_gt_gaussians": a.hidden, "sh_degree": 3, "loss": "0.8 L1 + 0.2 (1 - SSIM)",
                    "cap_max": a.cap, "intersections": I, "visible": V, "raster_pairs": pairs,
+                   "tile_entries_walked_max": tw_max, "tile_entries_walked_total": tw_total,
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -181,7 +181,7 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
     ensure(recA_, (n + 1) * 16); ensure(recB_, (n + 1) * 16); ensure(recC_, (n + 1) * 8);  // row N: zero record
     ensure(counts_, (size_t)T * 4); ensure(exact_, (size_t)T * 4); ensure(bins_, (size_t)T * 8);
     ensure(items_, (size_t)T * 4);
-    ensure(keys_, (size_t)capacity_ * 8);
+    ensure(keys_, (size_t)capacity_ * key_bytes_);
     ensure(srect_, n * 8);
     ensure(dkeys_, n * 4); ensure(dkeys_out_, n * 4); ensure(dvals_, n * 4); ensure(sorted_ids_, n * 4);
     ensure(temp_, msk::lists_temp_bytes(N));
@@ -362,7 +362,8 @@ msk::ListBufs Engine::list_bufs() {
     L.tile_counts = counts_.as<uint32_t>();
     L.exact_off = exact_.as<uint32_t>();
     L.bins = bins_.as<int2>();
-    L.keys = keys_.as<uint64_t>();
+    L.keys = keys_.p;
+    L.key_bytes = key_bytes_;
     L.srect = srect_.as<ushort4>();
     L.ids = ids_.as<uint32_t>();
     L.view = {ids_.as<uint32_t>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};

@@ -104,6 +104,9 @@ public:
     int persist_grid() const { return persist_grid_; }
     // Pixels per lane of the persistent raster kernels (1 or 4; the image is the same).
     void set_raster_ppl(int p) { ppl_ = p == 4 ? 4 : 1; }
+    // Width of the tile-list keys: 8 carries the Gaussian id with the depth rank, 4 looks
+    // it up after the per-tile sort (the lists are the same).
+    void set_list_key_bytes(int b) { key_bytes_ = b == 8 ? 8 : 4; }
     int raster_ppl() const { return ppl_; }
 
 private:
@@ -123,6 +126,7 @@ private:
     int64_t gen_ = 0;   // bumps whenever a buffer is re-allocated
     int persist_grid_ = 0;
     int ppl_ = 1;
+    int key_bytes_ = 4;
     // view constants: host copy, device copy in use, own device block [ViewConsts | RasterView]
     msk::ViewConsts vch_{};
     const msk::ViewConsts* vcd_ = nullptr;

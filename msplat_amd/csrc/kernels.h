@@ -41,7 +41,9 @@ struct ListBufs {
     uint32_t* tile_counts;     // [T]
     uint32_t* exact_off;       // [T]
     int2* bins;                // [T] (start, end) of each tile's list
-    uint64_t* keys;            // [capacity] (depth rank << 32 | Gaussian id), bucketed per tile
+    void* keys;                // [capacity] list keys bucketed per tile: (depth rank << 32 | Gaussian id)
+                               // with key_bytes 8, the depth rank alone with key_bytes 4
+    int key_bytes = 4;
     uint32_t* ids;             // [capacity] sorted per-tile Gaussian ids (the packed lists)
     EntryView view;            // ids + the projection's records
     uint32_t* hist;            // [chunks x T] per-chunk tile histograms -> bucket offsets

@@ -503,14 +503,21 @@ __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ 
     }
 }
 
-// Each hit's key is (rank << 32) | Gaussian id: sorting by key orders by depth rank and the
-// id comes along, so the per-tile sort needs no gather.
+// List keys: K = uint64_t holds (rank << 32) | Gaussian id, so the per-tile sort carries the
+// id along; K = uint32_t holds the rank only and the sort looks the id up (sorted_ids[rank]).
+template <class K> __device__ __forceinline__ K list_key(uint32_t r, uint32_t g);
+template <> __device__ __forceinline__ uint64_t list_key<uint64_t>(uint32_t r, uint32_t g) {
+    return ((uint64_t)r << 32) | (uint64_t)g;
+}
+template <> __device__ __forceinline__ uint32_t list_key<uint32_t>(uint32_t r, uint32_t) { return r; }
+
+template <class K>
 __global__ __launch_bounds__(HIST_THREADS) void k_tile_scatter(int N, int G, const uint32_t* __restrict__ sorted_ids,
                                                                const ushort4* __restrict__ srect,
                                                                const ViewConsts* __restrict__ vc,
                                                                const uint32_t* __restrict__ mat,
                                                                const uint32_t* __restrict__ exact_off,
-                                                               uint64_t* __restrict__ keys, uint32_t cap) {
+                                                               K* __restrict__ keys, uint32_t cap) {
     extern __shared__ uint32_t cur[];
     const int tbx = vc->tbx, T = vc->T;
     const uint32_t* row = mat + (size_t)blockIdx.x * T;
@@ -520,7 +527,7 @@ __global__ __launch_bounds__(HIST_THREADS) void k_tile_scatter(int N, int G, con
     for (int r = r0 + threadIdx.x; r < r1; r += HIST_THREADS) {
         const ushort4 q = srect[r];
         if (q.x >= q.z || q.y >= q.w) continue;
-        const uint64_t key = ((uint64_t)(uint32_t)r << 32) | (uint64_t)sorted_ids[r];
+        const K key = list_key<K>((uint32_t)r, sizeof(K) == 8 ? sorted_ids[r] : 0u);
         for (int y = q.y; y < q.w; y++)
             for (int x = q.x; x < q.z; x++) {
                 const uint32_t dst = atomicAdd(&cur[y * tbx + x], 1u);
@@ -582,33 +589,44 @@ template <class K> __device__ void wave_sort_lds(K* a, uint32_t p, int lane) {
         }
 }
 
-constexpr int SORT_THREADS = 128;
+constexpr int SORT_THREADS_MAX = 256;
 constexpr int WIN = 64;
 constexpr int WAVE_LDS_KEYS = 2048;
 
 // Work item = up to SORT_ITEM_WINDOWS windows of one tile; resident workgroups take items
 // from the queue (totals[5]) so that a tile of 100k entries is sorted by many workgroups.
-__global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(const ViewConsts* __restrict__ vc, int nblk,
+// 64-bit keys: 2 waves per workgroup (their LDS window buffers are twice as large).
+template <class K> struct SortCfg;
+template <> struct SortCfg<uint32_t> { static constexpr int threads = 256; };
+template <> struct SortCfg<uint64_t> { static constexpr int threads = 128; };
+
+template <class K>
+__global__ __launch_bounds__(SortCfg<K>::threads) void k_sort_pack(const ViewConsts* __restrict__ vc, int nblk,
                                                             const uint32_t* __restrict__ counts,
                                                             const uint32_t* __restrict__ exact_off,
                                                             const uint32_t* __restrict__ item_off,
                                                             uint32_t* __restrict__ totals,
                                                             const uint32_t* __restrict__ matT,
-                                                            const uint64_t* __restrict__ keys,
+                                                            const K* __restrict__ keys,
+                                                            const uint32_t* __restrict__ sorted_ids,
                                                             uint32_t* __restrict__ ids, int N, uint32_t cap) {
+    constexpr int SORT_THREADS = SortCfg<K>::threads;
     extern __shared__ uint32_t sm[];
     uint32_t* off = sm;                          // [nblk + 1] segment starts of this tile
-    uint64_t* wl = reinterpret_cast<uint64_t*>(sm + ((nblk + 4) & ~3));  // [waves][WAVE_LDS_KEYS]
+    K* wl = reinterpret_cast<K*>(sm + ((nblk + 4) & ~3));  // [waves][WAVE_LDS_KEYS]
     __shared__ int s_tile;
     const int T = vc->T;
     const uint32_t total = totals[4];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    uint64_t* wlds = wl + wid * WAVE_LDS_KEYS;
-    const uint64_t pad = ~0ull;
-    // a key's low half is the Gaussian id (< N for every written slot)
-    auto id_of = [&](uint64_t v) -> uint32_t {
-        const uint32_t g = (uint32_t)v;
-        return g < (uint32_t)N ? g : (uint32_t)N;
+    K* wlds = wl + wid * WAVE_LDS_KEYS;
+    const K pad = (K)~(K)0;
+    // 64-bit: the low half is the Gaussian id; 32-bit: the rank, looked up in sorted_ids
+    auto id_of = [&](K v) -> uint32_t {
+        if (sizeof(K) == 8) {
+            const uint32_t g = (uint32_t)v;
+            return g < (uint32_t)N ? g : (uint32_t)N;
+        }
+        return (uint32_t)v < (uint32_t)N ? sorted_ids[(uint32_t)v] : (uint32_t)N;
     };
     for (;;) {
         const uint32_t it = next_item(totals + 5);
@@ -652,15 +670,15 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(const ViewConsts* __
             const uint32_t hi = w + 1 < nwin ? seg_start_at_or_after((w + 1) * WIN) : count;
             if (hi <= lo) continue;
             const uint32_t n = hi - lo;
-            const uint64_t* ks = keys + base + lo;
+            const K* ks = keys + base + lo;
             uint32_t* out = ids + base + lo;
             if (n <= 64) {
-                uint64_t v = (uint32_t)lane < n ? ks[lane] : pad;
+                K v = (uint32_t)lane < n ? ks[lane] : pad;
                 v = wave_sort64(v, lane);
                 if ((uint32_t)lane < n) out[lane] = id_of(v);
             } else if (n <= 128) {
                 const uint32_t e0 = 2 * lane;
-                uint64_t v0 = e0 < n ? ks[e0] : pad, v1 = e0 + 1 < n ? ks[e0 + 1] : pad;
+                K v0 = e0 < n ? ks[e0] : pad, v1 = e0 + 1 < n ? ks[e0 + 1] : pad;
                 for (uint32_t k = 2; k <= 128; k <<= 1)
                     for (uint32_t j = k >> 1; j > 0; j >>= 1) reg_stage(v0, v1, e0, k, j, lane);
                 if (e0 < n) out[e0] = id_of(v0);
@@ -681,9 +699,10 @@ __global__ __launch_bounds__(SORT_THREADS) void k_sort_pack(const ViewConsts* __
     }
 }
 
-// LDS of the per-tile sort: segment offsets + one u64 window buffer per wave
-static size_t sort_lds_bytes(int nblk) {
-    return ((size_t)(nblk + 4) & ~(size_t)3) * 4 + (size_t)(SORT_THREADS / 64) * WAVE_LDS_KEYS * 8;
+// LDS of the per-tile sort: segment offsets + one window buffer per wave
+static size_t sort_lds_bytes(int nblk, int key_bytes) {
+    const int waves = (key_bytes == 8 ? SortCfg<uint64_t>::threads : SortCfg<uint32_t>::threads) / 64;
+    return ((size_t)(nblk + 4) & ~(size_t)3) * 4 + (size_t)waves * WAVE_LDS_KEYS * key_bytes;
 }
 
 static int hist_blocks(int N) {
@@ -797,7 +816,7 @@ static size_t lds_limit() {
 }
 
 bool lists_supported(int N, int T) {
-    const size_t sort_lds = sort_lds_bytes(hist_blocks(N));
+    const size_t sort_lds = std::max(sort_lds_bytes(hist_blocks(N), 4), sort_lds_bytes(hist_blocks(N), 8));
     return (size_t)T * 4 <= lds_limit() && sort_lds <= lds_limit();
 }
 
@@ -855,19 +874,34 @@ void lists_scatter(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc
     if (N <= 0) return;
     const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
     const size_t lds = (size_t)g.T * 4;
-    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_scatter, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_tile_scatter, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, L.srect, vc, L.hist,
-                       L.exact_off, L.keys, (uint32_t)L.capacity);
+    if (L.key_bytes == 8) {
+        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_scatter<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_tile_scatter<uint64_t>, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, L.srect,
+                           vc, L.hist, L.exact_off, reinterpret_cast<uint64_t*>(L.keys), (uint32_t)L.capacity);
+    } else {
+        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_scatter<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_tile_scatter<uint32_t>, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, L.srect,
+                           vc, L.hist, L.exact_off, reinterpret_cast<uint32_t*>(L.keys), (uint32_t)L.capacity);
+    }
 }
 
 void lists_sort_pack(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N <= 0) return;
     const int nblk = hist_blocks(N);
-    const size_t lds = sort_lds_bytes(nblk);
-    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_sort_pack, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_sort_pack, dim3(L.sort_grid), dim3(SORT_THREADS), lds, s, vc, nblk, L.tile_counts,
-                       L.exact_off, L.item_off, L.totals, L.hist + (size_t)nblk * g.T, L.keys, L.ids, N,
-                       (uint32_t)L.capacity);
+    const uint32_t* matT = L.hist + (size_t)nblk * g.T;
+    if (L.key_bytes == 8) {
+        const size_t lds = sort_lds_bytes(nblk, 8);
+        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_sort_pack<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_sort_pack<uint64_t>, dim3(L.sort_grid), dim3(SortCfg<uint64_t>::threads), lds, s, vc, nblk,
+                           L.tile_counts, L.exact_off, L.item_off, L.totals, matT,
+                           reinterpret_cast<const uint64_t*>(L.keys), L.sorted_ids, L.ids, N, (uint32_t)L.capacity);
+    } else {
+        const size_t lds = sort_lds_bytes(nblk, 4);
+        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_sort_pack<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(k_sort_pack<uint32_t>, dim3(L.sort_grid / 2), dim3(SortCfg<uint32_t>::threads), lds, s, vc,
+                           nblk, L.tile_counts, L.exact_off, L.item_off, L.totals, matT,
+                           reinterpret_cast<const uint32_t*>(L.keys), L.sorted_ids, L.ids, N, (uint32_t)L.capacity);
+    }
 }
 
 // ------------------------------------------------------------------------------------
