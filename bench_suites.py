@@ -10,14 +10,16 @@ c3_bf16     3M clustered Gaussians at 1920x1080: it/s with all-fp32 storage and 
             bfloat16 SH coefficients + moments, per-stage times and HBM bytes saved.
 c2          batched multi-view: 2M clustered Gaussians, 64 cameras whose sizes are drawn from
             640x480 / 1024x768 / 1600x1040 / 1920x1080, 8 views per step, 500 steps from the
-            same initial state in three modes: the views one after another through the
+            same initial state in four modes: the views one after another through the
             single-view kernels ("sequential", the baseline), one persistent raster launch
-            per direction over all views ("persistent"), and that recorded once as a HIP graph
-            and replayed, with the residency autotuned ("graph"); views/s of each and the
+            per direction over all views ("persistent"), that recorded once as a HIP graph
+            and replayed ("graph_default"), and the graph with the launch geometry autotuned
+            ("graph"); views/s of each and the
             batch loss / held-out PSNR after 500 steps (equal up to float summation order).
 c4          learned relocation proposal: 2M clustered Gaussians at 1600x1040, 8 withheld
             views; the plain GPU form (one view per step, opacity-weighted relocation), 8-view
-            batches, and 8-view batches with the learned proposal (judged by held-out loss),
+            batches (Adam steps scaled by sqrt(8)), and 8-view batches with the learned
+            proposal (judged by held-out loss),
             all from the same state; steps and seconds for each to reach the plain form's
             final held-out PSNR within 0.1 dB, and the trajectories.
 bf16_error  error against the FP64 oracle of the fp32 and the bf16-SH GPU paths, after 1
@@ -235,11 +237,11 @@ def c2(a):
     ds = Dataset(f"synthetic:c1:n={n},cams=64,mixed=1", eval_mode=True, test_every=8)
     sizes = sorted({(c["width"], c["height"]) for c in (ds.camera(i, image=False) for i in range(ds.num_train))})
     res = {}
-    for mode in ("sequential", "persistent", "graph"):
+    for mode in ("sequential", "persistent", "graph_default", "graph"):
         tr = GaussianTrainer(ds, TrainingConfig(iterations=30000, cap_max=n))
         tr.set_option("mcmc", 0)
         tr.set_option("batch_sequential", 1 if mode == "sequential" else 0)
-        tr.set_option("batch_graph", 1 if mode == "graph" else 0)
+        tr.set_option("batch_graph", 1 if mode.startswith("graph") else 0)
         if mode == "graph":
             tr.set_option("autotune", 1)
         L.msplat_sync()
@@ -283,11 +285,13 @@ def c4(a):
     steps = int(os.environ.get("MSPLAT_C4_STEPS", "1200"))
     every = 100
     ds = Dataset(f"synthetic:c1:n={n},cams=72", eval_mode=True, test_every=9)
+    lr_scale = float(os.environ.get("MSPLAT_C4_LR_SCALE", str(np.sqrt(views))))
     runs = {}
     for mode in ("plain", "batched", "batched_learned"):
         tr = GaussianTrainer(ds, TrainingConfig(iterations=steps, cap_max=int(1.5 * n)))
         if mode != "plain":
             tr.set_option("batch_graph", 1)
+            tr.set_option("batch_lr_scale", lr_scale)
         if mode == "batched_learned":
             tr.set_option("learned_relocation", 1)
         traj = []
@@ -311,6 +315,7 @@ def c4(a):
     target = runs["plain"]["trajectory"][-1]["psnr"] - 0.1
     reach = {m: next((p for p in r["trajectory"] if p["psnr"] >= target), None) for m, r in runs.items()}
     print(json.dumps({"suite": "c4", "gaussians": n, "view": "1600x1040", "heldout_views": ds.num_test,
-                      "steps": steps, "target_psnr": round(target, 3), "reaches_target": reach,
+                      "steps": steps, "batched_lr_scale": round(lr_scale, 4),
+                      "target_psnr": round(target, 3), "reaches_target": reach,
                       "final_psnr": {m: r["trajectory"][-1]["psnr"] for m, r in runs.items()},
                       "runs": runs}), flush=True)

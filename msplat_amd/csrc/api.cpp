@@ -787,6 +787,11 @@ int msplat_trainer_set_option(MsplatTrainer t, const char* name, double value) {
         TR(t)->model->set_batch_sequential(value != 0.0);
         return 0;
     }
+    if (n == "batch_lr_scale") {
+        if (!(value > 0.0) || value > 64.0) throw std::invalid_argument("batch_lr_scale must be in (0, 64]");
+        TR(t)->model->set_batch_lr_scale((float)value);
+        return 0;
+    }
     if (n == "mcmc") {
         TR(t)->model->set_mcmc_enabled(value != 0.0);
         return 0;

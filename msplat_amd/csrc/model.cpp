@@ -300,6 +300,8 @@ void Model::full_iteration_batch(const TrainCamera* const* cams, int B, int step
         views[v].inv_n = 1.0f / (float)(cam.height * cam.width * 3 * B);
         hp = step_params(cam, step, ssim_weight, os_.adam_step_count);
     }
+    // large-batch learning-rate scaling of the Adam steps (1 = the single-view rates)
+    for (int g = 0; g < 6; g++) hp.step_size[g] *= batch_lr_scale_;
     if (!batch_) {
         batch_.reset(new BatchRunner(eng_.stream()));
         batch_->set_persist_grid(persist_grid_);

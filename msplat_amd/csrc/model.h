@@ -106,6 +106,9 @@ public:
     // Relocation / growth on their schedule (on by default; off only for benchmarks that
     // compare step throughput at a fixed Gaussian count).
     void set_mcmc_enabled(bool on) { mcmc_on_ = on; }
+    // Factor on every Adam step size of batched steps (large-batch scaling; 1 by default).
+    void set_batch_lr_scale(float f) { batch_lr_scale_ = f; }
+    float batch_lr_scale() const { return batch_lr_scale_; }
     void set_batch_autotune(bool on) {
         batch_autotune_ = on;
         if (batch_) batch_->set_autotune(on);
@@ -160,6 +163,7 @@ private:
     bool batch_graph_ = false;
     bool batch_sequential_ = false;
     bool mcmc_on_ = true;
+    float batch_lr_scale_ = 1.0f;
     bool batch_autotune_ = false;
     int batch_n_ = 0;
     int persist_grid_ = 0;
