@@ -1,8 +1,9 @@
 // batch.cpp — multi-view batched step (no counterpart in the reference, which trains one
 // view per step: Model::fullIteration, core/src/model.cpp:901-980). Semantics: every view
 // runs the single-view forward/backward on the same parameters; the projection VJPs and SH
-// colour gradients are summed in view order; one Adam/SGLD update follows, SH Adam for the
-// Gaussians that had a finite colour gradient in at least one view.
+// colour gradients are summed over the views in order of decreasing size (ties in view
+// order); one Adam/SGLD update follows, SH Adam for the Gaussians that had a finite colour
+// gradient in at least one view.
 #include "batch.h"
 
 #include <algorithm>
@@ -15,6 +16,7 @@ bool BatchRunner::Key::operator==(const Key& o) const {
     if (N != o.N || B != o.B || du != o.du || degree != o.degree || bf16 != o.bf16 || grid != o.grid || ppl != o.ppl)
         return false;
     if (g.W != o.g.W || g.H != o.g.H || g.T != o.g.T || gens != o.gens || nanc != o.nanc) return false;
+    if (slot_geom != o.slot_geom) return false;
     for (int k = 0; k < 8; k++)
         if (state[k] != o.state[k]) return false;
     for (int k = 0; k < 7; k++)
@@ -170,32 +172,42 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
         slots_.back()->set_persist_grid(persist_grid_);
         slots_.back()->set_raster_ppl(ppl_);
     }
-    sizes_.assign(B, {0, 0});
-    bool graphable = graph_on_ && !sequential_;
-    msk::Geom gmax{};
     for (int v = 0; v < B; v++) {
         const MsplatCamera& c = views[v].cam;
         if (c.width <= 0 || c.height <= 0 || !views[v].gt) throw std::invalid_argument("batched view without image");
-        const msk::Geom g = msk::geom_of(c.width, c.height);
-        if (!msk::lists_supported(st.N, g.T)) throw std::invalid_argument("view too large for the tile lists");
-        sizes_[v] = {c.width, c.height};
-        gmax.W = std::max(gmax.W, g.W);
-        gmax.H = std::max(gmax.H, g.H);
+        if (!msk::lists_supported(st.N, msk::geom_of(c.width, c.height).T))
+            throw std::invalid_argument("view too large for the tile lists");
     }
+    // slot k runs view order[k]: views by decreasing size (ties in view order), so that each
+    // slot's recorded geometry tracks the sizes that rank of the batch actually takes
+    std::vector<int> order((size_t)B);
+    for (int v = 0; v < B; v++) order[v] = v;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return (int64_t)views[a].cam.width * views[a].cam.height > (int64_t)views[b].cam.width * views[b].cam.height;
+    });
+    sizes_.assign(B, {0, 0});
+    slot_of_view_.assign(B, 0);
+    for (int k = 0; k < B; k++) {
+        sizes_[k] = {views[order[k]].cam.width, views[order[k]].cam.height};
+        slot_of_view_[order[k]] = k;
+    }
+    bool graphable = graph_on_ && !sequential_;
+    if ((int)hwv_.size() < B) hwv_.resize(B, msk::Geom{});
     if (graphable) {
-        hw_.W = std::max(hw_.W, gmax.W);
-        hw_.H = std::max(hw_.H, gmax.H);
-        hw_ = msk::geom_of(hw_.W, hw_.H);
-        if (!msk::lists_supported(st.N, hw_.T)) graphable = false;
+        for (int k = 0; k < B; k++) {
+            hwv_[k] = msk::geom_of(std::max(hwv_[k].W, sizes_[k].first), std::max(hwv_[k].H, sizes_[k].second));
+            if (!msk::lists_supported(st.N, hwv_[k].T)) graphable = false;
+        }
+        hw_ = hwv_[0];
     }
-    // per-view buffers: the recorded geometry (graph) or each view's own
-    for (int v = 0; v < B; v++) {
-        const msk::Geom g = msk::geom_of(sizes_[v].first, sizes_[v].second);
-        slots_[v]->prepare(st.N, graphable ? hw_ : g, g, true);
+    // per-slot buffers: the slot's recorded geometry (graph) or its view's own
+    for (int k = 0; k < B; k++) {
+        const msk::Geom g = msk::geom_of(sizes_[k].first, sizes_[k].second);
+        slots_[k]->prepare(st.N, graphable ? hwv_[k] : g, g, true);
     }
     int Tmax = 0;
-    for (int v = 0; v < B; v++)
-        Tmax = std::max(Tmax, graphable ? hw_.T : msk::geom_of(sizes_[v].first, sizes_[v].second).T);
+    for (int k = 0; k < B; k++)
+        Tmax = std::max(Tmax, graphable ? hwv_[k].T : msk::geom_of(sizes_[k].first, sizes_[k].second).T);
     const size_t items = (size_t)B * Tmax;
     wka_.ensure(items * 4); wva_.ensure(items * 4); wkb_.ensure(items * 4); wvb_.ensure(items * 4);
     wtemp_.ensure(msk::work_temp_bytes((int)items));
@@ -207,14 +219,14 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
     // host image of the step block
     std::vector<uint8_t> h(bytes, 0);
     std::memcpy(h.data(), &hp, sizeof(hp));
-    for (int v = 0; v < B; v++) {
-        const msk::ViewConsts vc =
-            Engine::make_view(views[v].cam, views[v].gt, hp.background, hp.ssim_weight, views[v].inv_n);
-        const msk::ViewConsts* d = reinterpret_cast<const msk::ViewConsts*>(block_.as<uint8_t>() + off_vc) + v;
-        slots_[v]->bind_view(vc, d);
-        std::memcpy(h.data() + off_vc + sizeof(vc) * v, &vc, sizeof(vc));
-        const msk::RasterView rv = slots_[v]->raster_view(true);
-        std::memcpy(h.data() + off_rv + sizeof(rv) * v, &rv, sizeof(rv));
+    for (int k = 0; k < B; k++) {
+        const BatchView& bv = views[order[k]];
+        const msk::ViewConsts vc = Engine::make_view(bv.cam, bv.gt, hp.background, hp.ssim_weight, bv.inv_n);
+        const msk::ViewConsts* d = reinterpret_cast<const msk::ViewConsts*>(block_.as<uint8_t>() + off_vc) + k;
+        slots_[k]->bind_view(vc, d);
+        std::memcpy(h.data() + off_vc + sizeof(vc) * k, &vc, sizeof(vc));
+        const msk::RasterView rv = slots_[k]->raster_view(true);
+        std::memcpy(h.data() + off_rv + sizeof(rv) * k, &rv, sizeof(rv));
     }
     ring_.upload(s_, block_.p, h.data(), bytes);
     if (sequential_) {
@@ -223,9 +235,10 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
         void* zs[7] = {A.v_mean, A.v_scale, A.v_quat, A.v_opac, A.v_dc, A.v_rest, A.vis};
         const size_t zb[7] = {n * 12, n * 12, n * 16, n * 4, n * 12, n * (size_t)std::max(nb3, 1) * 4, n * 4};
         for (int k = 0; k < 7; k++) msk::fill_zero(s_, zs[k], zb[k]);
-        for (int v = 0; v < B; v++) {
-            Engine& e = *slots_[v];
-            e.own_view(Engine::make_view(views[v].cam, views[v].gt, hp.background, hp.ssim_weight, views[v].inv_n));
+        for (int k = 0; k < B; k++) {
+            Engine& e = *slots_[k];
+            const BatchView& bv = views[order[k]];
+            e.own_view(Engine::make_view(bv.cam, bv.gt, hp.background, hp.ssim_weight, bv.inv_n));
             e.front(P, du, true);
             e.raster_fwd_own();
             e.loss_to(0);
@@ -250,6 +263,11 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
     k.grid = persist_grid_;
     k.ppl = ppl_;
     k.g = hw_;
+    k.slot_geom.clear();
+    for (int j = 0; j < B; j++) {
+        k.slot_geom.push_back(hwv_[j].W);
+        k.slot_geom.push_back(hwv_[j].H);
+    }
     int64_t gens = 0;
     // per slot: buffer generation and depth-chunk count (chunked views record their own path)
     for (int v = 0; v < B; v++) gens = gens * 1000003 + slots_[v]->generation() * 4096 + slots_[v]->chunks();
@@ -289,8 +307,9 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
 }
 
 double BatchRunner::loss(int v) {
-    if (v < 0 || v >= (int)sizes_.size()) return 0.0;
-    return slots_[v]->loss_slot(0, sizes_[v].first, sizes_[v].second);
+    if (v < 0 || v >= (int)slot_of_view_.size()) return 0.0;
+    const int k = slot_of_view_[v];
+    return slots_[k]->loss_slot(0, sizes_[k].first, sizes_[k].second);
 }
 
 }  // namespace msplat_hip

@@ -1,6 +1,7 @@
 // batch.h — the multi-view batched training step: B views per step, each with its own
-// engine slot (lists, images, gradients), one persistent raster launch over all views'
-// tiles in each direction, gradient sums in view order and one update. The whole
+// engine slot (lists, images, gradients; slots take the views by decreasing size), one
+// persistent raster launch over all views' tiles in each direction, gradient sums in slot
+// order and one update. The whole
 // iteration can be recorded once as a HIP graph and replayed: every per-step input the
 // kernels read (cameras, targets, step scalars, the raster view table) sits in one device
 // block that is rewritten before each replay.
@@ -57,13 +58,14 @@ public:
     int64_t captures() const { return captures_; }
     // Tile-count geometry the recorded graph is launched with (grows monotonically).
     msk::Geom launch_geom() const { return hw_; }
-    // Engine of view slot v of the last step.
+    // Engine of slot v of the last step (slots hold the views by decreasing size).
     Engine* slot(int v) { return v >= 0 && v < (int)slots_.size() ? slots_[v].get() : nullptr; }
 
 private:
     struct Key {
         int N = -1, B = 0, du = -1, degree = -1, bf16 = 0, grid = 0, ppl = 1;
         msk::Geom g{};
+        std::vector<int> slot_geom;   // recorded (W, H) per slot
         int64_t gens = 0;
         const void* state[8] = {};
         const void* mom[12] = {};
@@ -77,7 +79,9 @@ private:
 
     hipStream_t s_;
     std::vector<std::unique_ptr<Engine>> slots_;
-    std::vector<std::pair<int, int>> sizes_;
+    std::vector<std::pair<int, int>> sizes_;   // per slot
+    std::vector<int> slot_of_view_;
+    std::vector<msk::Geom> hwv_;               // high-water launch geometry per slot (graph mode)
     DBuf block_;                                     // [MsplatStepParams | ViewConsts x B | RasterView x B]
     DBuf wka_, wva_, wkb_, wvb_, wtemp_, wmisc_;     // batch work queue
     ConstRing ring_;
