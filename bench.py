@@ -57,8 +57,6 @@ def parse():
     p.add_argument("--sh-bf16", action="store_true", help="bfloat16 SH storage for the c1 suite")
     p.add_argument("--raster-ppl", type=int, default=0, choices=[0, 1, 4],
                    help="pixels per lane of the persistent raster kernels (0: library default)")
-    p.add_argument("--list-key-bytes", type=int, default=0, choices=[0, 4, 8],
-                   help="tile-list key width (0: library default)")
     return p.parse_args()
 
 
@@ -129,8 +127,6 @@ def main():
         tr.set_option("sh_bf16", 1)
     if a.raster_ppl:
         tr.set_option("raster_ppl", a.raster_ppl)
-    if a.list_key_bytes:
-        tr.set_option("list_key_bytes", a.list_key_bytes)
     eng = tr.engine_handle
     log(f"setup {time.time() - t_setup:.1f}s ({ds.num_train} cameras)")
 

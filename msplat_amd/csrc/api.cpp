@@ -800,11 +800,6 @@ int msplat_trainer_set_option(MsplatTrainer t, const char* name, double value) {
         TR(t)->model->set_batch_graph(value != 0.0);
         return 0;
     }
-    if (n == "list_key_bytes") {
-        if (value != 4.0 && value != 8.0) throw std::invalid_argument("list_key_bytes is 4 or 8");
-        TR(t)->model->engine().set_list_key_bytes((int)value);
-        return 0;
-    }
     if (n == "raster_ppl") {
         if (value != 1.0 && value != 4.0) throw std::invalid_argument("raster_ppl is 1 or 4");
         TR(t)->model->engine().set_raster_ppl((int)value);

@@ -39,7 +39,7 @@ void DBuf::release() {
 }
 
 static const char* kStageNames[ST_COUNT] = {"zero_grads",  "project_sh_fwd", "depth_rank", "tile_count_scan",
-                                            "tile_scatter", "tile_sort_pack", "raster_fwd", "ssim_loss",
+                                            "tile_bucket", "raster_fwd", "ssim_loss",
                                             "raster_bwd",  "sh_rest_adam",   "gaussian_update"};
 
 static std::mutex g_engines_mu;
@@ -120,9 +120,9 @@ void Engine::trim() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
-                   &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &keys_, &ids_, &hist_, &img_,
+                   &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_,
                    &finalT_, &finalIdx_, &vrend_, &hbuf_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
-                   &prefixT_, &afterC_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_, &items_, &srect_};
+                   &prefixT_, &afterC_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
     debug_valid_ = false;
     gen_++;
@@ -139,9 +139,9 @@ Engine::~Engine() {
     hipEventDestroy(mark_[1]);
     DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
-                   &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &keys_, &ids_, &hist_, &img_, &finalT_, &finalIdx_,
+                   &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_, &finalT_, &finalIdx_,
                    &vrend_, &hbuf_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
-                   &afterC_, &nanc_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_, &items_, &srect_};
+                   &afterC_, &nanc_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
     if (stream_ && owns_stream_) hipStreamDestroy(stream_);
 }
@@ -180,8 +180,6 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
     ensure(radii_, n * 4); ensure(tiles_, n * 4); ensure(depths_, n * 4); ensure(rect_, n * 8); ensure(aabb_, n * 8);
     ensure(recA_, (n + 1) * 16); ensure(recB_, (n + 1) * 16); ensure(recC_, (n + 1) * 8);  // row N: zero record
     ensure(counts_, (size_t)T * 4); ensure(exact_, (size_t)T * 4); ensure(bins_, (size_t)T * 8);
-    ensure(items_, (size_t)T * 4);
-    ensure(keys_, (size_t)capacity_ * key_bytes_);
     ensure(srect_, n * 8);
     ensure(dkeys_, n * 4); ensure(dkeys_out_, n * 4); ensure(dvals_, n * 4); ensure(sorted_ids_, n * 4);
     ensure(temp_, msk::lists_temp_bytes(N));
@@ -327,10 +325,8 @@ void Engine::front(const msk::Params& P, int degrees_to_use, bool train) {
     stage_event(ST_RANK);
     msk::lists_count(stream_, P.N, pj, vcd_, launch_, L);
     stage_event(ST_COUNTS);
-    msk::lists_scatter(stream_, P.N, pj, vcd_, launch_, L);
-    stage_event(ST_SCATTER);
-    msk::lists_sort_pack(stream_, P.N, vcd_, launch_, L);
-    stage_event(ST_SORT);
+    msk::lists_bucket(stream_, P.N, vcd_, launch_, L);
+    stage_event(ST_BUCKET);
 }
 
 void Engine::raster_fwd_own() {
@@ -362,15 +358,11 @@ msk::ListBufs Engine::list_bufs() {
     L.tile_counts = counts_.as<uint32_t>();
     L.exact_off = exact_.as<uint32_t>();
     L.bins = bins_.as<int2>();
-    L.keys = keys_.p;
-    L.key_bytes = key_bytes_;
     L.srect = srect_.as<ushort4>();
     L.ids = ids_.as<uint32_t>();
     L.view = {ids_.as<uint32_t>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
     L.hist = hist_.as<uint32_t>();
     L.totals = totals_.as<uint32_t>();
-    L.item_off = items_.as<uint32_t>();
-    L.sort_grid = std::max(persist_grid_, 1);
     L.capacity = capacity_;
     L.num_tiles = tbx_ * tby_;
     L.dkeys = dkeys_.as<uint32_t>();

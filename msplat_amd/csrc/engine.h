@@ -20,7 +20,7 @@ struct DBuf {
 };
 
 enum Stage {
-    ST_ZERO = 0, ST_PROJECT, ST_RANK, ST_COUNTS, ST_SCATTER, ST_SORT, ST_RASTER_FWD, ST_SSIM, ST_RASTER_BWD,
+    ST_ZERO = 0, ST_PROJECT, ST_RANK, ST_COUNTS, ST_BUCKET, ST_RASTER_FWD, ST_SSIM, ST_RASTER_BWD,
     ST_SH_ADAM, ST_UPDATE, ST_COUNT
 };
 
@@ -106,7 +106,6 @@ public:
     void set_raster_ppl(int p) { ppl_ = p == 4 ? 4 : 1; }
     // Width of the tile-list keys: 8 carries the Gaussian id with the depth rank, 4 looks
     // it up after the per-tile sort (the lists are the same).
-    void set_list_key_bytes(int b) { key_bytes_ = b == 8 ? 8 : 4; }
     int raster_ppl() const { return ppl_; }
 
 private:
@@ -126,7 +125,6 @@ private:
     int64_t gen_ = 0;   // bumps whenever a buffer is re-allocated
     int persist_grid_ = 0;
     int ppl_ = 1;
-    int key_bytes_ = 4;
     // view constants: host copy, device copy in use, own device block [ViewConsts | RasterView]
     msk::ViewConsts vch_{};
     const msk::ViewConsts* vcd_ = nullptr;
@@ -137,11 +135,11 @@ private:
     DBuf v_xy_, v_conic_, v_rgb_, v_opac_;
     DBuf dbg_vm_, dbg_vs_, dbg_vq_, dbg_vo_;
     // per-tile
-    DBuf counts_, exact_, bins_, totals_, items_;
+    DBuf counts_, exact_, bins_, totals_;
     DBuf dkeys_, dkeys_out_, dvals_, sorted_ids_, temp_, srect_;
     DBuf wka_, wva_, wkb_, wvb_, wtemp_, wmisc_;   // own persistent-raster work queue
     // per-intersection
-    DBuf keys_, ids_, hist_;
+    DBuf ids_, hist_;
     // per-pixel
     DBuf img_, finalT_, finalIdx_, vrend_, hbuf_, dbuf_, partials_, loss_;
     DBuf chunkT_, chunkC_, chunkIdx_, prefixT_, afterC_;

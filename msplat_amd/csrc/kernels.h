@@ -41,16 +41,10 @@ struct ListBufs {
     uint32_t* tile_counts;     // [T]
     uint32_t* exact_off;       // [T]
     int2* bins;                // [T] (start, end) of each tile's list
-    void* keys;                // [capacity] list keys bucketed per tile: (depth rank << 32 | Gaussian id)
-                               // with key_bytes 8, the depth rank alone with key_bytes 4
-    int key_bytes = 4;
     uint32_t* ids;             // [capacity] sorted per-tile Gaussian ids (the packed lists)
     EntryView view;            // ids + the projection's records
     uint32_t* hist;            // [chunks x T] per-chunk tile histograms -> bucket offsets
-    uint32_t* totals;          // [8]: total, padded total (reference layout), overflow flag, max count,
-                               //      sort work items, their queue head
-    uint32_t* item_off;        // [T] first sort work item of each tile
-    int sort_grid;             // resident workgroups of the per-tile sort
+    uint32_t* totals;          // [8]: total, padded total (reference layout), overflow flag, max count
     int64_t capacity;
     int num_tiles;
     // global depth order
@@ -136,8 +130,8 @@ size_t lists_hist_bytes(int N, int T);
 bool lists_supported(int N, int T);
 void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L);
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);
-void lists_scatter(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);
-void lists_sort_pack(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L);
+// Scatter of every tile hit into its tile's bucket, per-segment sort and id packing.
+void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L);
 // Stable LSD radix sort of n (key, value) pairs over the low 8 * passes key bits; with an
 // even number of passes the result is back in (ka, va). temp: lists_temp_bytes(n).
 void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, void* temp,

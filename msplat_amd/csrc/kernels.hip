@@ -14,6 +14,7 @@
 
 #include "kernels.h"
 #include "ms_math.h"
+#include "lane_ops.h"
 
 namespace msk {
 
@@ -66,6 +67,34 @@ __device__ __forceinline__ float wave_sum_to_lane63(float v) {
     v += dpp_row_shr8(v);
     v += dpp_bcast15(v);
     v += dpp_bcast31(v);
+    return v;
+}
+
+// Sums of four 64-lane vectors a, b, c, d at once: half-waves swapped and added
+// (v_permlane32_swap), then rows (v_permlane16_swap), then within each 16-lane row: ten
+// instructions for four sums instead of 24. Every swap trades whole halves / rows, so each
+// row ends up holding one vector's partial sums; lane 15 of the row has that vector's total,
+// and wave_sum4_tag() (the same swaps applied to the tags 0, 1, 2, 3) says which vector it is.
+__device__ __forceinline__ int wave_sum4_tag() {
+    float a = 0.f, b = 1.f, c = 2.f, d = 3.f;
+    permlane32_swap(a, b);
+    permlane32_swap(c, d);
+    permlane16_swap(a, c);
+    return (int)a;
+}
+
+__device__ __forceinline__ float wave_sum4_rows(float a, float b, float c, float d) {
+    permlane32_swap(a, b);
+    const float ab = a + b;  // one vector per half-wave
+    permlane32_swap(c, d);
+    float cd = c + d;
+    float abr = ab;
+    permlane16_swap(abr, cd);
+    float v = abr + cd;      // one vector per row
+    v += dpp_row_shr1(v);
+    v += dpp_row_shr2(v);
+    v += dpp_row_shr4(v);
+    v += dpp_row_shr8(v);
     return v;
 }
 
@@ -311,7 +340,9 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
                     rad = ms_f2i(radius);
                     area = a;
                     depth = pvz;
-                    rect = make_ushort4((unsigned short)x0, (unsigned short)y0, (unsigned short)x1, (unsigned short)y1);
+                    // only listed Gaussians (radius > 0) keep a non-empty rectangle
+                    if (rad > 0)
+                        rect = make_ushort4((unsigned short)x0, (unsigned short)y0, (unsigned short)x1, (unsigned short)y1);
                     aabb = make_float2(ax, ay);
                     const float op = 1.f / (1.f + ms_expf(-opac[i]));  // pack-time sigmoid, metal:2563
                     if (ms_finite(op)) {
@@ -380,13 +411,12 @@ Geom geom_of(int W, int H) {
 //  2. The ranks are cut into G-wide chunks, one workgroup each. A workgroup counts its
 //     chunk's tile hits in an LDS histogram (no global atomics); a column scan over the
 //     chunks gives every (chunk, tile) pair its offset inside the tile's bucket.
-//  3. The scatter writes each hit's rank at that offset + an LDS cursor. A tile's bucket is
-//     thus a run of chunk segments in chunk order: all ranks of segment b are below those
-//     of segment b + 1, only the order inside a segment is arbitrary.
-//  4. Each tile's bucket is cut into windows of about 64 entries whose boundaries fall on
-//     segment starts; sorting every window by rank (a bitonic network in registers with
-//     wave shuffles, LDS for the rare long window) sorts the bucket. Packing writes
-//     sorted_ids[rank], the Gaussian id.
+//  3. The bucket kernel (same chunks) writes each hit's chunk-local rank at that offset + an
+//     LDS cursor. A tile's bucket is thus a run of chunk segments in chunk order: all ranks
+//     of segment b are below those of segment b + 1, only the order inside a segment is
+//     arbitrary. The same workgroup then sorts each of its segments by rank (one lane per
+//     segment of up to 8 entries, one wave up to 128, the workgroup beyond) and replaces the
+//     ranks by Gaussian ids from an LDS copy of the chunk's slice of the depth order.
 // The resulting per-tile id lists are the reference's sorted lists exactly.
 __global__ void k_depth_keys(int N, const int32_t* __restrict__ radii, const float* __restrict__ depths,
                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
@@ -426,283 +456,216 @@ __global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const 
 }
 
 // Exclusive scan of each tile's column over the chunks; the column total is the tile count.
-// The offsets are also written tile-major (matT [T][nblk]) for the per-tile sort, which
-// reads one tile's offsets contiguously.
-__global__ void k_col_scan(const ViewConsts* __restrict__ vc, int nblk, uint32_t* __restrict__ mat,
-                           uint32_t* __restrict__ matT, uint32_t* __restrict__ counts) {
+// A workgroup takes 64 columns; its 8 waves each own one eighth of the chunks (column sums
+// first, combined through LDS, then the rewrite), so the chain of dependent loads per lane is
+// an eighth of the column.
+constexpr int COLSCAN_PARTS = 8;
+__global__ __launch_bounds__(64 * COLSCAN_PARTS) void k_col_scan(const ViewConsts* __restrict__ vc, int nblk,
+                                                                 uint32_t* __restrict__ mat,
+                                                                 uint32_t* __restrict__ counts) {
+    __shared__ uint32_t part_sum[COLSCAN_PARTS][64];
     const int T = vc->T;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (int)(threadIdx.x & 63), part = (int)(threadIdx.x >> 6);
+    const int t = (int)blockIdx.x * 64 + c;
+    const int per = (nblk + COLSCAN_PARTS - 1) / COLSCAN_PARTS;
+    const int b0 = min(nblk, part * per), b1 = min(nblk, b0 + per);
+    uint32_t sum = 0;
+    if (t < T) {
+        int b = b0;
+        for (; b + 8 <= b1; b += 8) {  // eight independent loads in flight
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = mat[(size_t)(b + u) * T + t];
+#pragma unroll
+            for (int u = 0; u < 8; u++) sum += v[u];
+        }
+        for (; b < b1; b++) sum += mat[(size_t)b * T + t];
+    }
+    part_sum[part][c] = sum;
+    __syncthreads();
+    uint32_t run = 0, total = 0;
+    for (int q = 0; q < COLSCAN_PARTS; q++) {
+        const uint32_t v = part_sum[q][c];
+        if (q < part) run += v;
+        total += v;
+    }
     if (t >= T) return;
-    uint32_t run = 0;
-    uint32_t* rowT = matT + (size_t)t * nblk;
-    int b = 0;
-    for (; b + 8 <= nblk; b += 8) {
+    int b = b0;
+    for (; b + 8 <= b1; b += 8) {
         uint32_t v[8];
 #pragma unroll
         for (int u = 0; u < 8; u++) v[u] = mat[(size_t)(b + u) * T + t];
 #pragma unroll
         for (int u = 0; u < 8; u++) {
             mat[(size_t)(b + u) * T + t] = run;
-            rowT[b + u] = run;
             run += v[u];
         }
     }
-    for (; b < nblk; b++) {
+    for (; b < b1; b++) {
         const size_t o = (size_t)b * T + t;
         const uint32_t v = mat[o];
         mat[o] = run;
-        rowT[b] = run;
         run += v;
     }
-    counts[t] = run;
+    if (part == 0) counts[t] = total;
 }
-
-// Windows of the per-tile sort grouped into work items of this many (heavy tiles spread
-// over many workgroups).
-constexpr uint32_t SORT_ITEM_WINDOWS = 16;
 
 __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ vc, const uint32_t* __restrict__ counts,
                                                  uint32_t* __restrict__ exact_off, int2* __restrict__ bins,
-                                                 uint32_t* __restrict__ item_off, uint32_t* __restrict__ totals,
-                                                 uint32_t cap) {
+                                                 uint32_t* __restrict__ totals, uint32_t cap) {
     __shared__ uint32_t lw[17];
     const int T = vc->T;
-    uint32_t run_ex = 0, run_pd = 0, run_it = 0, mx = 0;
+    uint32_t run_ex = 0, my_pd = 0, mx = 0;
     for (int c0 = 0; c0 < T; c0 += 1024) {
         const int t = c0 + threadIdx.x;
         const uint32_t val = t < T ? counts[t] : 0u;
-        const uint32_t pv = pad_count(val);
+        my_pd += pad_count(val);
         mx = max(mx, val);
-        const uint32_t nit = (val + 64u * SORT_ITEM_WINDOWS - 1u) / (64u * SORT_ITEM_WINDOWS);
-        uint32_t tot_e, tot_p, tot_i;
+        uint32_t tot_e;
         const uint32_t ex = block_exclusive_scan(val, lw, &tot_e);
-        const uint32_t pd = block_exclusive_scan(pv, lw, &tot_p);
-        const uint32_t io = block_exclusive_scan(nit, lw, &tot_i);
         if (t < T) {
-            item_off[t] = run_it + io;
             exact_off[t] = run_ex + ex;
             // never past the list buffers (the reference scatter drops overflow, metal:2474-2477)
             bins[t] = make_int2((int)min(run_ex + ex, cap), (int)min(run_ex + ex + val, cap));
         }
         run_ex += tot_e;
-        run_pd += tot_p;
-        run_it += tot_i;
     }
-    __shared__ uint32_t smax;
-    if (threadIdx.x == 0) smax = 0;
+    __shared__ uint32_t smax, spd;
+    if (threadIdx.x == 0) { smax = 0; spd = 0; }
     __syncthreads();
     atomicMax(&smax, mx);
+    atomicAdd(&spd, my_pd);
     __syncthreads();
     if (threadIdx.x == 0) {
         totals[0] = run_ex;
-        totals[1] = run_pd;   // what the reference's padded buckets would occupy
+        totals[1] = spd;      // what the reference's padded buckets would occupy
         totals[2] = run_ex > cap ? 1u : 0u;
         totals[3] = smax;
-        totals[4] = run_it;   // per-tile sort work items
-        totals[5] = 0u;       // their queue head
     }
 }
 
-// List keys: K = uint64_t holds (rank << 32) | Gaussian id, so the per-tile sort carries the
-// id along; K = uint32_t holds the rank only and the sort looks the id up (sorted_ids[rank]).
-template <class K> __device__ __forceinline__ K list_key(uint32_t r, uint32_t g);
-template <> __device__ __forceinline__ uint64_t list_key<uint64_t>(uint32_t r, uint32_t g) {
-    return ((uint64_t)r << 32) | (uint64_t)g;
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
+    const uint32_t lo = min(a, b);
+    b = max(a, b);
+    a = lo;
 }
-template <> __device__ __forceinline__ uint32_t list_key<uint32_t>(uint32_t r, uint32_t) { return r; }
 
-template <class K>
-__global__ __launch_bounds__(HIST_THREADS) void k_tile_scatter(int N, int G, const uint32_t* __restrict__ sorted_ids,
-                                                               const ushort4* __restrict__ srect,
-                                                               const ViewConsts* __restrict__ vc,
-                                                               const uint32_t* __restrict__ mat,
-                                                               const uint32_t* __restrict__ exact_off,
-                                                               K* __restrict__ keys, uint32_t cap) {
-    extern __shared__ uint32_t cur[];
+// Ascending sorting network for 8 keys (19 comparators).
+__device__ __forceinline__ void sort8(uint32_t v[8]) {
+    cswap(v[0], v[2]); cswap(v[1], v[3]); cswap(v[4], v[6]); cswap(v[5], v[7]);
+    cswap(v[0], v[4]); cswap(v[1], v[5]); cswap(v[2], v[6]); cswap(v[3], v[7]);
+    cswap(v[0], v[1]); cswap(v[2], v[3]); cswap(v[4], v[5]); cswap(v[6], v[7]);
+    cswap(v[2], v[4]); cswap(v[3], v[5]);
+    cswap(v[1], v[4]); cswap(v[3], v[6]);
+    cswap(v[1], v[2]); cswap(v[3], v[4]); cswap(v[5], v[6]);
+}
+
+constexpr int SEG_LANE = 8;                  // segments up to this long: one lane; longer: one wave
+constexpr int BUCKET_ROUND = HIST_THREADS;   // tiles examined per sorting round
+constexpr int RANK_WORDS = 64;               // bitmap words over a chunk's local ranks (64 x 32 >= MAX_CHUNK)
+
+// LDS: cursors [T] | chunk ids [MAX_CHUNK] | wave list [round] | one rank bitmap per wave
+static size_t bucket_lds_bytes(int T) {
+    return ((size_t)(T + 3) & ~(size_t)3) * 4 +
+           (size_t)(MAX_CHUNK + BUCKET_ROUND + (HIST_THREADS / 64) * RANK_WORDS) * 4;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__global__ __launch_bounds__(HIST_THREADS) void k_tile_bucket(int N, int G, const uint32_t* __restrict__ sorted_ids,
+                                                              const ushort4* __restrict__ srect,
+                                                              const ViewConsts* __restrict__ vc,
+                                                              const uint32_t* __restrict__ mat,
+                                                              const uint32_t* __restrict__ exact_off,
+                                                              const uint32_t* __restrict__ counts,
+                                                              uint32_t* __restrict__ ids, uint32_t cap) {
+    extern __shared__ uint32_t sm[];
+    __shared__ uint32_t n_wl;
     const int tbx = vc->tbx, T = vc->T;
+    uint32_t* cur = sm;
+    uint32_t* sid = sm + ((T + 3) & ~3);
+    uint32_t* wl = sid + MAX_CHUNK;
+    const int tid = (int)threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    uint32_t* bm = wl + BUCKET_ROUND + wid * RANK_WORDS;
     const uint32_t* row = mat + (size_t)blockIdx.x * T;
-    for (int t = threadIdx.x; t < T; t += HIST_THREADS) cur[t] = exact_off[t] + row[t];
+    const int r0 = (int)blockIdx.x * G, r1 = min(N, r0 + G);
+    for (int t = tid; t < T; t += HIST_THREADS) cur[t] = exact_off[t] + row[t];
+    for (int r = r0 + tid; r < r1; r += HIST_THREADS) sid[r - r0] = sorted_ids[r];
     __syncthreads();
-    const int r0 = blockIdx.x * G, r1 = min(N, r0 + G);
-    for (int r = r0 + threadIdx.x; r < r1; r += HIST_THREADS) {
+    // 1. every hit's chunk-local rank at its tile's cursor
+    for (int r = r0 + tid; r < r1; r += HIST_THREADS) {
         const ushort4 q = srect[r];
         if (q.x >= q.z || q.y >= q.w) continue;
-        const K key = list_key<K>((uint32_t)r, sizeof(K) == 8 ? sorted_ids[r] : 0u);
+        const uint32_t key = (uint32_t)(r - r0);
         for (int y = q.y; y < q.w; y++)
             for (int x = q.x; x < q.z; x++) {
                 const uint32_t dst = atomicAdd(&cur[y * tbx + x], 1u);
-                if (dst < cap) keys[dst] = key;
+                if (dst < cap) ids[dst] = key;
             }
     }
-}
-
-template <class K> __device__ __forceinline__ K kmin(K a, K b) { return a < b ? a : b; }
-template <class K> __device__ __forceinline__ K kmax(K a, K b) { return a < b ? b : a; }
-
-// One compare-exchange stage with partner distance j < 128 on the two consecutive
-// elements e0, e0 + 1 a lane holds (e0 = 2 lane).
-template <class K>
-__device__ __forceinline__ void reg_stage(K& v0, K& v1, uint32_t e0, uint32_t k, uint32_t j, int lane) {
-    const bool asc = (e0 & k) == 0;
-    if (j == 1) {
-        const K lo = kmin(v0, v1), hi = kmax(v0, v1);
-        v0 = asc ? lo : hi;
-        v1 = asc ? hi : lo;
-    } else {
-        const int m = (int)(j >> 1);
-        const K p0 = __shfl_xor(v0, m, 64), p1 = __shfl_xor(v1, m, 64);
-        const bool keep_min = (((lane & m) == 0) == asc);
-        v0 = keep_min ? kmin(v0, p0) : kmax(v0, p0);
-        v1 = keep_min ? kmin(v1, p1) : kmax(v1, p1);
-    }
-}
-
-// Bitonic sort of 64 keys, one per lane, ascending by lane.
-template <class K> __device__ __forceinline__ K wave_sort64(K v, int lane) {
+    __syncthreads();
+    // 2. sort every segment by rank and write the Gaussian ids in its place. A segment holds
+    // distinct local ranks below G: longer ones are sorted by marking their ranks in a
+    // wave-private bitmap and reading the set bits back in order.
+    const uint32_t pad = 0xFFFFFFFFu;
+    for (int t0 = 0; t0 < T; t0 += BUCKET_ROUND) {
+        if (tid == 0) n_wl = 0u;
+        __syncthreads();
+        const int t = t0 + tid;
+        if (t < T) {
+            const uint32_t base = exact_off[t], s0 = base + row[t], e = cur[t];
+            if (e > s0) {
+                if (base + counts[t] > cap) {  // the lists overflowed: this tile's entries become the empty record
+                    for (uint32_t i = s0; i < min(e, cap); i++) ids[i] = (uint32_t)N;
+                } else if (e - s0 <= (uint32_t)SEG_LANE) {
+                    const uint32_t n = e - s0;
+                    uint32_t v[8];
 #pragma unroll
-    for (int k = 2; k <= 64; k <<= 1)
+                    for (int i = 0; i < 8; i++) v[i] = (uint32_t)i < n ? ids[s0 + i] : pad;
+                    sort8(v);
 #pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const K p = __shfl_xor(v, j, 64);
-            const bool keep_min = (((lane & j) == 0) == ((lane & k) == 0));
-            v = keep_min ? kmin(v, p) : kmax(v, p);
-        }
-    return v;
-}
-
-// Bitonic sort of p (power of two, 256..2048) keys in a wave-private LDS region, by one
-// wave: LDS accesses of one wave are performed in issue order, so a wave-level fence
-// between stages is enough.
-template <class K> __device__ void wave_sort_lds(K* a, uint32_t p, int lane) {
-    for (uint32_t k = 2; k <= p; k <<= 1)
-        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
-            for (uint32_t q = lane; q < (p >> 1); q += 64) {
-                const uint32_t lo = ((q & ~(j - 1)) << 1) | (q & (j - 1));
-                const uint32_t hi = lo + j;
-                const bool asc = (lo & k) == 0;
-                const K x = a[lo], y = a[hi];
-                if ((x > y) == asc) { a[lo] = y; a[hi] = x; }
+                    for (int i = 0; i < 8; i++)
+                        if ((uint32_t)i < n) ids[s0 + i] = sid[v[i]];
+                } else {
+                    wl[atomicAdd(&n_wl, 1u)] = (uint32_t)t;
+                }
             }
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        }
-}
-
-constexpr int SORT_THREADS_MAX = 256;
-constexpr int WIN = 64;
-constexpr int WAVE_LDS_KEYS = 2048;
-
-// Work item = up to SORT_ITEM_WINDOWS windows of one tile; resident workgroups take items
-// from the queue (totals[5]) so that a tile of 100k entries is sorted by many workgroups.
-// 64-bit keys: 2 waves per workgroup (their LDS window buffers are twice as large).
-template <class K> struct SortCfg;
-template <> struct SortCfg<uint32_t> { static constexpr int threads = 256; };
-template <> struct SortCfg<uint64_t> { static constexpr int threads = 128; };
-
-template <class K>
-__global__ __launch_bounds__(SortCfg<K>::threads) void k_sort_pack(const ViewConsts* __restrict__ vc, int nblk,
-                                                            const uint32_t* __restrict__ counts,
-                                                            const uint32_t* __restrict__ exact_off,
-                                                            const uint32_t* __restrict__ item_off,
-                                                            uint32_t* __restrict__ totals,
-                                                            const uint32_t* __restrict__ matT,
-                                                            const K* __restrict__ keys,
-                                                            const uint32_t* __restrict__ sorted_ids,
-                                                            uint32_t* __restrict__ ids, int N, uint32_t cap) {
-    constexpr int SORT_THREADS = SortCfg<K>::threads;
-    extern __shared__ uint32_t sm[];
-    uint32_t* off = sm;                          // [nblk + 1] segment starts of this tile
-    K* wl = reinterpret_cast<K*>(sm + ((nblk + 4) & ~3));  // [waves][WAVE_LDS_KEYS]
-    __shared__ int s_tile;
-    const int T = vc->T;
-    const uint32_t total = totals[4];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    K* wlds = wl + wid * WAVE_LDS_KEYS;
-    const K pad = (K)~(K)0;
-    // 64-bit: the low half is the Gaussian id; 32-bit: the rank, looked up in sorted_ids
-    auto id_of = [&](K v) -> uint32_t {
-        if (sizeof(K) == 8) {
-            const uint32_t g = (uint32_t)v;
-            return g < (uint32_t)N ? g : (uint32_t)N;
-        }
-        return (uint32_t)v < (uint32_t)N ? sorted_ids[(uint32_t)v] : (uint32_t)N;
-    };
-    for (;;) {
-        const uint32_t it = next_item(totals + 5);
-        if (it >= total) break;
-        if (threadIdx.x == 0) {  // last tile whose first item is <= it (tiles without items share offsets)
-            int lo = 0, hi = T;
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (item_off[mid] <= it) lo = mid; else hi = mid;
-            }
-            s_tile = lo;
         }
         __syncthreads();
-        const int t = s_tile;
-        const uint32_t sub = it - item_off[t];
-        const uint32_t count = counts[t];
-        const uint32_t base = exact_off[t];
-        const uint32_t nwin = (count + WIN - 1) / WIN;
-        const uint32_t w0 = sub * SORT_ITEM_WINDOWS, w1 = min(nwin, w0 + SORT_ITEM_WINDOWS);
-        if (base + count > cap) {  // the lists overflowed: this item's entries become the empty record
-            const uint32_t e1 = min(count, w1 * WIN);
-            for (uint32_t i = w0 * WIN + threadIdx.x; i < e1; i += SORT_THREADS)
-                if (base + i < cap) ids[base + i] = (uint32_t)N;
-            continue;
+        const uint32_t nw = n_wl;
+        for (uint32_t k = (uint32_t)wid; k < nw; k += HIST_THREADS / 64) {
+            const int tw = (int)wl[k];
+            const uint32_t s0 = exact_off[tw] + row[tw], n = cur[tw] - s0;
+            uint32_t* seg = ids + s0;
+            bm[lane] = 0u;
+            wave_lds_sync();
+            for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
+                const uint32_t key = seg[i];
+                atomicOr(&bm[key >> 5], 1u << (key & 31u));
+            }
+            wave_lds_sync();
+            uint32_t bits = bm[lane];
+            const uint32_t c = (uint32_t)__popc(bits);
+            uint32_t x = c;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
+            }
+            uint32_t* out = seg + (x - c);  // this word's ranks follow those of the words below
+            while (bits) {
+                const uint32_t b = (uint32_t)__builtin_ctz(bits);
+                bits &= bits - 1u;
+                *out++ = sid[32u * (uint32_t)lane + b];
+            }
+            wave_lds_sync();
         }
-        const uint32_t* rowT = matT + (size_t)t * nblk;
-        for (int b = threadIdx.x; b < nblk; b += SORT_THREADS) off[b] = rowT[b];
-        if (threadIdx.x == 0) off[nblk] = count;
         __syncthreads();
-        // window w = entries [first segment start >= 64 w, first segment start >= 64 (w + 1))
-        auto seg_start_at_or_after = [&](uint32_t x) -> uint32_t {
-            int lo = 0, hi = nblk;  // off[nblk] = count >= x for x <= count
-            while (lo < hi) {
-                const int mid = (lo + hi) >> 1;
-                if (off[mid] >= x) hi = mid; else lo = mid + 1;
-            }
-            return off[lo];
-        };
-        for (uint32_t w = w0 + wid; w < w1; w += SORT_THREADS / 64) {
-            const uint32_t lo = seg_start_at_or_after(w * WIN);
-            const uint32_t hi = w + 1 < nwin ? seg_start_at_or_after((w + 1) * WIN) : count;
-            if (hi <= lo) continue;
-            const uint32_t n = hi - lo;
-            const K* ks = keys + base + lo;
-            uint32_t* out = ids + base + lo;
-            if (n <= 64) {
-                K v = (uint32_t)lane < n ? ks[lane] : pad;
-                v = wave_sort64(v, lane);
-                if ((uint32_t)lane < n) out[lane] = id_of(v);
-            } else if (n <= 128) {
-                const uint32_t e0 = 2 * lane;
-                K v0 = e0 < n ? ks[e0] : pad, v1 = e0 + 1 < n ? ks[e0 + 1] : pad;
-                for (uint32_t k = 2; k <= 128; k <<= 1)
-                    for (uint32_t j = k >> 1; j > 0; j >>= 1) reg_stage(v0, v1, e0, k, j, lane);
-                if (e0 < n) out[e0] = id_of(v0);
-                if (e0 + 1 < n) out[e0 + 1] = id_of(v1);
-            } else {
-                uint32_t p = 256;
-                while (p < n) p <<= 1;  // n <= MAX_CHUNK + WIN - 1 < WAVE_LDS_KEYS
-                for (uint32_t i = lane; i < p; i += 64) wlds[i] = i < n ? ks[i] : pad;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                wave_sort_lds(wlds, p, lane);
-                for (uint32_t i = lane; i < n; i += 64) out[i] = id_of(wlds[i]);
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
     }
-}
-
-// LDS of the per-tile sort: segment offsets + one window buffer per wave
-static size_t sort_lds_bytes(int nblk, int key_bytes) {
-    const int waves = (key_bytes == 8 ? SortCfg<uint64_t>::threads : SortCfg<uint32_t>::threads) / 64;
-    return ((size_t)(nblk + 4) & ~(size_t)3) * 4 + (size_t)waves * WAVE_LDS_KEYS * key_bytes;
 }
 
 static int hist_blocks(int N) {
@@ -800,8 +763,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
 
 size_t lists_temp_bytes(int N) { return ((size_t)256 * (size_t)rs_blocks(N > 0 ? N : 1) + 256 + 64) * 4; }
 
-// [chunks x T] histogram / bucket offsets, then the same offsets tile-major [T x chunks]
-size_t lists_hist_bytes(int N, int T) { return 2 * (size_t)hist_blocks(N) * (size_t)T * 4; }
+// [chunks x T] histogram / bucket offsets
+size_t lists_hist_bytes(int N, int T) { return (size_t)hist_blocks(N) * (size_t)T * 4; }
 
 static size_t g_lds_limit = 0;
 static size_t lds_limit() {
@@ -816,8 +779,8 @@ static size_t lds_limit() {
 }
 
 bool lists_supported(int N, int T) {
-    const size_t sort_lds = std::max(sort_lds_bytes(hist_blocks(N), 4), sort_lds_bytes(hist_blocks(N), 8));
-    return (size_t)T * 4 <= lds_limit() && sort_lds <= lds_limit();
+    (void)N;
+    return bucket_lds_bytes(T) <= lds_limit();
 }
 
 void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, void* temp,
@@ -837,12 +800,12 @@ void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t
     }
 }
 
-__global__ void k_gather_rect(int N, const uint32_t* __restrict__ sorted_ids, const int32_t* __restrict__ radii,
-                              const ushort4* __restrict__ rect, ushort4* __restrict__ srect) {
+// (the projection leaves the rectangle of every Gaussian with radius 0 empty)
+__global__ void k_gather_rect(int N, const uint32_t* __restrict__ sorted_ids, const ushort4* __restrict__ rect,
+                              ushort4* __restrict__ srect) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= N) return;
-    const uint32_t g = sorted_ids[r];
-    srect[r] = radii[g] > 0 ? rect[g] : make_ushort4(0, 0, 0, 0);
+    srect[r] = rect[sorted_ids[r]];
 }
 
 void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L) {
@@ -851,8 +814,7 @@ void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L) {
     hipLaunchKernelGGL(k_depth_keys, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.depths, L.dkeys_out,
                        L.sorted_ids);
     radix_sort_pairs(s, N, L.dkeys_out, L.sorted_ids, L.dkeys, L.dvals, L.temp, 4);
-    hipLaunchKernelGGL(k_gather_rect, dim3(cdiv(N, 256)), dim3(256), 0, s, N, L.sorted_ids, pj.radii, pj.rect,
-                       L.srect);
+    hipLaunchKernelGGL(k_gather_rect, dim3(cdiv(N, 256)), dim3(256), 0, s, N, L.sorted_ids, pj.rect, L.srect);
 }
 
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L) {
@@ -861,47 +823,22 @@ void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, 
         const size_t lds = (size_t)g.T * 4;
         if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_hist, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.srect, vc, L.hist);
-        hipLaunchKernelGGL(k_col_scan, dim3(cdiv(g.T, 64)), dim3(64), 0, s, vc, nblk, L.hist,
-                           L.hist + (size_t)nblk * g.T, L.tile_counts);
+        hipLaunchKernelGGL(k_col_scan, dim3(cdiv(g.T, 64)), dim3(64 * COLSCAN_PARTS), 0, s, vc, nblk, L.hist,
+                           L.tile_counts);
     } else {
         fill_zero(s, L.tile_counts, (size_t)g.T * 4);
     }
-    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, vc, L.tile_counts, L.exact_off, L.bins, L.item_off, L.totals,
+    hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, vc, L.tile_counts, L.exact_off, L.bins, L.totals,
                        (uint32_t)L.capacity);
 }
 
-void lists_scatter(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L) {
+void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N <= 0) return;
     const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
-    const size_t lds = (size_t)g.T * 4;
-    if (L.key_bytes == 8) {
-        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_scatter<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_tile_scatter<uint64_t>, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, L.srect,
-                           vc, L.hist, L.exact_off, reinterpret_cast<uint64_t*>(L.keys), (uint32_t)L.capacity);
-    } else {
-        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_scatter<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_tile_scatter<uint32_t>, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, L.srect,
-                           vc, L.hist, L.exact_off, reinterpret_cast<uint32_t*>(L.keys), (uint32_t)L.capacity);
-    }
-}
-
-void lists_sort_pack(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L) {
-    if (N <= 0) return;
-    const int nblk = hist_blocks(N);
-    const uint32_t* matT = L.hist + (size_t)nblk * g.T;
-    if (L.key_bytes == 8) {
-        const size_t lds = sort_lds_bytes(nblk, 8);
-        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_sort_pack<uint64_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_sort_pack<uint64_t>, dim3(L.sort_grid), dim3(SortCfg<uint64_t>::threads), lds, s, vc, nblk,
-                           L.tile_counts, L.exact_off, L.item_off, L.totals, matT,
-                           reinterpret_cast<const uint64_t*>(L.keys), L.sorted_ids, L.ids, N, (uint32_t)L.capacity);
-    } else {
-        const size_t lds = sort_lds_bytes(nblk, 4);
-        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_sort_pack<uint32_t>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        hipLaunchKernelGGL(k_sort_pack<uint32_t>, dim3(L.sort_grid / 2), dim3(SortCfg<uint32_t>::threads), lds, s, vc,
-                           nblk, L.tile_counts, L.exact_off, L.item_off, L.totals, matT,
-                           reinterpret_cast<const uint32_t*>(L.keys), L.sorted_ids, L.ids, N, (uint32_t)L.capacity);
-    }
+    const size_t lds = bucket_lds_bytes(g.T);
+    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_bucket, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_tile_bucket, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, L.srect, vc, L.hist,
+                       L.exact_off, L.tile_counts, L.ids, (uint32_t)L.capacity);
 }
 
 // ------------------------------------------------------------------------------------
@@ -927,11 +864,12 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
     for (int b0 = lo; b0 < hi; b0 += RB) {
         if (__syncthreads_and(done)) break;
         const int idx = b0 + threadIdx.x;
-        if (idx < hi) {
+        if (idx < hi) {  // colours stored clamped (max(c + 1/2, 0)): per entry, not per pixel
             const uint32_t g = ev.ids[idx];
+            const float4 b = ev.rec.B[g];
             sA[threadIdx.x] = ev.rec.A[g];
-            sB[threadIdx.x] = ev.rec.B[g];
-            sCx[threadIdx.x] = ev.rec.C[g].x;
+            sB[threadIdx.x] = make_float4(b.x, b.y, fmaxf(b.z + 0.5f, 0.f), fmaxf(b.w + 0.5f, 0.f));
+            sCx[threadIdx.x] = fmaxf(ev.rec.C[g].x + 0.5f, 0.f);
         }
         __syncthreads();
         if (done) continue;
@@ -951,9 +889,9 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
                 break;
             }
             const float vis = alpha * T;
-            C[0] = fmaf(fmaxf(b.z + 0.5f, 0.f), vis, C[0]);
-            C[1] = fmaf(fmaxf(b.w + 0.5f, 0.f), vis, C[1]);
-            C[2] = fmaf(fmaxf(sCx[e] + 0.5f, 0.f), vis, C[2]);
+            C[0] = fmaf(b.z, vis, C[0]);
+            C[1] = fmaf(b.w, vis, C[1]);
+            C[2] = fmaf(sCx[e], vis, C[2]);
             T = nT;
             last = b0 + e;
         }
@@ -1330,8 +1268,8 @@ void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& gm, const float*
 // ------------------------------------------------------------------------------------
 // A15/A16: rasterisation backward (rasterize_backward_kernel metal:931-1135; chunked
 // metal:3139-3366). Entries are walked back to front in LDS batches; each wave reduces
-// its 64 lanes' 9 gradient terms per entry with DPP, the 4 wave sums meet in LDS and one
-// thread per entry issues the global atomics.
+// its 64 lanes' 9 gradient terms of four entries at a time (wave_sum4_rows), the 4 wave
+// sums meet in LDS and one thread per entry issues the global atomics.
 constexpr int BB = 128;  // backward batch
 
 __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bool inside, float px, float py,
@@ -1341,78 +1279,84 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
     __shared__ float4 sA[BB];
     __shared__ float4 sB[BB];
     __shared__ float2 sC[BB];
+    __shared__ uint32_t sF[BB];
     __shared__ float red[4][BB][9];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    const float tfb0 = T_final * bg[0], tfb1 = T_final * bg[1], tfb2 = T_final * bg[2];
+    // the colour accumulated behind the current entry plus the background term T_final bg:
+    // the two only ever enter as their sum
+    float bb0 = fmaf(T_final, bg[0], buf0), bb1 = fmaf(T_final, bg[1], buf1), bb2 = fmaf(T_final, bg[2], buf2);
     const int top = min(hi, tile_max + 1);  // entries past every pixel's last contributor are skipped
+    const int tag = wave_sum4_tag();
     for (int bhi = top; bhi > lo; bhi -= BB) {
         const int blo = max(lo, bhi - BB);
         const int n = bhi - blo;
         __syncthreads();
-        if ((int)threadIdx.x < n) {
+        if ((int)threadIdx.x < n) {  // colours stored clamped; which of them pass the clamp in sF
             const int idx = bhi - 1 - threadIdx.x;  // slot 0 = furthest back
             const uint32_t g = ev.ids[idx];
+            const float4 b = ev.rec.B[g];
+            const float c0 = b.z + 0.5f, c1 = b.w + 0.5f, c2 = ev.rec.C[g].x + 0.5f;
             sA[threadIdx.x] = ev.rec.A[g];
-            sB[threadIdx.x] = ev.rec.B[g];
-            sC[threadIdx.x] = make_float2(ev.rec.C[g].x, __builtin_bit_cast(float, g));
+            sB[threadIdx.x] = make_float4(b.x, b.y, fmaxf(c0, 0.f), fmaxf(c1, 0.f));
+            sC[threadIdx.x] = make_float2(fmaxf(c2, 0.f), __builtin_bit_cast(float, g));
+            sF[threadIdx.x] = (c0 >= 0.f ? 1u : 0u) | (c1 >= 0.f ? 2u : 0u) | (c2 >= 0.f ? 4u : 0u);
         }
         __syncthreads();
-        for (int tt = 0; tt < n; tt++) {
-            const int e = bhi - 1 - tt;
-            bool valid = inside && e <= bin_final;
-            float alpha = 0.f, dx = 0.f, dy = 0.f;
-            const float4 a = sA[tt];
-            const float4 b = sB[tt];
-            if (valid) {
-                dx = a.x - px;
-                dy = a.y - py;
-                const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
-                if (sigma < 0.f || sigma >= 5.55f) {
-                    valid = false;
-                } else {
-                    alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
-                    if (alpha < 1.f / 255.f) valid = false;
+        // four entries at a time: their 4 x 9 terms are reduced together (wave_sum4_rows)
+        for (int t4 = 0; t4 < n; t4 += 4) {
+            float g[4][9];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) g[u][k] = 0.f;
+                const int tt = t4 + u;
+                if (tt >= n) continue;  // uniform
+                const int e = bhi - 1 - tt;
+                bool valid = inside && e <= bin_final;
+                float alpha = 0.f, dx = 0.f, dy = 0.f;
+                const float4 a = sA[tt];
+                const float4 b = sB[tt];
+                if (valid) {
+                    dx = a.x - px;
+                    dy = a.y - py;
+                    const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
+                    if (sigma < 0.f || sigma >= 5.55f) {
+                        valid = false;
+                    } else {
+                        alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
+                        if (alpha < 1.f / 255.f) valid = false;
+                    }
+                }
+                if (valid && alpha < 0.999f) {
+                    const float ra = 1.f / (1.f - alpha);
+                    T *= ra;
+                    const float fac = alpha * T;
+                    const float r0 = b.z, r1 = b.w, r2 = sC[tt].x;
+                    const float q0 = fmaf(r0, T, -ra * bb0);
+                    const float q1 = fmaf(r1, T, -ra * bb1);
+                    const float q2 = fmaf(r2, T, -ra * bb2);
+                    const float v_alpha = q0 * vo0 + q1 * vo1 + q2 * vo2;
+                    bb0 = fmaf(r0, fac, bb0);
+                    bb1 = fmaf(r1, fac, bb1);
+                    bb2 = fmaf(r2, fac, bb2);
+                    const float v_sigma = -alpha * v_alpha;
+                    g[u][0] = fac * vo0;
+                    g[u][1] = fac * vo1;
+                    g[u][2] = fac * vo2;
+                    const float hs = 0.5f * v_sigma;
+                    g[u][3] = hs * (dx * dx);
+                    g[u][4] = hs * (dx * dy);
+                    g[u][5] = hs * (dy * dy);
+                    g[u][6] = v_sigma * fmaf(a.w, dx, b.x * dy);
+                    g[u][7] = v_sigma * fmaf(b.x, dx, b.y * dy);
+                    g[u][8] = -v_sigma * (1.f - a.z);
                 }
             }
-            const bool contrib = valid && alpha < 0.999f;
-            if (__ballot(contrib) == 0ull) {
-                if (lane == 63) {
+            const int slot = t4 + tag;  // lane 15 of each row: the sum of entry t4 + tag
 #pragma unroll
-                    for (int k = 0; k < 9; k++) red[wid][tt][k] = 0.f;
-                }
-                continue;
-            }
-            float g[9] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            if (contrib) {
-                const float ra = 1.f / (1.f - alpha);
-                T *= ra;
-                const float fac = alpha * T;
-                const float cz = sC[tt].x;
-                const float r0 = fmaxf(b.z + 0.5f, 0.f), r1 = fmaxf(b.w + 0.5f, 0.f), r2 = fmaxf(cz + 0.5f, 0.f);
-                const float q0 = fmaf(r0, T, fmaf(-buf0, ra, -ra * tfb0));
-                const float q1 = fmaf(r1, T, fmaf(-buf1, ra, -ra * tfb1));
-                const float q2 = fmaf(r2, T, fmaf(-buf2, ra, -ra * tfb2));
-                const float v_alpha = q0 * vo0 + q1 * vo1 + q2 * vo2;
-                buf0 = fmaf(r0, fac, buf0);
-                buf1 = fmaf(r1, fac, buf1);
-                buf2 = fmaf(r2, fac, buf2);
-                const float v_sigma = -alpha * v_alpha;
-                g[0] = fac * vo0;
-                g[1] = fac * vo1;
-                g[2] = fac * vo2;
-                const float hs = 0.5f * v_sigma;
-                g[3] = hs * (dx * dx);
-                g[4] = hs * (dx * dy);
-                g[5] = hs * (dy * dy);
-                g[6] = v_sigma * fmaf(a.w, dx, b.x * dy);
-                g[7] = v_sigma * fmaf(b.x, dx, b.y * dy);
-                g[8] = -v_sigma * (1.f - a.z);
-            }
-#pragma unroll
-            for (int k = 0; k < 9; k++) g[k] = wave_sum_to_lane63(g[k]);
-            if (lane == 63) {
-#pragma unroll
-                for (int k = 0; k < 9; k++) red[wid][tt][k] = g[k];
+            for (int k = 0; k < 9; k++) {
+                const float v = wave_sum4_rows(g[0][k], g[1][k], g[2][k], g[3][k]);
+                if ((lane & 15) == 15) red[wid][slot][k] = v;
             }
         }
         __syncthreads();
@@ -1425,12 +1369,11 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
 #pragma unroll
             for (int k = 0; k < 9; k++) any |= (s[k] != 0.f);
             if (any) {
-                const float4 b = sB[tt];
-                const float2 c = sC[tt];
-                const uint32_t gid = __builtin_bit_cast(uint32_t, c.y);
-                if (b.z + 0.5f >= 0.f) atomicAdd(&G.v_rgb[3 * (size_t)gid + 0], s[0]);
-                if (b.w + 0.5f >= 0.f) atomicAdd(&G.v_rgb[3 * (size_t)gid + 1], s[1]);
-                if (c.x + 0.5f >= 0.f) atomicAdd(&G.v_rgb[3 * (size_t)gid + 2], s[2]);
+                const uint32_t f = sF[tt];
+                const uint32_t gid = __builtin_bit_cast(uint32_t, sC[tt].y);
+                if (f & 1u) atomicAdd(&G.v_rgb[3 * (size_t)gid + 0], s[0]);
+                if (f & 2u) atomicAdd(&G.v_rgb[3 * (size_t)gid + 1], s[1]);
+                if (f & 4u) atomicAdd(&G.v_rgb[3 * (size_t)gid + 2], s[2]);
                 atomicAdd(&G.v_conic[3 * (size_t)gid + 0], s[3]);
                 atomicAdd(&G.v_conic[3 * (size_t)gid + 1], s[4]);
                 atomicAdd(&G.v_conic[3 * (size_t)gid + 2], s[5]);

@@ -62,6 +62,37 @@ def test_render_parity(W, H):
     _compare_forward(eng, orc, hidden["means"].shape[0], cams[0])
 
 
+def _cluster_scene(n, spread_px, W, H, seed):
+    """n small Gaussians at depths 2.4-2.6 in front of camera 0 of small_scene, within
+    spread_px pixels of the image centre: most of a depth chunk's Gaussians land in the same
+    few tiles, so the per-chunk bucket segments are long."""
+    g, _, cams = small_scene(n=n, n_hidden=16, W=W, H=H)
+    c2w = scene.synth_cameras(11 + 7, 4, 3.0, 0.0)[0]
+    R = c2w[:3, :3] * np.array([1.0, -1.0, -1.0], np.float32)  # camera axes: x right, y down, z forward
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(2.4, 2.6, n)
+    u = rng.uniform(-spread_px, spread_px, (n, 2))
+    pc = np.stack([u[:, 0] * z / W, u[:, 1] * z / W, z], 1)  # fx = W
+    g["means"] = (pc @ R.T + c2w[:3, 3]).astype(np.float32)
+    g["scales"][:] = np.float32(-5.5)
+    return g, cams[0]
+
+
+# longest chunk segments of 1667, 125 and 60 entries (segments longer than 8 are sorted by a
+# wave through a bitmap of the chunk's ranks); segments of up to 8 by single lanes in all three
+@pytest.mark.parametrize("n,spread,min_count", [(5000, 6.0, 1500), (4000, 40.0, 150), (4000, 60.0, 100)],
+                         ids=["segments_1667", "segments_125", "segments_60"])
+def test_render_parity_clustered(n, spread, min_count):
+    DeviceState, Engine = _gpu()
+    W = H = 336
+    g, cam = _cluster_scene(n, spread, W, H, seed=3)
+    eng = Engine()
+    eng.render(DeviceState(g), cam, 3, scene.DEFAULT_BG)
+    orc = O.render(O.HostState(g), cam, 3, scene.DEFAULT_BG)
+    assert orc["tile_counts"].max() >= min_count
+    _compare_forward(eng, orc, n, cam)
+
+
 @pytest.mark.parametrize("W,H", [(160, 144), (336, 336)], ids=["chunked_K", "monolithic"])
 def test_train_step_parity(W, H):
     DeviceState, Engine = _gpu()

@@ -49,16 +49,13 @@ def _host_state(st, degree, cap=None):
     return O.HostState(g, moments=mom, capacity=cap)
 
 
-@pytest.mark.parametrize("key_bytes", [4, 8])
-def test_trainer_steps_match_oracle(key_bytes):
-    """Both tile-list key widths (rank only + id lookup; rank and id together)."""
+def test_trainer_steps_match_oracle():
     W, H = 160, 144
     g, hidden, _ = small_scene(W=W, H=H)
     ds, cams = _dataset(g, hidden, W, H)
     train_cams = [c for c in cams if not c["test"]]
     cfg = TrainingConfig(iterations=1000, sh_degree_interval=1, cap_max=2 * g["means"].shape[0])
     tr = GaussianTrainer(ds, cfg)
-    tr.set_option("list_key_bytes", key_bytes)
     for step in (1, 2, 3, 4):
         before = _state(tr)
         hps = [tr.step_params(ci, step, step) for ci in range(len(train_cams))]
