@@ -50,6 +50,13 @@ int msplat_dev_memset(void* dst, int value, size_t bytes);
  * calibration point for HBM traffic counters. Waits for completion. */
 int msplat_dev_copy_kernel(void* dst, const void* src, size_t bytes);
 
+/* Waves per workgroup of the tile-list bucket kernel: 0 (default) = 4, 2 or 1 as the LDS
+ * left by the view's tile count allows; 1, 2 or 4 forces fewer (a larger request falls back
+ * to the most that fits). Process-wide; the lists are the same for every choice, so tests
+ * use it to cover each variant. No counterpart in the reference (its list build has no
+ * such variants). Returns -1 for any other value. */
+int msplat_set_list_bucket_waves(int waves);
+
 /* Creates an engine bound to the current HIP device (buffers grow on demand and are
  * reused across steps). Replaces the reference's global MetalContext + FusedTensorCache
  * (core/metal/msplat_metal.mm:41-183, 492-625). NULL if no HIP device is usable. */

@@ -577,6 +577,12 @@ int msplat_dev_memset(void* dst, int value, size_t bytes) {
     return hipMemset(dst, value, bytes) == hipSuccess ? 0 : -2;
 }
 
+int msplat_set_list_bucket_waves(int waves) {
+    if (waves != 0 && waves != 1 && waves != 2 && waves != 4) return -1;
+    msk::set_bucket_waves(waves);
+    return 0;
+}
+
 int msplat_dev_copy_kernel(void* dst, const void* src, size_t bytes) {
     if (!dst || !src) return -1;
     msk::copy16(nullptr, src, dst, bytes);

@@ -132,6 +132,9 @@ void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L);
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);
 // Scatter of every tile hit into its tile's bucket, per-segment sort and id packing.
 void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L);
+// Waves per bucket workgroup: 0 = the most the tile count's LDS allows (4, 2 or 1); 1, 2 or 4
+// forced where the LDS allows it (process-wide; used by tests to cover every variant).
+void set_bucket_waves(int waves);
 // Stable LSD radix sort of n (key, value) pairs over the low 8 * passes key bits; with an
 // even number of passes the result is back in (ka, va). temp: lists_temp_bytes(n).
 void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, void* temp,

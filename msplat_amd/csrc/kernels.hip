@@ -10,6 +10,7 @@
 // same formulas bit for bit.
 //
 // Reference kernels are cited as metal:<line> (core/metal/msplat_metal.metal).
+#include <atomic>
 #include <utility>
 
 #include "kernels.h"
@@ -411,12 +412,10 @@ Geom geom_of(int W, int H) {
 //  2. The ranks are cut into G-wide chunks, one workgroup each. A workgroup counts its
 //     chunk's tile hits in an LDS histogram (no global atomics); a column scan over the
 //     chunks gives every (chunk, tile) pair its offset inside the tile's bucket.
-//  3. The bucket kernel (same chunks) writes each hit's chunk-local rank at that offset + an
-//     LDS cursor. A tile's bucket is thus a run of chunk segments in chunk order: all ranks
-//     of segment b are below those of segment b + 1, only the order inside a segment is
-//     arbitrary. The same workgroup then sorts each of its segments by rank (one lane per
-//     segment of up to 8 entries, one wave up to 128, the workgroup beyond) and replaces the
-//     ranks by Gaussian ids from an LDS copy of the chunk's slice of the depth order.
+//  3. The bucket kernel (same chunks) writes every hit's Gaussian id at that offset plus
+//     its place among the chunk's hits on the tile, walking the chunk's ranks in increasing
+//     order (per wave, after a per-wave count), so each segment is written sorted. A tile's
+//     bucket is its chunks' segments in chunk order: sorted by rank, with no sort pass.
 // The resulting per-tile id lists are the reference's sorted lists exactly.
 __global__ void k_depth_keys(int N, const int32_t* __restrict__ radii, const float* __restrict__ depths,
                              uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
@@ -544,127 +543,116 @@ __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ 
     }
 }
 
-__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b) {
-    const uint32_t lo = min(a, b);
-    b = max(a, b);
-    a = lo;
+// Bucket kernel: the chunk's ranks are split into W contiguous runs, one per wave. Pass A
+// counts every wave's hits per tile (16-bit fields, two waves per LDS word); their exclusive
+// prefix over the waves gives each wave its first slot inside the chunk's segment of a tile.
+// Pass B: each wave walks its ranks in increasing order, one rank at a time, its lanes
+// covering the rank's tile rectangle (distinct tiles, so one LDS atomic per lane and no two
+// lanes on one cursor); a wave's writes to a tile are thus in rank order and the segment is
+// sorted as written. W = 4, 2 or 1 by the LDS the tile count leaves (12, 8 or 4 bytes per
+// tile).
+static int bucket_waves(int T, size_t limit) {
+    const size_t t4 = ((size_t)T + 3) & ~(size_t)3;
+    if (t4 * 12 <= limit) return 4;
+    if (t4 * 8 <= limit) return 2;
+    return 1;
 }
 
-// Ascending sorting network for 8 keys (19 comparators).
-__device__ __forceinline__ void sort8(uint32_t v[8]) {
-    cswap(v[0], v[2]); cswap(v[1], v[3]); cswap(v[4], v[6]); cswap(v[5], v[7]);
-    cswap(v[0], v[4]); cswap(v[1], v[5]); cswap(v[2], v[6]); cswap(v[3], v[7]);
-    cswap(v[0], v[1]); cswap(v[2], v[3]); cswap(v[4], v[5]); cswap(v[6], v[7]);
-    cswap(v[2], v[4]); cswap(v[3], v[5]);
-    cswap(v[1], v[4]); cswap(v[3], v[6]);
-    cswap(v[1], v[2]); cswap(v[3], v[4]); cswap(v[5], v[6]);
+static size_t bucket_lds_bytes(int T, int waves) {
+    const size_t t4 = ((size_t)T + 3) & ~(size_t)3;
+    return t4 * 4 * (size_t)(waves == 1 ? 1 : 1 + waves / 2);
 }
 
-constexpr int SEG_LANE = 8;                  // segments up to this long: one lane; longer: one wave
-constexpr int BUCKET_ROUND = HIST_THREADS;   // tiles examined per sorting round
-constexpr int RANK_WORDS = 64;               // bitmap words over a chunk's local ranks (64 x 32 >= MAX_CHUNK)
-
-// LDS: cursors [T] | chunk ids [MAX_CHUNK] | wave list [round] | one rank bitmap per wave
-static size_t bucket_lds_bytes(int T) {
-    return ((size_t)(T + 3) & ~(size_t)3) * 4 +
-           (size_t)(MAX_CHUNK + BUCKET_ROUND + (HIST_THREADS / 64) * RANK_WORDS) * 4;
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-__global__ __launch_bounds__(HIST_THREADS) void k_tile_bucket(int N, int G, const uint32_t* __restrict__ sorted_ids,
-                                                              const ushort4* __restrict__ srect,
-                                                              const ViewConsts* __restrict__ vc,
-                                                              const uint32_t* __restrict__ mat,
-                                                              const uint32_t* __restrict__ exact_off,
-                                                              const uint32_t* __restrict__ counts,
-                                                              uint32_t* __restrict__ ids, uint32_t cap) {
+template <int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const uint32_t* __restrict__ sorted_ids,
+                                                            const ushort4* __restrict__ srect,
+                                                            const ViewConsts* __restrict__ vc,
+                                                            const uint32_t* __restrict__ mat,
+                                                            const uint32_t* __restrict__ exact_off,
+                                                            const uint32_t* __restrict__ counts,
+                                                            const uint32_t* __restrict__ totals,
+                                                            uint32_t* __restrict__ ids, uint32_t cap) {
     extern __shared__ uint32_t sm[];
-    __shared__ uint32_t n_wl;
     const int tbx = vc->tbx, T = vc->T;
-    uint32_t* cur = sm;
-    uint32_t* sid = sm + ((T + 3) & ~3);
-    uint32_t* wl = sid + MAX_CHUNK;
+    const int T4 = (T + 3) & ~3;
+    uint32_t* cs = sm;        // [T] start of this chunk's segment of each tile
+    uint32_t* pw = sm + T4;   // [T][WAVES / 2] per-wave cursors, 16 bits each
     const int tid = (int)threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    uint32_t* bm = wl + BUCKET_ROUND + wid * RANK_WORDS;
     const uint32_t* row = mat + (size_t)blockIdx.x * T;
     const int r0 = (int)blockIdx.x * G, r1 = min(N, r0 + G);
-    for (int t = tid; t < T; t += HIST_THREADS) cur[t] = exact_off[t] + row[t];
-    for (int r = r0 + tid; r < r1; r += HIST_THREADS) sid[r - r0] = sorted_ids[r];
-    __syncthreads();
-    // 1. every hit's chunk-local rank at its tile's cursor
-    for (int r = r0 + tid; r < r1; r += HIST_THREADS) {
-        const ushort4 q = srect[r];
-        if (q.x >= q.z || q.y >= q.w) continue;
-        const uint32_t key = (uint32_t)(r - r0);
-        for (int y = q.y; y < q.w; y++)
-            for (int x = q.x; x < q.z; x++) {
-                const uint32_t dst = atomicAdd(&cur[y * tbx + x], 1u);
-                if (dst < cap) ids[dst] = key;
-            }
+    const int per = (G + WAVES - 1) / WAVES;
+    const int w0 = min(r1, r0 + wid * per), w1 = min(r1, w0 + per);
+    const int word = WAVES == 4 ? (wid >> 1) : 0;
+    const uint32_t shift = WAVES > 1 ? 16u * (uint32_t)(wid & 1) : 0u;
+    const uint32_t one = WAVES > 1 ? (1u << shift) : 1u;
+    for (int t = tid; t < T; t += 64 * WAVES) {
+        cs[t] = exact_off[t] + row[t];
+        if (WAVES > 1)
+            for (int k = 0; k < WAVES / 2; k++) pw[(WAVES / 2) * t + k] = 0u;
     }
     __syncthreads();
-    // 2. sort every segment by rank and write the Gaussian ids in its place. A segment holds
-    // distinct local ranks below G: longer ones are sorted by marking their ranks in a
-    // wave-private bitmap and reading the set bits back in order.
-    const uint32_t pad = 0xFFFFFFFFu;
-    for (int t0 = 0; t0 < T; t0 += BUCKET_ROUND) {
-        if (tid == 0) n_wl = 0u;
+    if (WAVES > 1) {
+        // A. hits per (wave, tile)
+        for (int r = w0 + lane; r < w1; r += 64) {
+            const ushort4 q = srect[r];
+            for (int y = q.y; y < q.w; y++)
+                for (int x = q.x; x < q.z; x++) atomicAdd(&pw[(WAVES / 2) * (y * tbx + x) + word], one);
+        }
         __syncthreads();
-        const int t = t0 + tid;
-        if (t < T) {
-            const uint32_t base = exact_off[t], s0 = base + row[t], e = cur[t];
-            if (e > s0) {
-                if (base + counts[t] > cap) {  // the lists overflowed: this tile's entries become the empty record
-                    for (uint32_t i = s0; i < min(e, cap); i++) ids[i] = (uint32_t)N;
-                } else if (e - s0 <= (uint32_t)SEG_LANE) {
-                    const uint32_t n = e - s0;
-                    uint32_t v[8];
-#pragma unroll
-                    for (int i = 0; i < 8; i++) v[i] = (uint32_t)i < n ? ids[s0 + i] : pad;
-                    sort8(v);
-#pragma unroll
-                    for (int i = 0; i < 8; i++)
-                        if ((uint32_t)i < n) ids[s0 + i] = sid[v[i]];
+        // exclusive prefix over the waves (a chunk has < 2^16 ranks, so no field overflows)
+        for (int t = tid; t < T; t += 64 * WAVES) {
+            uint32_t c[4] = {0u, 0u, 0u, 0u};
+            for (int k = 0; k < WAVES / 2; k++) {
+                const uint32_t v = pw[(WAVES / 2) * t + k];
+                c[2 * k] = v & 0xFFFFu;
+                c[2 * k + 1] = v >> 16;
+            }
+            uint32_t run = 0u;
+            for (int w = 0; w < WAVES; w++) {
+                const uint32_t v = c[w];
+                c[w] = run;
+                run += v;
+            }
+            for (int k = 0; k < WAVES / 2; k++) pw[(WAVES / 2) * t + k] = c[2 * k] | (c[2 * k + 1] << 16);
+        }
+        __syncthreads();
+    }
+    // B. the wave's ranks in order; lanes cover the rectangle of one rank
+    const bool any_over = totals[0] > cap;  // some tile's list does not fit
+    for (int b0 = w0; b0 < w1; b0 += 64) {
+        const int r = b0 + lane;
+        uint2 qw = make_uint2(0u, 0u);
+        uint32_t gid = 0u;
+        if (r < w1) {
+            qw = *reinterpret_cast<const uint2*>(srect + r);
+            gid = sorted_ids[r];
+        }
+        const int nr = min(64, w1 - b0);
+        for (int j = 0; j < nr; j++) {  // wave-uniform
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)qw.x, j);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)qw.y, j);
+            const int x0 = (int)(lo & 0xFFFFu), y0 = (int)(lo >> 16), x1 = (int)(hi & 0xFFFFu), y1 = (int)(hi >> 16);
+            if (x1 <= x0 || y1 <= y0) continue;
+            const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)gid, j);
+            const int w = x1 - x0, area = w * (y1 - y0);
+            const float inv_w = 1.0f / (float)w;
+            for (int k = lane; k < area; k += 64) {
+                const int dy = (int)(((float)k + 0.5f) * inv_w);
+                const int t = (y0 + dy) * tbx + x0 + (k - dy * w);
+                uint32_t slot;
+                if (WAVES > 1) {
+                    const uint32_t old = atomicAdd(&pw[(WAVES / 2) * t + word], one);
+                    slot = cs[t] + ((old >> shift) & 0xFFFFu);
                 } else {
-                    wl[atomicAdd(&n_wl, 1u)] = (uint32_t)t;
+                    slot = atomicAdd(&cs[t], 1u);
+                }
+                if (slot < cap) {
+                    // overflowed lists: all of the tile's entries become the empty record
+                    const bool over = any_over && exact_off[t] + counts[t] > cap;
+                    ids[slot] = over ? (uint32_t)N : g;
                 }
             }
         }
-        __syncthreads();
-        const uint32_t nw = n_wl;
-        for (uint32_t k = (uint32_t)wid; k < nw; k += HIST_THREADS / 64) {
-            const int tw = (int)wl[k];
-            const uint32_t s0 = exact_off[tw] + row[tw], n = cur[tw] - s0;
-            uint32_t* seg = ids + s0;
-            bm[lane] = 0u;
-            wave_lds_sync();
-            for (uint32_t i = (uint32_t)lane; i < n; i += 64) {
-                const uint32_t key = seg[i];
-                atomicOr(&bm[key >> 5], 1u << (key & 31u));
-            }
-            wave_lds_sync();
-            uint32_t bits = bm[lane];
-            const uint32_t c = (uint32_t)__popc(bits);
-            uint32_t x = c;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const uint32_t y = __shfl_up(x, o, 64);
-                if (lane >= o) x += y;
-            }
-            uint32_t* out = seg + (x - c);  // this word's ranks follow those of the words below
-            while (bits) {
-                const uint32_t b = (uint32_t)__builtin_ctz(bits);
-                bits &= bits - 1u;
-                *out++ = sid[32u * (uint32_t)lane + b];
-            }
-            wave_lds_sync();
-        }
-        __syncthreads();
     }
 }
 
@@ -780,7 +768,7 @@ static size_t lds_limit() {
 
 bool lists_supported(int N, int T) {
     (void)N;
-    return bucket_lds_bytes(T) <= lds_limit();
+    return (size_t)T * 4 <= lds_limit() && bucket_lds_bytes(T, 1) <= lds_limit();
 }
 
 void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, void* temp,
@@ -832,13 +820,27 @@ void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, 
                        (uint32_t)L.capacity);
 }
 
+template <int WAVES>
+static void launch_bucket(hipStream_t s, int nblk, int N, int G, const ViewConsts* vc, const Geom& g, ListBufs& L) {
+    const size_t lds = bucket_lds_bytes(g.T, WAVES);
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void*)k_tile_bucket<WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(k_tile_bucket<WAVES>, dim3(nblk), dim3(64 * WAVES), lds, s, N, G, L.sorted_ids, L.srect, vc,
+                       L.hist, L.exact_off, L.tile_counts, L.totals, L.ids, (uint32_t)L.capacity);
+}
+
+static std::atomic<int> g_bucket_waves{0};
+void set_bucket_waves(int waves) { g_bucket_waves.store(waves == 1 || waves == 2 || waves == 4 ? waves : 0); }
+
 void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N <= 0) return;
     const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
-    const size_t lds = bucket_lds_bytes(g.T);
-    if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_bucket, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(k_tile_bucket, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.sorted_ids, L.srect, vc, L.hist,
-                       L.exact_off, L.tile_counts, L.ids, (uint32_t)L.capacity);
+    const int most = bucket_waves(g.T, lds_limit()), forced = g_bucket_waves.load();
+    switch (forced > 0 && forced < most ? forced : most) {
+        case 4: launch_bucket<4>(s, nblk, N, G, vc, g, L); break;
+        case 2: launch_bucket<2>(s, nblk, N, G, vc, g, L); break;
+        default: launch_bucket<1>(s, nblk, N, G, vc, g, L); break;
+    }
 }
 
 // ------------------------------------------------------------------------------------
@@ -872,28 +874,35 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
             sCx[threadIdx.x] = fmaxf(ev.rec.C[g].x + 0.5f, 0.f);
         }
         __syncthreads();
-        if (done) continue;
         const int n = min(RB, hi - b0);
-        for (int e = 0; e < n; e++) {
-            const float4 a = sA[e];
-            const float4 b = sB[e];
-            const float dx = a.x - px, dy = a.y - py;
-            const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
-            if (!ms_finite(sigma) || sigma < 0.f || sigma >= 5.55f) continue;
-            const float alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
-            if (alpha < 1.f / 255.f) continue;
-            const float nT = T * (1.f - alpha);
-            if (nT <= 1e-4f) {
-                last = b0 + e - 1;
-                done = true;
-                break;
+        // Branch-free per entry (selects instead of divergent skips), eight entries per
+        // unrolled group so that their independent sigma / exp chains overlap; the blended
+        // values and their order are those of the plain loop. A lane that saturates stops
+        // changing its state; the wave leaves when all of its lanes have.
+        for (int e0 = 0; e0 < n; e0 += 8) {
+            if (__ballot(!done) == 0ull) break;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int e = e0 + u;
+                if (e >= n) break;  // uniform
+                const float4 a = sA[e];
+                const float4 b = sB[e];
+                const float dx = a.x - px, dy = a.y - py;
+                const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
+                const bool in = ms_finite(sigma) && sigma >= 0.f && sigma < 5.55f;
+                const float alpha = fminf(0.999f, a.z * ms_expf_core(in ? -sigma : 0.f));
+                const bool use = !done && in && alpha >= 1.f / 255.f;
+                const float nT = T * (1.f - alpha);
+                const bool sat = use && nT <= 1e-4f;
+                const bool blend = use && !sat;
+                const float vis = alpha * T;
+                C[0] = blend ? fmaf(b.z, vis, C[0]) : C[0];
+                C[1] = blend ? fmaf(b.w, vis, C[1]) : C[1];
+                C[2] = blend ? fmaf(sCx[e], vis, C[2]) : C[2];
+                T = blend ? nT : T;
+                last = sat ? b0 + e - 1 : (blend ? b0 + e : last);
+                done = done || sat;
             }
-            const float vis = alpha * T;
-            C[0] = fmaf(b.z, vis, C[0]);
-            C[1] = fmaf(b.w, vis, C[1]);
-            C[2] = fmaf(sCx[e], vis, C[2]);
-            T = nT;
-            last = b0 + e;
         }
     }
 }

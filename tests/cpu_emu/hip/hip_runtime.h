@@ -58,6 +58,7 @@ struct uchar4 { unsigned char x, y, z, w; };
 inline float2 make_float2(float a, float b) { return {a, b}; }
 inline float4 make_float4(float a, float b, float c, float d) { return {a, b, c, d}; }
 inline int2 make_int2(int a, int b) { return {a, b}; }
+inline uint2 make_uint2(unsigned a, unsigned b) { return {a, b}; }
 inline ushort4 make_ushort4(unsigned short a, unsigned short b, unsigned short c, unsigned short d) {
     return {a, b, c, d};
 }
@@ -220,6 +221,11 @@ template <class T> inline T __shfl_xor(T v, int m, int width = 64) {
     T o;
     std::memcpy(&o, &r, sizeof(T));
     return o;
+}
+
+// v_readlane_b32: lane `src`'s value, for every lane (src must be the same in all lanes)
+inline int __builtin_amdgcn_readlane(int v, int src) {
+    return (int)(uint32_t)emu_exchange((uint64_t)(uint32_t)v, (unsigned)src & 63u);
 }
 
 template <class T> inline T __shfl_up(T v, unsigned d, int width = 64) {

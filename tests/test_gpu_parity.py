@@ -78,11 +78,22 @@ def _cluster_scene(n, spread_px, W, H, seed):
     return g, cams[0]
 
 
-# longest chunk segments of 1667, 125 and 60 entries (segments longer than 8 are sorted by a
-# wave through a bitmap of the chunk's ranks); segments of up to 8 by single lanes in all three
+@pytest.fixture
+def bucket_waves(request):
+    """Forces the bucket kernel's waves per workgroup for one test (0: by LDS)."""
+    from msplat_amd import _lib
+    L = _lib.lib()
+    assert L.msplat_set_list_bucket_waves(request.param) == 0
+    yield request.param
+    L.msplat_set_list_bucket_waves(0)
+
+
+# chunk segments (one tile's entries from one depth chunk) of up to 1667, 125 and 60 entries,
+# written in rank order by 1, 2 or 4 waves per chunk
+@pytest.mark.parametrize("bucket_waves", [1, 2, 4], indirect=True, ids=["1wave", "2waves", "4waves"])
 @pytest.mark.parametrize("n,spread,min_count", [(5000, 6.0, 1500), (4000, 40.0, 150), (4000, 60.0, 100)],
                          ids=["segments_1667", "segments_125", "segments_60"])
-def test_render_parity_clustered(n, spread, min_count):
+def test_render_parity_clustered(n, spread, min_count, bucket_waves):
     DeviceState, Engine = _gpu()
     W = H = 336
     g, cam = _cluster_scene(n, spread, W, H, seed=3)
