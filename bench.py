@@ -55,6 +55,8 @@ def parse():
     p.add_argument("--suite", default="c1", choices=["c1", "c0", "c1_full", "c2", "c3_bf16", "c4", "bf16_error"],
                    help="c1: the headline it/s line (default); the others print their own JSON line")
     p.add_argument("--sh-bf16", action="store_true", help="bfloat16 SH storage for the c1 suite")
+    p.add_argument("--bucket-waves", type=int, default=0, choices=[0, 1, 2, 4],
+                   help="waves per workgroup of the tile-list bucket kernel (0: as the LDS allows)")
     p.add_argument("--raster-ppl", type=int, default=0, choices=[0, 1, 4],
                    help="pixels per lane of the persistent raster kernels (0: library default)")
     return p.parse_args()
@@ -127,6 +129,8 @@ def main():
         tr.set_option("sh_bf16", 1)
     if a.raster_ppl:
         tr.set_option("raster_ppl", a.raster_ppl)
+    if a.bucket_waves:
+        _lib.lib().msplat_set_list_bucket_waves(a.bucket_waves)
     eng = tr.engine_handle
     log(f"setup {time.time() - t_setup:.1f}s ({ds.num_train} cameras)")
 

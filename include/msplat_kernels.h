@@ -50,11 +50,11 @@ int msplat_dev_memset(void* dst, int value, size_t bytes);
  * calibration point for HBM traffic counters. Waits for completion. */
 int msplat_dev_copy_kernel(void* dst, const void* src, size_t bytes);
 
-/* Waves per workgroup of the tile-list bucket kernel: 0 (default) = 4, 2 or 1 as the LDS
- * left by the view's tile count allows; 1, 2 or 4 forces fewer (a larger request falls back
- * to the most that fits). Process-wide; the lists are the same for every choice, so tests
- * use it to cover each variant. No counterpart in the reference (its list build has no
- * such variants). Returns -1 for any other value. */
+/* Waves per workgroup of the tile-list bucket kernel: 0 (default) = 2 where the LDS left by
+ * the view's tile count allows (8 bytes per tile), else 1; 1, 2 or 4 (12 bytes per tile) on
+ * request, falling back to fewer where the LDS does not allow it. Process-wide; the lists
+ * are the same for every choice, so tests use it to cover each variant. No counterpart in
+ * the reference (its list build has no such variants). Returns -1 for any other value. */
 int msplat_set_list_bucket_waves(int waves);
 
 /* Creates an engine bound to the current HIP device (buffers grow on demand and are

@@ -132,8 +132,8 @@ void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L);
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);
 // Scatter of every tile hit into its tile's bucket, per-segment sort and id packing.
 void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L);
-// Waves per bucket workgroup: 0 = the most the tile count's LDS allows (4, 2 or 1); 1, 2 or 4
-// forced where the LDS allows it (process-wide; used by tests to cover every variant).
+// Waves per bucket workgroup: 0 = 2 where the tile count's LDS allows, else 1; 1, 2 or 4 on
+// request where the LDS allows it (process-wide; tests use it to cover every variant).
 void set_bucket_waves(int waves);
 // Stable LSD radix sort of n (key, value) pairs over the low 8 * passes key bits; with an
 // even number of passes the result is back in (ka, va). temp: lists_temp_bytes(n).
@@ -177,6 +177,8 @@ void view_accumulate(hipStream_t s, const Params& P, const ViewConsts* vc, int d
 void batch_update(hipStream_t s, const Params& P, const BatchGrads& A, const MsplatStepParams* hpd, int degrees_to_use,
                   uint32_t* nan_counts);
 void fill_zero(hipStream_t s, void* p, size_t bytes);
+// Four ranges (any may be null or empty) zeroed by one launch.
+void fill_zero4(hipStream_t s, void* p0, size_t b0, void* p1, size_t b1, void* p2, size_t b2, void* p3, size_t b3);
 void copy16(hipStream_t s, const void* src, void* dst, size_t bytes);
 // bf16 storage helpers: shadows of f_dc / f_rest from the fp32 masters (rows [0, N)), and
 // element-wise conversions.

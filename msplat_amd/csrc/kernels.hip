@@ -99,6 +99,14 @@ __device__ __forceinline__ float wave_sum4_rows(float a, float b, float c, float
     return v;
 }
 
+// 1 / d for d in [0.001, 1] (1 - alpha in the raster backward): v_rcp_f32 (within 1 ulp)
+// refined by one Newton step, instead of the correctly rounded division sequence (about ten
+// instructions). The backward's gradients are compared at 1e-4, not bit for bit.
+__device__ __forceinline__ float recip_nr(float d) {
+    const float r = __builtin_amdgcn_rcpf(d);
+    return fmaf(fmaf(-d, r, 1.0f), r, r);
+}
+
 // Block-wide exclusive scan of uint32 over blockDim.x (multiple of 64, <= 1024) threads.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* lds_wave, uint32_t* total) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
@@ -137,6 +145,31 @@ __global__ void k_fill_zero(uint32_t* p, size_t n) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     size_t stride = (size_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = 0u;
+}
+
+// Up to four ranges in one launch (blockIdx.y picks the range).
+struct ZeroRanges {
+    uint32_t* p[4];
+    size_t n[4];
+};
+__global__ void k_fill_zero4(ZeroRanges r) {
+    uint32_t* p = r.p[blockIdx.y];
+    const size_t n = r.n[blockIdx.y];
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0u;
+}
+
+void fill_zero4(hipStream_t s, void* p0, size_t b0, void* p1, size_t b1, void* p2, size_t b2, void* p3, size_t b3) {
+    ZeroRanges r{{(uint32_t*)p0, (uint32_t*)p1, (uint32_t*)p2, (uint32_t*)p3}, {b0 / 4, b1 / 4, b2 / 4, b3 / 4}};
+    size_t most = 0;
+    for (int k = 0; k < 4; k++) {
+        if (!r.p[k]) r.n[k] = 0;
+        most = std::max(most, r.n[k]);
+    }
+    if (most == 0) return;
+    const unsigned blocks = (unsigned)std::max<size_t>(1, std::min<size_t>(2048, (most + 255) / 256));
+    hipLaunchKernelGGL(k_fill_zero4, dim3(blocks, 4), dim3(256), 0, s, r);
 }
 
 void fill_zero(hipStream_t s, void* p, size_t bytes) {
@@ -213,6 +246,11 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
     SHT* s_rest = reinterpret_cast<SHT*>(s_raw);
     const int i0 = blockIdx.x * PROJ_BLOCK;
     const int i = i0 + threadIdx.x;
+    if (i == 0) {  // row N: the transparent record that list entries past the last Gaussian read
+        rA[N] = make_float4(0.f, 0.f, 0.f, 0.f);
+        rB[N] = make_float4(0.f, 0.f, 0.f, 0.f);
+        rC[N] = make_float2(0.f, 0.f);
+    }
     if (deg_use > 0 && nb3 > 0) {
         // the block's rows are contiguous and start 16-byte aligned (i0 * nb3 * sizeof(SHT)
         // is a multiple of 16 for PROJ_BLOCK = 128)
@@ -377,7 +415,10 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
 
 void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, ProjOut& o,
                      uint32_t* nan_counts) {
-    if (P.N <= 0) return;
+    if (P.N <= 0) {  // no Gaussians: only the transparent record (row 0)
+        fill_zero4(s, o.rec.A, 16, o.rec.B, 16, o.rec.C, 8, nullptr, 0);
+        return;
+    }
     const int nb3 = (P.degree <= 0 ? 0 : (P.degree == 1 ? 3 : P.degree == 2 ? 8 : 15)) * 3;
     const int du = degrees_to_use < P.degree ? degrees_to_use : P.degree;
     const size_t esz = P.sh_bf16 ? 2 : 4;
@@ -551,10 +592,12 @@ __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ 
 // lanes on one cursor); a wave's writes to a tile are thus in rank order and the segment is
 // sorted as written. W = 4, 2 or 1 by the LDS the tile count leaves (12, 8 or 4 bytes per
 // tile).
-static int bucket_waves(int T, size_t limit) {
+// 2 waves when the LDS allows (measured at 1.5M Gaussians, 1600x1040: 4 waves 0.336 ms,
+// 2 waves 0.319 ms, 1 wave 0.464 ms per launch), else 1; 4 only on request.
+static int bucket_waves(int T, size_t limit, int request) {
     const size_t t4 = ((size_t)T + 3) & ~(size_t)3;
-    if (t4 * 12 <= limit) return 4;
-    if (t4 * 8 <= limit) return 2;
+    if (request == 4 && t4 * 12 <= limit) return 4;
+    if (request != 1 && t4 * 8 <= limit) return 2;
     return 1;
 }
 
@@ -617,8 +660,35 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
         }
         __syncthreads();
     }
-    // B. the wave's ranks in order; lanes cover the rectangle of one rank
+    // B. the wave's ranks in order; lanes cover the rectangle of one rank. Four ranks at a
+    // time: their cursor atomics are issued back to back (the LDS performs one wave's
+    // operations in issue order, so rank order is kept) before any result is waited for.
     const bool any_over = totals[0] > cap;  // some tile's list does not fit
+    // next slot of this wave in tile t's segment, in two halves (the LDS atomic, then the
+    // sum) so that several atomics can be in flight before the first sum waits for one
+    auto claim = [&](int t, uint32_t& base) -> uint32_t {
+        if (WAVES > 1) {
+            base = cs[t];
+            return atomicAdd(&pw[(WAVES / 2) * t + word], one);
+        }
+        base = 0u;
+        return atomicAdd(&cs[t], 1u);
+    };
+    auto slot_of = [&](uint32_t claimed, uint32_t base) -> uint32_t {
+        return WAVES > 1 ? base + ((claimed >> shift) & 0xFFFFu) : claimed;
+    };
+    auto cursor = [&](int t) -> uint32_t {
+        uint32_t base;
+        const uint32_t c = claim(t, base);
+        return slot_of(c, base);
+    };
+    auto put = [&](uint32_t slot, int t, uint32_t g) {
+        if (slot < cap) {
+            // overflowed lists: all of the tile's entries become the empty record
+            const bool over = any_over && exact_off[t] + counts[t] > cap;
+            ids[slot] = over ? (uint32_t)N : g;
+        }
+    };
     for (int b0 = w0; b0 < w1; b0 += 64) {
         const int r = b0 + lane;
         uint2 qw = make_uint2(0u, 0u);
@@ -628,28 +698,54 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
             gid = sorted_ids[r];
         }
         const int nr = min(64, w1 - b0);
-        for (int j = 0; j < nr; j++) {  // wave-uniform
-            const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)qw.x, j);
-            const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)qw.y, j);
-            const int x0 = (int)(lo & 0xFFFFu), y0 = (int)(lo >> 16), x1 = (int)(hi & 0xFFFFu), y1 = (int)(hi >> 16);
-            if (x1 <= x0 || y1 <= y0) continue;
-            const uint32_t g = (uint32_t)__builtin_amdgcn_readlane((int)gid, j);
-            const int w = x1 - x0, area = w * (y1 - y0);
-            const float inv_w = 1.0f / (float)w;
-            for (int k = lane; k < area; k += 64) {
-                const int dy = (int)(((float)k + 0.5f) * inv_w);
-                const int t = (y0 + dy) * tbx + x0 + (k - dy * w);
-                uint32_t slot;
-                if (WAVES > 1) {
-                    const uint32_t old = atomicAdd(&pw[(WAVES / 2) * t + word], one);
-                    slot = cs[t] + ((old >> shift) & 0xFFFFu);
-                } else {
-                    slot = atomicAdd(&cs[t], 1u);
+        for (int j0 = 0; j0 < nr; j0 += 4) {  // wave-uniform
+            int x0[4], y0[4], wd[4], area[4];
+            uint32_t g[4];
+            int most = 0;
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int j = min(j0 + u, nr - 1);
+                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)qw.x, j);
+                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)qw.y, j);
+                g[u] = (uint32_t)__builtin_amdgcn_readlane((int)gid, j);
+                x0[u] = (int)(lo & 0xFFFFu);
+                y0[u] = (int)(lo >> 16);
+                const int x1 = (int)(hi & 0xFFFFu), y1 = (int)(hi >> 16);
+                wd[u] = max(x1 - x0[u], 0);
+                area[u] = (j0 + u < nr) ? wd[u] * max(y1 - y0[u], 0) : 0;
+                most = max(most, area[u]);
+            }
+            if (most == 0) continue;
+            if (most <= 64) {
+                int t[4];
+                uint32_t claimed[4], base[4];
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    t[u] = 0;
+                    if (lane < area[u]) {
+                        const int dy = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)wd[u]));
+                        t[u] = (y0[u] + dy) * tbx + x0[u] + (lane - dy * wd[u]);
+                    }
                 }
-                if (slot < cap) {
-                    // overflowed lists: all of the tile's entries become the empty record
-                    const bool over = any_over && exact_off[t] + counts[t] > cap;
-                    ids[slot] = over ? (uint32_t)N : g;
+                // the wave's LDS operations are performed in issue order, so rank u's claims
+                // precede rank u + 1's on every tile (the wave barrier only keeps the order
+                // explicit for the compiler)
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if (lane < area[u]) claimed[u] = claim(t[u], base[u]);
+                    __builtin_amdgcn_wave_barrier();
+                }
+#pragma unroll
+                for (int u = 0; u < 4; u++)
+                    if (lane < area[u]) put(slot_of(claimed[u], base[u]), t[u], g[u]);
+            } else {
+                for (int u = 0; u < 4; u++) {  // a rectangle of more than 64 tiles: rank by rank
+                    const float inv_w = __builtin_amdgcn_rcpf((float)max(wd[u], 1));
+                    for (int k = lane; k < area[u]; k += 64) {
+                        const int dy = (int)(((float)k + 0.5f) * inv_w);
+                        const int t = (y0[u] + dy) * tbx + x0[u] + (k - dy * wd[u]);
+                        put(cursor(t), t, g[u]);
+                    }
                 }
             }
         }
@@ -835,8 +931,7 @@ void set_bucket_waves(int waves) { g_bucket_waves.store(waves == 1 || waves == 2
 void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N <= 0) return;
     const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
-    const int most = bucket_waves(g.T, lds_limit()), forced = g_bucket_waves.load();
-    switch (forced > 0 && forced < most ? forced : most) {
+    switch (bucket_waves(g.T, lds_limit(), g_bucket_waves.load())) {
         case 4: launch_bucket<4>(s, nblk, N, G, vc, g, L); break;
         case 2: launch_bucket<2>(s, nblk, N, G, vc, g, L); break;
         default: launch_bucket<1>(s, nblk, N, G, vc, g, L); break;
@@ -874,35 +969,28 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
             sCx[threadIdx.x] = fmaxf(ev.rec.C[g].x + 0.5f, 0.f);
         }
         __syncthreads();
+        if (done) continue;
         const int n = min(RB, hi - b0);
-        // Branch-free per entry (selects instead of divergent skips), eight entries per
-        // unrolled group so that their independent sigma / exp chains overlap; the blended
-        // values and their order are those of the plain loop. A lane that saturates stops
-        // changing its state; the wave leaves when all of its lanes have.
-        for (int e0 = 0; e0 < n; e0 += 8) {
-            if (__ballot(!done) == 0ull) break;
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int e = e0 + u;
-                if (e >= n) break;  // uniform
-                const float4 a = sA[e];
-                const float4 b = sB[e];
-                const float dx = a.x - px, dy = a.y - py;
-                const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
-                const bool in = ms_finite(sigma) && sigma >= 0.f && sigma < 5.55f;
-                const float alpha = fminf(0.999f, a.z * ms_expf_core(in ? -sigma : 0.f));
-                const bool use = !done && in && alpha >= 1.f / 255.f;
-                const float nT = T * (1.f - alpha);
-                const bool sat = use && nT <= 1e-4f;
-                const bool blend = use && !sat;
-                const float vis = alpha * T;
-                C[0] = blend ? fmaf(b.z, vis, C[0]) : C[0];
-                C[1] = blend ? fmaf(b.w, vis, C[1]) : C[1];
-                C[2] = blend ? fmaf(sCx[e], vis, C[2]) : C[2];
-                T = blend ? nT : T;
-                last = sat ? b0 + e - 1 : (blend ? b0 + e : last);
-                done = done || sat;
+        for (int e = 0; e < n; e++) {
+            const float4 a = sA[e];
+            const float4 b = sB[e];
+            const float dx = a.x - px, dy = a.y - py;
+            const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
+            if (!ms_finite(sigma) || sigma < 0.f || sigma >= 5.55f) continue;
+            const float alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
+            if (alpha < 1.f / 255.f) continue;
+            const float nT = T * (1.f - alpha);
+            if (nT <= 1e-4f) {
+                last = b0 + e - 1;
+                done = true;
+                break;
             }
+            const float vis = alpha * T;
+            C[0] = fmaf(b.z, vis, C[0]);
+            C[1] = fmaf(b.w, vis, C[1]);
+            C[2] = fmaf(sCx[e], vis, C[2]);
+            T = nT;
+            last = b0 + e;
         }
     }
 }
@@ -1313,6 +1401,15 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
         __syncthreads();
         // four entries at a time: their 4 x 9 terms are reduced together (wave_sum4_rows)
         for (int t4 = 0; t4 < n; t4 += 4) {
+            // entries behind every pixel's last contributor in this wave: only zeros to record
+            const int e_min = bhi - 1 - min(t4 + 3, n - 1);
+            if (__ballot(inside && bin_final >= e_min) == 0ull) {
+                if ((lane & 15) == 15) {
+#pragma unroll
+                    for (int k = 0; k < 9; k++) red[wid][t4 + tag][k] = 0.f;
+                }
+                continue;
+            }
             float g[4][9];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -1337,7 +1434,7 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
                     }
                 }
                 if (valid && alpha < 0.999f) {
-                    const float ra = 1.f / (1.f - alpha);
+                    const float ra = recip_nr(1.f - alpha);
                     T *= ra;
                     const float fac = alpha * T;
                     const float r0 = b.z, r1 = b.w, r2 = sC[tt].x;
@@ -1682,7 +1779,7 @@ __global__ __launch_bounds__(RB4) void k_raster_bwd_persist4(const RasterView* _
                     }
                     if (!(valid && alpha < 0.999f)) continue;
                     any = true;
-                    const float ra = 1.f / (1.f - alpha);
+                    const float ra = recip_nr(1.f - alpha);
                     T[q] *= ra;
                     const float fac = alpha * T[q];
                     const float tfb0 = Tf[q] * bgc[0], tfb1 = Tf[q] * bgc[1], tfb2 = Tf[q] * bgc[2];
@@ -2086,13 +2183,38 @@ __device__ __forceinline__ void reg_grads(int N, float opac_i, const float ls[3]
 
 // Adam on means / log-scales / quaternions / opacity with the non-finite gradient gate, then
 // SGLD on the updated parameters, then write-back of row i.
+// Adam moments of groups 0, 1, 2, 5 and the opacity logit of one Gaussian, loaded at the
+// start of an update kernel so that their latency overlaps the gradient computation.
+struct TailState {
+    float m0[3], v0[3], m1[3], v1[3], m2[4], v2[4], m5, v5, o;
+};
+
+__device__ __forceinline__ void tail_load(int i, const float* __restrict__ opac, const float* __restrict__ m0,
+                                          const float* __restrict__ v0, const float* __restrict__ m1,
+                                          const float* __restrict__ v1, const float* __restrict__ m2,
+                                          const float* __restrict__ v2, const float* __restrict__ m5,
+                                          const float* __restrict__ v5, TailState& t) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const size_t e = 3 * (size_t)i + k;
+        t.m0[k] = m0[e]; t.v0[k] = v0[e];
+        t.m1[k] = m1[e]; t.v1[k] = v1[e];
+    }
+    const float4 a = reinterpret_cast<const float4*>(m2)[i], b = reinterpret_cast<const float4*>(v2)[i];
+    t.m2[0] = a.x; t.m2[1] = a.y; t.m2[2] = a.z; t.m2[3] = a.w;
+    t.v2[0] = b.x; t.v2[1] = b.y; t.v2[2] = b.z; t.v2[3] = b.w;
+    t.m5 = m5[i];
+    t.v5 = v5[i];
+    t.o = opac[i];
+}
+
 __device__ __forceinline__ void adam_sgld_tail(int i, float p[3], float ls[3], float q[4], const float vm[3],
                                                const float vs[3], const float vq[4], float vo,
                                                float* __restrict__ means, float* __restrict__ scales,
                                                float* __restrict__ quats, float* __restrict__ opac,
                                                float* __restrict__ m0, float* __restrict__ v0, float* __restrict__ m1,
                                                float* __restrict__ v1, float* __restrict__ m2, float* __restrict__ v2,
-                                               float* __restrict__ m5, float* __restrict__ v5,
+                                               float* __restrict__ m5, float* __restrict__ v5, TailState& t,
                                                const MsplatStepParams& hp, uint32_t* __restrict__ nanc) {
     const float b1 = hp.beta1, b2 = hp.beta2, eps = hp.eps;
     // fused_adam (metal:1680-1713) with the non-finite gradient gate, groups 0, 1, 2, 5
@@ -2100,30 +2222,26 @@ __device__ __forceinline__ void adam_sgld_tail(int i, float p[3], float ls[3], f
     for (int k = 0; k < 3; k++) {
         const size_t e = 3 * (size_t)i + k;
         if (ms_finite(vm[k])) {
-            float mm_ = m0[e], vv = v0[e];
-            adam_el(p[k], mm_, vv, vm[k], hp.step_size[0], b1, b2, hp.bc2_sqrt[0], eps);
-            m0[e] = mm_; v0[e] = vv;
+            adam_el(p[k], t.m0[k], t.v0[k], vm[k], hp.step_size[0], b1, b2, hp.bc2_sqrt[0], eps);
+            m0[e] = t.m0[k]; v0[e] = t.v0[k];
         } else atomicAdd(&nanc[0], 1u);
         if (ms_finite(vs[k])) {
-            float mm_ = m1[e], vv = v1[e];
-            adam_el(ls[k], mm_, vv, vs[k], hp.step_size[1], b1, b2, hp.bc2_sqrt[1], eps);
-            m1[e] = mm_; v1[e] = vv;
+            adam_el(ls[k], t.m1[k], t.v1[k], vs[k], hp.step_size[1], b1, b2, hp.bc2_sqrt[1], eps);
+            m1[e] = t.m1[k]; v1[e] = t.v1[k];
         } else atomicAdd(&nanc[1], 1u);
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) {
         const size_t e = 4 * (size_t)i + k;
         if (ms_finite(vq[k])) {
-            float mm_ = m2[e], vv = v2[e];
-            adam_el(q[k], mm_, vv, vq[k], hp.step_size[2], b1, b2, hp.bc2_sqrt[2], eps);
-            m2[e] = mm_; v2[e] = vv;
+            adam_el(q[k], t.m2[k], t.v2[k], vq[k], hp.step_size[2], b1, b2, hp.bc2_sqrt[2], eps);
+            m2[e] = t.m2[k]; v2[e] = t.v2[k];
         } else atomicAdd(&nanc[2], 1u);
     }
-    float o = opac[i];
+    float o = t.o;
     if (ms_finite(vo)) {
-        float mm_ = m5[i], vv = v5[i];
-        adam_el(o, mm_, vv, vo, hp.step_size[5], b1, b2, hp.bc2_sqrt[5], eps);
-        m5[i] = mm_; v5[i] = vv;
+        adam_el(o, t.m5, t.v5, vo, hp.step_size[5], b1, b2, hp.bc2_sqrt[5], eps);
+        m5[i] = t.m5; v5[i] = t.v5;
     } else atomicAdd(&nanc[5], 1u);
     // SGLD (sgld_noise_gen metal:3871-3890, sgld_noise metal:3892-3967) on the updated state
     if (hp.sgld_enable) {
@@ -2185,6 +2303,8 @@ __global__ __launch_bounds__(256) void k_gaussian_update(
     float ls[3] = {scales[3 * (size_t)i], scales[3 * (size_t)i + 1], scales[3 * (size_t)i + 2]};
     const float4 q4 = reinterpret_cast<const float4*>(quats)[i];
     float q[4] = {q4.x, q4.y, q4.z, q4.w};
+    TailState ts;
+    tail_load(i, opac, m0, v0, m1, v1, m2, v2, m5, v5, ts);
     float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
     const float b1 = hp.beta1, b2 = hp.beta2, eps = hp.eps;
     if (radii[i] > 0) {
@@ -2213,7 +2333,7 @@ __global__ __launch_bounds__(256) void k_gaussian_update(
     if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
         const float vdc = Mom<BF>::ld(v3, 3 * (size_t)i) + Mom<BF>::ld(v3, 3 * (size_t)i + 1) +
                           Mom<BF>::ld(v3, 3 * (size_t)i + 2);
-        reg_grads(N, opac[i], ls, vdc, hp, vo, vs);
+        reg_grads(N, ts.o, ls, vdc, hp, vo, vs);
     }
     if (dbg_vm) {
 #pragma unroll
@@ -2222,7 +2342,8 @@ __global__ __launch_bounds__(256) void k_gaussian_update(
         for (int k = 0; k < 4; k++) dbg_vq[4 * (size_t)i + k] = vq[k];
         dbg_vo[i] = vo;
     }
-    adam_sgld_tail(i, p, ls, q, vm, vs, vq, vo, means, scales, quats, opac, m0, v0, m1, v1, m2, v2, m5, v5, hp, nanc);
+    adam_sgld_tail(i, p, ls, q, vm, vs, vq, vo, means, scales, quats, opac, m0, v0, m1, v1, m2, v2, m5, v5, ts, hp,
+                   nanc);
 }
 
 void gaussian_update(hipStream_t s, const Params& P, const MsplatCamera& cam, const ProjOut& pj, const GradBufs& G,
@@ -2329,6 +2450,8 @@ __global__ __launch_bounds__(256) void k_batch_update(
     float ls[3] = {scales[3 * (size_t)i], scales[3 * (size_t)i + 1], scales[3 * (size_t)i + 2]};
     const float4 q4 = reinterpret_cast<const float4*>(quats)[i];
     float q[4] = {q4.x, q4.y, q4.z, q4.w};
+    TailState ts;
+    tail_load(i, opac, m0, v0, m1, v1, m2, v2, m5, v5, ts);
     float vm[3], vs[3], vq[4];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -2363,9 +2486,10 @@ __global__ __launch_bounds__(256) void k_batch_update(
     if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
         const float vdc = Mom<BF>::ld(v3, 3 * (size_t)i) + Mom<BF>::ld(v3, 3 * (size_t)i + 1) +
                           Mom<BF>::ld(v3, 3 * (size_t)i + 2);
-        reg_grads(N, opac[i], ls, vdc, hp, vo, vs);
+        reg_grads(N, ts.o, ls, vdc, hp, vo, vs);
     }
-    adam_sgld_tail(i, p, ls, q, vm, vs, vq, vo, means, scales, quats, opac, m0, v0, m1, v1, m2, v2, m5, v5, hp, nanc);
+    adam_sgld_tail(i, p, ls, q, vm, vs, vq, vo, means, scales, quats, opac, m0, v0, m1, v1, m2, v2, m5, v5, ts, hp,
+                   nanc);
 }
 
 static int nuse3_of(const Params& P, int degrees_to_use) {

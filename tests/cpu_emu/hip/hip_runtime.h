@@ -223,6 +223,9 @@ template <class T> inline T __shfl_xor(T v, int m, int width = 64) {
     return o;
 }
 
+// v_rcp_f32 (an approximation to within 1 ulp; the exact quotient stands in for it)
+inline float __builtin_amdgcn_rcpf(float x) { return 1.0f / x; }
+
 // v_readlane_b32: lane `src`'s value, for every lane (src must be the same in all lanes)
 inline int __builtin_amdgcn_readlane(int v, int src) {
     return (int)(uint32_t)emu_exchange((uint64_t)(uint32_t)v, (unsigned)src & 63u);
