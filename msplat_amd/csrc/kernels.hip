@@ -603,7 +603,8 @@ static int bucket_waves(int T, size_t limit, int request) {
 
 static size_t bucket_lds_bytes(int T, int waves) {
     const size_t t4 = ((size_t)T + 3) & ~(size_t)3;
-    return t4 * 4 * (size_t)(waves == 1 ? 1 : 1 + waves / 2);
+    // + one 64-word line per wave that idle lanes point their atomics at
+    return t4 * 4 * (size_t)(waves == 1 ? 1 : 1 + waves / 2) + (size_t)waves * 64 * 4;
 }
 
 template <int WAVES>
@@ -620,6 +621,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
     const int T4 = (T + 3) & ~3;
     uint32_t* cs = sm;        // [T] start of this chunk's segment of each tile
     uint32_t* pw = sm + T4;   // [T][WAVES / 2] per-wave cursors, 16 bits each
+    uint32_t* idle = sm + T4 * (WAVES == 1 ? 1 : 1 + WAVES / 2) + 64 * (threadIdx.x >> 6);  // [64] per wave
     const int tid = (int)threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint32_t* row = mat + (size_t)blockIdx.x * T;
     const int r0 = (int)blockIdx.x * G, r1 = min(N, r0 + G);
@@ -628,10 +630,26 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
     const int word = WAVES == 4 ? (wid >> 1) : 0;
     const uint32_t shift = WAVES > 1 ? 16u * (uint32_t)(wid & 1) : 0u;
     const uint32_t one = WAVES > 1 ? (1u << shift) : 1u;
-    for (int t = tid; t < T; t += 64 * WAVES) {
-        cs[t] = exact_off[t] + row[t];
-        if (WAVES > 1)
-            for (int k = 0; k < WAVES / 2; k++) pw[(WAVES / 2) * t + k] = 0u;
+    {
+        constexpr int NT = 64 * WAVES, U = 8;  // U tiles per thread in flight
+        for (int t0 = 0; t0 < T; t0 += NT * U) {
+            uint32_t a[U], b[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * NT + tid;
+                a[u] = t < T ? exact_off[t] : 0u;
+                b[u] = t < T ? row[t] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                const int t = t0 + u * NT + tid;
+                if (t < T) {
+                    cs[t] = a[u] + b[u];
+                    if (WAVES > 1)
+                        for (int k = 0; k < WAVES / 2; k++) pw[(WAVES / 2) * t + k] = 0u;
+                }
+            }
+        }
     }
     __syncthreads();
     if (WAVES > 1) {
@@ -728,12 +746,15 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
                     }
                 }
                 // the wave's LDS operations are performed in issue order, so rank u's claims
-                // precede rank u + 1's on every tile (the wave barrier only keeps the order
-                // explicit for the compiler)
+                // precede rank u + 1's on every tile. Lanes past a rank's rectangle add 0 to
+                // a word of their own, so the claims need no branches (and no waits between)
 #pragma unroll
                 for (int u = 0; u < 4; u++) {
-                    if (lane < area[u]) claimed[u] = claim(t[u], base[u]);
-                    __builtin_amdgcn_wave_barrier();
+                    const bool on = lane < area[u];
+                    uint32_t* cur = on ? (WAVES > 1 ? &pw[(WAVES / 2) * t[u] + word] : &cs[t[u]]) : &idle[lane];
+                    base[u] = WAVES > 1 ? (on ? cs[t[u]] : 0u) : 0u;
+                    claimed[u] = atomicAdd(cur, on ? one : 0u);
+                    __builtin_amdgcn_sched_barrier(0);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; u++)
@@ -746,6 +767,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
                         const int t = (y0[u] + dy) * tbx + x0[u] + (k - dy * wd[u]);
                         put(cursor(t), t, g[u]);
                     }
+                    __builtin_amdgcn_sched_barrier(0);  // rank u's claims before rank u + 1's
                 }
             }
         }

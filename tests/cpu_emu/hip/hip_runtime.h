@@ -196,6 +196,9 @@ inline emu_f32x4 __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(emu_bf16x4 a, emu_bf1
 }
 
 inline void __builtin_amdgcn_wave_barrier() { emu_wave_sync(); }
+// A scheduling barrier: instructions before it issue before those after it. The lanes of a
+// wave issue in lockstep on the GPU; here they are brought to the same point.
+inline void __builtin_amdgcn_sched_barrier(int) { emu_wave_sync(); }
 #define __builtin_amdgcn_fence(order, scope) ((void)0)
 
 inline uint64_t __ballot(int p) {

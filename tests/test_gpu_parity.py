@@ -62,7 +62,7 @@ def test_render_parity(W, H):
     _compare_forward(eng, orc, hidden["means"].shape[0], cams[0])
 
 
-def _cluster_scene(n, spread_px, W, H, seed):
+def _cluster_scene(n, spread_px, W, H, seed, n_big=0):
     """n small Gaussians at depths 2.4-2.6 in front of camera 0 of small_scene, within
     spread_px pixels of the image centre: most of a depth chunk's Gaussians land in the same
     few tiles, so the per-chunk bucket segments are long."""
@@ -75,6 +75,7 @@ def _cluster_scene(n, spread_px, W, H, seed):
     pc = np.stack([u[:, 0] * z / W, u[:, 1] * z / W, z], 1)  # fx = W
     g["means"] = (pc @ R.T + c2w[:3, 3]).astype(np.float32)
     g["scales"][:] = np.float32(-5.5)
+    g["scales"][:n_big] = np.float32(-1.7)  # ~100 tiles each (the list capacity is 48 per Gaussian)
     return g, cams[0]
 
 
@@ -89,14 +90,17 @@ def bucket_waves(request):
 
 
 # chunk segments (one tile's entries from one depth chunk) of up to 1667, 125 and 60 entries,
-# written in rank order by 1, 2 or 4 waves per chunk
+# and Gaussians covering more than 64 tiles each, written in rank order by 1, 2 or 4 waves
+# per chunk
 @pytest.mark.parametrize("bucket_waves", [1, 2, 4], indirect=True, ids=["1wave", "2waves", "4waves"])
-@pytest.mark.parametrize("n,spread,min_count", [(5000, 6.0, 1500), (4000, 40.0, 150), (4000, 60.0, 100)],
-                         ids=["segments_1667", "segments_125", "segments_60"])
-def test_render_parity_clustered(n, spread, min_count, bucket_waves):
+@pytest.mark.parametrize("n,spread,n_big,min_count",
+                         [(5000, 6.0, 0, 1500), (4000, 40.0, 0, 150), (4000, 60.0, 0, 100),
+                          (4000, 60.0, 600, 600)],
+                         ids=["segments_1667", "segments_125", "segments_60", "rects_over_64_tiles"])
+def test_render_parity_clustered(n, spread, n_big, min_count, bucket_waves):
     DeviceState, Engine = _gpu()
     W = H = 336
-    g, cam = _cluster_scene(n, spread, W, H, seed=3)
+    g, cam = _cluster_scene(n, spread, W, H, seed=3, n_big=n_big)
     eng = Engine()
     eng.render(DeviceState(g), cam, 3, scene.DEFAULT_BG)
     orc = O.render(O.HostState(g), cam, 3, scene.DEFAULT_BG)
