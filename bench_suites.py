@@ -214,8 +214,10 @@ def bf16_error(a):
                 one[mode] = e
             out["after_1_step"] = one
             _log(f"after 1 step: {one}")
-    for ci, hp in plan[1:]:
+    for s, (ci, hp) in enumerate(plan[1:], start=2):
         O.train_step(ref, cams[ci][0], cams[ci][1], hp, want_ids=False)
+        if s % 100 == 0:
+            _log(f"FP64 oracle step {s} of {total}")
     two = {}
     eval_cam = cams[0][0]
     r64 = O.render(ref, eval_cam, 3, scene.DEFAULT_BG, want_ids=False)["img"]

@@ -473,7 +473,7 @@ __device__ __forceinline__ uint32_t pad_count(uint32_t c) {  // metal:2248-2256
 }
 
 constexpr int HIST_THREADS = 512;
-constexpr int MAX_CHUNK = 1984;  // Gaussians per chunk: every window fits 2048 keys
+constexpr int MAX_CHUNK = 1984;  // Gaussians per chunk at most (a chunk-local rank fits 16 bits)
 
 // srect [N]: tile rectangles in depth-rank order (empty for invisible Gaussians), so the
 // chunk kernels read them contiguously.
@@ -774,8 +774,14 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
     }
 }
 
+// Chunks of at most MAX_CHUNK ranks; a smaller scene is cut finer (down to MIN_CHUNK ranks,
+// up to about one chunk per resident bucket workgroup) so that the bucket kernel's serial
+// walk per wave stays short: 50k Gaussians make 391 chunks of 128, not 26 of 1924.
+constexpr int MIN_CHUNK = 128, CHUNK_TARGET = 768;
 static int hist_blocks(int N) {
-    int nb = (N + MAX_CHUNK - 1) / MAX_CHUNK;
+    const int by_max = (N + MAX_CHUNK - 1) / MAX_CHUNK;
+    const int by_min = (N + MIN_CHUNK - 1) / MIN_CHUNK;
+    const int nb = std::max(by_max, std::min(by_min, CHUNK_TARGET));
     return nb < 1 ? 1 : nb;
 }
 
@@ -1212,6 +1218,11 @@ __global__ __launch_bounds__(256) void k_ssim_h(const ViewConsts* __restrict__ v
     }
 }
 
+// The vertical backward takes 32 columns per workgroup: each of its halo rows is then one
+// full 128-byte line per plane (16 columns fetch half-used lines). Every pixel's arithmetic
+// is that of 16x16 tiles.
+constexpr int FW = 32;
+
 __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict__ vc, const float* __restrict__ img,
                                                     const float* __restrict__ hbuf, float* __restrict__ dbuf,
                                                     float* __restrict__ partials, uint32_t* __restrict__ nanc) {
@@ -1318,16 +1329,15 @@ __global__ __launch_bounds__(256) void k_ssim_vbwd(const ViewConsts* __restrict_
                                                    const float* __restrict__ dbuf, float* __restrict__ v_rend,
                                                    uint32_t* __restrict__ nanc) {
     constexpr int TH = SB + 10;
-    __shared__ float sh[9][TH][SB];
+    __shared__ float sh[9][TH][FW];
     const int W = vc->W, H = vc->H;
-    if ((int)blockIdx.x * SB >= W || (int)blockIdx.y * SB >= H) return;
+    if ((int)blockIdx.x * FW >= W || (int)blockIdx.y * SB >= H) return;
     const uint8_t* __restrict__ gt = vc->gt;
     const float ssim_w = vc->ssim_w, inv_n = vc->inv_n;
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int bx = blockIdx.x * SB, by = blockIdx.y * SB - 5;
+    const int bx = blockIdx.x * FW, by = blockIdx.y * SB - 5;
     const size_t P = (size_t)W * H;
-    for (int k = threadIdx.x; k < TH * SB; k += 256) {
-        const int sy = k / SB, sx = k % SB;
+    for (int k = threadIdx.x; k < TH * FW; k += 256) {
+        const int sy = k / FW, sx = k % FW;
         const int gy = by + sy, gx = bx + sx;
         const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
         const size_t pid = (size_t)gy * W + gx;
@@ -1335,28 +1345,31 @@ __global__ __launch_bounds__(256) void k_ssim_vbwd(const ViewConsts* __restrict_
         for (int q = 0; q < 9; q++) sh[q][sy][sx] = in ? dbuf[q * P + pid] : 0.f;
     }
     __syncthreads();
-    const int px = bx + lx, py = blockIdx.y * SB + ly;
-    if (px >= W || py >= H) return;
-    const size_t pid = (size_t)py * W + px;
-    for (int c = 0; c < 3; c++) {
-        float f1 = 0.f, f2 = 0.f, f3 = 0.f;
+    for (int k = threadIdx.x; k < SB * FW; k += 256) {
+        const int ly = k / FW, lx = k % FW;
+        const int px = bx + lx, py = blockIdx.y * SB + ly;
+        if (px >= W || py >= H) continue;
+        const size_t pid = (size_t)py * W + px;
+        for (int c = 0; c < 3; c++) {
+            float f1 = 0.f, f2 = 0.f, f3 = 0.f;
 #pragma unroll
-        for (int d = 0; d < 11; d++) {
-            const float w = c_gauss1d[10 - d];
-            f1 += w * sh[c * 3 + 0][ly + d][lx];
-            f2 += w * sh[c * 3 + 1][ly + d][lx];
-            f3 += w * sh[c * 3 + 2][ly + d][lx];
+            for (int d = 0; d < 11; d++) {
+                const float w = c_gauss1d[10 - d];
+                f1 += w * sh[c * 3 + 0][ly + d][lx];
+                f2 += w * sh[c * 3 + 1][ly + d][lx];
+                f3 += w * sh[c * 3 + 2][ly + d][lx];
+            }
+            const float rv = img[3 * pid + c];
+            const float gv = (float)gt[3 * pid + c] * (1.0f / 255.0f);
+            const float vs = f1 + rv * f2 + gv * f3;
+            const float vl = (gv > rv) ? -1.0f : ((gv < rv) ? 1.0f : 0.0f);
+            float v = inv_n * (-ssim_w * vs + (1.0f - ssim_w) * vl);
+            if (!ms_finite(v)) {
+                atomicAdd(&nanc[6], 1u);
+                v = 0.f;
+            }
+            v_rend[3 * pid + c] = v;
         }
-        const float rv = img[3 * pid + c];
-        const float gv = (float)gt[3 * pid + c] * (1.0f / 255.0f);
-        const float vs = f1 + rv * f2 + gv * f3;
-        const float vl = (gv > rv) ? -1.0f : ((gv < rv) ? 1.0f : 0.0f);
-        float v = inv_n * (-ssim_w * vs + (1.0f - ssim_w) * vl);
-        if (!ms_finite(v)) {
-            atomicAdd(&nanc[6], 1u);
-            v = 0.f;
-        }
-        v_rend[3 * pid + c] = v;
     }
 }
 
@@ -1377,10 +1390,10 @@ __global__ __launch_bounds__(256) void k_loss_final(const ViewConsts* __restrict
 
 void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& gm, const float* img, float* hbuf, float* dbuf,
                float* v_rend, float* loss_partials, double* loss_out, uint32_t* nan_counts) {
-    dim3 g((gm.W + SB - 1) / SB, (gm.H + SB - 1) / SB);
+    dim3 g((gm.W + SB - 1) / SB, (gm.H + SB - 1) / SB), gw((gm.W + FW - 1) / FW, (gm.H + SB - 1) / SB);
     hipLaunchKernelGGL(k_ssim_h, g, dim3(256), 0, s, vc, img, hbuf);
     hipLaunchKernelGGL(k_ssim_fused, g, dim3(256), 0, s, vc, img, hbuf, dbuf, loss_partials, nan_counts);
-    hipLaunchKernelGGL(k_ssim_vbwd, g, dim3(256), 0, s, vc, img, dbuf, v_rend, nan_counts);
+    hipLaunchKernelGGL(k_ssim_vbwd, gw, dim3(256), 0, s, vc, img, dbuf, v_rend, nan_counts);
     hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, s, vc, loss_partials, loss_out);
 }
 

@@ -89,14 +89,14 @@ def bucket_waves(request):
     L.msplat_set_list_bucket_waves(0)
 
 
-# chunk segments (one tile's entries from one depth chunk) of up to 1667, 125 and 60 entries,
-# and Gaussians covering more than 64 tiles each, written in rank order by 1, 2 or 4 waves
-# per chunk
+# tiles holding 1500 and more entries, 150+ and 100+ (chunk segments, one tile's entries from
+# one depth chunk of 125 ranks, of up to 125 entries), and Gaussians covering more than 64
+# tiles each, written in rank order by 1, 2 or 4 waves per chunk
 @pytest.mark.parametrize("bucket_waves", [1, 2, 4], indirect=True, ids=["1wave", "2waves", "4waves"])
 @pytest.mark.parametrize("n,spread,n_big,min_count",
                          [(5000, 6.0, 0, 1500), (4000, 40.0, 0, 150), (4000, 60.0, 0, 100),
                           (4000, 60.0, 600, 600)],
-                         ids=["segments_1667", "segments_125", "segments_60", "rects_over_64_tiles"])
+                         ids=["one_tile", "few_tiles", "many_tiles", "rects_over_64_tiles"])
 def test_render_parity_clustered(n, spread, n_big, min_count, bucket_waves):
     DeviceState, Engine = _gpu()
     W = H = 336
