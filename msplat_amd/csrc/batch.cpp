@@ -23,7 +23,7 @@ bool BatchRunner::Key::operator==(const Key& o) const {
         if (acc[k] != o.acc[k]) return false;
     for (int k = 0; k < 12; k++)
         if (mom[k] != o.mom[k]) return false;
-    for (int k = 0; k < 7; k++)
+    for (int k = 0; k < 3; k++)
         if (own[k] != o.own[k]) return false;
     return true;
 }
@@ -35,7 +35,7 @@ BatchRunner::~BatchRunner() {
     release_graph();
     for (auto& e : tev_)
         if (e) (void)hipEventDestroy(e);
-    DBuf* all[] = {&block_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_};
+    DBuf* all[] = {&block_, &wq_, &wmisc_};
     for (DBuf* b : all) b->release();
 }
 
@@ -74,11 +74,7 @@ void BatchRunner::enqueue(const msk::Params& P, int B, int du, int Tmax, msk::Ba
                                                        align256(sizeof(msk::ViewConsts) * B));
     w.B = B;
     w.Tmax = Tmax;
-    w.keys_a = wka_.as<uint32_t>();
-    w.vals_a = wva_.as<uint32_t>();
-    w.keys_b = wkb_.as<uint32_t>();
-    w.vals_b = wvb_.as<uint32_t>();
-    w.temp = wtemp_.p;
+    w.queue = wq_.as<uint32_t>();
     w.n_work = wmisc_.as<uint32_t>();
     w.ctr = wmisc_.as<uint32_t>() + 1;
     w.grid = persist_grid_;
@@ -209,8 +205,7 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
     for (int k = 0; k < B; k++)
         Tmax = std::max(Tmax, graphable ? hwv_[k].T : msk::geom_of(sizes_[k].first, sizes_[k].second).T);
     const size_t items = (size_t)B * Tmax;
-    wka_.ensure(items * 4); wva_.ensure(items * 4); wkb_.ensure(items * 4); wvb_.ensure(items * 4);
-    wtemp_.ensure(msk::work_temp_bytes((int)items));
+    wq_.ensure(items * 4);
     wmisc_.ensure(64);
     const size_t off_vc = align256(sizeof(MsplatStepParams));
     const size_t off_rv = off_vc + align256(sizeof(msk::ViewConsts) * B);
@@ -272,8 +267,8 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
     // per slot: buffer generation and depth-chunk count (chunked views record their own path)
     for (int v = 0; v < B; v++) gens = gens * 1000003 + slots_[v]->generation() * 4096 + slots_[v]->chunks();
     k.gens = gens;
-    const DBuf* own[7] = {&block_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_};
-    for (int j = 0; j < 7; j++) k.own[j] = own[j]->p;
+    const DBuf* own[3] = {&block_, &wq_, &wmisc_};
+    for (int j = 0; j < 3; j++) k.own[j] = own[j]->p;
     const void* ptrs[8] = {st.means, st.scales, st.quats, st.f_dc, st.f_rest, st.opac, st.f_dc_bf16, st.f_rest_bf16};
     for (int j = 0; j < 8; j++) k.state[j] = ptrs[j];
     for (int j = 0; j < 6; j++) {

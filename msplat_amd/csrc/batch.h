@@ -69,7 +69,7 @@ private:
         int64_t gens = 0;
         const void* state[8] = {};
         const void* mom[12] = {};
-        const void* own[7] = {};
+        const void* own[3] = {};
         const void* acc[7] = {};
         const void* nanc = nullptr;
         bool operator==(const Key& o) const;
@@ -83,7 +83,7 @@ private:
     std::vector<int> slot_of_view_;
     std::vector<msk::Geom> hwv_;               // high-water launch geometry per slot (graph mode)
     DBuf block_;                                     // [MsplatStepParams | ViewConsts x B | RasterView x B]
-    DBuf wka_, wva_, wkb_, wvb_, wtemp_, wmisc_;     // batch work queue
+    DBuf wq_, wmisc_;     // batch work queue
     ConstRing ring_;
     bool graph_on_ = false;
     bool sequential_ = false;

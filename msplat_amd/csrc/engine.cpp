@@ -122,7 +122,7 @@ void Engine::trim() {
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_,
                    &finalT_, &finalIdx_, &vrend_, &hbuf_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
-                   &prefixT_, &afterC_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_, &srect_};
+                   &prefixT_, &afterC_, &wq_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
     debug_valid_ = false;
     gen_++;
@@ -141,7 +141,7 @@ Engine::~Engine() {
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_, &finalT_, &finalIdx_,
                    &vrend_, &hbuf_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
-                   &afterC_, &nanc_, &wka_, &wva_, &wkb_, &wvb_, &wtemp_, &wmisc_, &consts_, &srect_};
+                   &afterC_, &nanc_, &wq_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
     if (stream_ && owns_stream_) hipStreamDestroy(stream_);
 }
@@ -186,8 +186,7 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
     ensure(ids_, (size_t)capacity_ * 4);
     ensure(hist_, msk::lists_hist_bytes(N, T));
     ensure(img_, P * 12); ensure(finalT_, P * 4); ensure(finalIdx_, P * 4);
-    ensure(wka_, (size_t)T * 4); ensure(wva_, (size_t)T * 4); ensure(wkb_, (size_t)T * 4); ensure(wvb_, (size_t)T * 4);
-    ensure(wtemp_, msk::work_temp_bytes(T));
+    ensure(wq_, (size_t)T * 4);
     ensure(wmisc_, 64);
     ensure(consts_, sizeof(msk::ViewConsts) + sizeof(msk::RasterView) + 64);
     if (K_ > 1) {
@@ -258,11 +257,7 @@ msk::RasterWork Engine::own_work() {
     w.views = reinterpret_cast<const msk::RasterView*>(consts_.as<uint8_t>() + sizeof(msk::ViewConsts));
     w.B = 1;
     w.Tmax = launch_.T;
-    w.keys_a = wka_.as<uint32_t>();
-    w.vals_a = wva_.as<uint32_t>();
-    w.keys_b = wkb_.as<uint32_t>();
-    w.vals_b = wvb_.as<uint32_t>();
-    w.temp = wtemp_.p;
+    w.queue = wq_.as<uint32_t>();
     w.n_work = wmisc_.as<uint32_t>();
     w.ctr = wmisc_.as<uint32_t>() + 1;
     w.grid = persist_grid_;

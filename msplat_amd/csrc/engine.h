@@ -137,7 +137,7 @@ private:
     // per-tile
     DBuf counts_, exact_, bins_, totals_;
     DBuf dkeys_, dkeys_out_, dvals_, sorted_ids_, temp_, srect_;
-    DBuf wka_, wva_, wkb_, wvb_, wtemp_, wmisc_;   // own persistent-raster work queue
+    DBuf wq_, wmisc_;   // own persistent-raster work queue
     // per-intersection
     DBuf ids_, hist_;
     // per-pixel

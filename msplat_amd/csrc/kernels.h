@@ -115,8 +115,7 @@ struct RasterView {
 struct RasterWork {
     const RasterView* views;   // device [B]
     int B, Tmax;
-    uint32_t *keys_a, *vals_a, *keys_b, *vals_b;   // [B Tmax] each
-    void* temp;                // work_temp_bytes(B Tmax)
+    uint32_t* queue;           // [B Tmax] work items, longest tile lists first
     uint32_t* n_work;          // [1]
     uint32_t* ctr;             // [2] forward / backward queue heads
     int grid;                  // resident workgroups
@@ -139,7 +138,6 @@ void set_bucket_waves(int waves);
 // even number of passes the result is back in (ka, va). temp: lists_temp_bytes(n).
 void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, void* temp,
                       int passes);
-size_t work_temp_bytes(int items);
 // Fills the persistent raster work queue (ordered by decreasing tile list length) and
 // resets its heads; the views' lists must be built.
 void raster_work_build(hipStream_t s, const RasterWork& w);

@@ -837,17 +837,26 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
     __shared__ uint32_t lw[RS_THREADS / 64 + 1];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    // this thread's keys and values first: their loads overlap each other and the scans
+    uint32_t kr[RS_ITEMS], vr[RS_ITEMS];
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const int i = blockIdx.x * RS_TILE + r * RS_THREADS + threadIdx.x;
+        kr[r] = i < N ? kin[i] : 0u;
+        vr[r] = i < N ? vin[i] : 0u;
+    }
     uint32_t tot;
     const uint32_t dbase = block_exclusive_scan(rowtot[threadIdx.x], lw, &tot);
     run[threadIdx.x] = dbase + counts[(size_t)threadIdx.x * nblk + blockIdx.x];
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
 #pragma unroll
         for (int w = 0; w < RS_THREADS / 64; w++) wcnt[w][threadIdx.x] = 0u;
         __syncthreads();
         const int i = blockIdx.x * RS_TILE + r * RS_THREADS + threadIdx.x;
         const bool valid = i < N;
-        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t key = kr[r];
         const uint32_t d = (key >> shift) & 255u;
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -862,7 +871,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
             uint32_t pos = run[d] + rank;
             for (int w = 0; w < wid; w++) pos += wcnt[w][d];
             kout[pos] = key;
-            vout[pos] = vin[i];
+            vout[pos] = vr[r];
         }
         __syncthreads();
         uint32_t add = 0u;
@@ -1027,28 +1036,49 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
 // tile list length so that the longest tiles start first and the launch ends on short ones.
 constexpr int WORK_TILE_BITS = 20;
 
-__global__ void k_work_keys(const RasterView* __restrict__ views, int B, int Tmax, uint32_t* __restrict__ keys,
-                            uint32_t* __restrict__ vals, uint32_t* __restrict__ n_work, uint32_t* __restrict__ ctr) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) {
-        uint32_t n = 0;
-        for (int v = 0; v < B; v++) n += views[v].chunked ? 0u : (uint32_t)views[v].vc->T;
-        *n_work = n;
+// One workgroup: every valid (view, tile) item is binned by its list length (8-entry bins,
+// longest first, all lists of 2040 entries or more in the first bin) with LDS counters, and
+// placed bin by bin. The order inside a bin is arbitrary; it only balances the persistent
+// raster kernels, whose outputs do not depend on it.
+constexpr int WQ_THREADS = 1024, WQ_BINS = 256;
+__global__ __launch_bounds__(WQ_THREADS) void k_work_queue(const RasterView* __restrict__ views, int B, int Tmax,
+                                                          uint32_t* __restrict__ queue, uint32_t* __restrict__ n_work,
+                                                          uint32_t* __restrict__ ctr) {
+    __shared__ uint32_t cnt[WQ_BINS];
+    __shared__ uint32_t lw[17];
+    const int tid = (int)threadIdx.x, n = B * Tmax;
+    if (tid < WQ_BINS) cnt[tid] = 0u;
+    __syncthreads();
+    auto bin_of = [&](int k, uint32_t& code) -> int {  // -1: not an item
+        const int v = k / Tmax, t = k % Tmax;
+        const RasterView& rv = views[v];
+        if (rv.chunked || t >= rv.vc->T) return -1;
+        const int2 r = rv.bins[t];
+        const uint32_t len = (uint32_t)max(0, r.y - r.x);
+        code = ((uint32_t)v << WORK_TILE_BITS) | (uint32_t)t;
+        return WQ_BINS - 1 - (int)min(len >> 3, (uint32_t)(WQ_BINS - 1));
+    };
+    for (int k = tid; k < n; k += WQ_THREADS) {
+        uint32_t code;
+        const int b = bin_of(k, code);
+        if (b >= 0) atomicAdd(&cnt[b], 1u);
+    }
+    __syncthreads();
+    uint32_t total = 0u;
+    const uint32_t c = tid < WQ_BINS ? cnt[tid] : 0u;
+    const uint32_t start = block_exclusive_scan(c, lw, &total);
+    if (tid < WQ_BINS) cnt[tid] = start;
+    __syncthreads();
+    for (int k = tid; k < n; k += WQ_THREADS) {
+        uint32_t code;
+        const int b = bin_of(k, code);
+        if (b >= 0) queue[atomicAdd(&cnt[b], 1u)] = code;
+    }
+    if (tid == 0) {
+        *n_work = total;
         ctr[0] = 0u;
         ctr[1] = 0u;
     }
-    if (k >= B * Tmax) return;
-    const int v = k / Tmax, t = k % Tmax;
-    const RasterView& rv = views[v];
-    if (rv.chunked || t >= rv.vc->T) {
-        keys[k] = 0xFFFFu;
-        vals[k] = 0xFFFFFFFFu;
-        return;
-    }
-    const int2 r = rv.bins[t];
-    const uint32_t len = (uint32_t)max(0, r.y - r.x);
-    keys[k] = 0xFFFEu - min(len, 0xFFFEu);
-    vals[k] = ((uint32_t)v << WORK_TILE_BITS) | (uint32_t)t;
 }
 
 __global__ __launch_bounds__(RB) void k_raster_fwd_persist(const RasterView* __restrict__ views,
@@ -1143,15 +1173,9 @@ __global__ void k_raster_merge(const ViewConsts* __restrict__ vc, int K, const f
     img[3 * pid + 2] = fminf(fmaxf(fmaf(bg2, Tr, C2), 0.f), 1.f);
 }
 
-size_t work_temp_bytes(int items) { return lists_temp_bytes(items); }
-
 void raster_work_build(hipStream_t s, const RasterWork& w) {
-    const int n = w.B * w.Tmax;
-    if (n <= 0) return;
-    hipLaunchKernelGGL(k_work_keys, dim3(cdiv(n, 256)), dim3(256), 0, s, w.views, w.B, w.Tmax, w.keys_a, w.vals_a,
-                       w.n_work, w.ctr);
-    // keys are 16 bits: two stable 8-bit passes leave the order in (keys_a, vals_a)
-    radix_sort_pairs(s, n, w.keys_a, w.vals_a, w.keys_b, w.vals_b, w.temp, 2);
+    if (w.B * w.Tmax <= 0) return;
+    hipLaunchKernelGGL(k_work_queue, dim3(1), dim3(WQ_THREADS), 0, s, w.views, w.B, w.Tmax, w.queue, w.n_work, w.ctr);
 }
 
 void raster_forward(hipStream_t s, const ListBufs& L, const ViewConsts* vc, const Geom& g, RasterBufs& R,
@@ -1864,16 +1888,16 @@ __global__ __launch_bounds__(RB4) void k_raster_bwd_persist4(const RasterView* _
 
 void raster_backward_views(hipStream_t s, const RasterWork& w) {
     if (w.ppl == 4)
-        hipLaunchKernelGGL(k_raster_bwd_persist4, dim3(w.grid), dim3(RB4), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
+        hipLaunchKernelGGL(k_raster_bwd_persist4, dim3(w.grid), dim3(RB4), 0, s, w.views, w.queue, w.n_work, w.ctr);
     else
-        hipLaunchKernelGGL(k_raster_bwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
+        hipLaunchKernelGGL(k_raster_bwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.queue, w.n_work, w.ctr);
 }
 
 void raster_forward_views(hipStream_t s, const RasterWork& w) {
     if (w.ppl == 4)
-        hipLaunchKernelGGL(k_raster_fwd_persist4, dim3(w.grid), dim3(RB4), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
+        hipLaunchKernelGGL(k_raster_fwd_persist4, dim3(w.grid), dim3(RB4), 0, s, w.views, w.queue, w.n_work, w.ctr);
     else
-        hipLaunchKernelGGL(k_raster_fwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.vals_a, w.n_work, w.ctr);
+        hipLaunchKernelGGL(k_raster_fwd_persist, dim3(w.grid), dim3(RB), 0, s, w.views, w.queue, w.n_work, w.ctr);
 }
 
 // ------------------------------------------------------------------------------------
