@@ -1969,7 +1969,7 @@ __device__ __forceinline__ void sh_basis_all(float x, float y, float z, float* b
 // Each workgroup updates 1024 consecutive coefficients: the Gaussians they belong to get
 // their basis values and colour gradient staged in LDS once, then every thread updates 4
 // coefficients with 16-byte loads and stores of parameter, exp_avg and exp_avg_sq.
-constexpr int SH_EPB = 2048;  // coefficients per workgroup of k_sh_rest_adam (8 per thread)
+constexpr int SH_EPB = 1024;
 
 // Moment storage of the SH groups: fp32, or bf16 (rounded to nearest even on store).
 template <bool BF> struct Mom;
@@ -2032,64 +2032,56 @@ __global__ __launch_bounds__(256) void k_sh_rest_adam(int N, int nuse3, const fl
         }
     }
     __syncthreads();
-    // two groups of four coefficients per thread, both loaded before either is updated
-    constexpr int HALF = SH_EPB / 2;
-    float p[2][4], m[2][4], v[2][4];
-#pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const size_t e = e0 + (size_t)h * HALF + 4 * (size_t)threadIdx.x;
-        if (e + 4 <= total) {
-            const float4 a = reinterpret_cast<const float4*>(f_rest)[e >> 2];
-            p[h][0] = a.x; p[h][1] = a.y; p[h][2] = a.z; p[h][3] = a.w;
-            Mom<BF>::ld4(m4, e, m[h]);
-            Mom<BF>::ld4(v4, e, v[h]);
-        } else {
-            for (int q = 0; q < 4; q++) {
-                const bool in = e + q < total;
-                p[h][q] = in ? f_rest[e + q] : 0.f;
-                m[h][q] = in ? Mom<BF>::ld(m4, e + q) : 0.f;
-                v[h][q] = in ? Mom<BF>::ld(v4, e + q) : 0.f;
-            }
+    const size_t e = e0 + 4 * (size_t)threadIdx.x;
+    if (e >= total) return;
+    float p[4], m[4], v[4];
+    const bool vec = e + 4 <= total;
+    if (vec) {
+        const float4 a = reinterpret_cast<const float4*>(f_rest)[e >> 2];
+        p[0] = a.x; p[1] = a.y; p[2] = a.z; p[3] = a.w;
+        Mom<BF>::ld4(m4, e, m);
+        Mom<BF>::ld4(v4, e, v);
+    } else {
+        for (int q = 0; q < 4; q++) {
+            const bool in = e + q < total;
+            p[q] = in ? f_rest[e + q] : 0.f;
+            m[q] = in ? Mom<BF>::ld(m4, e + q) : 0.f;
+            v[q] = in ? Mom<BF>::ld(v4, e + q) : 0.f;
         }
     }
+    bool touched = false;
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        const size_t e = e0 + (size_t)h * HALF + 4 * (size_t)threadIdx.x;
-        if (e >= total) break;
-        bool touched = false;
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const size_t el = e + q;
-            if (el >= total) break;
-            const int g = (int)(el / NB3);
-            const int r = (int)(el - (size_t)g * NB3);
-            const int k = g - g0;
-            if (s_b[k][15] == 0.f || r >= nuse3) continue;
-            const int b = r / 3, c = r - 3 * b;
-            const float g_el = s_b[k][b] * s_vc[k][c];
-            adam_el(p[h][q], m[h][q], v[h][q], g_el, step, b1, b2, bc2s, eps);
-            touched = true;
+    for (int q = 0; q < 4; q++) {
+        const size_t el = e + q;
+        if (el >= total) break;
+        const int g = (int)(el / NB3);
+        const int r = (int)(el - (size_t)g * NB3);
+        const int k = g - g0;
+        if (s_b[k][15] == 0.f || r >= nuse3) continue;
+        const int b = r / 3, c = r - 3 * b;
+        const float g_el = s_b[k][b] * s_vc[k][c];
+        adam_el(p[q], m[q], v[q], g_el, step, b1, b2, bc2s, eps);
+        touched = true;
+    }
+    if (!touched) return;  // nothing of these four elements was updated
+    if (vec) {
+        reinterpret_cast<float4*>(f_rest)[e >> 2] = make_float4(p[0], p[1], p[2], p[3]);
+        Mom<BF>::st4(m4, e, m);
+        Mom<BF>::st4(v4, e, v);
+        if (BF) {
+            uint2 h;
+            h.x = (uint32_t)ms_bf16_bits(p[0]) | ((uint32_t)ms_bf16_bits(p[1]) << 16);
+            h.y = (uint32_t)ms_bf16_bits(p[2]) | ((uint32_t)ms_bf16_bits(p[3]) << 16);
+            reinterpret_cast<uint2*>(shadow)[e >> 2] = h;
         }
-        if (!touched) continue;  // nothing of these four elements was updated
-        if (e + 4 <= total) {
-            reinterpret_cast<float4*>(f_rest)[e >> 2] = make_float4(p[h][0], p[h][1], p[h][2], p[h][3]);
-            Mom<BF>::st4(m4, e, m[h]);
-            Mom<BF>::st4(v4, e, v[h]);
-            if (BF) {
-                uint2 hb;
-                hb.x = (uint32_t)ms_bf16_bits(p[h][0]) | ((uint32_t)ms_bf16_bits(p[h][1]) << 16);
-                hb.y = (uint32_t)ms_bf16_bits(p[h][2]) | ((uint32_t)ms_bf16_bits(p[h][3]) << 16);
-                reinterpret_cast<uint2*>(shadow)[e >> 2] = hb;
+    } else {
+        for (int q = 0; q < 4; q++)
+            if (e + q < total) {
+                f_rest[e + q] = p[q];
+                Mom<BF>::st(m4, e + q, m[q]);
+                Mom<BF>::st(v4, e + q, v[q]);
+                if (BF) shadow[e + q] = ms_bf16_bits(p[q]);
             }
-        } else {
-            for (int q = 0; q < 4; q++)
-                if (e + q < total) {
-                    f_rest[e + q] = p[h][q];
-                    Mom<BF>::st(m4, e + q, m[h][q]);
-                    Mom<BF>::st(v4, e + q, v[h][q]);
-                    if (BF) shadow[e + q] = ms_bf16_bits(p[h][q]);
-                }
-        }
     }
 }
 
