@@ -1069,10 +1069,18 @@ __global__ __launch_bounds__(WQ_THREADS) void k_work_queue(const RasterView* __r
         code = ((uint32_t)v << WORK_TILE_BITS) | (uint32_t)t;
         return WQ_BINS - 1 - (int)min(len >> 3, (uint32_t)(WQ_BINS - 1));
     };
-    for (int k = tid; k < n; k += WQ_THREADS) {
-        uint32_t code;
-        const int b = bin_of(k, code);
-        if (b >= 0) atomicAdd(&cnt[b], 1u);
+    constexpr int U = 8;  // items per thread in flight
+    for (int k0 = 0; k0 < n; k0 += WQ_THREADS * U) {
+        int b[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u * WQ_THREADS + tid;
+            uint32_t code;
+            b[u] = k < n ? bin_of(k, code) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (b[u] >= 0) atomicAdd(&cnt[b[u]], 1u);
     }
     __syncthreads();
     uint32_t total = 0u;
@@ -1080,10 +1088,17 @@ __global__ __launch_bounds__(WQ_THREADS) void k_work_queue(const RasterView* __r
     const uint32_t start = block_exclusive_scan(c, lw, &total);
     if (tid < WQ_BINS) cnt[tid] = start;
     __syncthreads();
-    for (int k = tid; k < n; k += WQ_THREADS) {
-        uint32_t code;
-        const int b = bin_of(k, code);
-        if (b >= 0) queue[atomicAdd(&cnt[b], 1u)] = code;
+    for (int k0 = 0; k0 < n; k0 += WQ_THREADS * U) {
+        int b[U];
+        uint32_t code[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int k = k0 + u * WQ_THREADS + tid;
+            b[u] = k < n ? bin_of(k, code[u]) : -1;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (b[u] >= 0) queue[atomicAdd(&cnt[b[u]], 1u)] = code[u];
     }
     if (tid == 0) {
         *n_work = total;
