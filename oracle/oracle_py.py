@@ -157,7 +157,7 @@ class HostState:
         return {k: self.p[k][: self.N] for k in GROUP_SHAPES}
 
 
-def _run(fn_name, st: HostState, cam, gt, hp, want_ids=True, render_args=None):
+def _run(fn_name, st: HostState, cam, gt, hp, want_ids=True, render_args=None, ids_cap=None):
     L = lib()
     f64 = st.dtype == np.float64
     O = OutF64 if f64 else OutF32
@@ -179,7 +179,8 @@ def _run(fn_name, st: HostState, cam, gt, hp, want_ids=True, render_args=None):
                     "v_conic": np.zeros((N, 3), st.dtype), "v_rgb": np.zeros((N, 3), st.dtype),
                     "v_opac": np.zeros(N, st.dtype), "v_mean": np.zeros((N, 3), st.dtype),
                     "v_scale": np.zeros((N, 3), st.dtype), "v_quat": np.zeros((N, 4), st.dtype)})
-    ids_cap = int(N) * 48 if want_ids else 0
+    # the lists are copied out only when they fit (48 N, the list capacity, by default)
+    ids_cap = (int(N) * 48 if ids_cap is None else int(ids_cap)) if want_ids else 0
     ids = np.zeros(max(ids_cap, 1), np.int32)
     o = O()
     for k, a in out.items():
@@ -216,9 +217,9 @@ def train_step(st: HostState, cam, gt: np.ndarray, hp, want_ids=True):
                 want_ids)
 
 
-def render(st: HostState, cam, degrees_to_use: int, bg, want_ids=True):
+def render(st: HostState, cam, degrees_to_use: int, bg, want_ids=True, ids_cap=None):
     return _run("oracle_render_f64" if st.dtype == np.float64 else "oracle_render_f32", st, cam, None, None,
-                want_ids, render_args=(degrees_to_use, bg))
+                want_ids, render_args=(degrees_to_use, bg), ids_cap=ids_cap)
 
 
 def train_step_batch(st: HostState, cams, gts, hps, want_images=False):

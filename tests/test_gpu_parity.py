@@ -108,6 +108,50 @@ def test_render_parity_clustered(n, spread, n_big, min_count, bucket_waves):
     _compare_forward(eng, orc, n, cam)
 
 
+@pytest.mark.parametrize("bucket_waves", [1, 2, 4], indirect=True, ids=["1wave", "2waves", "4waves"])
+def test_list_overflow_is_contained(bucket_waves):
+    """More tile hits than the list capacity (48 per Gaussian, msplat_metal.mm:518): tile
+    counts and clamped bins stay exact, the tiles whose lists fit hold the oracle's lists, the
+    tile that crosses the capacity holds only the empty record (id N), tiles past it are empty,
+    and every tile that lost entries renders as background."""
+    DeviceState, Engine = _gpu()
+    W = H = 336
+    n = 64
+    g, cam = _cluster_scene(n, 40.0, W, H, seed=5, n_big=n)  # 64 Gaussians of ~100 tiles each
+    eng = Engine()
+    eng.render(DeviceState(g), cam, 3, scene.DEFAULT_BG)
+    orc = O.render(O.HostState(g), cam, 3, scene.DEFAULT_BG, ids_cap=1000 * n)  # the oracle's full lists
+    st = eng.stats()
+    cap = 48 * n
+    assert st["num_isect"] == orc["num_isect"] > cap and st["overflow"] == 1
+    T = st["num_tiles"]
+    counts = orc["tile_counts"].astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    assert_exact("tile_counts", eng.read("tile_counts", np.int32, (T,)), orc["tile_counts"])
+    want_bins = np.stack([np.minimum(off, cap), np.minimum(off + counts, cap)], 1).astype(np.int32)
+    assert_exact("tile_bins", eng.read("tile_bins", np.int32, (T, 2)), want_bins)
+    ids = eng.read("ids", np.int32, (cap,))
+    lost = []
+    for t in range(T):
+        a, c = int(off[t]), int(counts[t])
+        if c == 0:
+            continue
+        if a + c <= cap:
+            assert_exact(f"ids of tile {t}", ids[a:a + c], orc["ids"][a:a + c])
+        else:
+            lost.append(t)
+            if a < cap:
+                assert (ids[a:cap] == n).all(), f"tile {t}: entries below the capacity are not the empty record"
+    assert len(lost) > 1
+    img = eng.raster(W, H)["img"]
+    tbx = (W + 15) // 16
+    bg = np.array(scene.DEFAULT_BG, np.float32)
+    for t in lost:
+        ty, tx = divmod(t, tbx)
+        tile = img[16 * ty:16 * ty + 16, 16 * tx:16 * tx + 16]
+        assert (tile == bg).all(), f"tile {t} does not show the background"
+
+
 @pytest.mark.parametrize("W,H", [(160, 144), (336, 336)], ids=["chunked_K", "monolithic"])
 def test_train_step_parity(W, H):
     DeviceState, Engine = _gpu()
