@@ -2390,34 +2390,44 @@ __global__ __launch_bounds__(256) void k_gaussian_update(
     float q[4] = {q4.x, q4.y, q4.z, q4.w};
     TailState ts;
     tail_load(i, opac, m0, v0, m1, v1, m2, v2, m5, v5, ts);
+    // Every row of the gradient and projection records is loaded before the visibility test
+    // (all are sized N), so their latency overlaps instead of following the radius load.
+    const int rad = radii[i];
+    const float4 a = rA[i], b = rB[i];
+    const float2 vxy = G.v_xy[i];
+    const float vcn[3] = {G.v_conic[3 * (size_t)i], G.v_conic[3 * (size_t)i + 1], G.v_conic[3 * (size_t)i + 2]};
+    const float vc[3] = {G.v_rgb[3 * (size_t)i], G.v_rgb[3 * (size_t)i + 1], G.v_rgb[3 * (size_t)i + 2]};
+    float dc[3], dm[3], dv[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const size_t k = 3 * (size_t)i + c;
+        dc[c] = f_dc[k]; dm[c] = Mom<BF>::ld(m3, k); dv[c] = Mom<BF>::ld(v3, k);
+    }
+    float vo = G.v_opac[i];
     float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
     const float b1 = hp.beta1, b2 = hp.beta2, eps = hp.eps;
-    if (radii[i] > 0) {
-        const float4 a = rA[i], b = rB[i];
+    if (rad > 0) {
         const float conic[3] = {a.w, b.x, b.y};
-        const float2 vxy = G.v_xy[i];
-        const float vcn[3] = {G.v_conic[3 * (size_t)i], G.v_conic[3 * (size_t)i + 1], G.v_conic[3 * (size_t)i + 2]};
         proj_vjp(p, ls, q, cam, conic, vxy.x, vxy.y, vcn, vm, vs, vq);
-        const float vc[3] = {G.v_rgb[3 * (size_t)i], G.v_rgb[3 * (size_t)i + 1], G.v_rgb[3 * (size_t)i + 2]};
         if (ms_finite(vc[0]) && ms_finite(vc[1]) && ms_finite(vc[2])) {
 #pragma unroll
             for (int c = 0; c < 3; c++) {
                 const size_t k = 3 * (size_t)i + c;
-                float pp = f_dc[k], mm_ = Mom<BF>::ld(m3, k), vv = Mom<BF>::ld(v3, k);
-                adam_el(pp, mm_, vv, SH_C0 * vc[c], hp.step_size[3], b1, b2, hp.bc2_sqrt[3], eps);
-                f_dc[k] = pp;
-                Mom<BF>::st(m3, k, mm_);
-                Mom<BF>::st(v3, k, vv);
-                if (BF) dc_shadow[k] = ms_bf16_bits(pp);
+                adam_el(dc[c], dm[c], dv[c], SH_C0 * vc[c], hp.step_size[3], b1, b2, hp.bc2_sqrt[3], eps);
+                f_dc[k] = dc[c];
+                Mom<BF>::st(m3, k, dm[c]);
+                Mom<BF>::st(v3, k, dv[c]);
+                if (BF) {
+                    dc_shadow[k] = ms_bf16_bits(dc[c]);
+                    dv[c] = ms_bf16_round(dv[c]);  // the value the store holds
+                }
             }
         } else {
             atomicAdd(&nanc[3], 1u);
         }
     }
-    float vo = G.v_opac[i];
     if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
-        const float vdc = Mom<BF>::ld(v3, 3 * (size_t)i) + Mom<BF>::ld(v3, 3 * (size_t)i + 1) +
-                          Mom<BF>::ld(v3, 3 * (size_t)i + 2);
+        const float vdc = dv[0] + dv[1] + dv[2];  // second moments of the colour after this step
         reg_grads(N, ts.o, ls, vdc, hp, vo, vs);
     }
     if (dbg_vm) {
