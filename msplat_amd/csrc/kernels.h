@@ -129,15 +129,11 @@ size_t lists_hist_bytes(int N, int T);
 bool lists_supported(int N, int T);
 void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L);
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);
-// Scatter of every tile hit into its tile's bucket, per-segment sort and id packing.
+// Writes every tile hit's Gaussian id into its tile's bucket, in depth-rank order.
 void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L);
 // Waves per bucket workgroup: 0 = 2 where the tile count's LDS allows, else 1; 1, 2 or 4 on
 // request where the LDS allows it (process-wide; tests use it to cover every variant).
 void set_bucket_waves(int waves);
-// Stable LSD radix sort of n (key, value) pairs over the low 8 * passes key bits; with an
-// even number of passes the result is back in (ka, va). temp: lists_temp_bytes(n).
-void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, void* temp,
-                      int passes);
 // Fills the persistent raster work queue (ordered by decreasing tile list length) and
 // resets its heads; the views' lists must be built.
 void raster_work_build(hipStream_t s, const RasterWork& w);

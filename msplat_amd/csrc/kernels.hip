@@ -458,13 +458,6 @@ Geom geom_of(int W, int H) {
 //     order (per wave, after a per-wave count), so each segment is written sorted. A tile's
 //     bucket is its chunks' segments in chunk order: sorted by rank, with no sort pass.
 // The resulting per-tile id lists are the reference's sorted lists exactly.
-__global__ void k_depth_keys(int N, const int32_t* __restrict__ radii, const float* __restrict__ depths,
-                             uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
-    keys[i] = radii[i] > 0 ? __builtin_bit_cast(uint32_t, depths[i]) : 0xFFFFFFFFu;
-    vals[i] = (uint32_t)i;
-}
 
 __device__ __forceinline__ uint32_t pad_count(uint32_t c) {  // metal:2248-2256
     if (c == 0u) return 0u;
@@ -804,8 +797,19 @@ constexpr int RS_THREADS = 256, RS_ITEMS = 4, RS_TILE = RS_THREADS * RS_ITEMS;
 
 static int rs_blocks(int N) { return (N + RS_TILE - 1) / RS_TILE; }
 
+// Sort key of Gaussian i: its depth's bits (positive floats order as their bits), invisible
+// Gaussians last.
+__device__ __forceinline__ uint32_t depth_key(const int32_t* __restrict__ radii, const float* __restrict__ depths,
+                                              int i) {
+    return radii[i] > 0 ? __builtin_bit_cast(uint32_t, depths[i]) : 0xFFFFFFFFu;
+}
+
+// FIRST: the keys are made from (radii, depths) instead of read from `keys`.
+template <bool FIRST>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(int N, const uint32_t* __restrict__ keys, int shift,
-                                                        uint32_t* __restrict__ counts, int nblk) {
+                                                        uint32_t* __restrict__ counts, int nblk,
+                                                        const int32_t* __restrict__ radii,
+                                                        const float* __restrict__ depths) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0u;
     __syncthreads();
@@ -813,7 +817,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_hist(int N, const uint32_t* _
 #pragma unroll
     for (int k = 0; k < RS_ITEMS; k++) {
         const int i = base + k * RS_THREADS;
-        if (i < N) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        if (i < N) atomicAdd(&h[((FIRST ? depth_key(radii, depths, i) : keys[i]) >> shift) & 255u], 1u);
     }
     __syncthreads();
     counts[(size_t)threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
@@ -839,22 +843,37 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_rowscan(uint32_t* __restrict_
     if (threadIdx.x == 0) rowtot[blockIdx.x] = tot;
 }
 
+// FIRST: keys made from (radii, depths), values the Gaussian indices. LAST: the keys are not
+// written; instead each Gaussian's tile rectangle is gathered to its rank (srect).
+template <bool FIRST, bool LAST>
 __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t* __restrict__ kin,
                                                            const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
                                                            uint32_t* __restrict__ vout, int shift,
                                                            const uint32_t* __restrict__ counts,
-                                                           const uint32_t* __restrict__ rowtot, int nblk) {
+                                                           const uint32_t* __restrict__ rowtot, int nblk,
+                                                           const int32_t* __restrict__ radii,
+                                                           const float* __restrict__ depths,
+                                                           const ushort4* __restrict__ rect,
+                                                           ushort4* __restrict__ srect) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wcnt[RS_THREADS / 64][256];
     __shared__ uint32_t lw[RS_THREADS / 64 + 1];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     // this thread's keys and values first: their loads overlap each other and the scans
     uint32_t kr[RS_ITEMS], vr[RS_ITEMS];
+    ushort4 rq[RS_ITEMS];
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
         const int i = blockIdx.x * RS_TILE + r * RS_THREADS + threadIdx.x;
-        kr[r] = i < N ? kin[i] : 0u;
-        vr[r] = i < N ? vin[i] : 0u;
+        kr[r] = i < N ? (FIRST ? depth_key(radii, depths, i) : kin[i]) : 0u;
+        vr[r] = i < N ? (FIRST ? (uint32_t)i : vin[i]) : 0u;
+    }
+    if (LAST) {
+#pragma unroll
+        for (int r = 0; r < RS_ITEMS; r++) {
+            const int i = blockIdx.x * RS_TILE + r * RS_THREADS + threadIdx.x;
+            rq[r] = i < N ? rect[vr[r]] : make_ushort4(0, 0, 0, 0);
+        }
     }
     uint32_t tot;
     const uint32_t dbase = block_exclusive_scan(rowtot[threadIdx.x], lw, &tot);
@@ -881,7 +900,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
         if (valid) {
             uint32_t pos = run[d] + rank;
             for (int w = 0; w < wid; w++) pos += wcnt[w][d];
-            kout[pos] = key;
+            if (LAST) srect[pos] = rq[r];
+            else kout[pos] = key;
             vout[pos] = vr[r];
         }
         __syncthreads();
@@ -915,38 +935,38 @@ bool lists_supported(int N, int T) {
     return (size_t)T * 4 <= lds_limit() && bucket_lds_bytes(T, 1) <= lds_limit();
 }
 
-void radix_sort_pairs(hipStream_t s, int n, uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, void* temp,
-                      int passes) {
-    if (n <= 0) return;
-    const int nblk = rs_blocks(n);
-    uint32_t* counts = reinterpret_cast<uint32_t*>(temp);
-    uint32_t* rowtot = counts + (size_t)256 * nblk;
-    for (int pass = 0; pass < passes; pass++) {
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_rs_hist, dim3(nblk), dim3(RS_THREADS), 0, s, n, ka, shift, counts, nblk);
-        hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(RS_THREADS), 0, s, counts, nblk, rowtot);
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nblk), dim3(RS_THREADS), 0, s, n, ka, va, kb, vb, shift, counts, rowtot,
-                           nblk);
-        std::swap(ka, kb);
-        std::swap(va, vb);
-    }
-}
-
-// (the projection leaves the rectangle of every Gaussian with radius 0 empty)
-__global__ void k_gather_rect(int N, const uint32_t* __restrict__ sorted_ids, const ushort4* __restrict__ rect,
-                              ushort4* __restrict__ srect) {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= N) return;
-    srect[r] = rect[sorted_ids[r]];
-}
-
+// Depth ranks of all N Gaussians: four 8-bit passes ping-pong between (dkeys, dvals) and
+// (dkeys_out, sorted_ids); the first makes its keys from the projection, the last writes the
+// sorted ids and the rectangles in rank order (srect) and drops the keys.
 void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L) {
     if (N <= 0) return;
-    // keys start in (dkeys_out, sorted_ids); four passes ping-pong and end there
-    hipLaunchKernelGGL(k_depth_keys, dim3(cdiv(N, 256)), dim3(256), 0, s, N, pj.radii, pj.depths, L.dkeys_out,
-                       L.sorted_ids);
-    radix_sort_pairs(s, N, L.dkeys_out, L.sorted_ids, L.dkeys, L.dvals, L.temp, 4);
-    hipLaunchKernelGGL(k_gather_rect, dim3(cdiv(N, 256)), dim3(256), 0, s, N, L.sorted_ids, pj.rect, L.srect);
+    const int nblk = rs_blocks(N);
+    uint32_t* counts = reinterpret_cast<uint32_t*>(L.temp);
+    uint32_t* rowtot = counts + (size_t)256 * nblk;
+    uint32_t *ka = nullptr, *va = nullptr, *kb = L.dkeys, *vb = L.dvals;
+    for (int pass = 0; pass < 4; pass++) {
+        const int shift = 8 * pass;
+        if (pass == 0)
+            hipLaunchKernelGGL(k_rs_hist<true>, dim3(nblk), dim3(RS_THREADS), 0, s, N, ka, shift, counts, nblk,
+                               pj.radii, pj.depths);
+        else
+            hipLaunchKernelGGL(k_rs_hist<false>, dim3(nblk), dim3(RS_THREADS), 0, s, N, ka, shift, counts, nblk,
+                               pj.radii, pj.depths);
+        hipLaunchKernelGGL(k_rs_rowscan, dim3(256), dim3(RS_THREADS), 0, s, counts, nblk, rowtot);
+        if (pass == 0)
+            hipLaunchKernelGGL((k_rs_scatter<true, false>), dim3(nblk), dim3(RS_THREADS), 0, s, N, ka, va, kb, vb,
+                               shift, counts, rowtot, nblk, pj.radii, pj.depths, pj.rect, L.srect);
+        else if (pass < 3)
+            hipLaunchKernelGGL((k_rs_scatter<false, false>), dim3(nblk), dim3(RS_THREADS), 0, s, N, ka, va, kb, vb,
+                               shift, counts, rowtot, nblk, pj.radii, pj.depths, pj.rect, L.srect);
+        else
+            hipLaunchKernelGGL((k_rs_scatter<false, true>), dim3(nblk), dim3(RS_THREADS), 0, s, N, ka, va, kb, vb,
+                               shift, counts, rowtot, nblk, pj.radii, pj.depths, pj.rect, L.srect);
+        ka = kb;
+        va = vb;
+        kb = (pass % 2 == 0) ? L.dkeys_out : L.dkeys;
+        vb = (pass % 2 == 0) ? L.sorted_ids : L.dvals;
+    }
 }
 
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L) {
