@@ -875,9 +875,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
             rq[r] = i < N ? rect[vr[r]] : make_ushort4(0, 0, 0, 0);
         }
     }
+    // Items are ranked into an LDS copy of the workgroup's tile, ordered by digit, and written
+    // out from there, so each digit's run of items goes to consecutive addresses.
+    __shared__ uint32_t sk[RS_TILE], sv[RS_TILE];
+    __shared__ ushort4 sq[LAST ? RS_TILE : 1];
+    __shared__ uint32_t gbase[256], lbase[256];
     uint32_t tot;
     const uint32_t dbase = block_exclusive_scan(rowtot[threadIdx.x], lw, &tot);
-    run[threadIdx.x] = dbase + counts[(size_t)threadIdx.x * nblk + blockIdx.x];
+    const uint32_t c0 = counts[(size_t)threadIdx.x * nblk + blockIdx.x];
+    const uint32_t c1 = (int)blockIdx.x + 1 < nblk ? counts[(size_t)threadIdx.x * nblk + blockIdx.x + 1]
+                                                     : rowtot[threadIdx.x];
+    run[threadIdx.x] = dbase + c0;
+    gbase[threadIdx.x] = dbase + c0;  // first output slot of this workgroup's digit-d items
+    lbase[threadIdx.x] = block_exclusive_scan(c1 - c0, lw, &tot);  // and their first LDS slot
     const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
     for (int r = 0; r < RS_ITEMS; r++) {
@@ -900,9 +910,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
         if (valid) {
             uint32_t pos = run[d] + rank;
             for (int w = 0; w < wid; w++) pos += wcnt[w][d];
-            if (LAST) srect[pos] = rq[r];
-            else kout[pos] = key;
-            vout[pos] = vr[r];
+            const uint32_t loc = pos - gbase[d] + lbase[d];
+            sk[loc] = key;
+            sv[loc] = vr[r];
+            if (LAST) sq[loc] = rq[r];
         }
         __syncthreads();
         uint32_t add = 0u;
@@ -910,6 +921,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
         for (int w = 0; w < RS_THREADS / 64; w++) add += wcnt[w][threadIdx.x];
         run[threadIdx.x] += add;
         __syncthreads();
+    }
+    const int items = min(RS_TILE, N - (int)blockIdx.x * RS_TILE);
+    for (int j = threadIdx.x; j < items; j += RS_THREADS) {
+        const uint32_t key = sk[j];
+        const uint32_t d = (key >> shift) & 255u;
+        const uint32_t pos = gbase[d] + ((uint32_t)j - lbase[d]);
+        if (LAST) srect[pos] = sq[j];
+        else kout[pos] = key;
+        vout[pos] = sv[j];
     }
 }
 
