@@ -793,7 +793,7 @@ static int hist_blocks(int N) {
 // digit histograms, one scan of the digit-major (digit, workgroup) count matrix, and a
 // scatter that ranks each item among equal digits of earlier items with wave ballots. Only
 // kernels: no memset or host synchronisation, so a step that uses it can be captured.
-constexpr int RS_THREADS = 256, RS_ITEMS = 4, RS_TILE = RS_THREADS * RS_ITEMS;
+constexpr int RS_THREADS = 256, RS_ITEMS = 8, RS_TILE = RS_THREADS * RS_ITEMS;
 
 static int rs_blocks(int N) { return (N + RS_TILE - 1) / RS_TILE; }
 
