@@ -922,7 +922,12 @@ template <class T> inline void sgld_noise3(uint32_t seed, uint32_t idx, T n[3]) 
     n[2] = r23 * c23;
 }
 
-// sgld_noise_kernel, metal:3892-3967
+// sgld_noise_kernel, metal:3892-3967: the mean moves by R diag(exp(2 s)) R^T (noise_lr *
+// xyz_lr * w(o) * n), w = sigmoid(-100 ((1 - sigmoid(o)) - 0.995)), capped at length max_disp.
+// Pins (tests/test_oracle_pins.py): identity rotation and isotropic scale give s^2 * ns * n;
+// a 90-degree turn about z maps the local y scale onto world x; w = 1/2 at sigmoid(o) = 0.005;
+// a capped move has length max_disp along the uncapped direction; rotating the frame by Q
+// (q -> q_Q q, n -> Q n) rotates the move by Q.
 template <class T>
 inline void sgld_apply(T mean[3], const T ls[3], const T q[4], T opac_logit, const T n[3], T noise_lr, T xyz_lr, T max_disp) {
     T op_sig = T(1) / (T(1) + t_exp<T>(-opac_logit));
@@ -949,6 +954,25 @@ inline void sgld_apply(T mean[3], const T ls[3], const T q[4], T opac_logit, con
         v0 *= inv; v1 *= inv; v2 *= inv;
     }
     mean[0] += v0; mean[1] += v1; mean[2] += v2;
+}
+
+// mcmc_reg_grad_kernel, metal:3987-4026: gradients of the regulariser
+//   duty * (op_reg * mean_j sigmoid(o_j) + sc_reg * mean_{j,k} exp(clamp(s_jk, -15, 15)))
+// added to Gaussian i's opacity-logit and log-scale gradients (the mean over the N active
+// rows, and over 3N scale entries). duty is 1 for a row whose DC second moments sum to 0
+// (never visible yet) and reg_invis_scale otherwise. The clamp's own derivative is not
+// applied (exp of the clamped value, as the reference writes it).
+// Pin: FP64 central differences of that objective (tests/test_oracle_pins.py).
+template <class T>
+inline void reg_grad(int N, T opac_logit, const T ls[3], const T dc_sq[3], T op_reg, T sc_reg, T reg_invis_scale,
+                     T& v_opac, T v_scale[3]) {
+    T vdc = dc_sq[0] + dc_sq[1] + dc_sq[2];
+    T duty = (vdc == T(0)) ? T(1) : reg_invis_scale;
+    T invN = duty / T((float)N);
+    T sig = T(1) / (T(1) + t_exp<T>(-opac_logit));
+    v_opac += op_reg * sig * (T(1) - sig) * invN;
+    T inv3N = invN * T(1.0f / 3.0f);
+    for (int k = 0; k < 3; k++) v_scale[k] += sc_reg * t_exp<T>(t_clamp(ls[k], T(-15), T(15))) * inv3N;
 }
 
 // ---------------------------------------------------------------------------------------
@@ -1048,16 +1072,9 @@ void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const M
         }
         T vo = O.rg.v_opac[i];
         // mcmc_reg_grad_kernel, metal:3987-4026 (dispatched when op_reg or sc_reg > 0)
-        if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
-            const T* dsq = S.v[3] + 3 * (size_t)i;
-            T vdc = dsq[0] + dsq[1] + dsq[2];
-            T duty = (vdc == T(0)) ? T(1) : T(hp.reg_invis_scale);
-            T invN = duty / T((float)N);
-            T sig = T(1) / (T(1) + t_exp<T>(-S.opac[i]));
-            vo += T(hp.op_reg) * sig * (T(1) - sig) * invN;
-            T inv3N = invN * T(1.0f / 3.0f);
-            for (int k = 0; k < 3; k++) vs[k] += T(hp.sc_reg) * t_exp<T>(t_clamp(ls[k], T(-15), T(15))) * inv3N;
-        }
+        if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f)
+            reg_grad<T>(N, S.opac[i], ls, S.v[3] + 3 * (size_t)i, T(hp.op_reg), T(hp.sc_reg), T(hp.reg_invis_scale),
+                        vo, vs);
         // gradients as fed to Adam (photometric + regulariser)
         for (int k = 0; k < 3; k++) { O.v_mean[3 * (size_t)i + k] = vm[k]; O.v_scale[3 * (size_t)i + k] = vs[k]; }
         for (int k = 0; k < 4; k++) O.v_quat[4 * (size_t)i + k] = vq[k];
@@ -1174,16 +1191,9 @@ void train_step_batch(State<T>& S, const MsplatCamera* cams, const uint8_t* cons
         }
         T gvo = vo[i];
         T gvs[3] = {vs[3 * (size_t)i], vs[3 * (size_t)i + 1], vs[3 * (size_t)i + 2]};
-        if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f) {
-            const T* dsq = S.v[3] + 3 * (size_t)i;
-            T vdcs = dsq[0] + dsq[1] + dsq[2];
-            T duty = (vdcs == T(0)) ? T(1) : T(hp.reg_invis_scale);
-            T invN = duty / T((float)N);
-            T sig = T(1) / (T(1) + t_exp<T>(-S.opac[i]));
-            gvo += T(hp.op_reg) * sig * (T(1) - sig) * invN;
-            T inv3N = invN * T(1.0f / 3.0f);
-            for (int k = 0; k < 3; k++) gvs[k] += T(hp.sc_reg) * t_exp<T>(t_clamp(ls[k], T(-15), T(15))) * inv3N;
-        }
+        if (hp.op_reg > 0.0f || hp.sc_reg > 0.0f)
+            reg_grad<T>(N, S.opac[i], ls, S.v[3] + 3 * (size_t)i, T(hp.op_reg), T(hp.sc_reg), T(hp.reg_invis_scale),
+                        gvo, gvs);
         auto adam_g = [&](int grp, T& pp, T& mm, T& vv, T g) {
             if (!t_finite(g)) { nan_cnt[grp]++; return; }
             adam<T>(pp, mm, vv, g, T(hp.step_size[grp]), b1, b2, T(hp.bc2_sqrt[grp]), eps);

@@ -136,6 +136,24 @@ void oracle_sgld_noise_f32(int N, uint32_t seed, float* out) {
 }
 uint32_t oracle_pcg_hash32(uint32_t seed, uint32_t idx) { return pcg_hash32(seed, idx); }
 
+// One Gaussian's SGLD move (sgld_apply): mean[3] updated in place.
+void oracle_sgld_apply_f64(double* mean, const double* ls, const double* q, double opac_logit, const double* n,
+                           double noise_lr, double xyz_lr, double max_disp) {
+    sgld_apply<double>(mean, ls, q, opac_logit, n, noise_lr, xyz_lr, max_disp);
+}
+void oracle_sgld_apply_f32(float* mean, const float* ls, const float* q, float opac_logit, const float* n,
+                           float noise_lr, float xyz_lr, float max_disp) {
+    sgld_apply<float>(mean, ls, q, opac_logit, n, noise_lr, xyz_lr, max_disp);
+}
+
+// Regulariser gradients of N rows (reg_grad): v_opac[N] and v_scale[N 3] are added to.
+void oracle_reg_grad_f64(int N, const double* opac, const double* ls, const double* dc_sq, double op_reg,
+                         double sc_reg, double reg_invis_scale, double* v_opac, double* v_scale) {
+    for (int i = 0; i < N; i++)
+        reg_grad<double>(N, opac[i], ls + 3 * (size_t)i, dc_sq + 3 * (size_t)i, op_reg, sc_reg, reg_invis_scale,
+                         v_opac[i], v_scale + 3 * (size_t)i);
+}
+
 float oracle_ms_expf(float x) { return ms_expf(x); }
 float oracle_ms_logf(float x) { return ms_logf(x); }
 void oracle_ms_sincosf(float a, float* s, float* c) { ms_sincosf(a, s, c); }
