@@ -56,6 +56,11 @@ int msplat_dev_copy_kernel(void* dst, const void* src, size_t bytes);
  * are the same for every choice, so tests use it to cover each variant. No counterpart in
  * the reference (its list build has no such variants). Returns -1 for any other value. */
 int msplat_set_list_bucket_waves(int waves);
+/* The waves per workgroup the bucket kernel launches with for `tiles` tiles under a device
+ * LDS limit of `lds_limit` bytes per workgroup (request as above); 0 when even one wave's
+ * LDS does not fit (such a view is rejected). Host-only arithmetic (no device is touched);
+ * no counterpart in the reference. */
+int msplat_list_bucket_waves_for(int tiles, int64_t lds_limit, int request);
 
 /* Creates an engine bound to the current HIP device (buffers grow on demand and are
  * reused across steps). Replaces the reference's global MetalContext + FusedTensorCache

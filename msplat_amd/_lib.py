@@ -102,6 +102,7 @@ def bind(path: str) -> C.CDLL:
         "msplat_dev_memset": ([vp, C.c_int, C.c_size_t], C.c_int),
         "msplat_dev_copy_kernel": ([vp, vp, C.c_size_t], C.c_int),
         "msplat_set_list_bucket_waves": ([C.c_int], C.c_int),
+        "msplat_list_bucket_waves_for": ([C.c_int, i64, C.c_int], C.c_int),
         "msplat_engine_create": ([], vp),
         "msplat_engine_destroy": ([vp], None),
         "msplat_engine_train_step": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), vp,

@@ -263,10 +263,14 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
         k.slot_geom.push_back(hwv_[j].W);
         k.slot_geom.push_back(hwv_[j].H);
     }
-    int64_t gens = 0;
-    // per slot: buffer generation and depth-chunk count (chunked views record their own path)
-    for (int v = 0; v < B; v++) gens = gens * 1000003 + slots_[v]->generation() * 4096 + slots_[v]->chunks();
-    k.gens = gens;
+    // per slot: buffer generation and depth-chunk count (chunked views record their own path),
+    // compared entry by entry: a recorded graph is replayed only against the very buffers
+    // it was recorded with
+    k.gens.clear();
+    for (int v = 0; v < B; v++) {
+        k.gens.push_back(slots_[v]->generation());
+        k.gens.push_back(slots_[v]->chunks());
+    }
     const DBuf* own[3] = {&block_, &wq_, &wmisc_};
     for (int j = 0; j < 3; j++) k.own[j] = own[j]->p;
     const void* ptrs[8] = {st.means, st.scales, st.quats, st.f_dc, st.f_rest, st.opac, st.f_dc_bf16, st.f_rest_bf16};

@@ -66,7 +66,7 @@ private:
         int N = -1, B = 0, du = -1, degree = -1, bf16 = 0, grid = 0, ppl = 1;
         msk::Geom g{};
         std::vector<int> slot_geom;   // recorded (W, H) per slot
-        int64_t gens = 0;
+        std::vector<int64_t> gens;    // (buffer generation, depth-chunk count) per slot
         const void* state[8] = {};
         const void* mom[12] = {};
         const void* own[3] = {};

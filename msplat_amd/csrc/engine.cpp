@@ -572,6 +572,11 @@ int msplat_set_list_bucket_waves(int waves) {
     return 0;
 }
 
+int msplat_list_bucket_waves_for(int tiles, int64_t lds_limit, int request) {
+    if (tiles <= 0 || lds_limit <= 0) return 0;
+    return msk::bucket_waves(tiles, (size_t)lds_limit, request);
+}
+
 int msplat_dev_copy_kernel(void* dst, const void* src, size_t bytes) {
     if (!dst || !src) return -1;
     msk::copy16(nullptr, src, dst, bytes);

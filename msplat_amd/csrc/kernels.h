@@ -134,6 +134,9 @@ void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, Lis
 // Waves per bucket workgroup: 0 = 2 where the tile count's LDS allows, else 1; 1, 2 or 4 on
 // request where the LDS allows it (process-wide; tests use it to cover every variant).
 void set_bucket_waves(int waves);
+// Waves per workgroup of the bucket kernel for T tiles under an LDS limit of `limit` bytes
+// (request: 0 by LDS, 1, 2 or 4); 0 when the tile count does not fit.
+int bucket_waves(int T, size_t limit, int request);
 // Fills the persistent raster work queue (ordered by decreasing tile list length) and
 // resets its heads; the views' lists must be built.
 void raster_work_build(hipStream_t s, const RasterWork& w);

@@ -11,6 +11,8 @@
 //
 // Reference kernels are cited as metal:<line> (core/metal/msplat_metal.metal).
 #include <atomic>
+#include <stdexcept>
+#include <string>
 #include <utility>
 
 #include "kernels.h"
@@ -596,19 +598,20 @@ __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ 
 // lanes on one cursor); a wave's writes to a tile are thus in rank order and the segment is
 // sorted as written. W = 4, 2 or 1 by the LDS the tile count leaves (12, 8 or 4 bytes per
 // tile).
-// 2 waves when the LDS allows (measured at 1.5M Gaussians, 1600x1040: 4 waves 0.336 ms,
-// 2 waves 0.319 ms, 1 wave 0.464 ms per launch), else 1; 4 only on request.
-static int bucket_waves(int T, size_t limit, int request) {
-    const size_t t4 = ((size_t)T + 3) & ~(size_t)3;
-    if (request == 4 && t4 * 12 <= limit) return 4;
-    if (request != 1 && t4 * 8 <= limit) return 2;
-    return 1;
-}
-
 static size_t bucket_lds_bytes(int T, int waves) {
     const size_t t4 = ((size_t)T + 3) & ~(size_t)3;
     // + one 64-word line per wave that idle lanes point their atomics at
     return t4 * 4 * (size_t)(waves == 1 ? 1 : 1 + waves / 2) + (size_t)waves * 64 * 4;
+}
+
+// 2 waves when the LDS allows (measured at 1.5M Gaussians, 1600x1040: 4 waves 0.336 ms,
+// 2 waves 0.319 ms, 1 wave 0.464 ms per launch), else 1; 4 only on request. The choice is
+// by the kernel's whole dynamic LDS (cursors plus the idle-lane lines), so a launch never
+// asks for more than `limit`; 0 when even one wave does not fit.
+int bucket_waves(int T, size_t limit, int request) {
+    if (request == 4 && bucket_lds_bytes(T, 4) <= limit) return 4;
+    if (request != 1 && bucket_lds_bytes(T, 2) <= limit) return 2;
+    return bucket_lds_bytes(T, 1) <= limit ? 1 : 0;
 }
 
 template <int WAVES>
@@ -989,12 +992,25 @@ void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L) {
     }
 }
 
+// A launch that fails leaves the lists unwritten; the raster must not be queued behind it.
+static void check_launch(const char* what) {
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static void allow_lds(const void* fn, size_t lds, const char* what) {
+    if (lds <= 65536) return;
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N > 0) {
         const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
         const size_t lds = (size_t)g.T * 4;
-        if (lds > 65536) (void)hipFuncSetAttribute((const void*)k_tile_hist, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        allow_lds((const void*)k_tile_hist, lds, "tile histogram LDS");
         hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.srect, vc, L.hist);
+        check_launch("tile histogram launch");
         hipLaunchKernelGGL(k_col_scan, dim3(cdiv(g.T, 64)), dim3(64 * COLSCAN_PARTS), 0, s, vc, nblk, L.hist,
                            L.tile_counts);
     } else {
@@ -1007,10 +1023,10 @@ void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, 
 template <int WAVES>
 static void launch_bucket(hipStream_t s, int nblk, int N, int G, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     const size_t lds = bucket_lds_bytes(g.T, WAVES);
-    if (lds > 65536)
-        (void)hipFuncSetAttribute((const void*)k_tile_bucket<WAVES>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    allow_lds((const void*)k_tile_bucket<WAVES>, lds, "tile bucket LDS");
     hipLaunchKernelGGL(k_tile_bucket<WAVES>, dim3(nblk), dim3(64 * WAVES), lds, s, N, G, L.sorted_ids, L.srect, vc,
                        L.hist, L.exact_off, L.tile_counts, L.totals, L.ids, (uint32_t)L.capacity);
+    check_launch("tile bucket launch");
 }
 
 static std::atomic<int> g_bucket_waves{0};
@@ -1022,7 +1038,8 @@ void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, Lis
     switch (bucket_waves(g.T, lds_limit(), g_bucket_waves.load())) {
         case 4: launch_bucket<4>(s, nblk, N, G, vc, g, L); break;
         case 2: launch_bucket<2>(s, nblk, N, G, vc, g, L); break;
-        default: launch_bucket<1>(s, nblk, N, G, vc, g, L); break;
+        case 1: launch_bucket<1>(s, nblk, N, G, vc, g, L); break;
+        default: throw std::runtime_error("tile count exceeds the bucket kernel's LDS");
     }
 }
 

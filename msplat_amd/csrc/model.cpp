@@ -418,9 +418,16 @@ void Model::save_checkpoint(const std::string& path, int step) {
 int Model::load_checkpoint(const std::string& path) {
     CheckpointData c = read_checkpoint(path);
     if ((int64_t)c.num > cap_) throw std::runtime_error("checkpoint holds more Gaussians than cap_max");
+    // Every degree-dependent buffer (parameters, moments, batch sums, the SH schedule) is
+    // sized for the trainer's configured degree, so a checkpoint of another degree is
+    // rejected before any state changes (the reference ignores the header's degree,
+    // model.cpp:779-783, and would read the rest coefficients with the wrong stride).
+    if (c.g.degree != opt_.sh_degree || (int)c.sh_degree != c.g.degree)
+        throw std::runtime_error("checkpoint SH degree " + std::to_string(c.g.degree) + " (header " +
+                                 std::to_string(c.sh_degree) + ") differs from the trainer's " +
+                                 std::to_string(opt_.sh_degree));
     // moments of rows beyond the checkpoint's count start from zero (model.cpp:801-812)
     hipStream_t s = eng_.stream();
-    if (c.g.degree != degree_) allocate(c.g.degree);
     for (int k = 0; k < 6; k++) {
         const size_t rows = (size_t)cap_ * std::max(row_len(k, degree_), 1);
         msk::fill_zero(s, p_[k].p, rows * 4);

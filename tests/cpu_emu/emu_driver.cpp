@@ -194,6 +194,19 @@ static void scenario_mini(const char* dir) {
     const std::string ck = std::string(dir) + "/mini.msplat", ply = std::string(dir) + "/mini.ply";
     msplat_trainer_save_checkpoint(t, ck.c_str());
     CHECK(msplat_trainer_load_checkpoint(t, ck.c_str()) == 5);
+    {   // a checkpoint of another SH degree is rejected and leaves the trainer as it was
+        MsplatConfig c1 = cfg;
+        c1.shDegree = 1;
+        MsplatTrainer t1 = msplat_trainer_create(ds, c1);
+        CHECK(t1);
+        CHECK(msplat_trainer_step(t1).iteration == 1);
+        const std::vector<float> before = group(t1, 4, 0);
+        CHECK(msplat_trainer_load_checkpoint(t1, ck.c_str()) == -1);
+        CHECK(msplat_trainer_iteration(t1) == 1 && msplat_trainer_splat_count(t1) == 400);
+        CHECK(group(t1, 4, 0) == before);
+        CHECK(msplat_trainer_step(t1).iteration == 2);
+        msplat_trainer_destroy(t1);
+    }
     msplat_trainer_export_ply(t, ply.c_str());
     float pose[16];
     msplat_dataset_camera_pose(ds, 1, pose);

@@ -99,3 +99,31 @@ def test_dataset_errors_do_not_cross_the_boundary():
     assert not L.msplat_trainer_create(None, _lib.default_config())
     assert L.msplat_dataset_num_train(None) == 0
     assert L.msplat_trainer_iteration(None) == 0
+
+
+def test_bucket_wave_choice_fits_lds():
+    """The bucket kernel's launch never asks for more LDS than the device allows: the choice
+    counts its whole dynamic LDS (4 B per tile of segment starts, 4 B per tile per wave pair
+    of cursors, and one 256 B idle-lane line per wave). Band edges: 2560x2048 (20480 tiles)
+    on a 160 KiB part, 1920x1080 (8160 tiles) on a 64 KiB part."""
+    L = _lib.lib()
+
+    def lds(T, w):
+        t4 = (T + 3) & ~3
+        return t4 * 4 * (1 if w == 1 else 1 + w // 2) + w * 256
+    for limit in (65536, 163840):
+        for T in list(range(1, 60)) + list(range(8100, 8200)) + list(range(20400, 20500)) + [40000, 40960, 41000]:
+            for req in (0, 1, 2, 4):
+                w = L.msplat_list_bucket_waves_for(T, limit, req)
+                if w == 0:
+                    assert lds(T, 1) > limit
+                    continue
+                assert lds(T, w) <= limit, (T, limit, req, w)
+                if req == 0:  # two waves wherever they fit
+                    assert w == (2 if lds(T, 2) <= limit else 1)
+                if req == 1:
+                    assert w == 1
+    assert L.msplat_list_bucket_waves_for(20480, 163840, 0) == 1
+    assert L.msplat_list_bucket_waves_for(20352, 163840, 0) == 2
+    assert L.msplat_list_bucket_waves_for(8160, 65536, 0) == 1
+    assert L.msplat_list_bucket_waves_for(8160, 163840, 4) == 4
