@@ -48,7 +48,7 @@ def parse():
     p.add_argument("--cams", type=int, default=64)
     p.add_argument("--hidden", type=int, default=20_000)
     p.add_argument("--cap", type=int, default=3_000_000)
-    p.add_argument("--cpu-steps", type=int, default=2)
+    p.add_argument("--cpu-steps", type=int, default=3)
     p.add_argument("--stage-steps", type=int, default=10)
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="launch the plain copy kernel once on this many MiB (PMC traffic calibration)")
@@ -218,8 +218,20 @@ def main():
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle_py as O
         from msplat_amd import scene
-        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+        # the host cores this process may run on; OMP_NUM_THREADS, where the environment sets
+        # it, is the CPU share the machine gives this GPU's process and caps them
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+        omp = os.environ.get("OMP_NUM_THREADS", "")
+        threads = min(avail, int(omp)) if omp.isdigit() and int(omp) > 0 else avail
         threads = O.set_threads(threads)
+        cpu_model = ""
+        try:
+            for line in open("/proc/cpuinfo"):
+                if line.startswith("model name"):
+                    cpu_model = line.split(":", 1)[1].strip()
+                    break
+        except OSError:
+            pass
         groups = ("means", "scales", "quats", "f_dc", "f_rest", "opac")
         shapes = {"means": (N, 3), "scales": (N, 3), "quats": (N, 4), "f_dc": (N, 3), "f_rest": (N, 15, 3),
                   "opac": (N,)}
@@ -242,7 +254,9 @@ def main():
         med = statistics.median(times)
         cpu = {"value": round(1.0 / med, 4), "unit": "it/s", "cores": threads, "kind": "port",
                "sample": f"{a.cpu_steps} oracle training steps (FP32, OpenMP, {threads} threads) at N={N}, "
-                         f"{W}x{H}, SH degree 3, on the GPU trainer's state; median step time"}
+                         f"{W}x{H}, SH degree 3, on the GPU trainer's state; median step time",
+               "step_seconds": [round(t, 3) for t in times], "nproc": os.cpu_count(), "affinity_cpus": avail,
+               "omp_num_threads": omp or None, "cpu_model": cpu_model}
 
     out = {
         "metric": METRIC,
@@ -259,7 +273,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "single-view 3DGS training step (C1: 64-component clustered mixture)",
                    "gaussians": N, "width": W, "height": H, "fx": a.fx, "cameras": ds.num_train,
-                   "hidden_gt_gaussians": a.hidden, "sh_degree": 3, "loss": "0.8 L1 + 0.2 (1 - SSIM)",
+                   "hidden_gt_gaussians": a.hidden, "sh_degree": 3, "sh_degree_interval": 1,
+                   "loss": "0.8 L1 + 0.2 (1 - SSIM)",
                    "cap_max": a.cap, "intersections": I, "visible": V, "raster_pairs": pairs,
                    "tile_entries_walked_max": tw_max, "tile_entries_walked_total": tw_total,
                    "parallelism": "single GPU"},

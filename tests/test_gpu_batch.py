@@ -9,7 +9,7 @@ import pytest
 
 import oracle_py as O
 from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, scene
-from parity_util import assert_rel, small_scene
+from parity_util import assert_rel, compare_state, small_scene
 from test_gpu_trainer import GROUPS, _host_state, _state
 
 pytestmark = pytest.mark.gpu
@@ -45,19 +45,18 @@ def _check_batched_steps(tr, cams, views, steps):
     for step in steps:
         before = _state(tr)
         host = _host_state(before, 3)
+        host64 = _host_state(before, 3, dtype=np.float64)
         pre = [tr.step_params(ci, step, step) for ci in range(len(cams))]
         s = tr.step_batch(views)
         assert s.iteration == step
         idx, losses = tr.last_batch(losses=True)
         assert len(idx) == views
         hps = _batch_hps(pre, cams, idx)
-        loss, _ = O.train_step_batch(host, [cams[i]["cam"] for i in idx], [cams[i]["image"] for i in idx], hps)
+        bc, bg = [cams[i]["cam"] for i in idx], [cams[i]["image"] for i in idx]
+        loss, _ = O.train_step_batch(host, bc, bg, hps)
+        O.train_step_batch(host64, bc, bg, hps)
         assert_rel(f"batch loss@{step}", np.mean(losses), loss, 1e-5)
-        after = _state(tr)
-        for k in GROUPS:
-            assert_rel(f"{k}@{step}", after[k], host.p[k][: host.N].reshape(-1), 1e-4)
-            assert_rel(f"m.{k}@{step}", after["m." + k], host.m[k][: host.N].reshape(-1), 1e-4)
-            assert_rel(f"v.{k}@{step}", after["v." + k], host.v[k][: host.N].reshape(-1), 1e-4)
+        compare_state(f"@{step}", _state(tr), host, host64)
 
 
 @pytest.mark.parametrize("views", [3, 5])

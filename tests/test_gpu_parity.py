@@ -1,15 +1,16 @@
 """GPU parity: one training step (and one render) of the HIP path against the CPU oracle,
 from an identical state. Integer outputs (tile rectangles, counts, offsets, sorted per-tile
 id lists, contributor indices) must match bit for bit; images and the loss within 1e-5
-relative; gradients and the optimiser state within 1e-4 relative (max-norm relative error
-per array: float atomics make the GPU summation order differ from the oracle's).
+relative; gradients and the optimiser state per element within 1e-4 relative plus an
+absolute floor of 4x the output's FP64 error (float atomics make the GPU summation order
+differ from the oracle's; tests/parity_util.py).
 """
 import numpy as np
 import pytest
 
 from msplat_amd import scene
 import oracle_py as O
-from parity_util import assert_exact, assert_rel, small_scene
+from parity_util import assert_close, assert_exact, assert_rel, compare_state, oracle_pair, small_scene
 
 pytestmark = pytest.mark.gpu
 
@@ -168,8 +169,7 @@ def test_train_step_parity(W, H):
         hp = scene.step_params(step, scene.LR_INIT, W, H, step, 3, 3)
         host = dst.to_host()
         mom = dst.moments_host()
-        ost = O.HostState(host, moments=mom)
-        orc = O.train_step(ost, cam, gt, hp)
+        ost, orc, o64s, o64 = oracle_pair(host, mom, cam, gt, hp)
         eng.train_step(dst, cam, gt_dev, hp, debug=True)
         eng.sync()
         _compare_forward(eng, orc, N, cam)
@@ -177,11 +177,11 @@ def test_train_step_parity(W, H):
         gr = eng.grads(N, W, H)
         assert_exact("v_rendered", gr["v_rendered"], orc["v_rendered"])
         for k in ("v_xy", "v_conic", "v_rgb", "v_mean", "v_scale", "v_quat", "v_opac"):
-            assert_rel(k, gr[k], orc[k], TOL_GRAD)
+            assert_close(f"{k}@{step}", gr[k], orc[k], o64[k], rtol=TOL_GRAD)
         after = dst.to_host()
         m_gpu, v_gpu = dst.moments_host()
+        got = {}
         for k in O.GROUP_SHAPES:
-            assert_rel("param " + k, after[k], ost.view()[k], TOL_GRAD)
-            assert_rel("exp_avg " + k, m_gpu[k], ost.m[k][:N], TOL_GRAD)
-            assert_rel("exp_avg_sq " + k, v_gpu[k], ost.v[k][:N], 2 * TOL_GRAD)
+            got[k], got["m." + k], got["v." + k] = after[k], m_gpu[k], v_gpu[k]
+        compare_state(f"@{step}", got, ost, o64s, rtol=TOL_GRAD)
         assert list(eng.nan_counts()[:6]) == [0] * 6

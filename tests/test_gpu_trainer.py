@@ -11,7 +11,7 @@ import pytest
 import oracle_py as O
 from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, scene
 from msplat_amd import _lib
-from parity_util import assert_exact, assert_rel, small_scene
+from parity_util import assert_exact, assert_rel, compare_state, small_scene
 from test_relocation_fixture import check_eq9
 
 pytestmark = pytest.mark.gpu
@@ -39,14 +39,14 @@ def _state(tr, n=None):
     return out
 
 
-def _host_state(st, degree, cap=None):
+def _host_state(st, degree, cap=None, dtype=np.float32):
     n = st["means"].size // 3
     nr = (degree + 1) ** 2 - 1
     shape = {"means": (n, 3), "scales": (n, 3), "quats": (n, 4), "f_dc": (n, 3), "f_rest": (n, nr, 3), "opac": (n,)}
     g = {k: st[k].reshape(shape[k]) for k in GROUPS}
     g["degree"] = degree
     mom = ({k: st["m." + k].reshape(shape[k]) for k in GROUPS}, {k: st["v." + k].reshape(shape[k]) for k in GROUPS})
-    return O.HostState(g, moments=mom, capacity=cap)
+    return O.HostState(g, moments=mom, capacity=cap, dtype=dtype)
 
 
 def test_trainer_steps_match_oracle():
@@ -63,14 +63,13 @@ def test_trainer_steps_match_oracle():
         assert s.iteration == step and s.splat_count == g["means"].shape[0]
         ci = tr.last_camera
         host = _host_state(before, 3)
+        host64 = _host_state(before, 3, dtype=np.float64)
         gt = train_cams[ci]["image"]
         orc = O.train_step(host, train_cams[ci]["cam"], gt, hps[ci], want_ids=False)
+        O.train_step(host64, train_cams[ci]["cam"], gt, hps[ci], want_ids=False)
         after = _state(tr)
         assert_rel(f"loss@{step}", tr.last_loss, orc["loss"], 1e-5)
-        for k in GROUPS:
-            assert_rel(f"{k}@{step}", after[k], host.p[k][: host.N].reshape(-1), 1e-4)
-            assert_rel(f"m.{k}@{step}", after["m." + k], host.m[k][: host.N].reshape(-1), 1e-4)
-            assert_rel(f"v.{k}@{step}", after["v." + k], host.v[k][: host.N].reshape(-1), 1e-4)
+        compare_state(f"@{step}", after, host, host64)
     n_adam, lr = tr.optim_state()
     assert n_adam == 4
     assert abs(lr[0] - scene.means_lr(4)) <= 1e-6 * scene.means_lr(4)
