@@ -81,6 +81,22 @@ int msplat_engine_train_step(MsplatEngine* e, const MsplatDeviceState* st, const
 int msplat_engine_render(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam,
                          int degrees_to_use, const float background[3]);
 
+/* msplat_engine_render for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). No counterpart in
+ * the reference. It allocates nothing: every intermediate buffer (projection records, depth
+ * rank, tile lists, raster work queue, depth-chunk buffers) lives in a 64 MiB array in
+ * device memory that the library holds on each GPU it runs on until the process ends. The
+ * caller passes the state and device outputs img [H,W,3] f32, final_T [H,W] f32 and
+ * final_idx [H,W] i32, identical to the engine's buffers of the same name after
+ * msplat_engine_render; the work is enqueued on `stream` (a hipStream_t, NULL: the null
+ * stream) and nothing waits for it. A render whose buffers exceed the array (roughly
+ * 120k Gaussians at 1920x1080 on the monolithic path; depth-chunked views of fewer than 400
+ * tiles need about 3 KB per Gaussian, so about 18k) is rejected with -3 and a message on
+ * stderr. Calls on one GPU share its array and overlapping calls on one GPU (two streams,
+ * two threads) race. Returns 0, -1 bad arguments, -2 HIP error, -3 too large. */
+int msplat_render_noalloc(const MsplatDeviceState* st, const MsplatCamera* cam, int degrees_to_use,
+                          const float background[3], float* img, float* final_T, int32_t* final_idx, void* stream);
+
 /* Device pointer and size of an internal buffer of the last step/render: "radii",
  * "tiles_hit", "depths", "rect", "aabb", "recA", "recB", "recC", "tile_counts",
  * "tile_bins", "pkA", "pkB", "pkC", "img", "final_T", "final_idx", "v_rendered", "v_xy",

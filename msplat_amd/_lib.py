@@ -108,6 +108,8 @@ def bind(path: str) -> C.CDLL:
         "msplat_engine_train_step": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), vp,
                                       C.POINTER(MsplatStepParams), C.c_int], C.c_int),
         "msplat_engine_render": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), C.c_int, fp], C.c_int),
+        "msplat_render_noalloc": ([C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), C.c_int, fp, vp, vp, vp,
+                                   vp], C.c_int),
         "msplat_engine_buffer": ([vp, C.c_char_p, C.POINTER(vp), C.POINTER(C.c_size_t)], C.c_int),
         "msplat_engine_sync": ([vp], C.c_int),
         "msplat_engine_read": ([vp, C.c_char_p, vp, C.c_size_t], i64),

@@ -23,6 +23,7 @@ SOURCES = [
     "mcmc.hip",
     "mlp.hip",
     "eval.hip",
+    "render_noalloc.hip",
     "engine.cpp",
     "batch.cpp",
     "proposal.cpp",

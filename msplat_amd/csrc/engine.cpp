@@ -147,7 +147,7 @@ Engine::~Engine() {
 }
 
 // K_max rule of the depth-chunked raster path (reference msplat_metal.mm:1204-1224).
-static uint32_t k_max_rule(int64_t capacity, int num_tiles) {
+uint32_t k_max_rule(int64_t capacity, int num_tiles) {
     if (num_tiles >= 400) return 1;
     uint32_t avg = (uint32_t)(capacity / (num_tiles > 1 ? num_tiles : 1));
     uint32_t cons = avg * 6u;

@@ -159,6 +159,9 @@ private:
 
 // Resident workgroups of a persistent raster launch by default: 8 per CU.
 int default_persist_grid();
+// Depth chunks per tile of a view with `num_tiles` tiles and a list capacity of `capacity`
+// slots (1: the monolithic raster path).
+uint32_t k_max_rule(int64_t capacity, int num_tiles);
 msk::Params params_of(const MsplatDeviceState& st);
 
 // Every live engine (msplat_sync / msplat_cleanup act on all of them).

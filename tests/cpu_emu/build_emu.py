@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OUT = os.environ.get("MSPLAT_EMU_OUT", os.path.join(HERE, "_build"))
 EXE = os.path.join(OUT, "emu_driver")
 SHLIB = os.path.join(OUT, "libmsplat_emu.so")
-SOURCES = ["kernels.hip", "mcmc.hip", "mlp.hip", "eval.hip", "engine.cpp", "batch.cpp", "proposal.cpp", "model.cpp", "api.cpp", "scene.cpp", "io.cpp"]
+SOURCES = ["kernels.hip", "mcmc.hip", "mlp.hip", "eval.hip", "render_noalloc.hip", "engine.cpp", "batch.cpp", "proposal.cpp", "model.cpp", "api.cpp", "scene.cpp", "io.cpp"]
 FLAGS = ["-std=c++20", "-O1", "-g", "-fPIC", "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0", "-fno-omit-frame-pointer", "-fsanitize=address",
          "-ffp-contract=off", "-pthread", "-Wno-deprecated-declarations", "-I" + HERE, "-I" + INCLUDE, "-I" + CSRC]
 

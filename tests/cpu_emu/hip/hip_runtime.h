@@ -45,6 +45,7 @@
 #define __launch_bounds__(...)
 #define __shared__ static
 #define __constant__ static
+#define __align__(n) alignas(n)
 #define __HIP_EMU__ 1
 
 // ---- vector types ------------------------------------------------------------------
@@ -435,6 +436,31 @@ inline hipError_t hipMalloc(void** p, size_t bytes) {
     return hipSuccess;
 }
 template <class T> inline hipError_t hipMalloc(T** p, size_t bytes) { return hipMalloc(reinterpret_cast<void**>(p), bytes); }
+
+// File-scope device arrays are host globals here. HIP_SYMBOL hands a symbol's host address
+// and size to hipGetSymbolAddress, which maps it once to an emulated device allocation of
+// the same size (guarded and 0xFF-filled like any other, so code that used the symbol's
+// host name as a device address, read past its end, or relied on zero-initialised contents
+// would fault or misbehave here as well).
+struct EmuSymbol {
+    const void* host;
+    size_t bytes;
+};
+#define HIP_SYMBOL(x) (EmuSymbol{(const void*)&(x), sizeof(x)})
+inline hipError_t hipGetSymbolAddress(void** p, EmuSymbol sym) {
+    static std::mutex mu;
+    static std::map<const void*, void*> mapped;
+    std::lock_guard<std::mutex> g(mu);
+    auto it = mapped.find(sym.host);
+    if (it == mapped.end()) {
+        void* d = nullptr;
+        const hipError_t e = hipMalloc(&d, sym.bytes);
+        if (e != hipSuccess) return e;
+        it = mapped.emplace(sym.host, d).first;
+    }
+    *p = it->second;
+    return hipSuccess;
+}
 inline hipError_t hipFree(void* p) {
     emu_no_capture("hipFree");
     if (!p) return hipSuccess;
