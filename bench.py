@@ -11,8 +11,8 @@ relocation), as the reference's headline it/s is quoted.
 
 Prints one JSON line (the bench contract). `roofline` covers the dominant kernel: its
 per-launch time comes from HIP events on the trainer's stream, its work from the
-algorithmic bytes (HBM-bound kernels) or FLOPs (raster kernels, from the number of
-evaluated pixel-entry pairs); `traffic` is read from the committed rocprofv3 PMC summary
+algorithmic bytes (HBM-bound kernels) or FLOPs (raster kernels: 16 forward, 45 backward per
+valid pixel-entry pair, i.e. per entry a pixel takes, counted by the forward kernel); `traffic` is read from the committed rocprofv3 PMC summary
 when one exists for that kernel and workload. `cpu_baseline` times the CPU oracle on the
 same state and view.
 """
@@ -184,6 +184,9 @@ def main():
     bins = read("tile_bins", np.int32, 2 * T).reshape(T, 2)
     final_idx = read("final_idx", np.int32, W * H)
     pairs = pair_counts(final_idx, bins, W, H)
+    # valid pairs: entries each pixel took in the forward (sigma < 5.55, alpha >= 1/255, before
+    # saturation) -- the work the per-pair FLOP figures describe
+    valid = int(read("n_contrib", np.int32, W * H).astype(np.int64).sum())
     tw_max, tw_total = tile_work(final_idx, bins, W, H)
     L.msplat_engine_stats(eng, stats)
     I = int(stats[0])
@@ -193,8 +196,8 @@ def main():
         "project_sh_fwd": ("hbm", N * ((50 + 90 + 68) if a.sh_bf16 else (56 + 180 + 68))),
         "gaussian_update": ("hbm", N * (390 if a.sh_bf16 else 408)),
         "sh_rest_adam": ("hbm", V * (810 if a.sh_bf16 else 1080) + N * 28),
-        "raster_fwd": ("valu", pairs * 16),
-        "raster_bwd": ("valu", pairs * 45),
+        "raster_fwd": ("valu", valid * 16),
+        "raster_bwd": ("valu", valid * 45),
     }
     roofline = None
     if stages:
@@ -275,7 +278,8 @@ def main():
                    "gaussians": N, "width": W, "height": H, "fx": a.fx, "cameras": ds.num_train,
                    "hidden_gt_gaussians": a.hidden, "sh_degree": 3, "sh_degree_interval": 1,
                    "loss": "0.8 L1 + 0.2 (1 - SSIM)",
-                   "cap_max": a.cap, "intersections": I, "visible": V, "raster_pairs": pairs,
+                   "cap_max": a.cap, "intersections": I, "visible": V, "raster_pairs_walked": pairs,
+                   "raster_pairs_valid": valid,
                    "tile_entries_walked_max": tw_max, "tile_entries_walked_total": tw_total,
                    "parallelism": "single GPU"},
         "roofline": roofline,

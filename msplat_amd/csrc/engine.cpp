@@ -121,7 +121,7 @@ void Engine::trim() {
     DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_,
-                   &finalT_, &finalIdx_, &vrend_, &hbuf_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
+                   &finalT_, &finalIdx_, &ncontrib_, &vrend_, &hbuf_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
                    &prefixT_, &afterC_, &wq_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
     debug_valid_ = false;
@@ -139,7 +139,7 @@ Engine::~Engine() {
     hipEventDestroy(mark_[1]);
     DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
-                   &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_, &finalT_, &finalIdx_,
+                   &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_, &finalT_, &finalIdx_, &ncontrib_,
                    &vrend_, &hbuf_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
                    &afterC_, &nanc_, &wq_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
@@ -185,7 +185,7 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
     ensure(temp_, msk::lists_temp_bytes(N));
     ensure(ids_, (size_t)capacity_ * 4);
     ensure(hist_, msk::lists_hist_bytes(N, T));
-    ensure(img_, P * 12); ensure(finalT_, P * 4); ensure(finalIdx_, P * 4);
+    ensure(img_, P * 12); ensure(finalT_, P * 4); ensure(finalIdx_, P * 4); ensure(ncontrib_, P * 4);
     ensure(wq_, (size_t)T * 4);
     ensure(wmisc_, 64);
     ensure(consts_, sizeof(msk::ViewConsts) + sizeof(msk::RasterView) + 64);
@@ -236,6 +236,7 @@ msk::RasterView Engine::raster_view(bool train) const {
     r.G = train ? msk::GradBufs{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()}
                 : msk::GradBufs{nullptr, nullptr, nullptr, nullptr};
     r.chunked = K_ > 1 ? 1 : 0;
+    r.n_contrib = ncontrib_.as<int32_t>();
     return r;
 }
 
@@ -465,7 +466,7 @@ int Engine::buffer(const std::string& name, void** p, size_t* bytes) {
         {"aabb", &aabb_, n * 8}, {"recA", &recA_, n * 16}, {"recB", &recB_, n * 16}, {"recC", &recC_, n * 8},
         {"tile_counts", &counts_, T * 4}, {"tile_bins", &bins_, T * 8},
         {"ids", &ids_, (size_t)capacity_ * 4}, {"img", &img_, P * 12}, {"final_T", &finalT_, P * 4},
-        {"final_idx", &finalIdx_, P * 4}, {"v_rendered", &vrend_, P * 12}, {"v_xy", &v_xy_, n * 8},
+        {"final_idx", &finalIdx_, P * 4}, {"n_contrib", &ncontrib_, P * 4}, {"v_rendered", &vrend_, P * 12}, {"v_xy", &v_xy_, n * 8},
         {"v_conic", &v_conic_, n * 12}, {"v_rgb", &v_rgb_, n * 12}, {"v_opac", &v_opac_, n * 4},
         {"dbg_v_mean", &dbg_vm_, n * 12}, {"dbg_v_scale", &dbg_vs_, n * 12}, {"dbg_v_quat", &dbg_vq_, n * 16},
         {"dbg_v_opac", &dbg_vo_, n * 4}, {"nan_counts", &nanc_, 64}, {"totals", &totals_, 32}, {"sorted_ids", &sorted_ids_, n * 4},

@@ -109,6 +109,7 @@ struct RasterView {
     const float* v_rend;
     GradBufs G;
     int chunked;   // rasterised by the chunked kernels, not through the work queue
+    int32_t* n_contrib = nullptr;  // [H W] entries each pixel took (persistent forward; optional)
 };
 
 // Work queue of the persistent raster kernels over B views of at most Tmax tiles each.

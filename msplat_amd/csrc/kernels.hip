@@ -1054,29 +1054,98 @@ constexpr int RB = 256;
 __device__ __forceinline__ int tile_lx() { return (int)((threadIdx.x & 7) | ((threadIdx.x >> 3) & 8)); }
 __device__ __forceinline__ int tile_ly() { return (int)(((threadIdx.x >> 3) & 7) | ((threadIdx.x >> 4) & 8)); }
 
+// Quadrants of the entry's 16x16 tile (bit q: the 8x8 quadrant of wave q, x half q & 1, y
+// half q >> 1) holding a pixel centre that can take the entry. A pixel takes it only where
+// sigma < 5.55 and o exp(-sigma) >= 1/255, i.e. sigma <= ln(255 o) (the early-outs of
+// metal:580-588, 1081-1090); sigma is a convex quadratic, so its minimum over a quadrant's
+// box of pixel centres is 0 when the Gaussian's centre lies in the box and is otherwise
+// reached on one of the box's edges, each a one-dimensional quadratic minimised in closed
+// form. A quadrant is left out only when that minimum exceeds the threshold by a margin
+// (1e-4 of the largest term of sigma over the box, plus 1e-4) that covers the rounding of
+// both this test and the per-pixel evaluation: every pixel of the quadrant would reject the
+// entry, so leaving it out changes no result. Conics that are not positive definite or not
+// finite keep every quadrant (the per-pixel test decides).
+__device__ __forceinline__ uint32_t quad_mask(const float4 A, const float4 B, float ox, float oy) {
+    const float cx = A.x, cy = A.y, o = A.z, ca = A.w, cb = B.x, cc = B.y;
+    if (o < (1.f / 255.f) * (1.f - 1e-5f)) return 0u;  // alpha <= o below 1/255 everywhere
+    const float det = ca * cc - cb * cb;
+    if (!(ca > 0.f && cc > 0.f && det > 0.f && ms_finite(cx) && ms_finite(cy) && ms_finite(det) && o <= 1.f))
+        return 15u;
+    const float t = fminf(5.55f, logf(255.f * o));
+    const float ia = 1.f / ca, ic = 1.f / cc;
+    auto sig = [&](float dx, float dy) { return 0.5f * (ca * dx * dx + cc * dy * dy) + cb * dx * dy; };
+    uint32_t m = 0u;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float x0 = ox + 8.f * (float)(q & 1), y0 = oy + 8.f * (float)(q >> 1);
+        const float dlx = cx - (x0 + 7.f), dhx = cx - x0, dly = cy - (y0 + 7.f), dhy = cy - y0;
+        float mn = 0.f;
+        if (!(dlx <= 0.f && dhx >= 0.f && dly <= 0.f && dhy >= 0.f)) {
+            const float ya = fminf(fmaxf(-cb * dlx * ic, dly), dhy), yb = fminf(fmaxf(-cb * dhx * ic, dly), dhy);
+            const float xa = fminf(fmaxf(-cb * dly * ia, dlx), dhx), xb = fminf(fmaxf(-cb * dhy * ia, dlx), dhx);
+            mn = fminf(fminf(sig(dlx, ya), sig(dhx, yb)), fminf(sig(xa, dly), sig(xb, dhy)));
+        }
+        const float Dx = fmaxf(fabsf(dlx), fabsf(dhx)), Dy = fmaxf(fabsf(dly), fabsf(dhy));
+        const float big = 0.5f * (ca * Dx * Dx + cc * Dy * Dy) + fabsf(cb) * Dx * Dy;
+        if (!(mn >= t + 1e-4f * (big + 1.f))) m |= 1u << q;
+    }
+    return m;
+}
+
+// Indices (batch slots, increasing) of the staged entries whose mask holds this wave's
+// quadrant and that pass `extra`; returns their count. Wave-local: the caller synchronises
+// before reading the list.
+template <int NB, class Extra>
+__device__ __forceinline__ int wave_entry_list(const uint8_t* sM, uint8_t* list, int n, Extra extra) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    int cnt = 0;
+#pragma unroll
+    for (int j0 = 0; j0 < NB; j0 += 64) {
+        const int e = j0 + lane;
+        const bool take = e < n && ((sM[e] >> wid) & 1u) && extra(e);
+        const uint64_t bal = __ballot(take);
+        if (take) list[cnt + __builtin_popcountll(bal & below)] = (uint8_t)e;
+        cnt += __builtin_popcountll(bal);
+    }
+    return cnt;
+}
+
 __device__ __forceinline__ void composite_range(int lo, int hi, float px, float py, bool inside, const EntryView& ev,
-                                                float& T, float C[3], int& last) {
+                                                float& T, float C[3], int& last, int& taken) {
     __shared__ float4 sA[RB];
     __shared__ float4 sB[RB];
     __shared__ float sCx[RB];
+    __shared__ uint8_t sM[RB];
+    __shared__ uint8_t sL[4][RB];
+    const int wid = (int)(threadIdx.x >> 6);
+    const float ox = px - (float)tile_lx(), oy = py - (float)tile_ly();  // the tile's corner pixel
     T = 1.f;
     C[0] = C[1] = C[2] = 0.f;
     last = lo - 1;
+    taken = 0;
     bool done = !inside;
     for (int b0 = lo; b0 < hi; b0 += RB) {
         if (__syncthreads_and(done)) break;
         const int idx = b0 + threadIdx.x;
         if (idx < hi) {  // colours stored clamped (max(c + 1/2, 0)): per entry, not per pixel
             const uint32_t g = ev.ids[idx];
+            const float4 a = ev.rec.A[g];
             const float4 b = ev.rec.B[g];
-            sA[threadIdx.x] = ev.rec.A[g];
+            sA[threadIdx.x] = a;
             sB[threadIdx.x] = make_float4(b.x, b.y, fmaxf(b.z + 0.5f, 0.f), fmaxf(b.w + 0.5f, 0.f));
             sCx[threadIdx.x] = fmaxf(ev.rec.C[g].x + 0.5f, 0.f);
+            sM[threadIdx.x] = (uint8_t)quad_mask(a, b, ox, oy);
         }
         __syncthreads();
-        if (done) continue;
         const int n = min(RB, hi - b0);
-        for (int e = 0; e < n; e++) {
+        // only the entries some pixel of this wave's quadrant can take (the others would be
+        // rejected by every lane): same accepted entries in the same order
+        const int cnt = wave_entry_list<RB>(sM, sL[wid], n, [](int) { return true; });
+        __syncthreads();
+        if (done) continue;
+        for (int k = 0; k < cnt; k++) {
+            const int e = sL[wid][k];
             const float4 a = sA[e];
             const float4 b = sB[e];
             const float dx = a.x - px, dy = a.y - py;
@@ -1096,6 +1165,7 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
             C[2] = fmaf(sCx[e], vis, C[2]);
             T = nT;
             last = b0 + e;
+            taken++;
         }
     }
 }
@@ -1182,12 +1252,13 @@ __global__ __launch_bounds__(RB) void k_raster_fwd_persist(const RasterView* __r
         const bool inside = i < H && j < W;
         const int2 r = rv.bins[t];
         float T, C[3];
-        int last;
-        composite_range(r.x, r.y, (float)j, (float)i, inside, rv.pk, T, C, last);
+        int last, taken;
+        composite_range(r.x, r.y, (float)j, (float)i, inside, rv.pk, T, C, last, taken);
         if (inside) {
             const size_t pid = (size_t)i * W + j;
             rv.final_T[pid] = T;
             rv.final_idx[pid] = last;
+            if (rv.n_contrib) rv.n_contrib[pid] = taken;
             rv.img[3 * pid + 0] = fminf(fmaxf(fmaf(vc.bg[0], T, C[0]), 0.f), 1.f);
             rv.img[3 * pid + 1] = fminf(fmaxf(fmaf(vc.bg[1], T, C[1]), 0.f), 1.f);
             rv.img[3 * pid + 2] = fminf(fmaxf(fmaf(vc.bg[2], T, C[2]), 0.f), 1.f);
@@ -1218,8 +1289,8 @@ __global__ __launch_bounds__(RB) void k_raster_fwd_chunk(const ViewConsts* __res
         return;
     }
     float T, C[3];
-    int last;
-    composite_range(cs, ce, (float)j, (float)i, inside, pk, T, C, last);
+    int last, taken;
+    composite_range(cs, ce, (float)j, (float)i, inside, pk, T, C, last, taken);
     if (inside) {
         chunk_T[o] = T;
         chunk_C[3 * o] = C[0]; chunk_C[3 * o + 1] = C[1]; chunk_C[3 * o + 2] = C[2];
@@ -1519,12 +1590,20 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
     __shared__ float4 sB[BB];
     __shared__ float2 sC[BB];
     __shared__ uint32_t sF[BB];
+    __shared__ uint8_t sM[BB];
+    __shared__ uint8_t sL[4][BB];
+    __shared__ uint8_t sIn[4][BB];   // sIn[w][slot]: entry `slot` is on wave w's list
     __shared__ float red[4][BB][9];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const float ox = px - (float)tile_lx(), oy = py - (float)tile_ly();  // the tile's corner pixel
     // the colour accumulated behind the current entry plus the background term T_final bg:
     // the two only ever enter as their sum
     float bb0 = fmaf(T_final, bg[0], buf0), bb1 = fmaf(T_final, bg[1], buf1), bb2 = fmaf(T_final, bg[2], buf2);
     const int top = min(hi, tile_max + 1);  // entries past every pixel's last contributor are skipped
+    // the last contributor of any pixel of this wave: entries behind it give the wave zeros
+    int wmax = inside ? bin_final : INT_MIN;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) wmax = max(wmax, __shfl_xor(wmax, off));
     const int tag = wave_sum4_tag();
     for (int bhi = top; bhi > lo; bhi -= BB) {
         const int blo = max(lo, bhi - BB);
@@ -1533,32 +1612,32 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
         if ((int)threadIdx.x < n) {  // colours stored clamped; which of them pass the clamp in sF
             const int idx = bhi - 1 - threadIdx.x;  // slot 0 = furthest back
             const uint32_t g = ev.ids[idx];
+            const float4 a = ev.rec.A[g];
             const float4 b = ev.rec.B[g];
             const float c0 = b.z + 0.5f, c1 = b.w + 0.5f, c2 = ev.rec.C[g].x + 0.5f;
-            sA[threadIdx.x] = ev.rec.A[g];
+            sA[threadIdx.x] = a;
             sB[threadIdx.x] = make_float4(b.x, b.y, fmaxf(c0, 0.f), fmaxf(c1, 0.f));
             sC[threadIdx.x] = make_float2(fmaxf(c2, 0.f), __builtin_bit_cast(float, g));
             sF[threadIdx.x] = (c0 >= 0.f ? 1u : 0u) | (c1 >= 0.f ? 2u : 0u) | (c2 >= 0.f ? 4u : 0u);
+            sM[threadIdx.x] = (uint8_t)quad_mask(a, b, ox, oy);
         }
         __syncthreads();
-        // four entries at a time: their 4 x 9 terms are reduced together (wave_sum4_rows)
-        for (int t4 = 0; t4 < n; t4 += 4) {
-            // entries behind every pixel's last contributor in this wave: only zeros to record
-            const int e_min = bhi - 1 - min(t4 + 3, n - 1);
-            if (__ballot(inside && bin_final >= e_min) == 0ull) {
-                if ((lane & 15) == 15) {
+        // this wave's entries, back to front: its quadrant reachable (every lane rejects the
+        // others) and not behind the wave's last contributor
+        const int cnt = wave_entry_list<BB>(sM, sL[wid], n, [&](int e) { return bhi - 1 - e <= wmax; });
 #pragma unroll
-                    for (int k = 0; k < 9; k++) red[wid][t4 + tag][k] = 0.f;
-                }
-                continue;
-            }
+        for (int j0 = 0; j0 < BB; j0 += 64) sIn[wid][j0 + lane] = 0;
+        __syncthreads();
+        for (int k = lane; k < cnt; k += 64) sIn[wid][sL[wid][k]] = 1;
+        // four entries at a time: their 4 x 9 terms are reduced together (wave_sum4_rows)
+        for (int t4 = 0; t4 < cnt; t4 += 4) {
             float g[4][9];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
 #pragma unroll
                 for (int k = 0; k < 9; k++) g[u][k] = 0.f;
-                const int tt = t4 + u;
-                if (tt >= n) continue;  // uniform
+                if (t4 + u >= cnt) continue;  // uniform
+                const int tt = sL[wid][t4 + u];
                 const int e = bhi - 1 - tt;
                 bool valid = inside && e <= bin_final;
                 float alpha = 0.f, dx = 0.f, dy = 0.f;
@@ -1600,11 +1679,13 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
                     g[u][8] = -v_sigma * (1.f - a.z);
                 }
             }
-            const int slot = t4 + tag;  // lane 15 of each row: the sum of entry t4 + tag
+            // lane 15 of each row holds the sum of list entry t4 + tag
+            const bool wr = (lane & 15) == 15 && t4 + tag < cnt;
+            const int slot = wr ? sL[wid][t4 + tag] : 0;
 #pragma unroll
             for (int k = 0; k < 9; k++) {
                 const float v = wave_sum4_rows(g[0][k], g[1][k], g[2][k], g[3][k]);
-                if ((lane & 15) == 15) red[wid][slot][k] = v;
+                if (wr) red[wid][slot][k] = v;
             }
         }
         __syncthreads();
@@ -1612,7 +1693,12 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
             const int tt = threadIdx.x;
             float s[9];
 #pragma unroll
-            for (int k = 0; k < 9; k++) s[k] = red[0][tt][k] + red[1][tt][k] + red[2][tt][k] + red[3][tt][k];
+            for (int k = 0; k < 9; k++) {
+                float v = 0.f;
+#pragma unroll
+                for (int w = 0; w < 4; w++) v += sIn[w][tt] ? red[w][tt][k] : 0.f;
+                s[k] = v;
+            }
             bool any = false;
 #pragma unroll
             for (int k = 0; k < 9; k++) any |= (s[k] != 0.f);
