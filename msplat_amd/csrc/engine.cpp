@@ -121,7 +121,7 @@ void Engine::trim() {
     DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_,
-                   &finalT_, &finalIdx_, &ncontrib_, &vrend_, &hbuf_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
+                   &finalT_, &finalIdx_, &ncontrib_, &vrend_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
                    &prefixT_, &afterC_, &wq_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
     debug_valid_ = false;
@@ -140,7 +140,7 @@ Engine::~Engine() {
     DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_, &finalT_, &finalIdx_, &ncontrib_,
-                   &vrend_, &hbuf_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
+                   &vrend_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
                    &afterC_, &nanc_, &wq_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
     if (stream_ && owns_stream_) hipStreamDestroy(stream_);
@@ -196,7 +196,7 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
     }
     if (train) {
         ensure(v_xy_, n * 8); ensure(v_conic_, n * 12); ensure(v_rgb_, n * 12); ensure(v_opac_, n * 4);
-        ensure(vrend_, P * 12); ensure(hbuf_, P * 60); ensure(dbuf_, P * 36);
+        ensure(vrend_, P * 12); ensure(dbuf_, P * 36);
         const size_t nblk = (size_t)((launch.W + 15) / 16) * ((launch.H + 15) / 16);
         ensure(partials_, nblk * 4);
     }
@@ -329,7 +329,7 @@ void Engine::raster_fwd_own() {
 }
 
 void Engine::loss_to(int slot) {
-    msk::ssim_loss(stream_, vcd_, launch_, img_.as<float>(), hbuf_.as<float>(), dbuf_.as<float>(), vrend_.as<float>(),
+    msk::ssim_loss(stream_, vcd_, launch_, img_.as<float>(), dbuf_.as<float>(), vrend_.as<float>(),
                    partials_.as<float>(), loss_.as<double>() + slot, nanc_.as<uint32_t>());
     stage_event(ST_SSIM);
 }

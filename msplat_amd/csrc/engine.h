@@ -141,7 +141,7 @@ private:
     // per-intersection
     DBuf ids_, hist_;
     // per-pixel
-    DBuf img_, finalT_, finalIdx_, ncontrib_, vrend_, hbuf_, dbuf_, partials_, loss_;
+    DBuf img_, finalT_, finalIdx_, ncontrib_, vrend_, dbuf_, partials_, loss_;
     DBuf chunkT_, chunkC_, chunkIdx_, prefixT_, afterC_;
     DBuf nanc_;
     bool debug_valid_ = false;

@@ -146,8 +146,8 @@ void raster_work_build(hipStream_t s, const RasterWork& w);
 void raster_forward(hipStream_t s, const ListBufs& L, const ViewConsts* vc, const Geom& g, RasterBufs& R,
                     const RasterWork* work);
 void raster_forward_views(hipStream_t s, const RasterWork& w);
-void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& g, const float* img, float* hbuf /*[H W 15]*/,
-               float* dbuf /*[H W 9]*/, float* v_rend, float* loss_partials, double* loss_out, uint32_t* nan_counts);
+void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& g, const float* img, float* dbuf /*[H W 9]*/,
+               float* v_rend, float* loss_partials, double* loss_out, uint32_t* nan_counts);
 void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, const ViewConsts* vc, const Geom& g,
                      const float* v_rend, GradBufs& G, const RasterWork* work);
 void raster_backward_views(hipStream_t s, const RasterWork& w);

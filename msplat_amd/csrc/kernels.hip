@@ -624,16 +624,17 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
                                                             const uint32_t* __restrict__ totals,
                                                             uint32_t* __restrict__ ids, uint32_t cap) {
     extern __shared__ uint32_t sm[];
+    const int chunk = (int)blockIdx.x;
     const int tbx = vc->tbx, T = vc->T;
     const int T4 = (T + 3) & ~3;
     uint32_t* cs = sm;        // [T] start of this chunk's segment of each tile
     uint32_t* pw = sm + T4;   // [T][WAVES / 2] per-wave cursors, 16 bits each
     uint32_t* idle = sm + T4 * (WAVES == 1 ? 1 : 1 + WAVES / 2) + 64 * (threadIdx.x >> 6);  // [64] per wave
     const int tid = (int)threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const uint32_t* row = mat + (size_t)blockIdx.x * T;
-    const int r0 = (int)blockIdx.x * G, r1 = min(N, r0 + G);
-    const int per = (G + WAVES - 1) / WAVES;
-    const int w0 = min(r1, r0 + wid * per), w1 = min(r1, w0 + per);
+    const uint32_t* row = mat + (size_t)chunk * T;
+    const int r0 = chunk * G, r1 = min(N, r0 + G);
+    const int wper = (G + WAVES - 1) / WAVES;
+    const int w0 = min(r1, r0 + wid * wper), w1 = min(r1, w0 + wper);
     const int word = WAVES == 4 ? (wid >> 1) : 0;
     const uint32_t shift = WAVES > 1 ? 16u * (uint32_t)(wid & 1) : 0u;
     const uint32_t one = WAVES > 1 ? (1u << shift) : 1u;
@@ -1349,67 +1350,30 @@ void raster_forward(hipStream_t s, const ListBufs& L, const ViewConsts* vc, cons
 // Intermediate buffers are planar ([q][H][W]) so every store of a 64-lane wave is one
 // contiguous 256-byte run.
 constexpr int SB = 16;
-
-__global__ __launch_bounds__(256) void k_ssim_h(const ViewConsts* __restrict__ vc, const float* __restrict__ img,
-                                                float* __restrict__ hbuf) {
-    __shared__ float sg[3][SB][SB + 10];
-    __shared__ float sr[3][SB][SB + 10];
-    const int W = vc->W, H = vc->H;
-    if ((int)blockIdx.x * SB >= W || (int)blockIdx.y * SB >= H) return;
-    const uint8_t* __restrict__ gt = vc->gt;
-    const int lx = threadIdx.x & 15, ly = threadIdx.x >> 4;
-    const int bx = blockIdx.x * SB - 5, by = blockIdx.y * SB;
-    for (int c = 0; c < 3; c++)
-        for (int k = threadIdx.x; k < SB * (SB + 10); k += 256) {
-            const int sy = k / (SB + 10), sx = k % (SB + 10);
-            const int gy = by + sy, gx = bx + sx;
-            float gv = 0.f, rv = 0.f;
-            if (gx >= 0 && gx < W && gy >= 0 && gy < H) {
-                const size_t idx = ((size_t)gy * W + gx) * 3 + c;
-                gv = (float)gt[idx] * (1.0f / 255.0f);
-                rv = img[idx];
-            }
-            sg[c][sy][sx] = gv;
-            sr[c][sy][sx] = rv;
-        }
-    __syncthreads();
-    const int px = blockIdx.x * SB + lx, py = by + ly;
-    if (px >= W || py >= H) return;
-    const size_t P = (size_t)W * H, pid = (size_t)py * W + px;
-    for (int c = 0; c < 3; c++) {
-        float mx = 0.f, my = 0.f, sx = 0.f, sy = 0.f, cxy = 0.f;
-#pragma unroll
-        for (int d = 0; d < 11; d++) {
-            const float w = c_gauss1d[d];
-            const float gv = sg[c][ly][lx + d], rv = sr[c][ly][lx + d];
-            mx += w * gv;
-            my += w * rv;
-            sx += w * gv * gv;
-            sy += w * rv * rv;
-            cxy += w * gv * rv;
-        }
-        hbuf[(c * 5 + 0) * P + pid] = mx;
-        hbuf[(c * 5 + 1) * P + pid] = my;
-        hbuf[(c * 5 + 2) * P + pid] = sx;
-        hbuf[(c * 5 + 3) * P + pid] = sy;
-        hbuf[(c * 5 + 4) * P + pid] = cxy;
-    }
-}
-
 // The vertical backward takes 32 columns per workgroup: each of its halo rows is then one
 // full 128-byte line per plane (16 columns fetch half-used lines). Every pixel's arithmetic
 // is that of 16x16 tiles.
 constexpr int FW = 32;
 
+// Horizontal pass, vertical pass, SSIM and L1 per pixel, derivative fields and their
+// horizontal transpose, for one 16x16 tile: the horizontal statistics (metal:3388-3450) of
+// the tile's 26x26 halo are computed in LDS from the image and the target (15 bytes per
+// pixel read, 36x26 with the halo) instead of being written out and read back (60 bytes per
+// pixel each way, and a misaligned 26-column halo row spans two or three 128-byte lines);
+// every value is summed in the same order as a separate pass would, so the results are the
+// same bit for bit.
 __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict__ vc, const float* __restrict__ img,
-                                                    const float* __restrict__ hbuf, float* __restrict__ dbuf,
-                                                    float* __restrict__ partials, uint32_t* __restrict__ nanc) {
+                                                    float* __restrict__ dbuf, float* __restrict__ partials,
+                                                    uint32_t* __restrict__ nanc) {
     const int W = vc->W, H = vc->H;
     const int gx = (W + SB - 1) / SB;
     if ((int)blockIdx.x >= gx || (int)blockIdx.y * SB >= H) return;
     const uint8_t* __restrict__ gt = vc->gt;
     const float ssim_w = vc->ssim_w;
     constexpr int TD = SB + 10;  // 26
+    constexpr int TI = SB + 20;  // 36: input columns of the horizontal pass over the halo
+    __shared__ float sg[3][TD][TI];
+    __shared__ float sr[3][TD][TI];
     __shared__ float shp[TD][TD][5];
     __shared__ float sf[3][SB][TD];
     __shared__ float s_ssim[SB][SB];
@@ -1420,14 +1384,43 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
     const size_t P = (size_t)W * H;
     s_ssim[ly][lx] = 0.f;
     s_l1[ly][lx] = 0.f;
+    // target (x = gt / 255) and render of rows by .. by + 25, columns bx - 5 .. bx + 30 (0 outside)
+    for (int k = threadIdx.x; k < TD * TI; k += 256) {
+        const int sy = k / TI, sx = k % TI;
+        const int gy = by + sy, gxx = bx - 5 + sx;
+        const bool in = gxx >= 0 && gxx < W && gy >= 0 && gy < H;
+        const size_t pid = in ? (size_t)gy * W + gxx : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            sg[c][sy][sx] = in ? (float)gt[3 * pid + c] * (1.0f / 255.0f) : 0.f;
+            sr[c][sy][sx] = in ? img[3 * pid + c] : 0.f;
+        }
+    }
+    __syncthreads();
     for (int c = 0; c < 3; c++) {
+        // horizontal 11-tap statistics at the halo's pixels; zero outside the image (the
+        // vertical pass's padding)
         for (int k = threadIdx.x; k < TD * TD; k += 256) {
             const int sy = k / TD, sx = k % TD;
-            const int gy = by + sy, gx = bx + sx;
-            const bool in = gx >= 0 && gx < W && gy >= 0 && gy < H;
-            const size_t pid = (size_t)gy * W + gx;
+            const int gy = by + sy, gxx = bx + sx;
+            float mx = 0.f, my = 0.f, sxq = 0.f, syq = 0.f, cxy = 0.f;
+            if (gxx >= 0 && gxx < W && gy >= 0 && gy < H) {
 #pragma unroll
-            for (int q = 0; q < 5; q++) shp[sy][sx][q] = in ? hbuf[(c * 5 + q) * P + pid] : 0.f;
+                for (int d = 0; d < 11; d++) {
+                    const float w = c_gauss1d[d];
+                    const float gv = sg[c][sy][sx + d], rv = sr[c][sy][sx + d];
+                    mx += w * gv;
+                    my += w * rv;
+                    sxq += w * gv * gv;
+                    syq += w * rv * rv;
+                    cxy += w * gv * rv;
+                }
+            }
+            shp[sy][sx][0] = mx;
+            shp[sy][sx][1] = my;
+            shp[sy][sx][2] = sxq;
+            shp[sy][sx][3] = syq;
+            shp[sy][sx][4] = cxy;
         }
         __syncthreads();
         for (int k = threadIdx.x; k < SB * TD; k += 256) {
@@ -1460,9 +1453,8 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
             if (dx >= 5 && dx < 5 + SB) {
                 const int gpx = bx + dx, gpy = by + dy + 5;
                 if (gpx >= 0 && gpx < W && gpy >= 0 && gpy < H) {
-                    const size_t pid = (size_t)gpy * W + gpx;
                     s_ssim[dy][dx - 5] += (A * Bv) / (Cd * D);
-                    s_l1[dy][dx - 5] += fabsf((float)gt[3 * pid + c] * (1.0f / 255.0f) - img[3 * pid + c]);
+                    s_l1[dy][dx - 5] += fabsf(sg[c][dy + 5][dx + 5] - sr[c][dy + 5][dx + 5]);
                 }
             }
         }
@@ -1566,11 +1558,10 @@ __global__ __launch_bounds__(256) void k_loss_final(const ViewConsts* __restrict
     if (threadIdx.x == 0) *out = red[0];
 }
 
-void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& gm, const float* img, float* hbuf, float* dbuf,
+void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& gm, const float* img, float* dbuf,
                float* v_rend, float* loss_partials, double* loss_out, uint32_t* nan_counts) {
     dim3 g((gm.W + SB - 1) / SB, (gm.H + SB - 1) / SB), gw((gm.W + FW - 1) / FW, (gm.H + SB - 1) / SB);
-    hipLaunchKernelGGL(k_ssim_h, g, dim3(256), 0, s, vc, img, hbuf);
-    hipLaunchKernelGGL(k_ssim_fused, g, dim3(256), 0, s, vc, img, hbuf, dbuf, loss_partials, nan_counts);
+    hipLaunchKernelGGL(k_ssim_fused, g, dim3(256), 0, s, vc, img, dbuf, loss_partials, nan_counts);
     hipLaunchKernelGGL(k_ssim_vbwd, gw, dim3(256), 0, s, vc, img, dbuf, v_rend, nan_counts);
     hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, s, vc, loss_partials, loss_out);
 }
