@@ -122,7 +122,13 @@ int msplat_trainer_run_mcmc(MsplatTrainer t, int step, int32_t info[6]);
  * unknown name or a failure.
  *   "sh_bf16"  1 = store f_dc / f_rest coefficients and their Adam moments in bfloat16
  *              (fp32 masters, fp32 arithmetic); 0 = all fp32 (default). Converts the
- *              current state; checkpoints stay fp32 on disk. */
+ *              current state; checkpoints stay fp32 on disk.
+ *   "nan_inject_step"  the reference's MSPLAT_NAN_INJECT self-test (model.cpp:638-659): at
+ *              the end of the relocation pass of that step, 100 draws among the active
+ *              rows get NaN DC colours (-1: never; the environment variable
+ *              MSPLAT_NAN_INJECT sets it when the trainer is created).
+ * The other options (batching, proposal, raster geometry) are listed with the entry points
+ * they affect. */
 int msplat_trainer_set_option(MsplatTrainer t, const char* name, double value);
 /* Adam step counter and current learning rates (lr[6]). */
 int msplat_trainer_optim_state(MsplatTrainer t, int* adamSteps, float lr[6]);

@@ -796,6 +796,11 @@ int msplat_trainer_set_option(MsplatTrainer t, const char* name, double value) {
         TR(t)->model->set_mcmc_enabled(value != 0.0);
         return 0;
     }
+    if (n == "nan_inject_step") {
+        if (value < -1.0 || value > 2147483647.0) throw std::invalid_argument("nan_inject_step out of range");
+        TR(t)->model->set_nan_inject_step((int)value);
+        return 0;
+    }
     if (n == "batch_graph") {
         TR(t)->model->set_batch_graph(value != 0.0);
         return 0;

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <stdexcept>
 
@@ -58,6 +59,7 @@ Model::Model(const ModelOptions& opt, const HostGaussians& init, float scale, co
     if (init.degree != opt.sh_degree) throw std::invalid_argument("initial Gaussians have the wrong SH degree");
     for (int k = 0; k < 3; k++) translation_[k] = translation ? translation[k] : 0.f;
     cap_ = opt.cap_max;
+    if (const char* e = std::getenv("MSPLAT_NAN_INJECT")) nan_inject_step_ = std::atoi(e);
     allocate(opt.sh_degree);
 
     // Seed selection (setupOptimizers, model.cpp:176-204): fresh seeds above 75% of the cap
@@ -351,7 +353,26 @@ msk::McmcInfo Model::mcmc_after_train(int step) {
                                        prop_.enabled ? &hooks : nullptr);
     if (prop_.enabled) proposal_after_event(P, hooks);
     num_active_ = info.num_active;
+    if (step == nan_inject_step_) nan_inject(step);
     return info;
+}
+
+void Model::nan_inject(int step) {
+    if (num_active_ <= 0) return;
+    // 100 draws with replacement among the active rows (the reference draws among its alive
+    // list with mt19937(1234); this build's own stream: splitmix64 from 1234)
+    static const float kNaN3[3] = {NAN, NAN, NAN};
+    uint64_t s = 1234;
+    hipStream_t st = eng_.stream();
+    for (int k = 0; k < 100; k++) {
+        const int64_t i = (int64_t)(mix64(s) % (uint64_t)num_active_);
+        hip_check(hipMemcpyAsync(p_[3].as<float>() + 3 * i, kNaN3, sizeof(kNaN3), hipMemcpyHostToDevice, st),
+                  "nan inject");
+    }
+    refresh_shadows();
+    hip_check(hipStreamSynchronize(st), "nan inject");
+    std::printf("MCMC nan-inject step=%d poisoned=100 (featuresDc)\n", step);
+    std::fflush(stdout);
 }
 
 const float* Model::render(const TrainCamera& cam, int step) {

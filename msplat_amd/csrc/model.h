@@ -132,6 +132,11 @@ public:
     int64_t proposal_read(const std::string& name, void* host, int64_t bytes) const;
     // Model::mcmcAfterTrain's schedule (model.cpp:498-501): is `step` a relocation step?
     bool is_mcmc_step(int step) const;
+    // Self-test of the non-finite gates (model.cpp:638-659): at the end of the relocation
+    // pass of `step` (-1: never; the MSPLAT_NAN_INJECT environment variable sets it at
+    // construction) 100 draws among the active rows get NaN DC colours. The gates keep
+    // those rows invisible and the next pass culls them.
+    void set_nan_inject_step(int step) { nan_inject_step_ = step; }
     // Resident workgroups of the batched persistent raster launches (0 = default).
     void set_persist_grid(int g) {
         persist_grid_ = g;
@@ -153,6 +158,8 @@ private:
     OptimState os_;
     int cap_ = 0, degree_ = 0, num_active_ = 0;
     uint32_t reloc_count_ = 0;
+    int nan_inject_step_ = -1;
+    void nan_inject(int step);
     float scale_ = 1.0f, translation_[3] = {0, 0, 0};
     DBuf p_[6], m_[6], v_[6];
     bool sh_bf16_ = false;
