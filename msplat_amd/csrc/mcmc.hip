@@ -11,8 +11,6 @@
 // advancing only when a draw happens.
 #include <algorithm>
 
-#include <hipcub/hipcub.hpp>
-
 #include "mcmc.h"
 #include "ms_math.h"
 
@@ -209,6 +207,94 @@ void eq9_cases(hipStream_t s, int n, const float* o_old, const int* cnt, float* 
     if (n > 0) hipLaunchKernelGGL(k_eq9_cases, dim3(cdiv2(n, 64)), dim3(64), 0, s, n, o_old, cnt, o_new, s_new);
 }
 
+// ---- device-wide prefix sums (u32 counts, u64 weights) --------------------------------
+// Three launches: each workgroup scans a 2048-item tile (8 consecutive items per thread,
+// wave prefix by shuffles, waves combined through LDS) and writes the tile's total; one
+// workgroup scans the totals; the tiles add their offsets. Integer sums: the result does not
+// depend on the order of evaluation.
+constexpr int SC_THREADS = 256, SC_ITEMS = 8, SC_TILE = SC_THREADS * SC_ITEMS;
+
+template <class T> __device__ __forceinline__ T wave_incl_scan(T v) {
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+// Exclusive prefix of v over the workgroup (blockDim.x <= 1024) and the workgroup's total.
+template <class T> __device__ __forceinline__ T block_excl_scan(T v, T& total) {
+    __shared__ T ws[16];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+    const T incl = wave_incl_scan(v);
+    if (lane == 63) ws[wid] = incl;
+    __syncthreads();
+    T before = 0, all = 0;
+    for (int w = 0; w < nw; w++) {
+        const T x = ws[w];
+        if (w < wid) before += x;
+        all += x;
+    }
+    __syncthreads();
+    total = all;
+    return before + incl - v;
+}
+
+template <class T, bool INCLUSIVE>
+__global__ __launch_bounds__(SC_THREADS) void k_scan_tiles(int n, const T* __restrict__ in, T* __restrict__ out,
+                                                           T* __restrict__ sums) {
+    const int base = blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+    T x[SC_ITEMS], run = 0;
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        x[k] = base + k < n ? in[base + k] : (T)0;
+        run += x[k];
+    }
+    T total;
+    T pre = block_excl_scan<T>(run, total);
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++) {
+        if (INCLUSIVE) pre += x[k];
+        if (base + k < n) out[base + k] = pre;
+        if (!INCLUSIVE) pre += x[k];
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of the tile totals in place, one workgroup.
+template <class T> __global__ __launch_bounds__(1024) void k_scan_sums(int nb, T* __restrict__ sums) {
+    T carry = 0;
+    for (int b0 = 0; b0 < nb; b0 += 1024) {
+        const int b = b0 + (int)threadIdx.x;
+        const T v = b < nb ? sums[b] : (T)0;
+        T total;
+        const T pre = block_excl_scan<T>(v, total);
+        if (b < nb) sums[b] = carry + pre;
+        carry += total;
+    }
+}
+
+template <class T> __global__ __launch_bounds__(SC_THREADS) void k_scan_add(int n, T* __restrict__ out,
+                                                                           const T* __restrict__ sums) {
+    const int base = blockIdx.x * SC_TILE + threadIdx.x * SC_ITEMS;
+    const T off = sums[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SC_ITEMS; k++)
+        if (base + k < n) out[base + k] += off;
+}
+
+// out = prefix sums of in[0..n) (exclusive or inclusive); `sums` holds ceil(n / 2048) items.
+template <class T, bool INCLUSIVE>
+static void device_scan(hipStream_t s, int n, const T* in, T* out, T* sums) {
+    if (n <= 0) return;
+    const int nb = (n + SC_TILE - 1) / SC_TILE;
+    hipLaunchKernelGGL((k_scan_tiles<T, INCLUSIVE>), dim3(nb), dim3(SC_THREADS), 0, s, n, in, out, sums);
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(1024), 0, s, nb, sums);
+    hipLaunchKernelGGL(k_scan_add<T>, dim3(nb), dim3(SC_THREADS), 0, s, n, out, (const T*)sums);
+}
+
 static void check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
 }
@@ -258,10 +344,8 @@ static bool draw_and_apply(hipStream_t s, McmcScratch& S, const Rows& R, uint64_
                            uint32_t& count, const float* mlp_out = nullptr, uint32_t* pair_src = nullptr,
                            uint32_t* pair_dst = nullptr) {
     uint64_t* cdf = grow<uint64_t>(S, 8, (size_t)ncand);
-    size_t tb = 0;
-    check(hipcub::DeviceScan::InclusiveSum(nullptr, tb, w, cdf, ncand, s), "scan");
-    void* tmp = grow<uint8_t>(S, 9, tb);
-    check(hipcub::DeviceScan::InclusiveSum(tmp, tb, w, cdf, ncand, s), "scan");
+    uint64_t* sums = grow<uint64_t>(S, 9, (size_t)ncand / SC_TILE + 1);
+    device_scan<uint64_t, true>(s, ncand, w, cdf, sums);
     uint64_t total = 0;
     check(hipMemcpyAsync(&total, cdf + (ncand - 1), 8, hipMemcpyDeviceToHost, s), "copy");
     check(hipStreamSynchronize(s), "sync");
@@ -298,11 +382,9 @@ McmcInfo mcmc_relocate_and_grow(hipStream_t s, const Params& P, int cap_max, flo
     const float r2 = cull_radius > 0.0f ? cull_radius * cull_radius : -1.0f;
     hipLaunchKernelGGL(k_classify, dim3(cdiv2(N, 256)), dim3(256), 0, s, N, nr, P.means, P.scales, P.quats, P.f_dc,
                        P.f_rest, P.opac, r2, mlp_out, dead, alive, w, ctr);
-    size_t tb = 0;
-    check(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, dead, dpos, N, s), "scan");
-    void* tmp = grow<uint8_t>(S, 6, tb);
-    check(hipcub::DeviceScan::ExclusiveSum(tmp, tb, dead, dpos, N, s), "scan");
-    check(hipcub::DeviceScan::ExclusiveSum(tmp, tb, alive, apos, N, s), "scan");
+    uint32_t* sums = grow<uint32_t>(S, 6, (size_t)N / SC_TILE + 1);
+    device_scan<uint32_t, false>(s, N, dead, dpos, sums);
+    device_scan<uint32_t, false>(s, N, alive, apos, sums);
     uint32_t* dead_list = grow<uint32_t>(S, 12, N);
     uint32_t* alive_list = grow<uint32_t>(S, 13, N);
     uint64_t* alive_w = grow<uint64_t>(S, 14, N);
