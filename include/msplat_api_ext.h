@@ -50,6 +50,14 @@ MsplatDataset msplat_dataset_create_empty(void);
  * Returns its index within the train or test list, or -1. */
 int msplat_dataset_add_camera(MsplatDataset ds, const float c2w[16], float fx, float fy, float cx, float cy, int width,
                               int height, const uint8_t* rgb, bool test);
+/* Training mask of camera `index` of the train (test = false) or held-out list: host uint8
+ * [height, width], white = use, black = ignore. Binarised as Camera::loadMask does
+ * (reference input_data.cpp:81-111): a value of 128 or more (above one half) keeps the
+ * pixel; an all-white mask is dropped. Steps on a masked camera pass the mask to the loss
+ * kernels and normalise the loss by the kept pixels (model.cpp:929-934); a mask that keeps
+ * no pixel trains as unmasked, as the reference's step does. Returns the count of kept
+ * pixels, or -1. */
+int64_t msplat_dataset_set_mask(MsplatDataset ds, int index, bool test, const uint8_t* mask);
 /* Initial Gaussians (row-major host arrays; f_rest [n, (degree+1)^2-1, 3]). Returns 0 / -1. */
 int msplat_dataset_set_init_gaussians(MsplatDataset ds, int n, int degree, const float* means, const float* log_scales,
                                       const float* quats, const float* opac_logits, const float* f_dc,

@@ -76,6 +76,17 @@ void msplat_engine_destroy(MsplatEngine* e);
 int msplat_engine_train_step(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam,
                              const uint8_t* gt, const MsplatStepParams* hp, int debug);
 
+/* msplat_engine_train_step with a per-pixel training mask: the mask / has_mask arguments of
+ * msplat_train_step (reference bindings.h:71-95, msplat_metal.mm:1118-1740; kernels
+ * msplat_metal.metal:3602-3653 and 3815-3834). mask is a device uint8 [H,W] image or NULL
+ * (the unmasked step). A pixel whose mask value is below 128 owns no loss: its SSIM
+ * derivative fields and its SSIM and L1 loss terms are zero and its own L1 gradient is
+ * dropped, while SSIM gradients of unmasked neighbours' windows still reach it; the loss
+ * value still adds the SSIM weight for it. The caller sets hp->loss_inv_n to
+ * 1 / (3 * unmasked pixels), as the reference's model does (model.cpp:929-934). */
+int msplat_engine_train_step_masked(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam,
+                                    const uint8_t* gt, const uint8_t* mask, const MsplatStepParams* hp, int debug);
+
 /* Forward render only: msplat_render (reference bindings.h:58-67). The image is left in
  * the engine buffer "img" (device float [H,W,3]). */
 int msplat_engine_render(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam,

@@ -107,6 +107,8 @@ def bind(path: str) -> C.CDLL:
         "msplat_engine_destroy": ([vp], None),
         "msplat_engine_train_step": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), vp,
                                       C.POINTER(MsplatStepParams), C.c_int], C.c_int),
+        "msplat_engine_train_step_masked": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), vp, vp,
+                                             C.POINTER(MsplatStepParams), C.c_int], C.c_int),
         "msplat_engine_render": ([vp, C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), C.c_int, fp], C.c_int),
         "msplat_render_noalloc": ([C.POINTER(MsplatDeviceState), C.POINTER(MsplatCamera), C.c_int, fp, vp, vp, vp,
                                    vp], C.c_int),
@@ -171,6 +173,7 @@ def bind(path: str) -> C.CDLL:
         "msplat_dataset_camera_info": ([vp, C.c_int, C.c_bool, fp, fp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                         C.POINTER(C.c_uint8)], C.c_int),
         "msplat_dataset_set_init_gaussians": ([vp, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp], C.c_int),
+        "msplat_dataset_set_mask": ([vp, C.c_int, C.c_bool, C.POINTER(C.c_uint8)], C.c_int64),
         "msplat_trainer_step_params": ([vp, C.c_int, C.c_int, C.c_int, C.POINTER(MsplatStepParams)], C.c_int),
         "msplat_trainer_last_camera": ([vp], C.c_int),
         "msplat_trainer_step_batch": ([vp, C.c_int], MsplatStats),

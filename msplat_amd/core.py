@@ -110,8 +110,8 @@ class Dataset:
     @classmethod
     def from_arrays(cls, cameras, gaussians: dict):
         """cameras: iterable of dicts with c2w (4x4, OpenGL), fx, fy, cx, cy, image (H, W, 3
-        uint8) and optional test flag; gaussians: means, scales (log), quats, opac (logit),
-        f_dc, f_rest, degree."""
+        uint8), optional test flag and optional mask (H, W uint8 training mask, white = use);
+        gaussians: means, scales (log), quats, opac (logit), f_dc, f_rest, degree."""
         L = _lib.lib()
         self = cls.__new__(cls)
         self._h = L.msplat_dataset_create_empty()
@@ -125,6 +125,8 @@ class Dataset:
                                             img.ctypes.data_as(C.POINTER(C.c_uint8)), bool(cam.get("test", False)))
             if r < 0:
                 raise RuntimeError("could not add camera")
+            if cam.get("mask") is not None:
+                self.set_mask(r, cam["mask"], bool(cam.get("test", False)))
         g = {k: np.ascontiguousarray(gaussians[k], np.float32) for k in
              ("means", "scales", "quats", "opac", "f_dc", "f_rest")}
         n = g["means"].shape[0]
@@ -165,6 +167,18 @@ class Dataset:
                                          img.ctypes.data_as(C.POINTER(C.c_uint8)))
             out["image"] = img
         return out
+
+    def set_mask(self, index: int, mask: np.ndarray, test: bool = False) -> int:
+        """Training mask (H, W uint8, white = use, black = ignore) of one camera, binarised at
+        one half; an all-white mask is dropped. Returns the count of kept pixels."""
+        m = np.ascontiguousarray(mask, np.uint8)
+        info = self.camera(index, test, image=False)
+        if m.shape != (info["height"], info["width"]):
+            raise ValueError(f"mask shape {m.shape} != {(info['height'], info['width'])}")
+        r = _lib.lib().msplat_dataset_set_mask(self._h, int(index), bool(test), m.ctypes.data_as(C.POINTER(C.c_uint8)))
+        if r < 0:
+            raise RuntimeError("could not set the mask")
+        return int(r)
 
     def camera_pose(self, index: int) -> np.ndarray:
         out = np.zeros(16, np.float32)

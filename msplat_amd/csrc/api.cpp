@@ -50,6 +50,34 @@ struct DatasetImpl {
         list.push_back(c);
         return (int)list.size() - 1;
     }
+    // Training mask of a camera from host uint8 [h w] values (white = use, black = ignore),
+    // binarised as Camera::loadMask does (reference input_data.cpp:81-111: a value above
+    // one half keeps the pixel, stored as 255 / 0); an all-white mask is dropped. Returns
+    // the count of kept pixels.
+    int64_t set_mask(int index, bool is_test, const uint8_t* m) {
+        auto& list = is_test ? test : train;
+        if (index < 0 || index >= (int)list.size()) throw std::invalid_argument("camera index out of range");
+        TrainCamera& c = list[(size_t)index];
+        const size_t n = (size_t)c.width * c.height;
+        std::vector<uint8_t> keep(n);
+        int64_t kept = 0;
+        for (size_t i = 0; i < n; i++) {
+            const bool k = m[i] >= 128;  // v / 255 > 0.5
+            keep[i] = k ? 255 : 0;
+            kept += k ? 1 : 0;
+        }
+        if (kept == (int64_t)n) {
+            c.mask = nullptr;
+            c.unmasked = -1;
+            return kept;
+        }
+        gt.emplace_back(new DBuf());
+        gt.back()->ensure(n);
+        hip_check(hipMemcpy(gt.back()->p, keep.data(), n, hipMemcpyHostToDevice), "mask upload");
+        c.mask = gt.back()->as<uint8_t>();
+        c.unmasked = kept;
+        return kept;
+    }
 };
 
 static HostGaussians synth_set(int kind, uint64_t seed, int n, int degree) {
@@ -437,6 +465,13 @@ int msplat_dataset_add_camera(MsplatDataset ds, const float c2w[16], float fx, f
     MS_GUARD_BEGIN
     if (!ds || !c2w || !rgb) return -1;
     return DS(ds)->add_camera(c2w, fx, fy, cx, cy, width, height, rgb, nullptr, test);
+    MS_GUARD_END(-1)
+}
+
+int64_t msplat_dataset_set_mask(MsplatDataset ds, int index, bool test, const uint8_t* mask) {
+    MS_GUARD_BEGIN
+    if (!ds || !mask) return -1;
+    return DS(ds)->set_mask(index, test, mask);
     MS_GUARD_END(-1)
 }
 

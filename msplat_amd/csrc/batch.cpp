@@ -216,7 +216,7 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
     std::memcpy(h.data(), &hp, sizeof(hp));
     for (int k = 0; k < B; k++) {
         const BatchView& bv = views[order[k]];
-        const msk::ViewConsts vc = Engine::make_view(bv.cam, bv.gt, hp.background, hp.ssim_weight, bv.inv_n);
+        const msk::ViewConsts vc = Engine::make_view(bv.cam, bv.gt, hp.background, hp.ssim_weight, bv.inv_n, bv.mask);
         const msk::ViewConsts* d = reinterpret_cast<const msk::ViewConsts*>(block_.as<uint8_t>() + off_vc) + k;
         slots_[k]->bind_view(vc, d);
         std::memcpy(h.data() + off_vc + sizeof(vc) * k, &vc, sizeof(vc));
@@ -233,7 +233,7 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
         for (int k = 0; k < B; k++) {
             Engine& e = *slots_[k];
             const BatchView& bv = views[order[k]];
-            e.own_view(Engine::make_view(bv.cam, bv.gt, hp.background, hp.ssim_weight, bv.inv_n));
+            e.own_view(Engine::make_view(bv.cam, bv.gt, hp.background, hp.ssim_weight, bv.inv_n, bv.mask));
             e.front(P, du, true);
             e.raster_fwd_own();
             e.loss_to(0);

@@ -17,6 +17,7 @@ struct BatchView {
     MsplatCamera cam;
     const uint8_t* gt;   // device [H W 3]
     float inv_n;         // loss normalisation of this view
+    const uint8_t* mask = nullptr;  // device [H W] training mask or null
 };
 
 class BatchRunner {

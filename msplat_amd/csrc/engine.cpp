@@ -203,7 +203,7 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
 }
 
 msk::ViewConsts Engine::make_view(const MsplatCamera& cam, const uint8_t* gt, const float bg[3], float ssim_w,
-                                  float inv_n) {
+                                  float inv_n, const uint8_t* mask) {
     msk::ViewConsts v{};
     v.cam = cam;
     const msk::Geom g = msk::geom_of(cam.width, cam.height);
@@ -216,6 +216,7 @@ msk::ViewConsts Engine::make_view(const MsplatCamera& cam, const uint8_t* gt, co
     v.ssim_w = ssim_w;
     v.inv_n = inv_n;
     v.gt = gt;
+    v.mask = mask;
     return v;
 }
 
@@ -379,7 +380,7 @@ msk::Params params_of(const MsplatDeviceState& st) {
 
 
 int Engine::forward_backward(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt,
-                             const MsplatStepParams& hp, int slot) {
+                             const MsplatStepParams& hp, int slot, const uint8_t* mask) {
     if (st.N < 0 || st.degree < 0 || st.degree > 3 || cam.width <= 0 || cam.height <= 0 || !gt) return -1;
     if (slot < 0 || slot >= kLossSlots) return -1;
     if (st.sh_bf16 && (!st.f_dc_bf16 || (st.degree > 0 && !st.f_rest_bf16))) return -1;
@@ -391,7 +392,7 @@ int Engine::forward_backward(const MsplatDeviceState& st, const MsplatCamera& ca
     last_W = cam.width;
     last_H = cam.height;
     stage_event(-1);
-    upload_view(make_view(cam, gt, hp.background, hp.ssim_weight, hp.loss_inv_n), true);
+    upload_view(make_view(cam, gt, hp.background, hp.ssim_weight, hp.loss_inv_n, mask), true);
     front(P, hp.degrees_to_use, true);
     raster_fwd_own();
     loss_to(slot);
@@ -418,8 +419,8 @@ double Engine::loss_slot(int k, int W, int H) {
 }
 
 int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt,
-                       const MsplatStepParams& hp, bool debug) {
-    const int rc = forward_backward(st, cam, gt, hp, 0);
+                       const MsplatStepParams& hp, bool debug, const uint8_t* mask) {
+    const int rc = forward_backward(st, cam, gt, hp, 0, mask);
     if (rc != 0) return rc;
     const msk::Params P = params_of(st);
     const size_t n = (size_t)(st.N > 0 ? st.N : 1);
@@ -606,9 +607,14 @@ void msplat_engine_destroy(MsplatEngine* e) {
 
 int msplat_engine_train_step(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam, const uint8_t* gt,
                              const MsplatStepParams* hp, int debug) {
+    return msplat_engine_train_step_masked(e, st, cam, gt, nullptr, hp, debug);
+}
+
+int msplat_engine_train_step_masked(MsplatEngine* e, const MsplatDeviceState* st, const MsplatCamera* cam,
+                                    const uint8_t* gt, const uint8_t* mask, const MsplatStepParams* hp, int debug) {
     if (!e || !st || !cam || !hp) return -1;
     try {
-        return e->e->train_step(*st, *cam, gt, *hp, debug != 0);
+        return e->e->train_step(*st, *cam, gt, *hp, debug != 0, mask);
     } catch (const std::exception& ex) {
         fprintf(stderr, "msplat: %s\n", ex.what());
         return -2;

@@ -47,13 +47,14 @@ public:
     // shared: run on this stream (owned by the caller) instead of a stream of its own.
     explicit Engine(hipStream_t shared = nullptr);
     ~Engine();
+    // mask: device uint8 [H W] training mask (pixels below 128 own no loss) or null.
     int train_step(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt, const MsplatStepParams& hp,
-                   bool debug);
+                   bool debug, const uint8_t* mask = nullptr);
     // The view-dependent part of a step: gradient accumulators cleared, forward, loss (into
     // loss slot `slot`), raster backward. Leaves the projection, records and raster
     // gradients of this view in the engine's buffers.
     int forward_backward(const MsplatDeviceState& st, const MsplatCamera& cam, const uint8_t* gt,
-                         const MsplatStepParams& hp, int slot);
+                         const MsplatStepParams& hp, int slot, const uint8_t* mask = nullptr);
     // Adds this view's projection VJP and SH colour gradients (after its backward).
     void accumulate(const MsplatDeviceState& st, int degrees_to_use, msk::BatchGrads& acc);
     // Loss slot k as the mean over its view's pixels (waits for the stream).
@@ -80,7 +81,7 @@ public:
     // Buffers for N Gaussians and launch geometry `launch`; `view` is this view's own.
     void prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool train);
     static msk::ViewConsts make_view(const MsplatCamera& cam, const uint8_t* gt, const float bg[3], float ssim_w,
-                                     float inv_n);
+                                     float inv_n, const uint8_t* mask = nullptr);
     // The view's constants live at `d` (already on the device, written by the caller).
     void bind_view(const msk::ViewConsts& h, const msk::ViewConsts* d);
     // The view's constants in the engine's own block (with its raster view table), so that

@@ -63,6 +63,7 @@ struct ViewConsts {
     float bg[3];
     float ssim_w, inv_n;
     const uint8_t* gt;   // [H W 3] target, uint8
+    const uint8_t* mask; // [H W] training mask (< 128: the pixel owns no loss) or null
 };
 
 // Launch geometry: a view's own extents (eager launches) or the largest extents of the

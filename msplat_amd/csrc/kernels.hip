@@ -1346,7 +1346,8 @@ void raster_forward(hipStream_t s, const ListBufs& L, const ViewConsts* vc, cons
 }
 
 // ------------------------------------------------------------------------------------
-// A12-A14: SSIM + L1 loss forward and backward (metal:3388-3450, 3563-3681, 3764-3850).
+// A12-A14: SSIM + L1 loss forward and backward (metal:3388-3450, 3563-3681, 3764-3850),
+// with the optional training mask (metal:3602-3653, 3815-3834).
 // Intermediate buffers are planar ([q][H][W]) so every store of a 64-lane wave is one
 // contiguous 256-byte run.
 constexpr int SB = 16;
@@ -1369,6 +1370,7 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
     const int gx = (W + SB - 1) / SB;
     if ((int)blockIdx.x >= gx || (int)blockIdx.y * SB >= H) return;
     const uint8_t* __restrict__ gt = vc->gt;
+    const uint8_t* __restrict__ mask = vc->mask;
     const float ssim_w = vc->ssim_w;
     constexpr int TD = SB + 10;  // 26
     constexpr int TI = SB + 20;  // 36: input columns of the horizontal pass over the halo
@@ -1447,15 +1449,17 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
             const float iCD = 1.0f / (Cd * D);
             const float dmu = 2.0f * Bv * (mu_x * Cd - A * mu_y) / (Cd * Cd * D);
             const float dsyq = -A * Bv * iCD / D, dsxy = 2.0f * A * iCD;
-            sf[0][dy][dx] = dmu - 2.0f * mu_y * dsyq - mu_x * dsxy;
-            sf[1][dy][dx] = 2.0f * dsyq;
-            sf[2][dy][dx] = dsxy;
-            if (dx >= 5 && dx < 5 + SB) {
-                const int gpx = bx + dx, gpy = by + dy + 5;
-                if (gpx >= 0 && gpx < W && gpy >= 0 && gpy < H) {
-                    s_ssim[dy][dx - 5] += (A * Bv) / (Cd * D);
-                    s_l1[dy][dx - 5] += fabsf(sg[c][dy + 5][dx + 5] - sr[c][dy + 5][dx + 5]);
-                }
+            // training mask: a masked centre pixel owns no loss, so its derivative fields
+            // and its loss terms are multiplied by zero (by one otherwise: exact)
+            const int gpx = bx + dx, gpy = by + dy + 5;
+            const bool inimg = gpx >= 0 && gpx < W && gpy >= 0 && gpy < H;
+            const float mval = (mask && inimg && mask[(size_t)gpy * W + gpx] < 128) ? 0.f : 1.f;
+            sf[0][dy][dx] = mval * (dmu - 2.0f * mu_y * dsyq - mu_x * dsxy);
+            sf[1][dy][dx] = mval * (2.0f * dsyq);
+            sf[2][dy][dx] = mval * dsxy;
+            if (dx >= 5 && dx < 5 + SB && inimg) {
+                s_ssim[dy][dx - 5] += mval * (A * Bv) / (Cd * D);
+                s_l1[dy][dx - 5] += mval * fabsf(sg[c][dy + 5][dx + 5] - sr[c][dy + 5][dx + 5]);
             }
         }
         __syncthreads();
@@ -1503,6 +1507,7 @@ __global__ __launch_bounds__(256) void k_ssim_vbwd(const ViewConsts* __restrict_
     const int W = vc->W, H = vc->H;
     if ((int)blockIdx.x * FW >= W || (int)blockIdx.y * SB >= H) return;
     const uint8_t* __restrict__ gt = vc->gt;
+    const uint8_t* __restrict__ mask = vc->mask;
     const float ssim_w = vc->ssim_w, inv_n = vc->inv_n;
     const int bx = blockIdx.x * FW, by = blockIdx.y * SB - 5;
     const size_t P = (size_t)W * H;
@@ -1520,6 +1525,9 @@ __global__ __launch_bounds__(256) void k_ssim_vbwd(const ViewConsts* __restrict_
         const int px = bx + lx, py = blockIdx.y * SB + ly;
         if (px >= W || py >= H) continue;
         const size_t pid = (size_t)py * W + px;
+        // a masked pixel drops its own L1 gradient; SSIM gradients of its neighbours'
+        // windows still flow through it
+        const float m_center = (mask && mask[pid] < 128) ? 0.f : 1.f;
         for (int c = 0; c < 3; c++) {
             float f1 = 0.f, f2 = 0.f, f3 = 0.f;
 #pragma unroll
@@ -1532,7 +1540,7 @@ __global__ __launch_bounds__(256) void k_ssim_vbwd(const ViewConsts* __restrict_
             const float rv = img[3 * pid + c];
             const float gv = (float)gt[3 * pid + c] * (1.0f / 255.0f);
             const float vs = f1 + rv * f2 + gv * f3;
-            const float vl = (gv > rv) ? -1.0f : ((gv < rv) ? 1.0f : 0.0f);
+            const float vl = m_center * ((gv > rv) ? -1.0f : ((gv < rv) ? 1.0f : 0.0f));
             float v = inv_n * (-ssim_w * vs + (1.0f - ssim_w) * vl);
             if (!ms_finite(v)) {
                 atomicAdd(&nanc[6], 1u);

@@ -257,7 +257,9 @@ MsplatStepParams Model::step_params(const TrainCamera& cam, int step, float ssim
     hp.beta2 = os_.beta2;
     hp.eps = os_.eps;
     hp.ssim_weight = ssim_weight;
-    hp.loss_inv_n = 1.0f / (float)(cam.height * cam.width * 3);
+    // masked training normalises by the pixels that own loss (reference model.cpp:929-934)
+    hp.loss_inv_n = (cam.mask && cam.unmasked > 0) ? 1.0f / (float)(cam.unmasked * 3)
+                                                   : 1.0f / (float)(cam.height * cam.width * 3);
     hp.op_reg = step >= 500 ? opt_.opacity_reg : 0.0f;
     hp.sc_reg = step >= 500 ? opt_.scale_reg : 0.0f;
     hp.reg_invis_scale = 1.0f;
@@ -277,7 +279,7 @@ void Model::full_iteration(const TrainCamera& cam, int step, float ssim_weight) 
     const MsplatCamera c = prepare(cam);
     os_.adam_step_count++;
     const MsplatStepParams hp = step_params(cam, step, ssim_weight, os_.adam_step_count);
-    const int rc = eng_.train_step(state(), c, cam.gt, hp, false);
+    const int rc = eng_.train_step(state(), c, cam.gt, hp, false, (cam.mask && cam.unmasked > 0) ? cam.mask : nullptr);
     if (rc != 0) throw std::runtime_error("training step rejected its inputs (rc=" + std::to_string(rc) + ")");
     if (prop_.enabled) proposal_accumulate(eng_);
 }
@@ -299,7 +301,9 @@ void Model::full_iteration_batch(const TrainCamera* const* cams, int B, int step
         const TrainCamera& cam = *cams[v];
         views[v].cam = prepare(cam);
         views[v].gt = cam.gt;
-        views[v].inv_n = 1.0f / (float)(cam.height * cam.width * 3 * B);
+        const bool masked = cam.mask && cam.unmasked > 0;
+        views[v].mask = masked ? cam.mask : nullptr;
+        views[v].inv_n = 1.0f / (float)((masked ? cam.unmasked : (int64_t)cam.height * cam.width) * 3 * B);
         hp = step_params(cam, step, ssim_weight, os_.adam_step_count);
     }
     // large-batch learning-rate scaling of the Adam steps (1 = the single-view rates)

@@ -22,6 +22,10 @@ struct TrainCamera {
     float fx = 0, fy = 0, cx = 0, cy = 0;
     int width = 0, height = 0;
     const uint8_t* gt = nullptr;
+    // training mask (device uint8 [H W], 255 use / 0 ignore) and its count of used pixels;
+    // the step is masked only when both are set (mask && unmasked > 0)
+    const uint8_t* mask = nullptr;
+    int64_t unmasked = -1;
 };
 
 // Host copy of a Gaussian set, row-major: means[n,3], scales[n,3] (log), quats[n,4] (w,x,y,z),

@@ -133,8 +133,13 @@ class Engine:
         except Exception:
             pass
 
-    def train_step(self, st: DeviceState, cam, gt: DeviceArray, hp, debug=False):
-        rc = self.L.msplat_engine_train_step(self.h, st.struct(), cam, gt.ptr, hp, 1 if debug else 0)
+    def train_step(self, st: DeviceState, cam, gt: DeviceArray, hp, debug=False, mask: DeviceArray = None):
+        """One step; mask: optional device uint8 [H, W] training mask (below 128: ignored)."""
+        if mask is None:
+            rc = self.L.msplat_engine_train_step(self.h, st.struct(), cam, gt.ptr, hp, 1 if debug else 0)
+        else:
+            rc = self.L.msplat_engine_train_step_masked(self.h, st.struct(), cam, gt.ptr, mask.ptr, hp,
+                                                        1 if debug else 0)
         if rc != 0:
             raise RuntimeError(f"msplat_engine_train_step failed ({rc})")
 

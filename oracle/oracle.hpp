@@ -545,12 +545,18 @@ inline bool ssim_loss_counted(int gx, int gy, int W, int H) {
 // Zero padding; the derivative fields at columns outside the image come from zeroed
 // statistics (f1 = 0, f2 = -2/C2, f3 = 2/C2) and reach pixels within 5 columns of the
 // left/right borders through the horizontal transpose convolution (reference behaviour,
-// reproduced). Returns the loss sum in double. Pins: SSIM map against an independent
-// numpy zero-padded SSIM; gradient by FP64 finite differences away from side borders,
-// and the border term against its closed form.
+// reproduced). Returns the loss sum in double. Optional training mask [H W] (metal:
+// 3602-3653, 3815-3834): a pixel whose mask value is below 128 has its derivative fields and
+// its SSIM and L1 loss terms multiplied by zero (its loss still adds w) and its own L1
+// gradient dropped. Pins: SSIM map against an independent numpy zero-padded SSIM;
+// gradient by FP64 finite differences away from side borders, and the border term against
+// its closed form; the masked loss and gradient against the same numpy formulation with
+// masked pixels' terms removed (finite differences of that objective).
 template <class T>
-double ssim_loss(int W, int H, const T* rend, const uint8_t* gt, T ssim_w, T inv_n, std::vector<T>& v_rend) {
+double ssim_loss(int W, int H, const T* rend, const uint8_t* gt, T ssim_w, T inv_n, std::vector<T>& v_rend,
+                 const uint8_t* mask = nullptr) {
     const size_t P = (size_t)W * H;
+    auto M = [&](int y, int x) -> T { return (mask && mask[(size_t)y * W + x] < 128) ? T(0) : T(1); };
     std::vector<T> hb(15 * P);
     auto X = [&](int y, int x, int c) -> T { return T((float)gt[((size_t)y * W + x) * 3 + c] * (1.0f / 255.0f)); };
     auto Y = [&](int y, int x, int c) -> T { return rend[((size_t)y * W + x) * 3 + c]; };
@@ -603,13 +609,14 @@ double ssim_loss(int W, int H, const T* rend, const uint8_t* gt, T ssim_w, T inv
                 T iCD = T(1) / (Cd * D);
                 T dmu = T(2) * B * (mu_x * Cd - A * mu_y) / (Cd * Cd * D);
                 T dsyq = -A * B * iCD / D, dsxy = T(2) * A * iCD;
-                fo[0] = dmu - T(2) * mu_y * dsyq - mu_x * dsxy;
-                fo[1] = T(2) * dsyq;
-                fo[2] = dsxy;
+                const T mv = (xx >= 0 && xx < W) ? M(y, xx) : T(1);
+                fo[0] = mv * (dmu - T(2) * mu_y * dsyq - mu_x * dsxy);
+                fo[1] = mv * (T(2) * dsyq);
+                fo[2] = mv * dsxy;
                 if (xx >= 0 && xx < W) {
-                    ssim_sum_c[c] = (A * B) / (Cd * D);
+                    ssim_sum_c[c] = mv * (A * B) / (Cd * D);
                     T d = X(y, xx, c) - Y(y, xx, c);
-                    l1_c[c] = std::fabs(d);
+                    l1_c[c] = mv * std::fabs(d);
                 }
             }
             if (xx >= 0 && xx < W) {
@@ -653,7 +660,7 @@ double ssim_loss(int W, int H, const T* rend, const uint8_t* gt, T ssim_w, T inv
                 }
                 T rv = Y(y, x, c), gv = X(y, x, c);
                 T v_ssim = cf[0] + rv * cf[1] + gv * cf[2];
-                T v_l1 = (gv > rv) ? T(-1) : ((gv < rv) ? T(1) : T(0));
+                T v_l1 = M(y, x) * ((gv > rv) ? T(-1) : ((gv < rv) ? T(1) : T(0)));
                 T v = inv_n * (-ssim_w * v_ssim + (T(1) - ssim_w) * v_l1);
                 if (!t_finite(v)) v = T(0);
                 v_rend[((size_t)y * W + x) * 3 + c] = v;
@@ -1012,7 +1019,8 @@ template <class T> struct StepOut {
 };
 
 template <class T>
-void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const MsplatStepParams& hp, StepOut<T>& O) {
+void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const MsplatStepParams& hp, StepOut<T>& O,
+                const uint8_t* mask = nullptr) {
     const int N = S.N, W = cam.width, H = cam.height;
     const int nb = num_sh_bases(S.degree) - 1;
     std::vector<T> dc_h, rest_h;
@@ -1023,7 +1031,7 @@ void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const M
     build_lists<T>(N, O.proj, W, H, O.lists);
     uint32_t K = k_max_rule(O.lists.capacity, O.lists.num_tiles);
     raster_forward<T>(O.lists, W, H, hp.background, K, O.raster);
-    O.loss_sum = ssim_loss<T>(W, H, O.raster.img.data(), gt, T(hp.ssim_weight), T(hp.loss_inv_n), O.v_rend);
+    O.loss_sum = ssim_loss<T>(W, H, O.raster.img.data(), gt, T(hp.ssim_weight), T(hp.loss_inv_n), O.v_rend, mask);
     raster_backward<T>(O.lists, O.raster, hp.background, O.v_rend.data(), N, O.rg);
     O.v_mean.assign(3 * (size_t)N, T(0));
     O.v_scale.assign(3 * (size_t)N, T(0));
@@ -1101,7 +1109,8 @@ void train_step(State<T>& S, const MsplatCamera& cam, const uint8_t* gt, const M
 // Multi-view batched step (a feature of this build; the reference trains one view per
 // step). Every view renders the same parameters; its raster, SSIM and projection gradients
 // are summed over the views in view order (hp[v].loss_inv_n carries 1 / (3 H_v W_v B), so
-// the objective is the mean of the per-view losses), and one update follows: Adam on the
+// the objective is the mean of the per-view losses; with a training mask, 1 / (3 U_v B) for
+// U_v unmasked pixels), and one update follows: Adam on the
 // DC and rest coefficients of Gaussians that received a finite colour gradient in at least
 // one view (bases up to degrees_to_use), the regulariser, Adam on means / scales / quats /
 // opacity over all N, then SGLD. With one view it is train_step.
@@ -1113,7 +1122,7 @@ template <class T> struct BatchOut {
 
 template <class T>
 void train_step_batch(State<T>& S, const MsplatCamera* cams, const uint8_t* const* gts, const MsplatStepParams* hps,
-                      int B, BatchOut<T>& O) {
+                      int B, BatchOut<T>& O, const uint8_t* const* masks = nullptr) {
     const int N = S.N;
     const int nb = num_sh_bases(S.degree) - 1;
     const MsplatStepParams& hp = hps[0];
@@ -1136,7 +1145,8 @@ void train_step_batch(State<T>& S, const MsplatCamera* cams, const uint8_t* cons
         build_lists<T>(N, V.proj, W, H, V.lists);
         const uint32_t K = k_max_rule(V.lists.capacity, V.lists.num_tiles);
         raster_forward<T>(V.lists, W, H, hp.background, K, V.raster);
-        V.loss_sum = ssim_loss<T>(W, H, V.raster.img.data(), gts[v], T(hp.ssim_weight), T(hps[v].loss_inv_n), V.v_rend);
+        V.loss_sum = ssim_loss<T>(W, H, V.raster.img.data(), gts[v], T(hp.ssim_weight), T(hps[v].loss_inv_n), V.v_rend,
+                                  masks ? masks[v] : nullptr);
         raster_backward<T>(V.lists, V.raster, hp.background, V.v_rend.data(), N, V.rg);
         O.loss_mean += V.loss_sum / ((double)W * H) / B;
 #pragma omp parallel for schedule(static) reduction(+ : nan3)

@@ -83,20 +83,21 @@ int oracle_set_threads(int n) {
 #endif
 }
 
+// One training step; mask (uint8 [H W], pixels below 128 own no loss) may be null.
 int oracle_train_step_f32(OracleStateF32* s, const MsplatCamera* cam, const uint8_t* gt, const MsplatStepParams* hp,
-                          OracleOutF32* o) {
+                          OracleOutF32* o, const uint8_t* mask) {
     State<float> S = make_state<float>(s);
     StepOut<float> O;
-    train_step<float>(S, *cam, gt, *hp, O);
+    train_step<float>(S, *cam, gt, *hp, O, mask);
     export_out(O, o, true);
     return O.lists.overflow ? 1 : 0;
 }
 
 int oracle_train_step_f64(OracleStateF64* s, const MsplatCamera* cam, const uint8_t* gt, const MsplatStepParams* hp,
-                          OracleOutF64* o) {
+                          OracleOutF64* o, const uint8_t* mask) {
     State<double> S = make_state<double>(s);
     StepOut<double> O;
-    train_step<double>(S, *cam, gt, *hp, O);
+    train_step<double>(S, *cam, gt, *hp, O, mask);
     export_out(O, o, true);
     return O.lists.overflow ? 1 : 0;
 }
@@ -117,7 +118,8 @@ int oracle_render_f64(OracleStateF64* s, const MsplatCamera* cam, int degrees_to
     return O.lists.overflow ? 1 : 0;
 }
 
-// SSIM + L1 loss on a given image (used to pin the loss against an independent SSIM).
+// SSIM + L1 loss on a given image (used to pin the loss against an independent SSIM); the
+// masked variants take a uint8 [H W] training mask.
 double oracle_ssim_loss_f64(int W, int H, const double* rend, const uint8_t* gt, double w, double inv_n, double* v_rend) {
     std::vector<double> v;
     double l = ssim_loss<double>(W, H, rend, gt, w, inv_n, v);
@@ -127,6 +129,20 @@ double oracle_ssim_loss_f64(int W, int H, const double* rend, const uint8_t* gt,
 double oracle_ssim_loss_f32(int W, int H, const float* rend, const uint8_t* gt, float w, float inv_n, float* v_rend) {
     std::vector<float> v;
     double l = ssim_loss<float>(W, H, rend, gt, w, inv_n, v);
+    if (v_rend) std::memcpy(v_rend, v.data(), v.size() * sizeof(float));
+    return l;
+}
+double oracle_ssim_loss_masked_f64(int W, int H, const double* rend, const uint8_t* gt, const uint8_t* mask, double w,
+                                   double inv_n, double* v_rend) {
+    std::vector<double> v;
+    double l = ssim_loss<double>(W, H, rend, gt, w, inv_n, v, mask);
+    if (v_rend) std::memcpy(v_rend, v.data(), v.size() * sizeof(double));
+    return l;
+}
+double oracle_ssim_loss_masked_f32(int W, int H, const float* rend, const uint8_t* gt, const uint8_t* mask, float w,
+                                   float inv_n, float* v_rend) {
+    std::vector<float> v;
+    double l = ssim_loss<float>(W, H, rend, gt, w, inv_n, v, mask);
     if (v_rend) std::memcpy(v_rend, v.data(), v.size() * sizeof(float));
     return l;
 }
@@ -166,14 +182,15 @@ void oracle_eq9(float o_old, const float* s_old, int n, float* o_new, float* s_n
 }
 
 // Batched step (FP32 or FP64 state); outputs per view: image and raster gradients when the
-// pointers are set; returns the mean of the per-view losses.
+// pointers are set; masks (per view uint8 [H W] or null; the array itself may be null);
+// returns the mean of the per-view losses.
 #define ORACLE_BATCH(SUF, NAME, FT)                                                                         \
     double oracle_train_step_batch_##NAME(OracleState##SUF* s, const MsplatCamera* cams,                    \
                                          const uint8_t* const* gts, const MsplatStepParams* hps, int B,     \
-                                         FT* const* imgs, FT* const* v_xy) {                                \
+                                         FT* const* imgs, FT* const* v_xy, const uint8_t* const* masks) {   \
         State<FT> S = make_state<FT>(s);                                                                    \
         BatchOut<FT> O;                                                                                     \
-        train_step_batch<FT>(S, cams, gts, hps, B, O);                                                      \
+        train_step_batch<FT>(S, cams, gts, hps, B, O, masks);                                               \
         for (int v = 0; v < B; v++) {                                                                       \
             if (imgs && imgs[v]) cp(imgs[v], O.views[v].raster.img);                                        \
             if (v_xy && v_xy[v]) cp(v_xy[v], O.views[v].rg.v_xy);                                          \

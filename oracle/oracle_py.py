@@ -58,9 +58,9 @@ def lib():
         from msplat_amd._lib import MsplatCamera, MsplatStepParams  # struct layouts only
         L = C.CDLL(LIB_PATH)
         L.oracle_train_step_f32.argtypes = [C.POINTER(StateF32), C.POINTER(MsplatCamera), C.c_void_p,
-                                            C.POINTER(MsplatStepParams), C.POINTER(OutF32)]
+                                            C.POINTER(MsplatStepParams), C.POINTER(OutF32), C.c_void_p]
         L.oracle_train_step_f64.argtypes = [C.POINTER(StateF64), C.POINTER(MsplatCamera), C.c_void_p,
-                                            C.POINTER(MsplatStepParams), C.POINTER(OutF64)]
+                                            C.POINTER(MsplatStepParams), C.POINTER(OutF64), C.c_void_p]
         L.oracle_render_f32.argtypes = [C.POINTER(StateF32), C.POINTER(MsplatCamera), C.c_int,
                                         C.POINTER(C.c_float), C.POINTER(OutF32)]
         L.oracle_render_f64.argtypes = [C.POINTER(StateF64), C.POINTER(MsplatCamera), C.c_int,
@@ -71,6 +71,12 @@ def lib():
         L.oracle_ssim_loss_f32.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_float, C.c_float,
                                            C.c_void_p]
         L.oracle_ssim_loss_f32.restype = C.c_double
+        L.oracle_ssim_loss_masked_f64.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+                                                  C.c_double, C.c_void_p]
+        L.oracle_ssim_loss_masked_f64.restype = C.c_double
+        L.oracle_ssim_loss_masked_f32.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                                                  C.c_float, C.c_void_p]
+        L.oracle_ssim_loss_masked_f32.restype = C.c_double
         L.oracle_sgld_noise_f32.argtypes = [C.c_int, C.c_uint32, C.c_void_p]
         L.oracle_pcg_hash32.argtypes = [C.c_uint32, C.c_uint32]
         L.oracle_pcg_hash32.restype = C.c_uint32
@@ -100,9 +106,9 @@ def lib():
         L.oracle_project_vjp_f64.argtypes = [C.c_int, vp, vp, vp, C.POINTER(MsplatCamera), vp, vp, vp, vp, vp, vp]
         L.oracle_sh_color_f64.argtypes = [C.c_int, vp, vp, vp, vp]
         L.oracle_ssim_window.argtypes = [vp]
-        L.oracle_train_step_batch_f32.argtypes = [C.POINTER(StateF32), vp, vp, vp, C.c_int, vp, vp]
+        L.oracle_train_step_batch_f32.argtypes = [C.POINTER(StateF32), vp, vp, vp, C.c_int, vp, vp, vp]
         L.oracle_train_step_batch_f32.restype = C.c_double
-        L.oracle_train_step_batch_f64.argtypes = [C.POINTER(StateF64), vp, vp, vp, C.c_int, vp, vp]
+        L.oracle_train_step_batch_f64.argtypes = [C.POINTER(StateF64), vp, vp, vp, C.c_int, vp, vp, vp]
         L.oracle_train_step_batch_f64.restype = C.c_double
         L.oracle_eval_sums.argtypes = [C.c_int, C.c_int, vp, vp, vp]
         _lib = L
@@ -157,7 +163,7 @@ class HostState:
         return {k: self.p[k][: self.N] for k in GROUP_SHAPES}
 
 
-def _run(fn_name, st: HostState, cam, gt, hp, want_ids=True, render_args=None, ids_cap=None):
+def _run(fn_name, st: HostState, cam, gt, hp, want_ids=True, render_args=None, ids_cap=None, mask=None):
     L = lib()
     f64 = st.dtype == np.float64
     O = OutF64 if f64 else OutF32
@@ -192,7 +198,7 @@ def _run(fn_name, st: HostState, cam, gt, hp, want_ids=True, render_args=None, i
     o.ids_cap = ids_cap
     s = st.struct()
     if train:
-        getattr(L, fn_name)(s, cam, gt.ctypes.data, hp, o)
+        getattr(L, fn_name)(s, cam, gt.ctypes.data, hp, o, None if mask is None else mask.ctypes.data)
     else:
         deg_use, bg = render_args
         bgarr = (C.c_float * 3)(*bg)
@@ -210,11 +216,13 @@ def _run(fn_name, st: HostState, cam, gt, hp, want_ids=True, render_args=None, i
     return out
 
 
-def train_step(st: HostState, cam, gt: np.ndarray, hp, want_ids=True):
-    """One training step on the host state (updated in place); returns the intermediates."""
+def train_step(st: HostState, cam, gt: np.ndarray, hp, want_ids=True, mask=None):
+    """One training step on the host state (updated in place); returns the intermediates.
+    mask: optional uint8 [H, W] training mask (pixels below 128 own no loss)."""
     gt = np.ascontiguousarray(gt, np.uint8)
+    mask = None if mask is None else np.ascontiguousarray(mask, np.uint8)
     return _run("oracle_train_step_f64" if st.dtype == np.float64 else "oracle_train_step_f32", st, cam, gt, hp,
-                want_ids)
+                want_ids, mask=mask)
 
 
 def render(st: HostState, cam, degrees_to_use: int, bg, want_ids=True, ids_cap=None):
@@ -222,9 +230,10 @@ def render(st: HostState, cam, degrees_to_use: int, bg, want_ids=True, ids_cap=N
                 want_ids, render_args=(degrees_to_use, bg), ids_cap=ids_cap)
 
 
-def train_step_batch(st: HostState, cams, gts, hps, want_images=False):
+def train_step_batch(st: HostState, cams, gts, hps, want_images=False, masks=None):
     """One multi-view batched step on the host state (updated in place): per-view cameras,
-    uint8 ground truths and step scalars. Returns (mean per-view loss, per-view images)."""
+    uint8 ground truths, step scalars and optional training masks (uint8 [H, W] or None per
+    view). Returns (mean per-view loss, per-view images)."""
     from msplat_amd._lib import MsplatCamera, MsplatStepParams
     L = lib()
     B = len(cams)
@@ -236,8 +245,12 @@ def train_step_batch(st: HostState, cams, gts, hps, want_images=False):
     imgs = [np.zeros((c.height, c.width, 3), st.dtype) for c in cams] if want_images else None
     img_ptrs = (C.c_void_p * B)(*[a.ctypes.data for a in imgs]) if want_images else None
     fn = L.oracle_train_step_batch_f64 if f64 else L.oracle_train_step_batch_f32
+    mk = None
+    if masks is not None:
+        masks = [None if m is None else np.ascontiguousarray(m, np.uint8) for m in masks]
+        mk = (C.c_void_p * B)(*[None if m is None else m.ctypes.data for m in masks])
     loss = fn(st.struct(), C.addressof(cam_arr), C.addressof(gt_ptrs), C.addressof(hp_arr), B,
-              C.addressof(img_ptrs) if want_images else None, None)
+              C.addressof(img_ptrs) if want_images else None, None, C.addressof(mk) if mk is not None else None)
     return loss, imgs
 
 

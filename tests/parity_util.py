@@ -110,18 +110,18 @@ def report_lines():
 GROUPS = ("means", "scales", "quats", "f_dc", "f_rest", "opac")
 
 
-def oracle_pair(g, moments, cam, gt, hp, sh_bf16=False, want_ids=True, fp32_ref=False):
+def oracle_pair(g, moments, cam, gt, hp, sh_bf16=False, want_ids=True, fp32_ref=False, mask=None):
     """One oracle step from the same state in FP32 (in the given SH storage mode) and in FP64.
     Returns (state32, out32, state64, out64[, state32 all-fp32, out32 all-fp32])."""
     import oracle_py as O
     h32 = O.HostState(g, moments=moments, sh_bf16=sh_bf16)
-    o32 = O.train_step(h32, cam, gt, hp, want_ids=want_ids)
+    o32 = O.train_step(h32, cam, gt, hp, want_ids=want_ids, mask=mask)
     h64 = O.HostState(g, dtype=np.float64, moments=moments)
-    o64 = O.train_step(h64, cam, gt, hp, want_ids=False)
+    o64 = O.train_step(h64, cam, gt, hp, want_ids=False, mask=mask)
     if not fp32_ref:
         return h32, o32, h64, o64
     hr = O.HostState(g, moments=moments)
-    orr = O.train_step(hr, cam, gt, hp, want_ids=False)
+    orr = O.train_step(hr, cam, gt, hp, want_ids=False, mask=mask)
     return h32, o32, h64, o64, hr, orr
 
 

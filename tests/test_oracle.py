@@ -92,6 +92,103 @@ def test_ssim_gradient_finite_differences_and_border_term():
         assert abs(v[yy, xx, c] - (fd + extra)) <= 2e-6 * abs(v[yy, xx, c]) + 1e-12
 
 
+def independent_masked_loss_sum(rend, gt_u8, w, keep):
+    """Sum over pixels of the masked pixel loss in FP64: a kept pixel adds
+    w (1 - mean_c SSIM) + (1 - w) mean_c |x - y|, a masked one adds only w (the reference's
+    loss value counts every pixel, msplat_metal.mm:1738, while the masked pixel's terms are
+    multiplied by zero)."""
+    x = gt_u8.astype(np.float64) / 255.0
+    y = rend.astype(np.float64)
+    g = gauss_window()
+
+    def blur(a):
+        a = ndimage.correlate1d(a, g, axis=1, mode="constant", cval=0.0)
+        return ndimage.correlate1d(a, g, axis=0, mode="constant", cval=0.0)
+    ss = np.zeros(x.shape[:2])
+    for c in range(3):
+        mx, my = blur(x[..., c]), blur(y[..., c])
+        sxx = np.maximum(blur(x[..., c] ** 2) - mx * mx, 0)
+        syy = np.maximum(blur(y[..., c] ** 2) - my * my, 0)
+        sxy = blur(x[..., c] * y[..., c]) - mx * my
+        ss += ((2 * mx * my + SSIM_C1) * (2 * sxy + SSIM_C2)) / ((mx * mx + my * my + SSIM_C1) * (sxx + syy + SSIM_C2))
+    l1 = np.abs(x - y).sum(-1)
+    k = keep.astype(np.float64)
+    return float(np.sum(w + k * (-w * ss / 3 + (1 - w) * l1 / 3)))
+
+
+def _mask_case(W, H, seed):
+    """A mask with a masked block, scattered masked pixels and values either side of 128."""
+    rng = np.random.default_rng(seed)
+    m = np.where(rng.random((H, W)) < 0.2, rng.integers(0, 128, (H, W)), rng.integers(128, 256, (H, W)))
+    m[H // 4: H // 2, W // 3: W // 2] = 127
+    m[0, 0], m[0, 1] = 128, 127
+    return m.astype(np.uint8)
+
+
+def test_ssim_masked_loss_matches_independent_formulation():
+    """oracle ssim_loss with a training mask (metal:3602-3653): masked pixels' terms are
+    dropped from the sum; the threshold is 128."""
+    L = O.lib()
+    rend, gt = _ssim_case(seed=3)
+    H, W = gt.shape[:2]
+    m = _mask_case(W, H, 4)
+    keep = m >= 128
+    assert 0 < keep.sum() < W * H
+    loss = L.oracle_ssim_loss_masked_f64(W, H, _p(rend), _p(gt), _p(m), 0.2, 1.0 / (3 * keep.sum()), None)
+    ref = independent_masked_loss_sum(rend, gt, 0.2, keep)
+    assert abs(loss - ref) < 1e-7 * ref
+    # the unmasked case of the masked entry point is the unmasked loss, bit for bit
+    ones = np.full((H, W), 255, np.uint8)
+    v1, v2 = np.zeros(rend.shape), np.zeros(rend.shape)
+    a = L.oracle_ssim_loss_masked_f64(W, H, _p(rend), _p(gt), _p(ones), 0.2, 1e-3, _p(v1))
+    b = L.oracle_ssim_loss_f64(W, H, _p(rend), _p(gt), 0.2, 1e-3, _p(v2))
+    assert a == b and np.array_equal(v1, v2)
+    # an all-black mask leaves w per pixel
+    zeros = np.zeros((H, W), np.uint8)
+    assert L.oracle_ssim_loss_masked_f64(W, H, _p(rend), _p(gt), _p(zeros), 0.2, 1e-3, None) == pytest.approx(
+        0.2 * W * H, rel=1e-12)
+
+
+def test_ssim_masked_gradient_finite_differences():
+    """The masked gradient (metal:3646-3648 fields, 3819-3834 L1 term) is the derivative of
+    the masked objective: finite differences of the independent numpy loss, plus the
+    reference's side-border term (unmasked: it comes from columns outside the image)."""
+    L = O.lib()
+    rend, gt = _ssim_case(seed=5)
+    H, W = gt.shape[:2]
+    m = _mask_case(W, H, 6)
+    keep = m >= 128
+    w, inv_n = 0.2, 1.0 / (3 * keep.sum())
+    v = np.zeros(rend.shape)
+    L.oracle_ssim_loss_masked_f64(W, H, _p(rend), _p(gt), _p(m), w, inv_n, _p(v))
+    g = gauss_window()
+    rng = np.random.default_rng(7)
+    eps = 1e-6
+    # samples at masked pixels, at kept pixels next to the masked block, and at random
+    pts = [(H // 4 + 2, W // 3 + 3), (H // 2 - 1, W // 2 - 1), (H // 4 - 1, W // 3), (H // 2, W // 2)]
+    pts += [(int(rng.integers(0, H)), int(rng.integers(0, W))) for _ in range(24)]
+    n_masked = 0
+    for yy, xx in pts:
+        c = int(rng.integers(0, 3))
+        r1 = rend.copy(); r1[yy, xx, c] += eps
+        r2 = rend.copy(); r2[yy, xx, c] -= eps
+        fd = (independent_masked_loss_sum(r1, gt, w, keep) - independent_masked_loss_sum(r2, gt, w, keep)) / (2 * eps)
+        fd *= 3 * inv_n
+        rows = sum(g[k] for k in range(11) if 0 <= yy - 5 + k < H)
+        cols = sum(g[k] for k in range(11) if not (0 <= xx - 5 + k < W))
+        x = gt[yy, xx, c] / 255.0
+        extra = inv_n * (-w) * (2.0 / SSIM_C2) * (x - rend[yy, xx, c]) * rows * cols
+        assert abs(v[yy, xx, c] - (fd + extra)) <= 5e-6 * abs(v[yy, xx, c]) + 1e-11, (yy, xx, c)
+        n_masked += int(not keep[yy, xx])
+    assert n_masked >= 2
+    # a masked pixel deep inside the block has no L1 term and no own SSIM term: its
+    # gradient is what its neighbours' windows give, with or without its own mask bit
+    m2 = m.copy(); m2[H // 4 + 2, W // 3 + 3] = 255
+    v2 = np.zeros(rend.shape)
+    L.oracle_ssim_loss_masked_f64(W, H, _p(rend), _p(gt), _p(m2), w, inv_n, _p(v2))
+    assert not np.allclose(v2[H // 4 + 2, W // 3 + 3], v[H // 4 + 2, W // 3 + 3])
+
+
 # ------------------------------------------------------------- raster (oracle raster fwd/bwd)
 def _raster_case(n, W, H, seed, single_tile=False):
     rng = np.random.default_rng(seed)
