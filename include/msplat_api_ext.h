@@ -104,13 +104,15 @@ int64_t msplat_trainer_proposal_read(MsplatTrainer t, const char* name, void* ho
  * configuration and replayed when msplat_trainer_set_option(t, "batch_graph", 1) is set.
  * Returns 0 / -1. */
 int msplat_trainer_batch_info(MsplatTrainer t, int64_t out[4]);
-/* Launch-geometry autotuner of the batched steps (msplat_trainer_set_option(t, "autotune",
- * 1)): the candidates timed so far (pixels per lane of the persistent raster kernels, their
- * resident workgroups, ms per megapixel of the batch) and, once tuning has finished, the chosen pair
- * in chosen[0..1] (zeros before). Returns the number of timed candidates (at most max
- * written), or -1. */
-int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* ppl, int32_t* grids, float* ms, int max,
-                                 int32_t chosen[2]);
+/* Launch autotuner of the batched steps (msplat_trainer_set_option(t, "autotune", 1)): the
+ * candidates timed so far, four int32 each in cands[4k..4k+3] = {pixels per lane of the
+ * persistent raster kernels (the raster sub-tile: 1 of a 256-lane workgroup per tile, or 4 of
+ * one wave per tile), their resident workgroups, rank chunks of the tile-list kernels
+ * (0 = default 768), waves per bucket workgroup (0 = default)}, with ms per megapixel of
+ * the batch in ms[k]; once tuning has finished the chosen candidate in chosen[0..3] (zeros
+ * before). Stage one times 7 raster choices, stage two 4 list choices around the best.
+ * Returns the number of timed candidates (at most max written), or -1. */
+int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* cands, float* ms, int max, int32_t chosen[4]);
 
 /* ---- Trainer introspection (tests, benchmarks) --------------------------------------- */
 /* The scalars Model::fullIteration would hand to the kernels for (camera, step, adam step). */

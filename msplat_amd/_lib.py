@@ -180,7 +180,7 @@ def bind(path: str) -> C.CDLL:
         "msplat_trainer_last_batch": ([vp, C.POINTER(i32), C.POINTER(f64), C.c_int], C.c_int),
         "msplat_trainer_batch_info": ([vp, C.POINTER(C.c_int64)], C.c_int),
         "msplat_trainer_proposal_info": ([vp, C.POINTER(f64)], C.c_int),
-        "msplat_trainer_autotune_info": ([vp, C.POINTER(i32), C.POINTER(i32), fp, C.c_int, C.POINTER(i32)],
+        "msplat_trainer_autotune_info": ([vp, C.POINTER(i32), fp, C.c_int, C.POINTER(i32)],
                                          C.c_int),
         "msplat_trainer_proposal_params": ([vp, fp, C.c_int], C.c_int),
         "msplat_trainer_proposal_read": ([vp, C.c_char_p, vp, C.c_int64], C.c_int64),

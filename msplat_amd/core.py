@@ -267,15 +267,15 @@ class GaussianTrainer:
         return out
 
     def autotune_info(self) -> dict:
-        """Timed (pixels per lane, workgroups, ms) candidates and the chosen pair of the
-        batched-step autotuner."""
-        p = (C.c_int32 * 16)()
-        g = (C.c_int32 * 16)()
-        ms = np.zeros(16, np.float32)
-        ch = (C.c_int32 * 2)()
-        n = _lib.lib().msplat_trainer_autotune_info(self._h, p, g, fptr(ms), 16, ch)
-        return {"timed": [(int(p[k]), int(g[k]), float(ms[k])) for k in range(max(n, 0))],
-                "chosen": (int(ch[0]), int(ch[1]))}
+        """Timed candidates of the batched-step autotuner, each (pixels per lane, raster
+        workgroups, list rank chunks, bucket waves, ms per megapixel), and the chosen
+        (pixels per lane, workgroups, chunks, waves) once tuning has finished."""
+        cands = (C.c_int32 * (4 * 32))()
+        ms = np.zeros(32, np.float32)
+        ch = (C.c_int32 * 4)()
+        n = _lib.lib().msplat_trainer_autotune_info(self._h, cands, fptr(ms), 32, ch)
+        return {"timed": [tuple(int(cands[4 * k + j]) for j in range(4)) + (float(ms[k]),) for k in range(max(n, 0))],
+                "chosen": tuple(int(ch[j]) for j in range(4))}
 
     def batch_info(self) -> dict:
         """Recorded-graph counters of the batched steps (msplat_trainer_batch_info)."""

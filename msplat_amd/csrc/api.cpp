@@ -635,19 +635,28 @@ int msplat_trainer_batch_info(MsplatTrainer t, int64_t out[4]) {
     return 0;
 }
 
-int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* ppl, int32_t* grids, float* ms, int max, int32_t chosen[2]) {
+int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* cands, float* ms, int max, int32_t chosen[4]) {
     if (!t) return -1;
     msplat_hip::BatchRunner* b = TR(t)->model->batch_runner();
+    if (chosen)
+        for (int j = 0; j < 4; j++) chosen[j] = 0;
     if (!b) return 0;
     const auto& r = b->tune_results();
     for (int k = 0; k < (int)r.size() && k < max; k++) {
-        if (ppl) ppl[k] = r[k].ppl;
-        if (grids) grids[k] = r[k].grid;
+        if (cands) {
+            cands[4 * k + 0] = r[k].c.ppl;
+            cands[4 * k + 1] = r[k].c.grid;
+            cands[4 * k + 2] = r[k].c.chunks;
+            cands[4 * k + 3] = r[k].c.waves;
+        }
         if (ms) ms[k] = r[k].ms;
     }
-    if (chosen) {
-        chosen[0] = b->tuned() ? b->raster_ppl() : 0;
-        chosen[1] = b->tuned() ? b->persist_grid() : 0;
+    if (chosen && b->tuned()) {
+        const auto c = b->choice();
+        chosen[0] = c.ppl;
+        chosen[1] = c.grid;
+        chosen[2] = c.chunks;
+        chosen[3] = c.waves;
     }
     return (int)r.size();
 }

@@ -13,7 +13,8 @@
 namespace msplat_hip {
 
 bool BatchRunner::Key::operator==(const Key& o) const {
-    if (N != o.N || B != o.B || du != o.du || degree != o.degree || bf16 != o.bf16 || grid != o.grid || ppl != o.ppl)
+    if (N != o.N || B != o.B || du != o.du || degree != o.degree || bf16 != o.bf16 || grid != o.grid || ppl != o.ppl ||
+        chunks != o.chunks || waves != o.waves)
         return false;
     if (g.W != o.g.W || g.H != o.g.H || g.T != o.g.T || gens != o.gens || nanc != o.nanc) return false;
     if (slot_geom != o.slot_geom) return false;
@@ -53,6 +54,21 @@ void BatchRunner::set_persist_grid(int g) {
 void BatchRunner::set_raster_ppl(int p) {
     ppl_ = p == 4 ? 4 : 1;
     for (auto& e : slots_) e->set_raster_ppl(ppl_);
+}
+
+void BatchRunner::set_list_choice(int chunks, int waves) {
+    chunks_ = chunks > 0 ? chunks : 0;
+    bucket_waves_ = (waves == 1 || waves == 2 || waves == 4) ? waves : 0;
+    for (auto& e : slots_) {
+        e->set_list_chunks(chunks_);
+        e->set_list_bucket_waves(bucket_waves_);
+    }
+}
+
+void BatchRunner::apply(const Choice& c) {
+    set_raster_ppl(c.ppl);
+    set_persist_grid(c.grid);
+    set_list_choice(c.chunks, c.waves);
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -109,9 +125,19 @@ void BatchRunner::set_autotune(bool on) {
     tune_cands_.clear();
     if (on) {
         const int cus = default_persist_grid() / 8;  // the default is 8 workgroups per CU
-        for (int k : {2, 4, 8, 16}) tune_cands_.push_back({1, cus * k});
-        for (int k : {8, 16, 32}) tune_cands_.push_back({4, cus * k});
+        for (int k : {2, 4, 8, 16}) tune_cands_.push_back({1, cus * k, 0, 0});
+        for (int k : {8, 16, 32}) tune_cands_.push_back({4, cus * k, 0, 0});
     }
+    tune_stage1_ = (int)tune_cands_.size();
+}
+
+// Stage two's candidates around the best of stage one (whose default list choice is
+// already timed): rank chunks 384 / 1536 (default 768) and 4 waves per bucket workgroup.
+static std::vector<BatchRunner::Choice> stage_two(const BatchRunner::Choice& b) {
+    std::vector<BatchRunner::Choice> out;
+    for (const auto& cw : {std::make_pair(384, 0), std::make_pair(1536, 0), std::make_pair(0, 4), std::make_pair(384, 4)})
+        out.push_back({b.ppl, b.grid, cw.first, cw.second});
+    return out;
 }
 
 // Each candidate runs kTuneSteps steps: the first records its graph (untimed), the others
@@ -126,8 +152,7 @@ void BatchRunner::run(const MsplatDeviceState& st, const BatchView* views, int B
         return;
     }
     const int c = tune_step_ / kTuneSteps;
-    set_raster_ppl(tune_cands_[c].first);
-    set_persist_grid(tune_cands_[c].second);
+    apply(tune_cands_[c]);
     const bool timed = (tune_step_ % kTuneSteps) != 0;
     if (timed) {
         if (!tev_[0]) {
@@ -144,17 +169,25 @@ void BatchRunner::run(const MsplatDeviceState& st, const BatchView* views, int B
         hip_check(hipEventElapsedTime(&ms, tev_[0], tev_[1]), "hipEventElapsedTime");
         double px = 0.0;
         for (auto& wh : sizes_) px += (double)wh.first * wh.second;
-        if ((int)tune_ms_.size() <= c) tune_ms_.push_back({tune_cands_[c].first, tune_cands_[c].second, 0.f});
+        if ((int)tune_ms_.size() <= c) tune_ms_.push_back({tune_cands_[c], 0.f});
         tune_ms_[c].ms += (float)(ms / (px * 1e-6));  // ms per megapixel, summed over the timed steps
     }
-    if (++tune_step_ >= kTuneSteps * (int)tune_cands_.size()) {
-        for (auto& r : tune_ms_) r.ms /= (float)(kTuneSteps - 1);
+    auto best_of = [&](int upto) {
         TuneResult best = tune_ms_.front();
-        for (auto& r : tune_ms_)
-            if (r.ms < best.ms) best = r;
-        set_raster_ppl(best.ppl);
-        set_persist_grid(best.grid);
-        tune_done_ = true;
+        for (int k = 0; k < upto; k++)
+            if (tune_ms_[k].ms < best.ms) best = tune_ms_[k];
+        return best;
+    };
+    if (++tune_step_ >= kTuneSteps * (int)tune_cands_.size()) {
+        if ((int)tune_cands_.size() == tune_stage1_) {
+            // stage one done: averages, then stage two around its best
+            for (auto& r : tune_ms_) r.ms /= (float)(kTuneSteps - 1);
+            for (const Choice& ch : stage_two(best_of(tune_stage1_).c)) tune_cands_.push_back(ch);
+        } else {
+            for (int k = tune_stage1_; k < (int)tune_ms_.size(); k++) tune_ms_[k].ms /= (float)(kTuneSteps - 1);
+            apply(best_of((int)tune_ms_.size()).c);
+            tune_done_ = true;
+        }
     }
 }
 
@@ -167,6 +200,8 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
         slots_.emplace_back(new Engine(s_));
         slots_.back()->set_persist_grid(persist_grid_);
         slots_.back()->set_raster_ppl(ppl_);
+        slots_.back()->set_list_chunks(chunks_);
+        slots_.back()->set_list_bucket_waves(bucket_waves_);
     }
     for (int v = 0; v < B; v++) {
         const MsplatCamera& c = views[v].cam;
@@ -196,9 +231,11 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
         }
         hw_ = hwv_[0];
     }
-    // per-slot buffers: the slot's recorded geometry (graph) or its view's own
+    // per-slot buffers: the slot's recorded geometry (graph) or its view's own; a recorded
+    // slot buckets its lists by tile-count class (its views may be smaller than its geometry)
     for (int k = 0; k < B; k++) {
         const msk::Geom g = msk::geom_of(sizes_[k].first, sizes_[k].second);
+        slots_[k]->set_size_classes(graphable);
         slots_[k]->prepare(st.N, graphable ? hwv_[k] : g, g, true);
     }
     int Tmax = 0;
@@ -257,6 +294,8 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
     k.bf16 = st.sh_bf16;
     k.grid = persist_grid_;
     k.ppl = ppl_;
+    k.chunks = chunks_;
+    k.waves = bucket_waves_;
     k.g = hw_;
     k.slot_geom.clear();
     for (int j = 0; j < B; j++) {

@@ -41,19 +41,31 @@ public:
     // Work-queue residency of the persistent raster kernels (0 = default).
     void set_persist_grid(int g);
     int persist_grid() const { return persist_grid_; }
-    // Autotuner of the launch geometry the results do not depend on (pixels per lane and
-    // residency of the persistent raster kernels): the first steps cycle through the
-    // candidates, each timed per megapixel over three steps after the one that records its
-    // graph, and the fastest is kept for the remaining steps.
+    // Autotuner of the launch choices the results do not depend on. Stage one: pixels per
+    // lane (the raster sub-tile a lane covers: 1 pixel of a 256-lane workgroup per tile, or
+    // 2x2 pixels of one wave per tile) x residency of the persistent raster kernels. Stage
+    // two, with the best of stage one: rank chunks of the tile-list count / bucket kernels
+    // x waves per bucket workgroup. The first steps cycle through the candidates, each timed
+    // per megapixel over three steps after the one that records its graph, and the fastest
+    // is kept for the remaining steps.
     void set_autotune(bool on);
     bool tuned() const { return tune_done_; }
+    struct Choice {
+        int ppl = 1, grid = 0, chunks = 0, waves = 0;
+        bool operator==(const Choice& o) const {
+            return ppl == o.ppl && grid == o.grid && chunks == o.chunks && waves == o.waves;
+        }
+    };
     struct TuneResult {
-        int ppl, grid;
+        Choice c;
         float ms;
     };
     const std::vector<TuneResult>& tune_results() const { return tune_ms_; }
+    Choice choice() const { return {ppl_, persist_grid_, chunks_, bucket_waves_}; }
     void set_raster_ppl(int p);
     int raster_ppl() const { return ppl_; }
+    // Rank chunks of the tile-list kernels and waves per bucket workgroup (0: defaults).
+    void set_list_choice(int chunks, int waves);
     // Counters: steps replayed from a recorded graph, graphs recorded.
     int64_t replays() const { return replays_; }
     int64_t captures() const { return captures_; }
@@ -64,7 +76,7 @@ public:
 
 private:
     struct Key {
-        int N = -1, B = 0, du = -1, degree = -1, bf16 = 0, grid = 0, ppl = 1;
+        int N = -1, B = 0, du = -1, degree = -1, bf16 = 0, grid = 0, ppl = 1, chunks = 0, waves = 0;
         msk::Geom g{};
         std::vector<int> slot_geom;   // recorded (W, H) per slot
         std::vector<int64_t> gens;    // (buffer generation, depth-chunk count) per slot
@@ -93,9 +105,11 @@ private:
     hipGraphExec_t exec_ = nullptr;
     bool autotune_ = false, tune_done_ = false;
     int tune_step_ = 0;
-    std::vector<std::pair<int, int>> tune_cands_;  // (pixels per lane, grid)
+    std::vector<Choice> tune_cands_;
     std::vector<TuneResult> tune_ms_;
-    int ppl_ = 1;
+    int tune_stage1_ = 0;  // candidates of stage one
+    int ppl_ = 1, chunks_ = 0, bucket_waves_ = 0;
+    void apply(const Choice& c);
     hipEvent_t tev_[2] = {nullptr, nullptr};
     void run_inner(const MsplatDeviceState& st, const BatchView* views, int B, const MsplatStepParams& hp,
                    msk::BatchGrads A, uint32_t* nan_counts);

@@ -184,7 +184,7 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
     ensure(dkeys_, n * 4); ensure(dkeys_out_, n * 4); ensure(dvals_, n * 4); ensure(sorted_ids_, n * 4);
     ensure(temp_, msk::lists_temp_bytes(N));
     ensure(ids_, (size_t)capacity_ * 4);
-    ensure(hist_, msk::lists_hist_bytes(N, T));
+    ensure(hist_, msk::lists_hist_bytes(N, T, chunk_target_));
     ensure(img_, P * 12); ensure(finalT_, P * 4); ensure(finalIdx_, P * 4); ensure(ncontrib_, P * 4);
     ensure(wq_, (size_t)T * 4);
     ensure(wmisc_, 64);
@@ -362,6 +362,9 @@ msk::ListBufs Engine::list_bufs() {
     L.sorted_ids = sorted_ids_.as<uint32_t>();
     L.temp = temp_.p;
     L.temp_bytes = temp_.cap;
+    L.chunk_target = chunk_target_;
+    L.bucket_waves = bucket_waves_;
+    L.size_classes = size_classes_;
     return L;
 }
 

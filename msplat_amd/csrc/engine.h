@@ -108,6 +108,13 @@ public:
     // Width of the tile-list keys: 8 carries the Gaussian id with the depth rank, 4 looks
     // it up after the per-tile sort (the lists are the same).
     int raster_ppl() const { return ppl_; }
+    // Tile-list launch choices (the lists are the same for every choice): rank chunks of the
+    // count and bucket kernels (0: default), waves per bucket workgroup (0: process-wide
+    // default), and one bucket launch per tile-count class (for recorded sequences that serve
+    // views smaller than their launch geometry).
+    void set_list_chunks(int c) { chunk_target_ = c > 0 ? c : 0; }
+    void set_list_bucket_waves(int w) { bucket_waves_ = (w == 1 || w == 2 || w == 4) ? w : 0; }
+    void set_size_classes(bool on) { size_classes_ = on; }
 
 private:
     void upload_view(const msk::ViewConsts& h, bool train);
@@ -126,6 +133,8 @@ private:
     int64_t gen_ = 0;   // bumps whenever a buffer is re-allocated
     int persist_grid_ = 0;
     int ppl_ = 1;
+    int chunk_target_ = 0, bucket_waves_ = 0;
+    bool size_classes_ = false;
     // view constants: host copy, device copy in use, own device block [ViewConsts | RasterView]
     msk::ViewConsts vch_{};
     const msk::ViewConsts* vcd_ = nullptr;

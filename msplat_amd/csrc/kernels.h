@@ -52,6 +52,13 @@ struct ListBufs {
     ushort4* srect;            // [N] tile rectangles in depth-rank order
     void* temp;                // radix-sort scratch
     size_t temp_bytes;
+    // launch choices (the lists are the same for every choice)
+    int chunk_target = 0;      // rank chunks of the count / bucket kernels (0: the default 768)
+    int bucket_waves = 0;      // waves per bucket workgroup (0: the process-wide choice)
+    // one bucket launch per tile-count class, each with the LDS of its class, of which only
+    // the one holding the view's tile count does work: for a recorded launch sequence that
+    // serves views smaller than its launch geometry
+    bool size_classes = false;
 };
 
 // Constants of one view of a step, read by the kernels from device memory, so that a
@@ -127,7 +134,7 @@ struct RasterWork {
 void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, ProjOut& o,
                      uint32_t* nan_counts);
 size_t lists_temp_bytes(int N);
-size_t lists_hist_bytes(int N, int T);
+size_t lists_hist_bytes(int N, int T, int chunk_target = 0);
 bool lists_supported(int N, int T);
 void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L);
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);

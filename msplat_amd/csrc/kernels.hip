@@ -11,6 +11,7 @@
 //
 // Reference kernels are cited as metal:<line> (core/metal/msplat_metal.metal).
 #include <atomic>
+#include <climits>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -622,10 +623,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
                                                             const uint32_t* __restrict__ exact_off,
                                                             const uint32_t* __restrict__ counts,
                                                             const uint32_t* __restrict__ totals,
-                                                            uint32_t* __restrict__ ids, uint32_t cap) {
+                                                            uint32_t* __restrict__ ids, uint32_t cap, int tmin,
+                                                            int tmax) {
     extern __shared__ uint32_t sm[];
     const int chunk = (int)blockIdx.x;
     const int tbx = vc->tbx, T = vc->T;
+    if (T < tmin || T > tmax) return;  // another launch's tile-count class
     const int T4 = (T + 3) & ~3;
     uint32_t* cs = sm;        // [T] start of this chunk's segment of each tile
     uint32_t* pw = sm + T4;   // [T][WAVES / 2] per-wave cursors, 16 bits each
@@ -723,6 +726,12 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
             qw = *reinterpret_cast<const uint2*>(srect + r);
             gid = sorted_ids[r];
         }
+        // Loads and stores share one completion counter: the first read of gid inside the
+        // walk below would wait for every store issued so far, once per step. Consuming the
+        // loaded values here makes that one wait per 64 ranks (measured 7% faster at 1.5M
+        // Gaussians, 1600x1040). The branch is never taken: a rectangle's x0 is below its
+        // x1 <= 65535 (or the rectangle is all zeros), so its first word is never all ones.
+        if (__builtin_amdgcn_readfirstlane((int)(qw.x & qw.y & gid)) == -1) return;
         const int nr = min(64, w1 - b0);
         for (int j0 = 0; j0 < nr; j0 += 4) {  // wave-uniform
             int x0[4], y0[4], wd[4], area[4];
@@ -786,10 +795,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const 
 // up to about one chunk per resident bucket workgroup) so that the bucket kernel's serial
 // walk per wave stays short: 50k Gaussians make 391 chunks of 128, not 26 of 1924.
 constexpr int MIN_CHUNK = 128, CHUNK_TARGET = 768;
-static int hist_blocks(int N) {
+static int hist_blocks(int N, int target = 0) {
     const int by_max = (N + MAX_CHUNK - 1) / MAX_CHUNK;
     const int by_min = (N + MIN_CHUNK - 1) / MIN_CHUNK;
-    const int nb = std::max(by_max, std::min(by_min, CHUNK_TARGET));
+    const int nb = std::max(by_max, std::min(by_min, target > 0 ? target : CHUNK_TARGET));
     return nb < 1 ? 1 : nb;
 }
 
@@ -940,7 +949,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
 size_t lists_temp_bytes(int N) { return ((size_t)256 * (size_t)rs_blocks(N > 0 ? N : 1) + 256 + 64) * 4; }
 
 // [chunks x T] histogram / bucket offsets
-size_t lists_hist_bytes(int N, int T) { return (size_t)hist_blocks(N) * (size_t)T * 4; }
+size_t lists_hist_bytes(int N, int T, int chunk_target) { return (size_t)hist_blocks(N, chunk_target) * (size_t)T * 4; }
 
 static size_t g_lds_limit = 0;
 static size_t lds_limit() {
@@ -1007,7 +1016,7 @@ static void allow_lds(const void* fn, size_t lds, const char* what) {
 
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N > 0) {
-        const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
+        const int nblk = hist_blocks(N, L.chunk_target), G = (N + nblk - 1) / nblk;
         const size_t lds = (size_t)g.T * 4;
         allow_lds((const void*)k_tile_hist, lds, "tile histogram LDS");
         hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.srect, vc, L.hist);
@@ -1021,26 +1030,46 @@ void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, 
                        (uint32_t)L.capacity);
 }
 
+// One launch for the views whose tile count lies in [tmin, tmax], with the LDS of tmax tiles.
 template <int WAVES>
-static void launch_bucket(hipStream_t s, int nblk, int N, int G, const ViewConsts* vc, const Geom& g, ListBufs& L) {
-    const size_t lds = bucket_lds_bytes(g.T, WAVES);
+static void launch_bucket(hipStream_t s, int nblk, int N, int G, const ViewConsts* vc, int tiles, int tmin, int tmax,
+                          ListBufs& L) {
+    const size_t lds = bucket_lds_bytes(tiles, WAVES);
     allow_lds((const void*)k_tile_bucket<WAVES>, lds, "tile bucket LDS");
     hipLaunchKernelGGL(k_tile_bucket<WAVES>, dim3(nblk), dim3(64 * WAVES), lds, s, N, G, L.sorted_ids, L.srect, vc,
-                       L.hist, L.exact_off, L.tile_counts, L.totals, L.ids, (uint32_t)L.capacity);
+                       L.hist, L.exact_off, L.tile_counts, L.totals, L.ids, (uint32_t)L.capacity, tmin, tmax);
     check_launch("tile bucket launch");
 }
 
 static std::atomic<int> g_bucket_waves{0};
 void set_bucket_waves(int waves) { g_bucket_waves.store(waves == 1 || waves == 2 || waves == 4 ? waves : 0); }
 
+static void bucket_class(hipStream_t s, int nblk, int N, int G, const ViewConsts* vc, int tiles, int tmin, int tmax,
+                         ListBufs& L) {
+    switch (bucket_waves(tiles, lds_limit(), L.bucket_waves > 0 ? L.bucket_waves : g_bucket_waves.load())) {
+        case 4: launch_bucket<4>(s, nblk, N, G, vc, tiles, tmin, tmax, L); break;
+        case 2: launch_bucket<2>(s, nblk, N, G, vc, tiles, tmin, tmax, L); break;
+        case 1: launch_bucket<1>(s, nblk, N, G, vc, tiles, tmin, tmax, L); break;
+        default: throw std::runtime_error("tile count exceeds the bucket kernel's LDS");
+    }
+}
+
 void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N <= 0) return;
-    const int nblk = hist_blocks(N), G = (N + nblk - 1) / nblk;
-    switch (bucket_waves(g.T, lds_limit(), g_bucket_waves.load())) {
-        case 4: launch_bucket<4>(s, nblk, N, G, vc, g, L); break;
-        case 2: launch_bucket<2>(s, nblk, N, G, vc, g, L); break;
-        case 1: launch_bucket<1>(s, nblk, N, G, vc, g, L); break;
-        default: throw std::runtime_error("tile count exceeds the bucket kernel's LDS");
+    const int nblk = hist_blocks(N, L.chunk_target), G = (N + nblk - 1) / nblk;
+    if (!L.size_classes) {
+        bucket_class(s, nblk, N, G, vc, g.T, 0, INT_MAX, L);
+        return;
+    }
+    // tile-count classes (0, 1024], (1024, 2048], ... up to the launch geometry's: the LDS of
+    // a launch (and so its resident workgroups) follows the class, not the largest view
+    // the sequence serves; the launches of the other classes exit at once
+    int lo = 0;
+    for (int hi = 1024;; hi *= 2) {
+        const int top = std::min(hi, g.T);
+        bucket_class(s, nblk, N, G, vc, top, lo + 1, top >= g.T ? INT_MAX : top, L);
+        if (top >= g.T) break;
+        lo = top;
     }
 }
 

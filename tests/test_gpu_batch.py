@@ -100,18 +100,26 @@ def test_sequential_batch_matches_oracle():
 
 
 def test_autotuned_graph_steps_match_oracle():
-    """Every autotuner candidate (persistent raster residency) trains identically."""
+    """Every autotuner candidate trains identically: stage one's raster choices (pixels per
+    lane x persistent residency), then stage two's list choices (rank chunks x bucket waves)
+    around the best, all recorded as graphs whose bucket launches are split by tile-count
+    class (views of 96x80 and 640x480 alternate in each slot: 30 and 1200 tiles, classes
+    (0, 1024] and (1024, 1200])."""
     g, hidden, _ = small_scene(n=2500, n_hidden=1200)
-    ds, cams = _uneven_dataset(g, hidden, n_cams=6, sizes=[(320, 320), (96, 80)])
+    ds, cams = _uneven_dataset(g, hidden, n_cams=6, sizes=[(640, 480), (96, 80)])
     tr = GaussianTrainer(ds, TrainingConfig(iterations=1000, sh_degree_interval=1, cap_max=2 * 2500))
     tr.set_option("batch_graph", 1)
     tr.set_option("autotune", 1)
-    _check_batched_steps(tr, cams, 2, tuple(range(1, 30)))
+    _check_batched_steps(tr, cams, 2, tuple(range(1, 48)))
     info = tr.autotune_info()
-    cands = [(p, g) for p, g, _ in info["timed"]]
-    assert len(cands) == 7 and len(set(cands)) == 7 and {p for p, _ in cands} == {1, 4}
+    cands = [c[:4] for c in info["timed"]]
+    assert len(cands) == 11 and len(set(cands)) == 11
+    assert {c[0] for c in cands[:7]} == {1, 4} and all(c[2:] == (0, 0) for c in cands[:7])
+    best1 = min(info["timed"][:7], key=lambda c: c[4])[:2]
+    assert all(c[:2] == best1 for c in cands[7:])
+    assert {c[2:] for c in cands[7:]} == {(384, 0), (1536, 0), (0, 4), (384, 4)}
     assert info["chosen"] in cands
-    assert all(ms > 0 for _, _, ms in info["timed"])
+    assert all(c[4] > 0 for c in info["timed"])
 
 
 def test_four_pixels_per_lane_match_oracle():
