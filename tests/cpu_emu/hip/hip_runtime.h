@@ -235,6 +235,12 @@ inline int __builtin_amdgcn_readlane(int v, int src) {
     return (int)(uint32_t)emu_exchange((uint64_t)(uint32_t)v, (unsigned)src & 63u);
 }
 
+// v_readfirstlane_b32: the value of the lowest lane that reaches the call
+inline int __builtin_amdgcn_readfirstlane(int v) {
+    const uint64_t live = __ballot(1);
+    return __builtin_amdgcn_readlane(v, live ? __builtin_ctzll(live) : 0);
+}
+
 template <class T> inline T __shfl_up(T v, unsigned d, int width = 64) {
     uint64_t b = 0;
     std::memcpy(&b, &v, sizeof(T));
