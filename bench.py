@@ -52,7 +52,7 @@ def parse():
     p.add_argument("--stage-steps", type=int, default=10)
     p.add_argument("--calib-copy-mb", type=int, default=0,
                    help="launch the plain copy kernel once on this many MiB (PMC traffic calibration)")
-    p.add_argument("--suite", default="c1", choices=["c1", "c0", "c1_full", "c2", "c3_bf16", "c4", "bf16_error"],
+    p.add_argument("--suite", default="c1", choices=["c1", "c0", "c1_full", "c2", "c3_bf16", "c4", "n1", "bf16_error"],
                    help="c1: the headline it/s line (default); the others print their own JSON line")
     p.add_argument("--sh-bf16", action="store_true", help="bfloat16 SH storage for the c1 suite")
     p.add_argument("--bucket-waves", type=int, default=0, choices=[0, 1, 2, 4],

@@ -24,6 +24,13 @@ c4          learned relocation proposal: 2M clustered Gaussians at 1600x1040, 8 
             all from the same state, evaluated about every 100 views; steps and seconds for
             each to reach the plain form's final held-out PSNR within 0.1 dB, and the
             trajectories.
+n1          the learned relocation proposal over many relocation events: 8-view batched steps
+            on 2M clustered Gaussians at 1600x1040 (cap 3M), relocation every 100 steps from
+            step 500 to the last step (1700 by default: 13 events), three arms from the same
+            initial state and camera order (plain relocation, learned forced once trained,
+            learned judged by held-out loss), replicated over three initial Gaussian sets
+            (paired seeds); held-out PSNR after every event, the paired differences, and the
+            network's losses on unseen outcomes against the best constant predictor's.
 bf16_error  error against the FP64 oracle of the fp32 and the bf16-SH GPU paths, after 1
             step (image, loss, raster gradients, updated parameters) and after 2000 steps
             (parameters, render, loss), on a small scene (20k Gaussians, 128x128, 8 views).
@@ -284,6 +291,56 @@ def c2(a):
                                                           res["sequential"]["views_per_s_after_60"], 3),
                       "note": "same initial state and camera order in every mode; relocation off "
                               "(fixed N)"}), flush=True)
+
+
+def n1(a):
+    from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, _lib
+    L = _lib.lib()
+    n, views = 2_000_000, 8
+    steps = int(os.environ.get("MSPLAT_N1_STEPS", "1700"))
+    seeds = [int(x) for x in os.environ.get("MSPLAT_N1_SEEDS", "1,2,3").split(",")]
+    arms = {"plain": None, "learned": 2, "judged": 0}
+    out = {}
+    for seed in seeds:
+        ds = Dataset(f"synthetic:c1:n={n},cams=72,seed={seed}", eval_mode=True, test_every=9)
+        res = {}
+        for arm, mode in arms.items():
+            tr = GaussianTrainer(ds, TrainingConfig(iterations=30000, cap_max=int(1.5 * n)))
+            tr.set_option("batch_graph", 1)
+            tr.set_option("batch_lr_scale", float(np.sqrt(views)))
+            if mode is not None:
+                tr.set_option("learned_relocation", 1)
+                tr.set_option("relocation_mode", mode)
+            traj = []
+            for s in range(1, steps + 1):
+                tr.step_batch(views)
+                if s >= 500 and (s % 100 == 0 or s == steps):   # just after each relocation event
+                    m = tr.evaluate()
+                    traj.append({"step": s, "psnr": m["psnr"], "gaussians": m["num_gaussians"]})
+            r = {"trajectory": traj, "final_psnr": traj[-1]["psnr"]}
+            if mode is not None:
+                r["proposal"] = tr.proposal_info()
+            res[arm] = r
+            _log(f"n1 seed {seed} {arm}: final {r['final_psnr']:.3f} dB {r.get('proposal', '')}")
+            del tr
+        out[seed] = res
+        del ds
+    def paired(arm):
+        d = [out[sd][arm]["final_psnr"] - out[sd]["plain"]["final_psnr"] for sd in seeds]
+        ev = [b["psnr"] - p["psnr"] for sd in seeds
+              for b, p in zip(out[sd][arm]["trajectory"], out[sd]["plain"]["trajectory"])]
+        return {"final_delta_db": [round(x, 4) for x in d], "final_delta_mean_db": round(float(np.mean(d)), 4),
+                "per_event_delta_mean_db": round(float(np.mean(ev)), 4),
+                "events_better": int(sum(x > 0 for x in ev)), "events": len(ev)}
+    print(json.dumps({"suite": "n1", "gaussians": n, "views_per_step": views, "steps": steps, "seeds": seeds,
+                      "relocation_events_per_run": sum(1 for s in range(500, steps + 1, 100)),
+                      "learned_vs_plain": paired("learned"), "judged_vs_plain": paired("judged"),
+                      "network_vs_constant": {str(sd): {k: out[sd]["learned"]["proposal"][k]
+                                                        for k in ("val_bce", "base_bce", "val_off", "base_off",
+                                                                  "learned_events", "events")} for sd in seeds},
+                      "judged_learned_events": {str(sd): out[sd]["judged"]["proposal"]["learned_events"]
+                                                for sd in seeds},
+                      "runs": out}), flush=True)
 
 
 def c4(a):
