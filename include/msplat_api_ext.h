@@ -36,8 +36,13 @@ void msplat_quantize_rgb(const float* src, int64_t n, uint8_t* dst);
  * reference's viewer builds them (core/src/msplat_api.mm:486-528). */
 void msplat_camera_from_fov(const float c2w[16], float fov_y, int width, int height, MsplatCamera* out);
 
-/* ---- Datasets built from memory -------------------------------------------------------
- * msplat_dataset_create also accepts "synthetic:<preset>[:key=value,...]":
+/* ---- Datasets ---------------------------------------------------------------------------
+ * msplat_dataset_create reads scene folders (COLMAP / Nerfstudio / Polycam, msplat_io.h):
+ * every camera's image decoded, resized and undistorted as Camera::loadImage does
+ * (downscaleFactor > 1 shrinks by area), uint8 ground truth on the device, every
+ * testEvery-th camera held out in eval mode, and the initial Gaussians seeded from the
+ * point cloud (kNN scales x 0.1, opacity 0.5, random rotations, DC colour from the points).
+ * It also accepts "synthetic:<preset>[:key=value,...]":
  *   presets  c0  (50k uniform-cube Gaussians, 16 cameras 256x256, f = 256)
  *            c1  (1.5M 64-component mixture, 64 jittered cameras 1600x1040, f = 1300)
  *   keys     n, hidden, cams, w, h, f, kind (0 cube / 1 mixture), seed, hidden_seed,
@@ -58,6 +63,13 @@ int msplat_dataset_add_camera(MsplatDataset ds, const float c2w[16], float fx, f
  * no pixel trains as unmasked, as the reference's step does. Returns the count of kept
  * pixels, or -1. */
 int64_t msplat_dataset_set_mask(MsplatDataset ds, int index, bool test, const uint8_t* mask);
+/* Camera::loadMask for every camera of a scene-folder dataset that has a name relative to
+ * its image folder (COLMAP): "<mask_dir>/<name>.png", else <name> with its extension
+ * replaced by .png, area-resized to the camera's size, kept above one half (input_data.cpp:
+ * 77-111); cameras without a file, or whose mask keeps every pixel, stay unmasked. The
+ * reference's CLI does this for --mask-dir (cli/msplat.mm:153-163). Returns the number of
+ * cameras masked, or -1. */
+int64_t msplat_dataset_load_masks(MsplatDataset ds, const char* mask_dir);
 /* Initial Gaussians (row-major host arrays; f_rest [n, (degree+1)^2-1, 3]). Returns 0 / -1. */
 int msplat_dataset_set_init_gaussians(MsplatDataset ds, int n, int degree, const float* means, const float* log_scales,
                                       const float* quats, const float* opac_logits, const float* f_dc,

@@ -174,6 +174,25 @@ def bind(path: str) -> C.CDLL:
                                         C.POINTER(C.c_uint8)], C.c_int),
         "msplat_dataset_set_init_gaussians": ([vp, C.c_int, C.c_int, fp, fp, fp, fp, fp, fp], C.c_int),
         "msplat_dataset_set_mask": ([vp, C.c_int, C.c_bool, C.POINTER(C.c_uint8)], C.c_int64),
+        "msplat_dataset_load_masks": ([vp, C.c_char_p], C.c_int64),
+        # host-side ingestion (msplat_io.h)
+        "msplat_image_decode": ([C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), vp], C.c_int),
+        "msplat_image_resize_area": ([vp, C.c_int, C.c_int, vp, C.c_int, C.c_int], C.c_int),
+        "msplat_image_undistort": ([vp, C.c_int, C.c_int, f32, f32, f32, f32, vp, vp, C.POINTER(C.c_int),
+                                    C.POINTER(C.c_int), vp], C.c_int),
+        "msplat_scene_load": ([C.c_char_p, C.c_char_p], vp),
+        "msplat_scene_free": ([vp], None),
+        "msplat_scene_num_cameras": ([vp], C.c_int),
+        "msplat_scene_camera": ([vp, C.c_int, vp, vp, vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_char_p, C.c_int,
+                                 C.c_char_p, C.c_int], C.c_int),
+        "msplat_scene_transform": ([vp, C.POINTER(C.c_float), vp], C.c_int),
+        "msplat_scene_num_points": ([vp], C.c_int64),
+        "msplat_scene_points": ([vp, vp, vp], C.c_int),
+        "msplat_scene_load_image": ([vp, C.c_int, f32], C.c_int),
+        "msplat_scene_image": ([vp, C.c_int, vp], C.c_int),
+        "msplat_scene_load_mask": ([vp, C.c_int, C.c_char_p, vp], C.c_int64),
+        "msplat_knn_mean_distance": ([vp, C.c_int64, C.c_int, vp], C.c_int),
+        "msplat_seed_gaussians": ([vp, vp, C.c_int64, C.c_int, f32, f32, vp, vp, vp, vp, vp, vp], C.c_int),
         "msplat_trainer_step_params": ([vp, C.c_int, C.c_int, C.c_int, C.POINTER(MsplatStepParams)], C.c_int),
         "msplat_trainer_last_camera": ([vp], C.c_int),
         "msplat_trainer_step_batch": ([vp, C.c_int], MsplatStats),

@@ -98,7 +98,8 @@ class TrainingStats:
 class Dataset:
     """Cameras with uint8 ground truth on the GPU (reference _core.pyi:68-90).
 
-    `path` is "synthetic:<preset>[:key=value,...]" (see include/msplat_api_ext.h); use
+    `path` is a scene folder (COLMAP, Nerfstudio or Polycam; include/msplat_io.h) or
+    "synthetic:<preset>[:key=value,...]" (include/msplat_api_ext.h); use
     Dataset.from_arrays for data already in memory."""
 
     def __init__(self, path: str, downscale_factor: float = 1.0, eval_mode: bool = False, test_every: int = 8):
@@ -167,6 +168,15 @@ class Dataset:
                                          img.ctypes.data_as(C.POINTER(C.c_uint8)))
             out["image"] = img
         return out
+
+    def load_masks(self, mask_dir: str) -> int:
+        """Training masks of a scene-folder dataset from `mask_dir` ("<name>.png" or the
+        name with its extension replaced by .png, white = use); returns the number of
+        cameras masked."""
+        r = _lib.lib().msplat_dataset_load_masks(self._h, str(mask_dir).encode())
+        if r < 0:
+            raise RuntimeError("could not load the masks (reason on stderr)")
+        return int(r)
 
     def set_mask(self, index: int, mask: np.ndarray, test: bool = False) -> int:
         """Training mask (H, W uint8, white = use, black = ignore) of one camera, binarised at

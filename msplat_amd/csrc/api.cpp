@@ -3,6 +3,7 @@
 // viewer 399-720). Every entry point catches C++ exceptions and reports failure through its
 // return value, so nothing unwinds across the C boundary.
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -17,6 +18,8 @@
 #include "../../include/msplat_api_ext.h"
 #include "../../include/msplat_c_api.h"
 #include "eval.h"
+#include "image_io.h"
+#include "ingest.h"
 #include "model.h"
 
 namespace msplat_hip {
@@ -26,6 +29,7 @@ namespace msplat_hip {
 
 struct DatasetImpl {
     std::vector<TrainCamera> train, test;
+    std::vector<std::string> train_rel, test_rel;  // names relative to the image folder (masks)
     std::vector<std::unique_ptr<DBuf>> gt;  // owns every camera's device ground truth
     HostGaussians init;
     bool has_init = false;
@@ -48,6 +52,7 @@ struct DatasetImpl {
         c.gt = gt.back()->as<uint8_t>();
         auto& list = is_test ? test : train;
         list.push_back(c);
+        (is_test ? test_rel : train_rel).push_back(std::string());
         return (int)list.size() - 1;
     }
     // Training mask of a camera from host uint8 [h w] values (white = use, black = ignore),
@@ -417,14 +422,68 @@ static ViewerImpl* VW(MsplatViewer v) { return static_cast<ViewerImpl*>(v); }
 
 extern "C" {
 
+// A scene folder (Dataset::Dataset, msplat_api.mm:25-41): the format's loader, every
+// camera's image as Camera::loadImage prepares it (uint8 ground truth on the device), the
+// train / held-out split (every testEvery-th camera held out in eval mode, input_data.cpp:
+// 196-205) and the initial Gaussians Model::Model seeds from the point cloud (kNN scales
+// x 0.1, opacity 0.5, SH degree 3: the trainer adapts the degree).
+static DatasetImpl* folder_dataset(const std::string& path, float downscale, bool eval_mode, int test_every) {
+    Scene sc = load_scene(path);
+    if (sc.cams.empty()) throw std::runtime_error("no cameras in " + path);
+    if (sc.points.count() == 0) throw std::runtime_error("no point cloud in " + path + " to seed the Gaussians from");
+    if (eval_mode && test_every <= 0) throw std::invalid_argument("test_every must be positive");
+    std::unique_ptr<DatasetImpl> ds(new DatasetImpl());
+    for (size_t i = 0; i < sc.cams.size(); i++) {
+        SceneCamera& c = sc.cams[i];
+        const std::vector<uint8_t> rgb = load_camera_image(c, downscale);
+        const bool held_out = eval_mode && (int)(i % (size_t)test_every) == 0;
+        const int k = ds->add_camera(c.c2w, c.fx, c.fy, c.cx, c.cy, c.width, c.height, rgb.data(), nullptr, held_out);
+        (held_out ? ds->test_rel : ds->train_rel)[(size_t)k] = c.relative;
+    }
+    const int64_t n = sc.points.count();
+    if (n > INT32_MAX) throw std::runtime_error("point cloud too large");
+    HostGaussians g;
+    g.resize((int)n, 3);
+    seed_gaussians(sc.points, 3, 0.1f, 0.5f, g.means.data(), g.scales.data(), g.quats.data(), g.opac.data(),
+                   g.f_dc.data(), g.f_rest.data());
+    ds->init = std::move(g);
+    ds->has_init = true;
+    ds->scale = sc.scale;
+    for (int k = 0; k < 3; k++) ds->translation[k] = sc.translation[k];
+    return ds.release();
+}
+
 MsplatDataset msplat_dataset_create(const char* path, float downscaleFactor, bool evalMode, int testEvery) {
     MS_GUARD_BEGIN
     if (!path) return nullptr;
     const std::string p(path);
     if (p.rfind("synthetic:", 0) == 0) return synthetic_dataset(p, downscaleFactor, evalMode, testEvery);
-    throw std::runtime_error("on-disk dataset formats (COLMAP / Nerfstudio / Polycam) are not supported by this "
-                             "build; use a synthetic:<preset> path or msplat_dataset_create_empty()");
+    return folder_dataset(p, downscaleFactor, evalMode, testEvery);
     MS_GUARD_END(nullptr)
+}
+
+// Camera::loadMask for every camera with a name relative to its image folder (COLMAP).
+int64_t msplat_dataset_load_masks(MsplatDataset ds, const char* mask_dir) {
+    MS_GUARD_BEGIN
+    if (!ds || !mask_dir) return -1;
+    DatasetImpl* d = DS(ds);
+    int64_t masked = 0;
+    for (int t = 0; t < 2; t++) {
+        auto& list = t ? d->test : d->train;
+        auto& rel = t ? d->test_rel : d->train_rel;
+        for (size_t i = 0; i < list.size(); i++) {
+            SceneCamera c;
+            c.relative = rel[i];
+            c.width = list[i].width;
+            c.height = list[i].height;
+            std::vector<uint8_t> m;
+            int64_t kept = 0;
+            if (!load_camera_mask(c, mask_dir, m, kept)) continue;
+            if (d->set_mask((int)i, t == 1, m.data()) < (int64_t)m.size()) masked++;
+        }
+    }
+    return masked;
+    MS_GUARD_END(-1)
 }
 
 MsplatDataset msplat_dataset_create_empty(void) {

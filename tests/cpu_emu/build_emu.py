@@ -21,7 +21,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OUT = os.environ.get("MSPLAT_EMU_OUT", os.path.join(HERE, "_build"))
 EXE = os.path.join(OUT, "emu_driver")
 SHLIB = os.path.join(OUT, "libmsplat_emu.so")
-SOURCES = ["kernels.hip", "mcmc.hip", "mlp.hip", "eval.hip", "render_noalloc.hip", "engine.cpp", "batch.cpp", "proposal.cpp", "model.cpp", "api.cpp", "scene.cpp", "io.cpp"]
+SOURCES = ["kernels.hip", "mcmc.hip", "mlp.hip", "eval.hip", "render_noalloc.hip", "engine.cpp", "batch.cpp", "proposal.cpp", "model.cpp", "api.cpp", "scene.cpp", "io.cpp", "image_io.cpp", "ingest.cpp", "io_api.cpp"]
 FLAGS = ["-std=c++20", "-O1", "-g", "-fPIC", "-U_FORTIFY_SOURCE", "-D_FORTIFY_SOURCE=0", "-fno-omit-frame-pointer", "-fsanitize=address",
          "-ffp-contract=off", "-pthread", "-Wno-deprecated-declarations", "-I" + HERE, "-I" + INCLUDE, "-I" + CSRC]
 
@@ -60,11 +60,11 @@ def build() -> str:
     paths.append(os.path.join(HERE, "emu_driver.cpp"))
     with ThreadPoolExecutor(max_workers=min(8, len(paths))) as ex:
         objs = list(ex.map(_compile, paths))
-    cmd = ["g++"] + FLAGS + objs + ["-o", EXE]
+    cmd = ["g++"] + FLAGS + objs + ["-o", EXE, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"emulation link failed:\n{r.stderr[-6000:]}")
-    cmd = ["g++"] + FLAGS + ["-shared"] + objs[:-1] + ["-o", SHLIB]
+    cmd = ["g++"] + FLAGS + ["-shared"] + objs[:-1] + ["-o", SHLIB, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"emulation shared-library link failed:\n{r.stderr[-6000:]}")
