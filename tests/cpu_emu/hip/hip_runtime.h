@@ -429,7 +429,23 @@ inline hipError_t hipMalloc(void** p, size_t bytes) {
     if (mprotect((void*)start, pages * EmuDevMem::kPage, PROT_READ | PROT_WRITE) != 0) return hipErrorOutOfMemory;
     const uintptr_t end = start + pages * EmuDevMem::kPage;
     const uintptr_t q = end - body;
-    std::memset((void*)q, 0xFF, body);
+    // Fresh device memory holds what the last user left there: fill it with all ones by
+    // default, or (MSPLAT_EMU_FILL) zeros or pseudo-random bytes, so that a result that depends
+    // on memory read before it is written differs between fills.
+    static const int fill_mode = [] {
+        const char* f = std::getenv("MSPLAT_EMU_FILL");
+        return !f ? 0 : std::strcmp(f, "zero") == 0 ? 1 : std::strcmp(f, "random") == 0 ? 2 : 0;
+    }();
+    if (fill_mode == 2) {
+        static std::atomic<uint64_t> seed{0x9E3779B97F4A7C15ull};
+        uint64_t x = seed.fetch_add(0x632BE59BD9B4E019ull);
+        for (size_t i = 0; i < body; i += 8) {
+            x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+            std::memcpy((char*)q + i, &x, std::min<size_t>(8, body - i));
+        }
+    } else {
+        std::memset((void*)q, fill_mode == 1 ? 0x00 : 0xFF, body);
+    }
     if (__asan_poison_memory_region) {
         if (q > start) __asan_poison_memory_region((void*)start, q - start);
         if (body > n) __asan_poison_memory_region((void*)(q + n), body - n);
