@@ -59,6 +59,8 @@ def parse():
                    help="waves per workgroup of the tile-list bucket kernel (0: as the LDS allows)")
     p.add_argument("--raster-ppl", type=int, default=0, choices=[0, 1, 4],
                    help="pixels per lane of the persistent raster kernels (0: library default)")
+    p.add_argument("--persist-grid", type=int, default=0,
+                   help="resident workgroups of the persistent raster kernels (0: library default)")
     return p.parse_args()
 
 
@@ -129,6 +131,8 @@ def main():
         tr.set_option("sh_bf16", 1)
     if a.raster_ppl:
         tr.set_option("raster_ppl", a.raster_ppl)
+    if a.persist_grid:
+        tr.set_option("persist_grid", a.persist_grid)
     if a.bucket_waves:
         _lib.lib().msplat_set_list_bucket_waves(a.bucket_waves)
     eng = tr.engine_handle

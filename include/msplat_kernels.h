@@ -50,17 +50,18 @@ int msplat_dev_memset(void* dst, int value, size_t bytes);
  * calibration point for HBM traffic counters. Waits for completion. */
 int msplat_dev_copy_kernel(void* dst, const void* src, size_t bytes);
 
-/* Waves per workgroup of the tile-list bucket kernel: 0 (default) = 2 where the LDS left by
- * the view's tile count allows (8 bytes per tile), else 1; 1, 2 or 4 (12 bytes per tile) on
- * request, falling back to fewer where the LDS does not allow it. Process-wide; the lists
- * are the same for every choice, so tests use it to cover each variant. No counterpart in
- * the reference (its list build has no such variants). Returns -1 for any other value. */
+/* Waves per workgroup of the block-list (level-1) bucket kernel of the tile-list build (one
+ * chunk of ranks per wave, 4 bytes of LDS per 4x4-tile block per wave): 0 (default) = 4
+ * where the LDS allows, else 2, else 1; 1, 2 or 4 on request, falling back to fewer where
+ * the LDS does not allow it. Process-wide; the lists are the same for every
+ * choice, so tests use it to cover each variant. No counterpart in the reference (its list
+ * build has no such variants). Returns -1 for any other value. */
 int msplat_set_list_bucket_waves(int waves);
-/* The waves per workgroup the bucket kernel launches with for `tiles` tiles under a device
+/* The waves per workgroup the bucket kernel launches with for `bins` blocks under a device
  * LDS limit of `lds_limit` bytes per workgroup (request as above); 0 when even one wave's
- * LDS does not fit (such a view is rejected). Host-only arithmetic (no device is touched);
- * no counterpart in the reference. */
-int msplat_list_bucket_waves_for(int tiles, int64_t lds_limit, int request);
+ * LDS does not fit. Host-only arithmetic (no device is touched); no counterpart in the
+ * reference. */
+int msplat_list_bucket_waves_for(int bins, int64_t lds_limit, int request);
 
 /* Creates an engine bound to the current HIP device (buffers grow on demand and are
  * reused across steps). Replaces the reference's global MetalContext + FusedTensorCache
@@ -101,8 +102,8 @@ int msplat_engine_render(MsplatEngine* e, const MsplatDeviceState* st, const Msp
  * final_idx [H,W] i32, identical to the engine's buffers of the same name after
  * msplat_engine_render; the work is enqueued on `stream` (a hipStream_t, NULL: the null
  * stream) and nothing waits for it. A render whose buffers exceed the array (roughly
- * 120k Gaussians at 1920x1080 on the monolithic path; depth-chunked views of fewer than 400
- * tiles need about 3 KB per Gaussian, so about 18k) is rejected with -3 and a message on
+ * 70k Gaussians at 1920x1080 on the monolithic path; depth-chunked views of fewer than 400
+ * tiles need about 3.4 KB per Gaussian, so about 16k) is rejected with -3 and a message on
  * stderr. Calls on one GPU share its array and overlapping calls on one GPU (two streams,
  * two threads) race. Returns 0, -1 bad arguments, -2 HIP error, -3 too large. */
 int msplat_render_noalloc(const MsplatDeviceState* st, const MsplatCamera* cam, int degrees_to_use,

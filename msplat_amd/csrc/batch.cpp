@@ -231,11 +231,9 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
         }
         hw_ = hwv_[0];
     }
-    // per-slot buffers: the slot's recorded geometry (graph) or its view's own; a recorded
-    // slot buckets its lists by tile-count class (its views may be smaller than its geometry)
+    // per-slot buffers: the slot's recorded geometry (graph) or its view's own
     for (int k = 0; k < B; k++) {
         const msk::Geom g = msk::geom_of(sizes_[k].first, sizes_[k].second);
-        slots_[k]->set_size_classes(graphable);
         slots_[k]->prepare(st.N, graphable ? hwv_[k] : g, g, true);
     }
     int Tmax = 0;

@@ -120,7 +120,7 @@ void Engine::trim() {
     if (stream_) (void)hipStreamSynchronize(stream_);
     DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
-                   &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_,
+                   &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &l1_, &l1m_, &blocks_, &hist_, &img_,
                    &finalT_, &finalIdx_, &ncontrib_, &vrend_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
                    &prefixT_, &afterC_, &wq_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
@@ -139,7 +139,7 @@ Engine::~Engine() {
     hipEventDestroy(mark_[1]);
     DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
                    &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
-                   &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &hist_, &img_, &finalT_, &finalIdx_, &ncontrib_,
+                   &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &l1_, &l1m_, &blocks_, &hist_, &img_, &finalT_, &finalIdx_, &ncontrib_,
                    &vrend_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
                    &afterC_, &nanc_, &wq_, &wmisc_, &consts_, &srect_};
     for (DBuf* b : all) b->release();
@@ -184,6 +184,9 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
     ensure(dkeys_, n * 4); ensure(dkeys_out_, n * 4); ensure(dvals_, n * 4); ensure(sorted_ids_, n * 4);
     ensure(temp_, msk::lists_temp_bytes(N));
     ensure(ids_, (size_t)capacity_ * 4);
+    ensure(l1_, (size_t)capacity_ * 4);
+    ensure(l1m_, (size_t)capacity_ * 2);
+    ensure(blocks_, msk::lists_block_bytes(launch));
     ensure(hist_, msk::lists_hist_bytes(N, T, chunk_target_));
     ensure(img_, P * 12); ensure(finalT_, P * 4); ensure(finalIdx_, P * 4); ensure(ncontrib_, P * 4);
     ensure(wq_, (size_t)T * 4);
@@ -353,6 +356,9 @@ msk::ListBufs Engine::list_bufs() {
     L.ids = ids_.as<uint32_t>();
     L.view = {ids_.as<uint32_t>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
     L.hist = hist_.as<uint32_t>();
+    L.l1_ids = l1_.as<uint32_t>();
+    L.l1_mask = l1m_.as<uint16_t>();
+    L.blocks = blocks_.as<uint32_t>();
     L.totals = totals_.as<uint32_t>();
     L.capacity = capacity_;
     L.num_tiles = tbx_ * tby_;
@@ -364,7 +370,6 @@ msk::ListBufs Engine::list_bufs() {
     L.temp_bytes = temp_.cap;
     L.chunk_target = chunk_target_;
     L.bucket_waves = bucket_waves_;
-    L.size_classes = size_classes_;
     return L;
 }
 

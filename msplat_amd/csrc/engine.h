@@ -114,7 +114,6 @@ public:
     // views smaller than their launch geometry).
     void set_list_chunks(int c) { chunk_target_ = c > 0 ? c : 0; }
     void set_list_bucket_waves(int w) { bucket_waves_ = (w == 1 || w == 2 || w == 4) ? w : 0; }
-    void set_size_classes(bool on) { size_classes_ = on; }
 
 private:
     void upload_view(const msk::ViewConsts& h, bool train);
@@ -134,7 +133,6 @@ private:
     int persist_grid_ = 0;
     int ppl_ = 1;
     int chunk_target_ = 0, bucket_waves_ = 0;
-    bool size_classes_ = false;
     // view constants: host copy, device copy in use, own device block [ViewConsts | RasterView]
     msk::ViewConsts vch_{};
     const msk::ViewConsts* vcd_ = nullptr;
@@ -149,7 +147,7 @@ private:
     DBuf dkeys_, dkeys_out_, dvals_, sorted_ids_, temp_, srect_;
     DBuf wq_, wmisc_;   // own persistent-raster work queue
     // per-intersection
-    DBuf ids_, hist_;
+    DBuf ids_, l1_, l1m_, blocks_, hist_;
     // per-pixel
     DBuf img_, finalT_, finalIdx_, ncontrib_, vrend_, dbuf_, partials_, loss_;
     DBuf chunkT_, chunkC_, chunkIdx_, prefixT_, afterC_;

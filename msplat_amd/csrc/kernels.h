@@ -43,7 +43,10 @@ struct ListBufs {
     int2* bins;                // [T] (start, end) of each tile's list
     uint32_t* ids;             // [capacity] sorted per-tile Gaussian ids (the packed lists)
     EntryView view;            // ids + the projection's records
-    uint32_t* hist;            // [chunks x T] per-chunk tile histograms -> bucket offsets
+    uint32_t* hist;            // [chunks x T] per-chunk tile histograms, then [chunks x blocks] block offsets
+    uint32_t* l1_ids;          // [capacity] block lists: Gaussian ids
+    uint16_t* l1_mask;         // [capacity] block lists: masks of the block's tiles the Gaussian covers
+    uint32_t* blocks;          // [2 blocks + 2] block list lengths, then their starts and the total
     uint32_t* totals;          // [8]: total, padded total (reference layout), overflow flag, max count
     int64_t capacity;
     int num_tiles;
@@ -55,10 +58,6 @@ struct ListBufs {
     // launch choices (the lists are the same for every choice)
     int chunk_target = 0;      // rank chunks of the count / bucket kernels (0: the default 768)
     int bucket_waves = 0;      // waves per bucket workgroup (0: the process-wide choice)
-    // one bucket launch per tile-count class, each with the LDS of its class, of which only
-    // the one holding the view's tile count does work: for a recorded launch sequence that
-    // serves views smaller than its launch geometry
-    bool size_classes = false;
 };
 
 // Constants of one view of a step, read by the kernels from device memory, so that a
@@ -135,6 +134,7 @@ void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int d
                      uint32_t* nan_counts);
 size_t lists_temp_bytes(int N);
 size_t lists_hist_bytes(int N, int T, int chunk_target = 0);
+size_t lists_block_bytes(const Geom& g);
 bool lists_supported(int N, int T);
 void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L);
 void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);

@@ -454,12 +454,19 @@ Geom geom_of(int W, int H) {
 //  1. One stable radix sort of the N Gaussians by depth bits gives the global (depth, id)
 //     order; r is a Gaussian's rank in it.
 //  2. The ranks are cut into G-wide chunks, one workgroup each. A workgroup counts its
-//     chunk's tile hits in an LDS histogram (no global atomics); a column scan over the
-//     chunks gives every (chunk, tile) pair its offset inside the tile's bucket.
-//  3. The bucket kernel (same chunks) writes every hit's Gaussian id at that offset plus
-//     its place among the chunk's hits on the tile, walking the chunk's ranks in increasing
-//     order (per wave, after a per-wave count), so each segment is written sorted. A tile's
-//     bucket is its chunks' segments in chunk order: sorted by rank, with no sort pass.
+//     chunk's tile hits in an LDS histogram (no global atomics), one row of a chunk x tile
+//     matrix; column sums give the tile counts, and one scan the tiles' list offsets.
+//  3. Level 1: the same chunks list every Gaussian once per 4x4-tile block its rectangle
+//     meets, with the mask of the block's tiles it covers, in per-block lists: a block
+//     histogram and column scan give every (chunk, block) pair its offset, and the bucket
+//     kernel writes each chunk's pairs there, walking the chunk's ranks in increasing order
+//     (per wave, after a per-wave count), so each block list is in rank order.
+//  4. Level 2: one workgroup per tile keeps, in order, the entries of its block's list
+//     whose mask has the tile's bit, and writes them front to back as the tile's list:
+//     sorted by rank, with no sort pass, and every line of the list written by one
+//     workgroup (writing each hit straight into its tile's list from its chunk scatters
+//     4-byte stores over the whole list buffer; the L2 then writes back partial lines,
+//     about 8x the list's bytes).
 // The resulting per-tile id lists are the reference's sorted lists exactly.
 
 __device__ __forceinline__ uint32_t pad_count(uint32_t c) {  // metal:2248-2256
@@ -473,7 +480,8 @@ constexpr int MAX_CHUNK = 1984;  // Gaussians per chunk at most (a chunk-local r
 
 // srect [N]: tile rectangles in depth-rank order (empty for invisible Gaussians), so the
 // chunk kernels read them contiguously.
-__global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const ushort4* __restrict__ srect,
+// mat [chunks][T1]: row stride T1 = the launch geometry's tile count.
+__global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, int T1, const ushort4* __restrict__ srect,
                                                             const ViewConsts* __restrict__ vc,
                                                             uint32_t* __restrict__ mat) {
     extern __shared__ uint32_t hist[];
@@ -487,18 +495,17 @@ __global__ __launch_bounds__(HIST_THREADS) void k_tile_hist(int N, int G, const 
             for (int x = q.x; x < q.z; x++) atomicAdd(&hist[y * tbx + x], 1u);
     }
     __syncthreads();
-    uint32_t* row = mat + (size_t)blockIdx.x * T;
+    uint32_t* row = mat + (size_t)blockIdx.x * T1;
     for (int t = threadIdx.x; t < T; t += HIST_THREADS) row[t] = hist[t];
 }
 
-// Exclusive scan of each tile's column over the chunks; the column total is the tile count.
-// A workgroup takes 64 columns; its 8 waves each own one eighth of the chunks (column sums
-// first, combined through LDS, then the rewrite), so the chain of dependent loads per lane is
-// an eighth of the column.
+// Column sums of the chunk x tile matrix: the tile counts. A workgroup takes 64 columns; its
+// 8 waves each own one eighth of the chunks (partial sums combined through LDS), so the chain
+// of dependent loads per lane is an eighth of the column.
 constexpr int COLSCAN_PARTS = 8;
-__global__ __launch_bounds__(64 * COLSCAN_PARTS) void k_col_scan(const ViewConsts* __restrict__ vc, int nblk,
-                                                                 uint32_t* __restrict__ mat,
-                                                                 uint32_t* __restrict__ counts) {
+__global__ __launch_bounds__(64 * COLSCAN_PARTS) void k_col_sum(const ViewConsts* __restrict__ vc, int nblk, int T1,
+                                                                const uint32_t* __restrict__ mat,
+                                                                uint32_t* __restrict__ counts) {
     __shared__ uint32_t part_sum[COLSCAN_PARTS][64];
     const int T = vc->T;
     const int c = (int)(threadIdx.x & 63), part = (int)(threadIdx.x >> 6);
@@ -511,39 +518,18 @@ __global__ __launch_bounds__(64 * COLSCAN_PARTS) void k_col_scan(const ViewConst
         for (; b + 8 <= b1; b += 8) {  // eight independent loads in flight
             uint32_t v[8];
 #pragma unroll
-            for (int u = 0; u < 8; u++) v[u] = mat[(size_t)(b + u) * T + t];
+            for (int u = 0; u < 8; u++) v[u] = mat[(size_t)(b + u) * T1 + t];
 #pragma unroll
             for (int u = 0; u < 8; u++) sum += v[u];
         }
-        for (; b < b1; b++) sum += mat[(size_t)b * T + t];
+        for (; b < b1; b++) sum += mat[(size_t)b * T1 + t];
     }
     part_sum[part][c] = sum;
     __syncthreads();
-    uint32_t run = 0, total = 0;
-    for (int q = 0; q < COLSCAN_PARTS; q++) {
-        const uint32_t v = part_sum[q][c];
-        if (q < part) run += v;
-        total += v;
-    }
-    if (t >= T) return;
-    int b = b0;
-    for (; b + 8 <= b1; b += 8) {
-        uint32_t v[8];
-#pragma unroll
-        for (int u = 0; u < 8; u++) v[u] = mat[(size_t)(b + u) * T + t];
-#pragma unroll
-        for (int u = 0; u < 8; u++) {
-            mat[(size_t)(b + u) * T + t] = run;
-            run += v[u];
-        }
-    }
-    for (; b < b1; b++) {
-        const size_t o = (size_t)b * T + t;
-        const uint32_t v = mat[o];
-        mat[o] = run;
-        run += v;
-    }
-    if (part == 0) counts[t] = total;
+    if (part != 0 || t >= T) return;
+    uint32_t total = 0;
+    for (int q = 0; q < COLSCAN_PARTS; q++) total += part_sum[q][c];
+    counts[t] = total;
 }
 
 __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ vc, const uint32_t* __restrict__ counts,
@@ -591,203 +577,366 @@ __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ 
     }
 }
 
-// Bucket kernel: the chunk's ranks are split into W contiguous runs, one per wave. Pass A
-// counts every wave's hits per tile (16-bit fields, two waves per LDS word); their exclusive
-// prefix over the waves gives each wave its first slot inside the chunk's segment of a tile.
-// Pass B: each wave walks its ranks in increasing order, one rank at a time, its lanes
-// covering the rank's tile rectangle (distinct tiles, so one LDS atomic per lane and no two
-// lanes on one cursor); a wave's writes to a tile are thus in rank order and the segment is
-// sorted as written. W = 4, 2 or 1 by the LDS the tile count leaves (12, 8 or 4 bytes per
-// tile).
-static size_t bucket_lds_bytes(int T, int waves) {
-    const size_t t4 = ((size_t)T + 3) & ~(size_t)3;
-    // + one 64-word line per wave that idle lanes point their atomics at
-    return t4 * 4 * (size_t)(waves == 1 ? 1 : 1 + waves / 2) + (size_t)waves * 64 * 4;
+// Block lists (level 1): the tiles are grouped into BLK x BLK blocks, and every Gaussian
+// whose rectangle meets a block is listed once for it, in rank order, with a 16-bit mask of
+// the block's tiles its rectangle covers. A Gaussian meets about 4x fewer blocks than tiles,
+// so this list is about 4x shorter than the tile lists, and a chunk's segment of a block's
+// list is about 4x longer than its segment of a tile's. When the tile lists overflow the
+// capacity, a Gaussian is listed for a block only with the bits of the block's tiles whose
+// lists end within the capacity (and not at all when no such bit is left): the level-1 list
+// then holds at most as many entries as those tiles, so it never passes the capacity.
+constexpr int BLK = 4;
+
+struct BlockGeom {
+    int nbx, nby, NB;
+};
+__device__ __forceinline__ BlockGeom block_geom(int tbx, int tby) {
+    BlockGeom b;
+    b.nbx = (tbx + BLK - 1) / BLK;
+    b.nby = (tby + BLK - 1) / BLK;
+    b.NB = b.nbx * b.nby;
+    return b;
 }
 
-// 2 waves when the LDS allows (measured at 1.5M Gaussians, 1600x1040: 4 waves 0.336 ms,
-// 2 waves 0.319 ms, 1 wave 0.464 ms per launch), else 1; 4 only on request. The choice is
-// by the kernel's whole dynamic LDS (cursors plus the idle-lane lines), so a launch never
-// asks for more than `limit`; 0 when even one wave does not fit.
-int bucket_waves(int T, size_t limit, int request) {
-    if (request == 4 && bucket_lds_bytes(T, 4) <= limit) return 4;
-    if (request != 1 && bucket_lds_bytes(T, 2) <= limit) return 2;
-    return bucket_lds_bytes(T, 1) <= limit ? 1 : 0;
+// Bits (BLK ly + lx) of block (bx, by)'s tiles inside the rectangle [x0, x1) x [y0, y1).
+__device__ __forceinline__ uint32_t block_cover(ushort4 q, int bx, int by) {
+    const int lx0 = max((int)q.x - bx * BLK, 0), lx1 = min((int)q.z - bx * BLK, BLK);
+    const int ly0 = max((int)q.y - by * BLK, 0), ly1 = min((int)q.w - by * BLK, BLK);
+    if (lx1 <= lx0 || ly1 <= ly0) return 0u;
+    const uint32_t row = ((1u << (lx1 - lx0)) - 1u) << lx0;
+    uint32_t m = 0u;
+    for (int ly = ly0; ly < ly1; ly++) m |= row << (ly * BLK);
+    return m;
+}
+
+// Bits of block (bx, by)'s tiles whose lists end within the capacity.
+__device__ __forceinline__ uint32_t block_complete(int bx, int by, int tbx, int tby,
+                                                   const uint32_t* __restrict__ exact_off,
+                                                   const uint32_t* __restrict__ counts, uint32_t cap) {
+    uint32_t m = 0u;
+    for (int ly = 0; ly < BLK; ly++)
+        for (int lx = 0; lx < BLK; lx++) {
+            const int tx = bx * BLK + lx, ty = by * BLK + ly;
+            if (tx >= tbx || ty >= tby) continue;
+            const int t = ty * tbx + tx;
+            if (exact_off[t] + counts[t] <= cap) m |= 1u << (ly * BLK + lx);
+        }
+    return m;
+}
+
+// The listed bits of rank rectangle q for block (bx, by): its cover, restricted to the tiles
+// whose lists fit when some list overflows; 0 means the pair is not listed.
+__device__ __forceinline__ uint32_t block_bits(ushort4 q, int bx, int by, bool any_over, int tbx, int tby,
+                                               const uint32_t* __restrict__ exact_off,
+                                               const uint32_t* __restrict__ counts, uint32_t cap) {
+    uint32_t m = block_cover(q, bx, by);
+    if (any_over && m != 0u) m &= block_complete(bx, by, tbx, tby, exact_off, counts, cap);
+    return m;
+}
+
+// Per chunk, the number of listed pairs of each block (one row of the chunk x block matrix,
+// row stride NBL = the launch geometry's block count).
+__global__ __launch_bounds__(HIST_THREADS) void k_block_hist(int N, int G, int NBL, const ushort4* __restrict__ srect,
+                                                             const ViewConsts* __restrict__ vc,
+                                                             const uint32_t* __restrict__ exact_off,
+                                                             const uint32_t* __restrict__ counts,
+                                                             const uint32_t* __restrict__ totals, uint32_t cap,
+                                                             uint32_t* __restrict__ bmat) {
+    extern __shared__ uint32_t hist[];
+    const int tbx = vc->tbx, tby = vc->tby;
+    const BlockGeom bg = block_geom(tbx, tby);
+    for (int b = threadIdx.x; b < bg.NB; b += HIST_THREADS) hist[b] = 0u;
+    __syncthreads();
+    const bool any_over = totals[0] > cap;
+    const int r0 = blockIdx.x * G, r1 = min(N, r0 + G);
+    for (int r = r0 + threadIdx.x; r < r1; r += HIST_THREADS) {
+        const ushort4 q = srect[r];
+        if (q.z <= q.x || q.w <= q.y) continue;
+        const int bx0 = q.x / BLK, bx1 = (q.z - 1) / BLK, by0 = q.y / BLK, by1 = (q.w - 1) / BLK;
+        for (int by = by0; by <= by1; by++)
+            for (int bx = bx0; bx <= bx1; bx++)
+                if (!any_over || block_bits(q, bx, by, true, tbx, tby, exact_off, counts, cap) != 0u)
+                    atomicAdd(&hist[by * bg.nbx + bx], 1u);
+    }
+    __syncthreads();
+    uint32_t* row = bmat + (size_t)blockIdx.x * NBL;
+    for (int b = threadIdx.x; b < bg.NB; b += HIST_THREADS) row[b] = hist[b];
+}
+
+// Exclusive scan of each block's column over the chunks (in place); the column total is the
+// block's list length. A workgroup takes 16 columns; its 16 thread groups each own one
+// sixteenth of the chunks (column sums first, combined through LDS, then the rewrite), with
+// eight loads in flight per thread.
+constexpr int BSCAN_COLS = 16, BSCAN_PARTS = 16;
+__global__ __launch_bounds__(BSCAN_COLS * BSCAN_PARTS) void k_block_scan(const ViewConsts* __restrict__ vc, int nblk,
+                                                                         int NBL, uint32_t* __restrict__ bmat,
+                                                                         uint32_t* __restrict__ btot) {
+    __shared__ uint32_t part_sum[BSCAN_PARTS][BSCAN_COLS];
+    const int NB = block_geom(vc->tbx, vc->tby).NB;
+    const int c = (int)(threadIdx.x % BSCAN_COLS), part = (int)(threadIdx.x / BSCAN_COLS);
+    const int b = (int)blockIdx.x * BSCAN_COLS + c;
+    const int per = (nblk + BSCAN_PARTS - 1) / BSCAN_PARTS;
+    const int k0 = min(nblk, part * per), k1 = min(nblk, k0 + per);
+    uint32_t sum = 0;
+    if (b < NB) {
+        int k = k0;
+        for (; k + 8 <= k1; k += 8) {
+            uint32_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = bmat[(size_t)(k + u) * NBL + b];
+#pragma unroll
+            for (int u = 0; u < 8; u++) sum += v[u];
+        }
+        for (; k < k1; k++) sum += bmat[(size_t)k * NBL + b];
+    }
+    part_sum[part][c] = sum;
+    __syncthreads();
+    uint32_t run = 0, total = 0;
+    for (int q = 0; q < BSCAN_PARTS; q++) {
+        const uint32_t v = part_sum[q][c];
+        if (q < part) run += v;
+        total += v;
+    }
+    if (b >= NB) return;
+    int k = k0;
+    for (; k + 8 <= k1; k += 8) {
+        uint32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = bmat[(size_t)(k + u) * NBL + b];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            bmat[(size_t)(k + u) * NBL + b] = run;
+            run += v[u];
+        }
+    }
+    for (; k < k1; k++) {
+        const size_t o = (size_t)k * NBL + b;
+        const uint32_t v = bmat[o];
+        bmat[o] = run;
+        run += v;
+    }
+    if (part == 0) btot[b] = total;
+}
+
+// Level-1 bucket kernel: one wave per chunk (WAVES consecutive chunks per workgroup, each
+// wave with its own block cursors in LDS), chunk c writing its listed pairs of block b from
+// boff[b] + (its column offset) on. The wave takes its chunk's ranks 64 at a time, numbers
+// their (rank, block) pairs in rank-major order (a wave scan of the ranks' block counts),
+// and then places 64 pairs per step, one per lane: the lane finds its pair's rank by a
+// binary search over the scan (lane shuffles), lanes with the same block find each other
+// by ballots over the block number's bits, the lowest of them advances the block's cursor
+// by their count, and each writes at the old cursor plus its place among them. Steps run
+// in pair order and a step orders a block's pairs by lane, so every block list is in rank
+// order. Lists are stored as two arrays, Gaussian ids and 16-bit tile masks. boff (the
+// exclusive prefix of the block lengths) is computed by every wave in LDS; wave 0 of
+// workgroup 0 also writes it out (NB + 1 values) for the compaction kernel.
+static size_t bucket_lds_bytes(int NB, int waves) {
+    const size_t t4 = ((size_t)NB + 3) & ~(size_t)3;
+    return t4 * 4 * (size_t)waves;
+}
+
+// 4 waves per workgroup when the LDS allows (the default and a request of 4), 2 on request
+// or when 4 do not fit, else 1. The choice is by the kernel's whole dynamic LDS (one cursor
+// array per wave), so a launch never asks for more than `limit`; 0 when even one wave does
+// not fit.
+int bucket_waves(int NB, size_t limit, int request) {
+    if ((request == 0 || request == 4) && bucket_lds_bytes(NB, 4) <= limit) return 4;
+    if (request != 1 && bucket_lds_bytes(NB, 2) <= limit) return 2;
+    return bucket_lds_bytes(NB, 1) <= limit ? 1 : 0;
 }
 
 template <int WAVES>
-__global__ __launch_bounds__(64 * WAVES) void k_tile_bucket(int N, int G, const uint32_t* __restrict__ sorted_ids,
-                                                            const ushort4* __restrict__ srect,
-                                                            const ViewConsts* __restrict__ vc,
-                                                            const uint32_t* __restrict__ mat,
-                                                            const uint32_t* __restrict__ exact_off,
-                                                            const uint32_t* __restrict__ counts,
-                                                            const uint32_t* __restrict__ totals,
-                                                            uint32_t* __restrict__ ids, uint32_t cap, int tmin,
-                                                            int tmax) {
+__global__ __launch_bounds__(64 * WAVES) void k_block_bucket(int N, int G, int nblk, int NBL,
+                                                             const uint32_t* __restrict__ sorted_ids,
+                                                             const ushort4* __restrict__ srect,
+                                                             const ViewConsts* __restrict__ vc,
+                                                             const uint32_t* __restrict__ bmat,
+                                                             const uint32_t* __restrict__ btot,
+                                                             const uint32_t* __restrict__ exact_off,
+                                                             const uint32_t* __restrict__ counts,
+                                                             const uint32_t* __restrict__ totals,
+                                                             uint32_t* __restrict__ l1_ids,
+                                                             uint16_t* __restrict__ l1_mask,
+                                                             uint32_t* __restrict__ boff_out, uint32_t cap) {
     extern __shared__ uint32_t sm[];
-    const int chunk = (int)blockIdx.x;
-    const int tbx = vc->tbx, T = vc->T;
-    if (T < tmin || T > tmax) return;  // another launch's tile-count class
-    const int T4 = (T + 3) & ~3;
-    uint32_t* cs = sm;        // [T] start of this chunk's segment of each tile
-    uint32_t* pw = sm + T4;   // [T][WAVES / 2] per-wave cursors, 16 bits each
-    uint32_t* idle = sm + T4 * (WAVES == 1 ? 1 : 1 + WAVES / 2) + 64 * (threadIdx.x >> 6);  // [64] per wave
-    const int tid = (int)threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const uint32_t* row = mat + (size_t)chunk * T;
-    const int r0 = chunk * G, r1 = min(N, r0 + G);
-    const int wper = (G + WAVES - 1) / WAVES;
-    const int w0 = min(r1, r0 + wid * wper), w1 = min(r1, w0 + wper);
-    const int word = WAVES == 4 ? (wid >> 1) : 0;
-    const uint32_t shift = WAVES > 1 ? 16u * (uint32_t)(wid & 1) : 0u;
-    const uint32_t one = WAVES > 1 ? (1u << shift) : 1u;
+    const int lane = (int)(threadIdx.x & 63), wid = (int)(threadIdx.x >> 6);
+    const int chunk = (int)blockIdx.x * WAVES + wid;
+    if (chunk >= nblk) return;  // wave-uniform; no workgroup barrier follows
+    const int tbx = vc->tbx, tby = vc->tby;
+    const BlockGeom bgm = block_geom(tbx, tby);
+    const int NB = bgm.NB, nbx = bgm.nbx;
+    uint32_t* cs = sm + (size_t)wid * ((NB + 3) & ~3);  // this wave's cursors
+    const uint32_t* row = bmat + (size_t)chunk * NBL;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    // cursors: block list start (prefix of the block lengths) + this chunk's column offset
     {
-        constexpr int NT = 64 * WAVES, U = 8;  // U tiles per thread in flight
-        for (int t0 = 0; t0 < T; t0 += NT * U) {
-            uint32_t a[U], b[U];
+        uint32_t run = 0u;
+        for (int b0 = 0; b0 < NB; b0 += 64) {
+            const int b = b0 + lane;
+            const uint32_t v = b < NB ? btot[b] : 0u;
+            uint32_t x = v;
 #pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int t = t0 + u * NT + tid;
-                a[u] = t < T ? exact_off[t] : 0u;
-                b[u] = t < T ? row[t] : 0u;
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(x, o, 64);
+                if (lane >= o) x += y;
             }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                const int t = t0 + u * NT + tid;
-                if (t < T) {
-                    cs[t] = a[u] + b[u];
-                    if (WAVES > 1)
-                        for (int k = 0; k < WAVES / 2; k++) pw[(WAVES / 2) * t + k] = 0u;
-                }
+            if (b < NB) {
+                cs[b] = run + x - v + row[b];
+                if (chunk == 0) boff_out[b] = run + x - v;
             }
+            run += __shfl(x, 63, 64);
         }
+        if (chunk == 0 && lane == 0) boff_out[NB] = run;
     }
-    __syncthreads();
-    if (WAVES > 1) {
-        // A. hits per (wave, tile)
-        for (int r = w0 + lane; r < w1; r += 64) {
-            const ushort4 q = srect[r];
-            for (int y = q.y; y < q.w; y++)
-                for (int x = q.x; x < q.z; x++) atomicAdd(&pw[(WAVES / 2) * (y * tbx + x) + word], one);
-        }
-        __syncthreads();
-        // exclusive prefix over the waves (a chunk has < 2^16 ranks, so no field overflows)
-        for (int t = tid; t < T; t += 64 * WAVES) {
-            uint32_t c[4] = {0u, 0u, 0u, 0u};
-            for (int k = 0; k < WAVES / 2; k++) {
-                const uint32_t v = pw[(WAVES / 2) * t + k];
-                c[2 * k] = v & 0xFFFFu;
-                c[2 * k + 1] = v >> 16;
-            }
-            uint32_t run = 0u;
-            for (int w = 0; w < WAVES; w++) {
-                const uint32_t v = c[w];
-                c[w] = run;
-                run += v;
-            }
-            for (int k = 0; k < WAVES / 2; k++) pw[(WAVES / 2) * t + k] = c[2 * k] | (c[2 * k + 1] << 16);
-        }
-        __syncthreads();
-    }
-    // B. the wave's ranks in order; lanes cover the rectangle of one rank. Four ranks at a
-    // time: their cursor atomics are issued back to back (the LDS performs one wave's
-    // operations in issue order, so rank order is kept) before any result is waited for.
-    const bool any_over = totals[0] > cap;  // some tile's list does not fit
-    // next slot of this wave in tile t's segment, in two halves (the LDS atomic, then the
-    // sum) so that several atomics can be in flight before the first sum waits for one
-    auto claim = [&](int t, uint32_t& base) -> uint32_t {
-        if (WAVES > 1) {
-            base = cs[t];
-            return atomicAdd(&pw[(WAVES / 2) * t + word], one);
-        }
-        base = 0u;
-        return atomicAdd(&cs[t], 1u);
-    };
-    auto slot_of = [&](uint32_t claimed, uint32_t base) -> uint32_t {
-        return WAVES > 1 ? base + ((claimed >> shift) & 0xFFFFu) : claimed;
-    };
-    auto cursor = [&](int t) -> uint32_t {
-        uint32_t base;
-        const uint32_t c = claim(t, base);
-        return slot_of(c, base);
-    };
-    auto put = [&](uint32_t slot, int t, uint32_t g) {
-        if (slot < cap) {
-            // overflowed lists: all of the tile's entries become the empty record
-            const bool over = any_over && exact_off[t] + counts[t] > cap;
-            ids[slot] = over ? (uint32_t)N : g;
-        }
-    };
-    for (int b0 = w0; b0 < w1; b0 += 64) {
-        const int r = b0 + lane;
-        uint2 qw = make_uint2(0u, 0u);
+    const bool any_over = totals[0] > cap;
+    const int nbits = NB > 1 ? 32 - __clz(NB - 1) : 1;  // bits of a block number
+    const int r0 = chunk * G, r1 = min(N, r0 + G);
+    for (int base = r0; base < r1; base += 64) {
+        const int r = base + lane;
+        ushort4 q = make_ushort4(0, 0, 0, 0);
         uint32_t gid = 0u;
-        if (r < w1) {
-            qw = *reinterpret_cast<const uint2*>(srect + r);
+        if (r < r1) {
+            q = srect[r];
             gid = sorted_ids[r];
         }
-        // Loads and stores share one completion counter: the first read of gid inside the
-        // walk below would wait for every store issued so far, once per step. Consuming the
-        // loaded values here makes that one wait per 64 ranks (measured 7% faster at 1.5M
-        // Gaussians, 1600x1040). The branch is never taken: a rectangle's x0 is below its
-        // x1 <= 65535 (or the rectangle is all zeros), so its first word is never all ones.
-        if (__builtin_amdgcn_readfirstlane((int)(qw.x & qw.y & gid)) == -1) return;
-        const int nr = min(64, w1 - b0);
-        for (int j0 = 0; j0 < nr; j0 += 4) {  // wave-uniform
-            int x0[4], y0[4], wd[4], area[4];
-            uint32_t g[4];
-            int most = 0;
+        const bool has = q.z > q.x && q.w > q.y;
+        const int bx0 = q.x / BLK, by0 = q.y / BLK;
+        const int wd = has ? (q.z - 1) / BLK - bx0 + 1 : 0;
+        const int area = has ? wd * ((q.w - 1) / BLK - by0 + 1) : 0;
+        // rank-major numbering of the 64 ranks' pairs
+        int pre = area;
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int j = min(j0 + u, nr - 1);
-                const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)qw.x, j);
-                const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)qw.y, j);
-                g[u] = (uint32_t)__builtin_amdgcn_readlane((int)gid, j);
-                x0[u] = (int)(lo & 0xFFFFu);
-                y0[u] = (int)(lo >> 16);
-                const int x1 = (int)(hi & 0xFFFFu), y1 = (int)(hi >> 16);
-                wd[u] = max(x1 - x0[u], 0);
-                area[u] = (j0 + u < nr) ? wd[u] * max(y1 - y0[u], 0) : 0;
-                most = max(most, area[u]);
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += y;
+        }
+        const int total = __shfl(pre, 63, 64);
+        pre -= area;  // exclusive
+        const uint32_t qlo = (uint32_t)q.x | ((uint32_t)q.y << 16), qhi = (uint32_t)q.z | ((uint32_t)q.w << 16);
+        for (int w0 = 0; w0 < total; w0 += 64) {
+            const int p = w0 + lane;
+            const bool valid = p < total;
+            // the pair's rank: the last lane whose first pair is at or before p (lanes with
+            // no pair share their successor's start, so the last one found has pairs)
+            int i = 0;
+#pragma unroll
+            for (int st = 32; st >= 1; st >>= 1) {
+                const int cand = i + st;
+                if (__shfl(pre, cand, 64) <= p) i = cand;
             }
-            if (most == 0) continue;
-            if (most <= 64) {
-                int t[4];
-                uint32_t claimed[4], base[4];
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    t[u] = 0;
-                    if (lane < area[u]) {
-                        const int dy = (int)(((float)lane + 0.5f) * __builtin_amdgcn_rcpf((float)wd[u]));
-                        t[u] = (y0[u] + dy) * tbx + x0[u] + (lane - dy * wd[u]);
-                    }
-                }
-                // the wave's LDS operations are performed in issue order, so rank u's claims
-                // precede rank u + 1's on every tile. Lanes past a rank's rectangle add 0 to
-                // a word of their own, so the claims need no branches (and no waits between)
-#pragma unroll
-                for (int u = 0; u < 4; u++) {
-                    const bool on = lane < area[u];
-                    uint32_t* cur = on ? (WAVES > 1 ? &pw[(WAVES / 2) * t[u] + word] : &cs[t[u]]) : &idle[lane];
-                    base[u] = WAVES > 1 ? (on ? cs[t[u]] : 0u) : 0u;
-                    claimed[u] = atomicAdd(cur, on ? one : 0u);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-#pragma unroll
-                for (int u = 0; u < 4; u++)
-                    if (lane < area[u]) put(slot_of(claimed[u], base[u]), t[u], g[u]);
-            } else {
-                for (int u = 0; u < 4; u++) {  // a rectangle of more than 64 tiles: rank by rank
-                    const float inv_w = __builtin_amdgcn_rcpf((float)max(wd[u], 1));
-                    for (int k = lane; k < area[u]; k += 64) {
-                        const int dy = (int)(((float)k + 0.5f) * inv_w);
-                        const int t = (y0[u] + dy) * tbx + x0[u] + (k - dy * wd[u]);
-                        put(cursor(t), t, g[u]);
-                    }
-                    __builtin_amdgcn_sched_barrier(0);  // rank u's claims before rank u + 1's
+            const int k = p - __shfl(pre, i, 64);
+            const uint32_t lo = (uint32_t)__shfl((int)qlo, i, 64), hi = (uint32_t)__shfl((int)qhi, i, 64);
+            const uint32_t g = (uint32_t)__shfl((int)gid, i, 64);
+            const int wdi = max(__shfl(wd, i, 64), 1);
+            const ushort4 qi = make_ushort4((unsigned short)(lo & 0xFFFFu), (unsigned short)(lo >> 16),
+                                            (unsigned short)(hi & 0xFFFFu), (unsigned short)(hi >> 16));
+            const int dy = k / wdi;
+            const int bx = qi.x / BLK + (k - dy * wdi), by = qi.y / BLK + dy;
+            const int b = by * nbx + bx;
+            const uint32_t m = valid ? block_bits(qi, bx, by, any_over, tbx, tby, exact_off, counts, cap) : 0u;
+            const bool on = m != 0u;
+            // lanes placing a pair of the same block
+            uint64_t peers = __ballot(on);
+            for (int bit = 0; bit < nbits; bit++) {
+                const bool set = ((b >> bit) & 1) != 0;
+                const uint64_t bal = __ballot(on && set);
+                peers &= set ? bal : ~bal;
+            }
+            const uint32_t rank = (uint32_t)__popcll(peers & below);
+            uint32_t start = 0u;
+            if (on && rank == 0u) start = atomicAdd(&cs[b], (uint32_t)__popcll(peers));
+            const int leader = on ? (int)__builtin_ctzll(peers) : lane;
+            start = (uint32_t)__shfl((int)start, leader, 64);
+            if (on) {
+                const uint32_t slot = start + rank;
+                if (slot < cap) {
+                    l1_ids[slot] = g;
+                    l1_mask[slot] = (uint16_t)m;
                 }
             }
         }
+    }
+}
+
+// Level 2: one workgroup per tile takes its block's list in rank order and keeps the
+// entries whose mask has the tile's bit (it reads every mask, and only the kept ids),
+// compacted by wave ballots (a wave's kept entries
+// are numbered by popcount, the 4 waves' counts meet in LDS), so the tile's list is written
+// front to back by one workgroup. Workgroups are numbered so that each XCD (workgroups
+// x, x + 8, ...) takes a contiguous run of blocks, and a block's 16 tiles run back to back
+// on one XCD, where the block list they all read stays in L2. A tile whose list crosses
+// the capacity holds the empty record N in its slots below it; the tiles past it have
+// empty, clamped bins.
+constexpr int COMPACT_THREADS = 256, COMPACT_PER = 4;
+__global__ __launch_bounds__(COMPACT_THREADS) void k_tile_compact(const ViewConsts* __restrict__ vc,
+                                                                  const uint32_t* __restrict__ boff,
+                                                                  const uint32_t* __restrict__ l1_ids,
+                                                                  const uint16_t* __restrict__ l1_mask,
+                                                                  const uint32_t* __restrict__ exact_off,
+                                                                  const uint32_t* __restrict__ counts,
+                                                                  uint32_t* __restrict__ ids, uint32_t cap,
+                                                                  uint32_t empty_id) {
+    constexpr int NW = COMPACT_THREADS / 64;
+    __shared__ uint32_t wcount[2][COMPACT_PER][NW];  // two rounds' counts: one barrier per round
+    const int tbx = vc->tbx, tby = vc->tby;
+    const BlockGeom bgm = block_geom(tbx, tby);
+    const int per = (int)(gridDim.x / 8) / (BLK * BLK);  // blocks per XCD
+    const int j = (int)(blockIdx.x >> 3);
+    const int b = (int)(blockIdx.x & 7) * per + j / (BLK * BLK), local = j % (BLK * BLK);
+    if (b >= bgm.NB) return;
+    const int tx = (b % bgm.nbx) * BLK + local % BLK, ty = (b / bgm.nbx) * BLK + local / BLK;
+    if (tx >= tbx || ty >= tby) return;
+    const int t = ty * tbx + tx;
+    const uint32_t off = exact_off[t], cnt = counts[t];
+    if (cnt == 0u) return;
+    const int tid = (int)threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    if (off + cnt > cap) {
+        for (uint32_t p = off + (uint32_t)tid; p < cap; p += COMPACT_THREADS) ids[p] = empty_id;
+        return;
+    }
+    const uint32_t lo = boff[b], n = boff[b + 1] - lo;
+    const uint16_t* masks = l1_mask + lo;
+    const uint32_t* lids = l1_ids + lo;
+    uint32_t* out = ids + off;
+    const uint32_t bit = 1u << local;
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    uint32_t run = 0u;
+    int par = 0;
+    for (uint32_t base = 0u; base < n; base += COMPACT_THREADS * COMPACT_PER) {
+        uint32_t mk[COMPACT_PER], v[COMPACT_PER];
+#pragma unroll
+        for (int k = 0; k < COMPACT_PER; k++) {
+            const uint32_t e = base + (uint32_t)(k * COMPACT_THREADS + tid);
+            mk[k] = e < n ? (uint32_t)masks[e] : 0u;
+        }
+        uint32_t rank[COMPACT_PER];
+        bool keep[COMPACT_PER];
+#pragma unroll
+        for (int k = 0; k < COMPACT_PER; k++) {  // only the kept entries' ids are read
+            keep[k] = (mk[k] & bit) != 0u;
+            v[k] = keep[k] ? lids[base + (uint32_t)(k * COMPACT_THREADS + tid)] : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < COMPACT_PER; k++) {
+            const uint64_t bal = __ballot(keep[k]);
+            rank[k] = (uint32_t)__popcll(bal & below);
+            if (lane == 0) wcount[par][k][wid] = (uint32_t)__popcll(bal);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < COMPACT_PER; k++) {  // entries in list order: round k, then wave, then lane
+            uint32_t before = 0u, total = 0u;
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                const uint32_t c = wcount[par][k][w];
+                before += w < wid ? c : 0u;
+                total += c;
+            }
+            const uint32_t p = run + before + rank[k];
+            if (keep[k] && p < cnt) out[p] = v[k];
+            run += total;
+        }
+        par ^= 1;
     }
 }
 
@@ -799,6 +948,17 @@ static int hist_blocks(int N, int target = 0) {
     const int by_max = (N + MAX_CHUNK - 1) / MAX_CHUNK;
     const int by_min = (N + MIN_CHUNK - 1) / MIN_CHUNK;
     const int nb = std::max(by_max, std::min(by_min, target > 0 ? target : CHUNK_TARGET));
+    return nb < 1 ? 1 : nb;
+}
+
+// The level-1 (block list) kernels cut the ranks finer: one wave per chunk and a few bytes
+// of LDS per block, so thousands of chunks fit at once (about 2048 chunks, at least 64
+// ranks each).
+constexpr int L1_MIN_CHUNK = 64, L1_CHUNK_TARGET = 2048;
+static int l1_blocks(int N) {
+    const int by_max = (N + MAX_CHUNK - 1) / MAX_CHUNK;
+    const int by_min = (N + L1_MIN_CHUNK - 1) / L1_MIN_CHUNK;
+    const int nb = std::max(by_max, std::min(by_min, L1_CHUNK_TARGET));
     return nb < 1 ? 1 : nb;
 }
 
@@ -948,8 +1108,15 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(int N, const uint32_t
 
 size_t lists_temp_bytes(int N) { return ((size_t)256 * (size_t)rs_blocks(N > 0 ? N : 1) + 256 + 64) * 4; }
 
-// [chunks x T] histogram / bucket offsets
-size_t lists_hist_bytes(int N, int T, int chunk_target) { return (size_t)hist_blocks(N, chunk_target) * (size_t)T * 4; }
+// [chunks x T] tile histogram rows, then (the same buffer) the [chunks x blocks] block
+// histogram and offsets
+size_t lists_hist_bytes(int N, int T, int chunk_target) {
+    return (size_t)std::max(hist_blocks(N, chunk_target), l1_blocks(N)) * (size_t)T * 4;  // blocks <= tiles
+}
+
+// 4x4-tile blocks of a tile grid
+static int blocks_of(const Geom& g) { return ((g.tbx + BLK - 1) / BLK) * ((g.tby + BLK - 1) / BLK); }
+size_t lists_block_bytes(const Geom& g) { return ((size_t)blocks_of(g) * 2 + 2) * 4; }
 
 static size_t g_lds_limit = 0;
 static size_t lds_limit() {
@@ -1019,9 +1186,9 @@ void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, 
         const int nblk = hist_blocks(N, L.chunk_target), G = (N + nblk - 1) / nblk;
         const size_t lds = (size_t)g.T * 4;
         allow_lds((const void*)k_tile_hist, lds, "tile histogram LDS");
-        hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, L.srect, vc, L.hist);
+        hipLaunchKernelGGL(k_tile_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, g.T, L.srect, vc, L.hist);
         check_launch("tile histogram launch");
-        hipLaunchKernelGGL(k_col_scan, dim3(cdiv(g.T, 64)), dim3(64 * COLSCAN_PARTS), 0, s, vc, nblk, L.hist,
+        hipLaunchKernelGGL(k_col_sum, dim3(cdiv(g.T, 64)), dim3(64 * COLSCAN_PARTS), 0, s, vc, nblk, g.T, L.hist,
                            L.tile_counts);
     } else {
         fill_zero(s, L.tile_counts, (size_t)g.T * 4);
@@ -1030,47 +1197,48 @@ void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, 
                        (uint32_t)L.capacity);
 }
 
-// One launch for the views whose tile count lies in [tmin, tmax], with the LDS of tmax tiles.
+// Level-1 bucket launch with WAVES waves (chunks) per workgroup (LDS by the launch's block
+// count NBL).
 template <int WAVES>
-static void launch_bucket(hipStream_t s, int nblk, int N, int G, const ViewConsts* vc, int tiles, int tmin, int tmax,
-                          ListBufs& L) {
-    const size_t lds = bucket_lds_bytes(tiles, WAVES);
-    allow_lds((const void*)k_tile_bucket<WAVES>, lds, "tile bucket LDS");
-    hipLaunchKernelGGL(k_tile_bucket<WAVES>, dim3(nblk), dim3(64 * WAVES), lds, s, N, G, L.sorted_ids, L.srect, vc,
-                       L.hist, L.exact_off, L.tile_counts, L.totals, L.ids, (uint32_t)L.capacity, tmin, tmax);
-    check_launch("tile bucket launch");
+static void launch_bucket(hipStream_t s, int nblk, int N, int G, int NBL, const ViewConsts* vc, ListBufs& L) {
+    const size_t lds = bucket_lds_bytes(NBL, WAVES);
+    allow_lds((const void*)k_block_bucket<WAVES>, lds, "block bucket LDS");
+    uint32_t* btot = L.blocks;
+    uint32_t* boff = L.blocks + NBL;
+    hipLaunchKernelGGL(k_block_bucket<WAVES>, dim3(cdiv(nblk, WAVES)), dim3(64 * WAVES), lds, s, N, G, nblk, NBL,
+                       L.sorted_ids, L.srect, vc, L.hist, btot, L.exact_off, L.tile_counts, L.totals, L.l1_ids,
+                       L.l1_mask, boff, (uint32_t)L.capacity);
+    check_launch("block bucket launch");
 }
 
 static std::atomic<int> g_bucket_waves{0};
 void set_bucket_waves(int waves) { g_bucket_waves.store(waves == 1 || waves == 2 || waves == 4 ? waves : 0); }
 
-static void bucket_class(hipStream_t s, int nblk, int N, int G, const ViewConsts* vc, int tiles, int tmin, int tmax,
-                         ListBufs& L) {
-    switch (bucket_waves(tiles, lds_limit(), L.bucket_waves > 0 ? L.bucket_waves : g_bucket_waves.load())) {
-        case 4: launch_bucket<4>(s, nblk, N, G, vc, tiles, tmin, tmax, L); break;
-        case 2: launch_bucket<2>(s, nblk, N, G, vc, tiles, tmin, tmax, L); break;
-        case 1: launch_bucket<1>(s, nblk, N, G, vc, tiles, tmin, tmax, L); break;
-        default: throw std::runtime_error("tile count exceeds the bucket kernel's LDS");
-    }
-}
-
 void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L) {
     if (N <= 0) return;
-    const int nblk = hist_blocks(N, L.chunk_target), G = (N + nblk - 1) / nblk;
-    if (!L.size_classes) {
-        bucket_class(s, nblk, N, G, vc, g.T, 0, INT_MAX, L);
-        return;
+    const int nblk = l1_blocks(N), G = (N + nblk - 1) / nblk;
+    const int NBL = blocks_of(g);
+    uint32_t* btot = L.blocks;
+    uint32_t* boff = L.blocks + NBL;
+    // level 1: block histogram (the tile histogram's buffer is free once the tile counts are
+    // summed), column offsets, bucket
+    const size_t lds = (size_t)NBL * 4;
+    allow_lds((const void*)k_block_hist, lds, "block histogram LDS");
+    hipLaunchKernelGGL(k_block_hist, dim3(nblk), dim3(HIST_THREADS), lds, s, N, G, NBL, L.srect, vc, L.exact_off,
+                       L.tile_counts, L.totals, (uint32_t)L.capacity, L.hist);
+    check_launch("block histogram launch");
+    hipLaunchKernelGGL(k_block_scan, dim3(cdiv(NBL, BSCAN_COLS)), dim3(BSCAN_COLS * BSCAN_PARTS), 0, s, vc, nblk, NBL,
+                       L.hist, btot);
+    switch (bucket_waves(NBL, lds_limit(), L.bucket_waves > 0 ? L.bucket_waves : g_bucket_waves.load())) {
+        case 4: launch_bucket<4>(s, nblk, N, G, NBL, vc, L); break;
+        case 2: launch_bucket<2>(s, nblk, N, G, NBL, vc, L); break;
+        case 1: launch_bucket<1>(s, nblk, N, G, NBL, vc, L); break;
+        default: throw std::runtime_error("block count exceeds the bucket kernel's LDS");
     }
-    // tile-count classes (0, 1024], (1024, 2048], ... up to the launch geometry's: the LDS of
-    // a launch (and so its resident workgroups) follows the class, not the largest view
-    // the sequence serves; the launches of the other classes exit at once
-    int lo = 0;
-    for (int hi = 1024;; hi *= 2) {
-        const int top = std::min(hi, g.T);
-        bucket_class(s, nblk, N, G, vc, top, lo + 1, top >= g.T ? INT_MAX : top, L);
-        if (top >= g.T) break;
-        lo = top;
-    }
+    // level 2: tile lists
+    hipLaunchKernelGGL(k_tile_compact, dim3(8 * cdiv(NBL, 8) * BLK * BLK), dim3(COMPACT_THREADS), 0, s, vc, boff,
+                       L.l1_ids, L.l1_mask, L.exact_off, L.tile_counts, L.ids, (uint32_t)L.capacity, (uint32_t)N);
+    check_launch("tile compaction launch");
 }
 
 // ------------------------------------------------------------------------------------

@@ -58,7 +58,7 @@ size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 struct Layout {
     size_t consts, nanc, totals, wmisc;
     size_t radii, tiles, depths, rect, aabb, recA, recB, recC;
-    size_t srect, dkeys, dkeys_out, dvals, sorted_ids, temp, hist, ids;
+    size_t srect, dkeys, dkeys_out, dvals, sorted_ids, temp, hist, ids, l1, l1m, blocks;
     size_t counts, exact, bins, wq;
     size_t chunkT, chunkC, chunkIdx;
     size_t total;
@@ -89,6 +89,9 @@ Layout layout(int N, const msk::Geom& g) {
     L.temp = take(msk::lists_temp_bytes(N));
     L.hist = take(msk::lists_hist_bytes(N, g.T));
     L.ids = take((size_t)L.capacity * 4);
+    L.l1 = take((size_t)L.capacity * 4);
+    L.l1m = take((size_t)L.capacity * 2);
+    L.blocks = take(msk::lists_block_bytes(g));
     L.counts = take(T * 4); L.exact = take(T * 4); L.bins = take(T * 8); L.wq = take(T * 4);
     if (L.K > 1) {
         L.chunkT = take((size_t)L.K * P * 4);
@@ -148,6 +151,9 @@ int render(const MsplatDeviceState& st, const MsplatCamera& cam, int degrees_to_
     lists.ids = static_cast<uint32_t*>(at(L.ids));
     lists.view = {lists.ids, pj.rec};
     lists.hist = static_cast<uint32_t*>(at(L.hist));
+    lists.l1_ids = static_cast<uint32_t*>(at(L.l1));
+    lists.l1_mask = static_cast<uint16_t*>(at(L.l1m));
+    lists.blocks = static_cast<uint32_t*>(at(L.blocks));
     lists.totals = static_cast<uint32_t*>(at(L.totals));
     lists.capacity = L.capacity;
     lists.num_tiles = g.T;

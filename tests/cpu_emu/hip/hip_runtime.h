@@ -215,6 +215,13 @@ inline uint64_t __ballot(int p) {
     return m;
 }
 
+// wave votes over the lanes that reach the call
+inline int __all(int p) {
+    const uint64_t live = __ballot(1);
+    return __ballot(p) == live ? 1 : 0;
+}
+inline int __any(int p) { return __ballot(p) != 0u ? 1 : 0; }
+
 template <class T> inline T __shfl_xor(T v, int m, int width = 64) {
     static_assert(sizeof(T) <= 8, "shuffle of a wide type");
     uint64_t b = 0;
@@ -222,6 +229,19 @@ template <class T> inline T __shfl_xor(T v, int m, int width = 64) {
     const unsigned l = emu_lane();
     const unsigned src = (l & ~(unsigned)(width - 1)) | ((l ^ (unsigned)m) & (unsigned)(width - 1));
     const uint64_t r = emu_exchange(b, src);
+    T o;
+    std::memcpy(&o, &r, sizeof(T));
+    return o;
+}
+
+// ds_bpermute: lane `src`'s value within the lane's width-wide segment (src per lane)
+template <class T> inline T __shfl(T v, int src, int width = 64) {
+    static_assert(sizeof(T) <= 8, "shuffle of a wide type");
+    uint64_t b = 0;
+    std::memcpy(&b, &v, sizeof(T));
+    const unsigned l = emu_lane();
+    const unsigned from = (l & ~(unsigned)(width - 1)) | ((unsigned)src & (unsigned)(width - 1));
+    const uint64_t r = emu_exchange(b, from);
     T o;
     std::memcpy(&o, &r, sizeof(T));
     return o;
@@ -315,6 +335,7 @@ inline int atomicMax(int* p, int v) {
 }
 inline int __clz(int x) { return x == 0 ? 32 : __builtin_clz((unsigned)x); }
 inline int __popc(unsigned x) { return __builtin_popcount(x); }
+inline int __popcll(unsigned long long x) { return __builtin_popcountll(x); }
 inline unsigned atomicOr(unsigned* p, unsigned v) { return __atomic_fetch_or(p, v, __ATOMIC_SEQ_CST); }
 inline uint64_t __umul64hi(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) >> 64); }
 

@@ -102,28 +102,31 @@ def test_dataset_errors_do_not_cross_the_boundary():
 
 
 def test_bucket_wave_choice_fits_lds():
-    """The bucket kernel's launch never asks for more LDS than the device allows: the choice
-    counts its whole dynamic LDS (4 B per tile of segment starts, 4 B per tile per wave pair
-    of cursors, and one 256 B idle-lane line per wave). Band edges: 2560x2048 (20480 tiles)
-    on a 160 KiB part, 1920x1080 (8160 tiles) on a 64 KiB part."""
+    """The level-1 bucket kernel's launch never asks for more LDS than the device allows: the
+    choice counts its whole dynamic LDS (each wave, one chunk, keeps 4 B of cursor per
+    block). Four waves by default wherever they fit. Band edges of the LDS formula at 160 KiB
+    and 64 KiB limits."""
     L = _lib.lib()
 
     def lds(T, w):
-        t4 = (T + 3) & ~3
-        return t4 * 4 * (1 if w == 1 else 1 + w // 2) + w * 256
+        return ((T + 3) & ~3) * 4 * w
     for limit in (65536, 163840):
-        for T in list(range(1, 60)) + list(range(8100, 8200)) + list(range(20400, 20500)) + [40000, 40960, 41000]:
+        for T in list(range(1, 60)) + list(range(4090, 4100)) + list(range(8100, 8200)) + list(range(10200, 10300)) + list(range(16350, 16400)) + [40000, 40960, 41000]:
             for req in (0, 1, 2, 4):
                 w = L.msplat_list_bucket_waves_for(T, limit, req)
                 if w == 0:
                     assert lds(T, 1) > limit
                     continue
                 assert lds(T, w) <= limit, (T, limit, req, w)
-                if req == 0:  # two waves wherever they fit
+                if req == 0:  # four waves wherever they fit
+                    assert w == (4 if lds(T, 4) <= limit else 2 if lds(T, 2) <= limit else 1)
+                if req == 2:
                     assert w == (2 if lds(T, 2) <= limit else 1)
                 if req == 1:
                     assert w == 1
-    assert L.msplat_list_bucket_waves_for(20480, 163840, 0) == 1
-    assert L.msplat_list_bucket_waves_for(20352, 163840, 0) == 2
-    assert L.msplat_list_bucket_waves_for(8160, 65536, 0) == 1
-    assert L.msplat_list_bucket_waves_for(8160, 163840, 4) == 4
+    assert L.msplat_list_bucket_waves_for(10240, 163840, 0) == 4
+    assert L.msplat_list_bucket_waves_for(10241, 163840, 0) == 2
+    assert L.msplat_list_bucket_waves_for(40960, 163840, 0) == 1
+    assert L.msplat_list_bucket_waves_for(40961, 163840, 0) == 0
+    assert L.msplat_list_bucket_waves_for(8160, 65536, 0) == 2
+    assert L.msplat_list_bucket_waves_for(425, 163840, 0) == 4
