@@ -1315,7 +1315,7 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
     __shared__ float4 sB[RB];
     __shared__ float sCx[RB];
     __shared__ uint8_t sM[RB];
-    __shared__ uint8_t sL[4][RB];
+    __shared__ __align__(4) uint8_t sL[4][RB + 4];  // + 4: a step reads its four indices as one word
     const int wid = (int)(threadIdx.x >> 6);
     const float ox = px - (float)tile_lx(), oy = py - (float)tile_ly();  // the tile's corner pixel
     T = 1.f;
@@ -1341,29 +1341,38 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
         // rejected by every lane): same accepted entries in the same order
         const int cnt = wave_entry_list<RB>(sM, sL[wid], n, [](int) { return true; });
         __syncthreads();
-        if (done) continue;
-        for (int k = 0; k < cnt; k++) {
-            const int e = sL[wid][k];
-            const float4 a = sA[e];
-            const float4 b = sB[e];
-            const float dx = a.x - px, dy = a.y - py;
-            const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
-            if (!ms_finite(sigma) || sigma < 0.f || sigma >= 5.55f) continue;
-            const float alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
-            if (alpha < 1.f / 255.f) continue;
-            const float nT = T * (1.f - alpha);
-            if (nT <= 1e-4f) {
-                last = b0 + e - 1;
-                done = true;
-                break;
+        // Four entries per step, without branches inside a step: their indices come in one
+        // LDS word and all their record reads are issued before the first is used, so a
+        // step waits on LDS twice instead of several times per entry. Every lane evaluates
+        // every entry; the per-pixel tests only select whether its result is taken, so the
+        // arithmetic of the taken entries is the same as entry by entry.
+        for (int k0 = 0; k0 < cnt; k0 += 4) {
+            if (__all(done)) break;  // wave-uniform
+            const uint32_t e4 = *reinterpret_cast<const uint32_t*>(&sL[wid][k0]);
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const bool have = k0 + u < cnt;  // the last step may hold fewer than four
+                const int e = have ? (int)((e4 >> (8 * u)) & 255u) : 0;
+                const float4 a = sA[e];
+                const float4 b = sB[e];
+                const float cz = sCx[e];
+                const float dx = a.x - px, dy = a.y - py;
+                const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
+                const bool in_range = ms_finite(sigma) && sigma >= 0.f && sigma < 5.55f;
+                const float alpha = fminf(0.999f, a.z * ms_expf_core(in_range ? -sigma : 0.f));
+                const bool take = have && !done && in_range && alpha >= 1.f / 255.f;
+                const float nT = T * (1.f - alpha);
+                const bool sat = take && nT <= 1e-4f;
+                const bool acc = take && !sat;
+                const float vis = alpha * T;
+                C[0] = acc ? fmaf(b.z, vis, C[0]) : C[0];
+                C[1] = acc ? fmaf(b.w, vis, C[1]) : C[1];
+                C[2] = acc ? fmaf(cz, vis, C[2]) : C[2];
+                T = acc ? nT : T;
+                last = acc ? b0 + e : (sat ? b0 + e - 1 : last);
+                taken += acc ? 1 : 0;
+                done = done || sat;
             }
-            const float vis = alpha * T;
-            C[0] = fmaf(b.z, vis, C[0]);
-            C[1] = fmaf(b.w, vis, C[1]);
-            C[2] = fmaf(sCx[e], vis, C[2]);
-            T = nT;
-            last = b0 + e;
-            taken++;
         }
     }
 }
@@ -1787,7 +1796,7 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
     __shared__ float2 sC[BB];
     __shared__ uint32_t sF[BB];
     __shared__ uint8_t sM[BB];
-    __shared__ uint8_t sL[4][BB];
+    __shared__ __align__(4) uint8_t sL[4][BB + 4];  // + 4: a step reads its four indices as one word
     __shared__ uint8_t sIn[4][BB];   // sIn[w][slot]: entry `slot` is on wave w's list
     __shared__ float red[4][BB][9];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -1825,55 +1834,51 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
         for (int j0 = 0; j0 < BB; j0 += 64) sIn[wid][j0 + lane] = 0;
         __syncthreads();
         for (int k = lane; k < cnt; k += 64) sIn[wid][sL[wid][k]] = 1;
-        // four entries at a time: their 4 x 9 terms are reduced together (wave_sum4_rows)
+        // four entries at a time: their 4 x 9 terms are reduced together (wave_sum4_rows).
+        // Their indices come in one LDS word and all their record reads are issued before
+        // the first is used; every lane evaluates every entry and the per-pixel tests only
+        // select whether its terms and transmittance update are taken (the same arithmetic
+        // as entry by entry for the entries taken, zeros for the others).
         for (int t4 = 0; t4 < cnt; t4 += 4) {
             float g[4][9];
+            const uint32_t e4 = *reinterpret_cast<const uint32_t*>(&sL[wid][t4]);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-#pragma unroll
-                for (int k = 0; k < 9; k++) g[u][k] = 0.f;
-                if (t4 + u >= cnt) continue;  // uniform
-                const int tt = sL[wid][t4 + u];
+                const bool have = t4 + u < cnt;  // the last step may hold fewer than four
+                const int tt = have ? (int)((e4 >> (8 * u)) & 255u) : 0;
                 const int e = bhi - 1 - tt;
-                bool valid = inside && e <= bin_final;
-                float alpha = 0.f, dx = 0.f, dy = 0.f;
                 const float4 a = sA[tt];
                 const float4 b = sB[tt];
-                if (valid) {
-                    dx = a.x - px;
-                    dy = a.y - py;
-                    const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
-                    if (sigma < 0.f || sigma >= 5.55f) {
-                        valid = false;
-                    } else {
-                        alpha = fminf(0.999f, a.z * ms_expf_core(-sigma));
-                        if (alpha < 1.f / 255.f) valid = false;
-                    }
-                }
-                if (valid && alpha < 0.999f) {
-                    const float ra = recip_nr(1.f - alpha);
-                    T *= ra;
-                    const float fac = alpha * T;
-                    const float r0 = b.z, r1 = b.w, r2 = sC[tt].x;
-                    const float q0 = fmaf(r0, T, -ra * bb0);
-                    const float q1 = fmaf(r1, T, -ra * bb1);
-                    const float q2 = fmaf(r2, T, -ra * bb2);
-                    const float v_alpha = q0 * vo0 + q1 * vo1 + q2 * vo2;
-                    bb0 = fmaf(r0, fac, bb0);
-                    bb1 = fmaf(r1, fac, bb1);
-                    bb2 = fmaf(r2, fac, bb2);
-                    const float v_sigma = -alpha * v_alpha;
-                    g[u][0] = fac * vo0;
-                    g[u][1] = fac * vo1;
-                    g[u][2] = fac * vo2;
-                    const float hs = 0.5f * v_sigma;
-                    g[u][3] = hs * (dx * dx);
-                    g[u][4] = hs * (dx * dy);
-                    g[u][5] = hs * (dy * dy);
-                    g[u][6] = v_sigma * fmaf(a.w, dx, b.x * dy);
-                    g[u][7] = v_sigma * fmaf(b.x, dx, b.y * dy);
-                    g[u][8] = -v_sigma * (1.f - a.z);
-                }
+                const float r2 = sC[tt].x;
+                const float dx = a.x - px, dy = a.y - py;
+                const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
+                const bool in_range = sigma >= 0.f && sigma < 5.55f;
+                const float alpha = fminf(0.999f, a.z * ms_expf_core(in_range ? -sigma : 0.f));
+                const bool use = have && inside && e <= bin_final && in_range && alpha >= 1.f / 255.f &&
+                                 alpha < 0.999f;
+                const float ra = recip_nr(1.f - (use ? alpha : 0.f));
+                const float Tn = use ? T * ra : T;
+                const float fac = alpha * Tn;
+                const float r0 = b.z, r1 = b.w;
+                const float q0 = fmaf(r0, Tn, -ra * bb0);
+                const float q1 = fmaf(r1, Tn, -ra * bb1);
+                const float q2 = fmaf(r2, Tn, -ra * bb2);
+                const float v_alpha = q0 * vo0 + q1 * vo1 + q2 * vo2;
+                bb0 = use ? fmaf(r0, fac, bb0) : bb0;
+                bb1 = use ? fmaf(r1, fac, bb1) : bb1;
+                bb2 = use ? fmaf(r2, fac, bb2) : bb2;
+                T = Tn;
+                const float v_sigma = -alpha * v_alpha;
+                const float hs = 0.5f * v_sigma;
+                g[u][0] = use ? fac * vo0 : 0.f;
+                g[u][1] = use ? fac * vo1 : 0.f;
+                g[u][2] = use ? fac * vo2 : 0.f;
+                g[u][3] = use ? hs * (dx * dx) : 0.f;
+                g[u][4] = use ? hs * (dx * dy) : 0.f;
+                g[u][5] = use ? hs * (dy * dy) : 0.f;
+                g[u][6] = use ? v_sigma * fmaf(a.w, dx, b.x * dy) : 0.f;
+                g[u][7] = use ? v_sigma * fmaf(b.x, dx, b.y * dy) : 0.f;
+                g[u][8] = use ? -v_sigma * (1.f - a.z) : 0.f;
             }
             // lane 15 of each row holds the sum of list entry t4 + tag
             const bool wr = (lane & 15) == 15 && t4 + tag < cnt;
