@@ -859,7 +859,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_block_bucket(int N, int G, int n
 }
 
 // Level 2: one workgroup per tile takes its block's list in rank order and keeps the
-// entries whose mask has the tile's bit (it reads every mask, and only the kept ids),
+// entries whose mask has the tile's bit (8 masks and ids per thread in flight per round),
 // compacted by wave ballots (a wave's kept entries
 // are numbered by popcount, the 4 waves' counts meet in LDS), so the tile's list is written
 // front to back by one workgroup. Workgroups are numbered so that each XCD (workgroups
@@ -867,7 +867,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_block_bucket(int N, int G, int n
 // on one XCD, where the block list they all read stays in L2. A tile whose list crosses
 // the capacity holds the empty record N in its slots below it; the tiles past it have
 // empty, clamped bins.
-constexpr int COMPACT_THREADS = 256, COMPACT_PER = 4;
+constexpr int COMPACT_THREADS = 256, COMPACT_PER = 8;
 __global__ __launch_bounds__(COMPACT_THREADS) void k_tile_compact(const ViewConsts* __restrict__ vc,
                                                                   const uint32_t* __restrict__ boff,
                                                                   const uint32_t* __restrict__ l1_ids,
@@ -903,19 +903,19 @@ __global__ __launch_bounds__(COMPACT_THREADS) void k_tile_compact(const ViewCons
     uint32_t run = 0u;
     int par = 0;
     for (uint32_t base = 0u; base < n; base += COMPACT_THREADS * COMPACT_PER) {
+        // masks and ids are read together (one wait per round rather than a mask wait and
+        // then an id wait)
         uint32_t mk[COMPACT_PER], v[COMPACT_PER];
 #pragma unroll
         for (int k = 0; k < COMPACT_PER; k++) {
             const uint32_t e = base + (uint32_t)(k * COMPACT_THREADS + tid);
             mk[k] = e < n ? (uint32_t)masks[e] : 0u;
+            v[k] = e < n ? lids[e] : 0u;
         }
         uint32_t rank[COMPACT_PER];
         bool keep[COMPACT_PER];
 #pragma unroll
-        for (int k = 0; k < COMPACT_PER; k++) {  // only the kept entries' ids are read
-            keep[k] = (mk[k] & bit) != 0u;
-            v[k] = keep[k] ? lids[base + (uint32_t)(k * COMPACT_THREADS + tid)] : 0u;
-        }
+        for (int k = 0; k < COMPACT_PER; k++) keep[k] = (mk[k] & bit) != 0u;
 #pragma unroll
         for (int k = 0; k < COMPACT_PER; k++) {
             const uint64_t bal = __ballot(keep[k]);
