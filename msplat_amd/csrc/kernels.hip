@@ -1347,7 +1347,7 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
         // every entry; the per-pixel tests only select whether its result is taken, so the
         // arithmetic of the taken entries is the same as entry by entry.
         for (int k0 = 0; k0 < cnt; k0 += 4) {
-            if (__all(done)) break;  // wave-uniform
+            if (__ballot(!done) == 0ull) break;  // wave-uniform: every pixel of the wave is done
             const uint32_t e4 = *reinterpret_cast<const uint32_t*>(&sL[wid][k0]);
 #pragma unroll
             for (int u = 0; u < 4; u++) {
@@ -1580,8 +1580,10 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
     const float ssim_w = vc->ssim_w;
     constexpr int TD = SB + 10;  // 26
     constexpr int TI = SB + 20;  // 36: input columns of the horizontal pass over the halo
-    __shared__ float sg[3][TD][TI];
-    __shared__ float sr[3][TD][TI];
+    // one channel of the target and render at a time (the channels are taken in turn), so
+    // the workgroup's LDS is 29 KB and five workgroups fit on a CU
+    __shared__ float sg[TD][TI];
+    __shared__ float sr[TD][TI];
     __shared__ float shp[TD][TD][5];
     __shared__ float sf[3][SB][TD];
     __shared__ float s_ssim[SB][SB];
@@ -1592,20 +1594,19 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
     const size_t P = (size_t)W * H;
     s_ssim[ly][lx] = 0.f;
     s_l1[ly][lx] = 0.f;
-    // target (x = gt / 255) and render of rows by .. by + 25, columns bx - 5 .. bx + 30 (0 outside)
-    for (int k = threadIdx.x; k < TD * TI; k += 256) {
-        const int sy = k / TI, sx = k % TI;
-        const int gy = by + sy, gxx = bx - 5 + sx;
-        const bool in = gxx >= 0 && gxx < W && gy >= 0 && gy < H;
-        const size_t pid = in ? (size_t)gy * W + gxx : 0;
-#pragma unroll
-        for (int c = 0; c < 3; c++) {
-            sg[c][sy][sx] = in ? (float)gt[3 * pid + c] * (1.0f / 255.0f) : 0.f;
-            sr[c][sy][sx] = in ? img[3 * pid + c] : 0.f;
-        }
-    }
-    __syncthreads();
     for (int c = 0; c < 3; c++) {
+        // channel c of the target (x = gt / 255) and render, rows by .. by + 25, columns
+        // bx - 5 .. bx + 30 (0 outside); the previous channel's readers passed the barrier
+        // that ends its iteration
+        for (int k = threadIdx.x; k < TD * TI; k += 256) {
+            const int sy = k / TI, sx = k % TI;
+            const int gy = by + sy, gxx = bx - 5 + sx;
+            const bool in = gxx >= 0 && gxx < W && gy >= 0 && gy < H;
+            const size_t pid = in ? (size_t)gy * W + gxx : 0;
+            sg[sy][sx] = in ? (float)gt[3 * pid + c] * (1.0f / 255.0f) : 0.f;
+            sr[sy][sx] = in ? img[3 * pid + c] : 0.f;
+        }
+        __syncthreads();
         // horizontal 11-tap statistics at the halo's pixels; zero outside the image (the
         // vertical pass's padding)
         for (int k = threadIdx.x; k < TD * TD; k += 256) {
@@ -1616,7 +1617,7 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
 #pragma unroll
                 for (int d = 0; d < 11; d++) {
                     const float w = c_gauss1d[d];
-                    const float gv = sg[c][sy][sx + d], rv = sr[c][sy][sx + d];
+                    const float gv = sg[sy][sx + d], rv = sr[sy][sx + d];
                     mx += w * gv;
                     my += w * rv;
                     sxq += w * gv * gv;
@@ -1665,7 +1666,7 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
             sf[2][dy][dx] = mval * dsxy;
             if (dx >= 5 && dx < 5 + SB && inimg) {
                 s_ssim[dy][dx - 5] += mval * (A * Bv) / (Cd * D);
-                s_l1[dy][dx - 5] += mval * fabsf(sg[c][dy + 5][dx + 5] - sr[c][dy + 5][dx + 5]);
+                s_l1[dy][dx - 5] += mval * fabsf(sg[dy + 5][dx + 5] - sr[dy + 5][dx + 5]);
             }
         }
         __syncthreads();

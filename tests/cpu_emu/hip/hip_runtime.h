@@ -215,13 +215,6 @@ inline uint64_t __ballot(int p) {
     return m;
 }
 
-// wave votes over the lanes that reach the call
-inline int __all(int p) {
-    const uint64_t live = __ballot(1);
-    return __ballot(p) == live ? 1 : 0;
-}
-inline int __any(int p) { return __ballot(p) != 0u ? 1 : 0; }
-
 template <class T> inline T __shfl_xor(T v, int m, int width = 64) {
     static_assert(sizeof(T) <= 8, "shuffle of a wide type");
     uint64_t b = 0;
