@@ -302,15 +302,13 @@ void Engine::flush_events() {
 }
 
 void Engine::front(const msk::Params& P, int degrees_to_use, bool train) {
-    const size_t n = (size_t)(P.N > 0 ? P.N : 1);
-    if (train) {
-        // accumulators of the raster backward (the reference blit-zeroes them, mm:1496-1512)
-        msk::fill_zero4(stream_, v_xy_.p, n * 8, v_conic_.p, n * 12, v_rgb_.p, n * 12, v_opac_.p, n * 4);
-        stage_event(ST_ZERO);
-    }
+    // accumulators of the raster backward (the reference blit-zeroes them, mm:1496-1512):
+    // zeroed by the projection, row by row
+    const msk::GradBufs zero{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
+    if (train) stage_event(ST_ZERO);
     msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
                     aabb_.as<float2>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
-    msk::project_forward(stream_, P, vcd_, degrees_to_use, pj, nanc_.as<uint32_t>());
+    msk::project_forward(stream_, P, vcd_, degrees_to_use, pj, nanc_.as<uint32_t>(), train ? &zero : nullptr);
     // (the projection also writes row N, the transparent record)
     stage_event(ST_PROJECT);
     lists_ = list_bufs();

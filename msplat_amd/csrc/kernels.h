@@ -130,8 +130,10 @@ struct RasterWork {
     int ppl = 1;               // pixels per lane: 1 (256-lane workgroup per tile) or 4 (one wave)
 };
 
+// zero (training steps): the raster backward's gradient accumulators, zeroed row by row by
+// the projection (one pass instead of a separate fill)
 void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, ProjOut& o,
-                     uint32_t* nan_counts);
+                     uint32_t* nan_counts, const GradBufs* zero = nullptr);
 size_t lists_temp_bytes(int N);
 size_t lists_hist_bytes(int N, int T, int chunk_target = 0);
 size_t lists_block_bytes(const Geom& g);

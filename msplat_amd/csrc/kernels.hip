@@ -244,7 +244,7 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
     const SHT* __restrict__ f_dc, const SHT* __restrict__ f_rest, const float* __restrict__ opac, int nb3,
     int deg_use, const ViewConsts* __restrict__ vc, int32_t* __restrict__ radii_o, int32_t* __restrict__ tiles_o,
     float* __restrict__ depth_o, ushort4* __restrict__ rect_o, float2* __restrict__ aabb_o, float4* __restrict__ rA,
-    float4* __restrict__ rB, float2* __restrict__ rC, uint32_t* __restrict__ nanc) {
+    float4* __restrict__ rB, float2* __restrict__ rC, uint32_t* __restrict__ nanc, GradBufs zero) {
     extern __shared__ uint4 s_raw[];  // PROJ_BLOCK * nb3 coefficients when deg_use > 0
     SHT* s_rest = reinterpret_cast<SHT*>(s_raw);
     const int i0 = blockIdx.x * PROJ_BLOCK;
@@ -269,6 +269,15 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
         __syncthreads();
     }
     if (i >= N) return;
+    if (zero.v_xy) {  // a training step: the raster backward's accumulators start at zero
+        zero.v_xy[i] = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            zero.v_conic[3 * (size_t)i + k] = 0.f;
+            zero.v_rgb[3 * (size_t)i + k] = 0.f;
+        }
+        zero.v_opac[i] = 0.f;
+    }
     const MsplatCamera cam = vc->cam;
     const int tbx = vc->tbx, tby = vc->tby;
 
@@ -417,7 +426,8 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
 }
 
 void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, ProjOut& o,
-                     uint32_t* nan_counts) {
+                     uint32_t* nan_counts, const GradBufs* zero) {
+    const GradBufs z = zero ? *zero : GradBufs{nullptr, nullptr, nullptr, nullptr};
     if (P.N <= 0) {  // no Gaussians: only the transparent record (row 0)
         fill_zero4(s, o.rec.A, 16, o.rec.B, 16, o.rec.C, 8, nullptr, 0);
         return;
@@ -429,11 +439,11 @@ void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int d
     if (P.sh_bf16)
         hipLaunchKernelGGL(k_project<uint16_t>, dim3(cdiv(P.N, PROJ_BLOCK)), dim3(PROJ_BLOCK), lds, s, P.N, P.means,
                            P.scales, P.quats, P.f_dc_h, P.f_rest_h, P.opac, nb3, du, vc, o.radii, o.tiles_hit,
-                           o.depths, o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
+                           o.depths, o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts, z);
     else
         hipLaunchKernelGGL(k_project<float>, dim3(cdiv(P.N, PROJ_BLOCK)), dim3(PROJ_BLOCK), lds, s, P.N, P.means,
                            P.scales, P.quats, P.f_dc, P.f_rest, P.opac, nb3, du, vc, o.radii, o.tiles_hit, o.depths,
-                           o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts);
+                           o.rect, o.aabb, o.rec.A, o.rec.B, o.rec.C, nan_counts, z);
 }
 
 Geom geom_of(int W, int H) {
