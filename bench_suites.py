@@ -15,7 +15,7 @@ c2          batched multi-view: 2M clustered Gaussians, 64 cameras whose sizes a
             per direction over all views ("persistent"), that recorded once as a HIP graph
             and replayed ("graph_default"), and the graph with the launch geometry autotuned
             ("graph"); views/s of each (over all steps, after step 10, and after step 60,
-            which is past the autotuner's 44 tuning steps) and the batch loss / held-out PSNR
+            which is past the autotuner's 48 tuning steps) and the batch loss / held-out PSNR
             after 500 steps (equal up to float summation order).
 c4          learned relocation proposal: 2M clustered Gaussians at 1600x1040, 8 withheld
             views; the plain GPU form (one view per step, opacity-weighted relocation), 8-view
@@ -270,7 +270,7 @@ def c2(a):
         t1 = time.perf_counter()
         _, losses = tr.last_batch(losses=True)
         m = tr.evaluate()
-        # steps 61.. in every mode: after the autotuner's 44 tuning steps in the tuned mode
+        # steps 61.. in every mode: after the autotuner's 48 tuning steps in the tuned mode
         r = {"views_per_s": round(steps * views / (t1 - t0), 2),
              "views_per_s_after_10": round((steps - 10) * views / (t1 - t10), 2),
              "views_per_s_after_60": round((steps - 60) * views / (t1 - t60), 2),

@@ -117,14 +117,19 @@ int64_t msplat_trainer_proposal_read(MsplatTrainer t, const char* name, void* ho
  * Returns 0 / -1. */
 int msplat_trainer_batch_info(MsplatTrainer t, int64_t out[4]);
 /* Launch autotuner of the batched steps (msplat_trainer_set_option(t, "autotune", 1)): the
- * candidates timed so far, four int32 each in cands[4k..4k+3] = {pixels per lane of the
+ * candidates timed so far, five int32 each in cands[5k..5k+4] = {pixels per lane of the
  * persistent raster kernels (the raster sub-tile: 1 of a 256-lane workgroup per tile, or 4 of
- * one wave per tile), their resident workgroups, rank chunks of the tile-list kernels
- * (0 = default 768), waves per bucket workgroup (0 = default)}, with ms per megapixel of
- * the batch in ms[k]; once tuning has finished the chosen candidate in chosen[0..3] (zeros
- * before). Stage one times 7 raster choices, stage two 4 list choices around the best.
- * Returns the number of timed candidates (at most max written), or -1. */
-int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* cands, float* ms, int max, int32_t chosen[4]);
+ * one wave per tile), their resident workgroups, rank chunks of the tile histogram
+ * (0 = default 768), waves per block-list bucket workgroup (0 = default 4), 1 = the step
+ * replayed from a recorded graph / 0 = launched eagerly}, with its score in ms[k]: the mean
+ * over its timed steps of the measured time / the time a least-squares line through all
+ * timed steps (ms = a + b * pixels of the batch) gives for that step's pixels (1 = an
+ * average step, lower is faster); once tuning has finished the chosen candidate in
+ * chosen[0..4] (zeros before).
+ * Stage one times 7 raster choices, stage two 4 list choices around the best, and, when
+ * batch_graph is on, stage three the best so far launched eagerly. Returns the number of
+ * timed candidates (at most max written), or -1. */
+int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* cands, float* ms, int max, int32_t chosen[5]);
 
 /* ---- Trainer introspection (tests, benchmarks) --------------------------------------- */
 /* The scalars Model::fullIteration would hand to the kernels for (camera, step, adam step). */

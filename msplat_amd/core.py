@@ -278,14 +278,16 @@ class GaussianTrainer:
 
     def autotune_info(self) -> dict:
         """Timed candidates of the batched-step autotuner, each (pixels per lane, raster
-        workgroups, list rank chunks, bucket waves, ms per megapixel), and the chosen
-        (pixels per lane, workgroups, chunks, waves) once tuning has finished."""
-        cands = (C.c_int32 * (4 * 32))()
+        workgroups, list rank chunks, bucket waves, recorded graph 1 / eager 0, score: its
+        steps' time relative to a least-squares fit of step time on pixels over all timed
+        steps, lower is faster), and the chosen (pixels per lane, workgroups, chunks, waves,
+        graph) once tuning has finished."""
+        cands = (C.c_int32 * (5 * 32))()
         ms = np.zeros(32, np.float32)
-        ch = (C.c_int32 * 4)()
+        ch = (C.c_int32 * 5)()
         n = _lib.lib().msplat_trainer_autotune_info(self._h, cands, fptr(ms), 32, ch)
-        return {"timed": [tuple(int(cands[4 * k + j]) for j in range(4)) + (float(ms[k]),) for k in range(max(n, 0))],
-                "chosen": tuple(int(ch[j]) for j in range(4))}
+        return {"timed": [tuple(int(cands[5 * k + j]) for j in range(5)) + (float(ms[k]),) for k in range(max(n, 0))],
+                "chosen": tuple(int(ch[j]) for j in range(5))}
 
     def batch_info(self) -> dict:
         """Recorded-graph counters of the batched steps (msplat_trainer_batch_info)."""

@@ -694,19 +694,20 @@ int msplat_trainer_batch_info(MsplatTrainer t, int64_t out[4]) {
     return 0;
 }
 
-int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* cands, float* ms, int max, int32_t chosen[4]) {
+int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* cands, float* ms, int max, int32_t chosen[5]) {
     if (!t) return -1;
     msplat_hip::BatchRunner* b = TR(t)->model->batch_runner();
     if (chosen)
-        for (int j = 0; j < 4; j++) chosen[j] = 0;
+        for (int j = 0; j < 5; j++) chosen[j] = 0;
     if (!b) return 0;
     const auto& r = b->tune_results();
     for (int k = 0; k < (int)r.size() && k < max; k++) {
         if (cands) {
-            cands[4 * k + 0] = r[k].c.ppl;
-            cands[4 * k + 1] = r[k].c.grid;
-            cands[4 * k + 2] = r[k].c.chunks;
-            cands[4 * k + 3] = r[k].c.waves;
+            cands[5 * k + 0] = r[k].c.ppl;
+            cands[5 * k + 1] = r[k].c.grid;
+            cands[5 * k + 2] = r[k].c.chunks;
+            cands[5 * k + 3] = r[k].c.waves;
+            cands[5 * k + 4] = r[k].c.graph;
         }
         if (ms) ms[k] = r[k].ms;
     }
@@ -716,6 +717,7 @@ int msplat_trainer_autotune_info(MsplatTrainer t, int32_t* cands, float* ms, int
         chosen[1] = c.grid;
         chosen[2] = c.chunks;
         chosen[3] = c.waves;
+        chosen[4] = c.graph;
     }
     return (int)r.size();
 }

@@ -69,6 +69,7 @@ void BatchRunner::apply(const Choice& c) {
     set_raster_ppl(c.ppl);
     set_persist_grid(c.grid);
     set_list_choice(c.chunks, c.waves);
+    graph_choice_ = c.graph != 0;
 }
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -122,28 +123,61 @@ void BatchRunner::set_autotune(bool on) {
     tune_done_ = !on;
     tune_step_ = 0;
     tune_ms_.clear();
+    tune_obs_.clear();
     tune_cands_.clear();
     if (on) {
         const int cus = default_persist_grid() / 8;  // the default is 8 workgroups per CU
-        for (int k : {2, 4, 8, 16}) tune_cands_.push_back({1, cus * k, 0, 0});
-        for (int k : {8, 16, 32}) tune_cands_.push_back({4, cus * k, 0, 0});
+        // the default first: a later candidate replaces the best only by a clear margin
+        for (int k : {8, 2, 4, 16}) tune_cands_.push_back({1, cus * k, 0, 0, 1});
+        for (int k : {8, 16, 32}) tune_cands_.push_back({4, cus * k, 0, 0, 1});
     }
     tune_stage1_ = (int)tune_cands_.size();
+    tune_stage2_ = 0;
+    tune_stage_lo_ = 0;
+    tune_stage_step_ = 0;
 }
 
 // Stage two's candidates around the best of stage one (whose default list choice is
-// already timed): rank chunks 384 / 1536 (default 768) and 4 waves per bucket workgroup.
+// already timed): tile-histogram rank chunks 384 / 1536 (default 768) and 2 waves per
+// block-list bucket workgroup (default 4).
 static std::vector<BatchRunner::Choice> stage_two(const BatchRunner::Choice& b) {
     std::vector<BatchRunner::Choice> out;
-    for (const auto& cw : {std::make_pair(384, 0), std::make_pair(1536, 0), std::make_pair(0, 4), std::make_pair(384, 4)})
-        out.push_back({b.ppl, b.grid, cw.first, cw.second});
+    for (const auto& cw : {std::make_pair(384, 0), std::make_pair(1536, 0), std::make_pair(0, 2), std::make_pair(384, 2)})
+        out.push_back({b.ppl, b.grid, cw.first, cw.second, 1});
     return out;
 }
 
 // Each candidate runs kTuneSteps steps: the first records its graph (untimed), the others
-// are timed and normalised by the pixels of their views, since the views (and their sizes)
-// change from step to step.
+// are timed (a stage's candidates take turns). The views (and their sizes) change from step to step, so a step's time is
+// judged against what a step of its pixel count takes: a least-squares line
+// ms = a + b * pixels through every timed step so far (all candidates: a is the per-step
+// cost that does not scale with pixels, the Gaussians' projection, lists and update), and a
+// candidate's score is the mean of its steps' measured / fitted ms (1 = an average step).
 constexpr int kTuneSteps = 4;
+
+void BatchRunner::tune_score() {
+    double n = 0.0, sx = 0.0, sy = 0.0, sxx = 0.0, sxy = 0.0;
+    for (const TuneObs& o : tune_obs_) {
+        n += 1.0;
+        sx += o.px;
+        sy += o.ms;
+        sxx += o.px * o.px;
+        sxy += o.px * o.ms;
+    }
+    const double den = n * sxx - sx * sx;
+    double b = den > 0.0 ? (n * sxy - sx * sy) / den : 0.0;
+    double a = n > 0.0 ? (sy - b * sx) / n : 0.0;
+    if (!(a > 0.0) || !(b >= 0.0)) {  // a degenerate fit: time per pixel
+        a = 0.0;
+        b = sx > 0.0 ? sy / sx : 1.0;
+    }
+    std::vector<double> sum(tune_ms_.size(), 0.0), cnt(tune_ms_.size(), 0.0);
+    for (const TuneObs& o : tune_obs_) {
+        sum[o.c] += o.ms / (a + b * o.px);
+        cnt[o.c] += 1.0;
+    }
+    for (size_t k = 0; k < tune_ms_.size(); k++) tune_ms_[k].ms = cnt[k] > 0.0 ? (float)(sum[k] / cnt[k]) : 0.f;
+}
 
 void BatchRunner::run(const MsplatDeviceState& st, const BatchView* views, int B, const MsplatStepParams& hp,
                       msk::BatchGrads A, uint32_t* nan_counts) {
@@ -151,9 +185,15 @@ void BatchRunner::run(const MsplatDeviceState& st, const BatchView* views, int B
         run_inner(st, views, B, hp, A, nan_counts);
         return;
     }
-    const int c = tune_step_ / kTuneSteps;
+    // Within a stage the candidates take turns, step by step (the scene and so the step time
+    // drift while tuning runs; taking turns spreads that drift over all of them): the first
+    // round records each candidate's graph, the others are timed.
+    const int lo = tune_stage_lo_, ns = (int)tune_cands_.size() - lo;
+    const int t = tune_step_ - tune_stage_step_;
+    const int c = lo + t % ns;
     apply(tune_cands_[c]);
-    const bool timed = (tune_step_ % kTuneSteps) != 0;
+    const bool timed = t >= ns;
+    while ((int)tune_ms_.size() < (int)tune_cands_.size()) tune_ms_.push_back({tune_cands_[tune_ms_.size()], 0.f});
     if (timed) {
         if (!tev_[0]) {
             hip_check(hipEventCreate(&tev_[0]), "hipEventCreate");
@@ -169,22 +209,30 @@ void BatchRunner::run(const MsplatDeviceState& st, const BatchView* views, int B
         hip_check(hipEventElapsedTime(&ms, tev_[0], tev_[1]), "hipEventElapsedTime");
         double px = 0.0;
         for (auto& wh : sizes_) px += (double)wh.first * wh.second;
-        if ((int)tune_ms_.size() <= c) tune_ms_.push_back({tune_cands_[c], 0.f});
-        tune_ms_[c].ms += (float)(ms / (px * 1e-6));  // ms per megapixel, summed over the timed steps
+        tune_obs_.push_back({c, (double)ms, px});
     }
+    // Scores within 1% of each other are within the noise of a few steps of changing views:
+    // the earlier candidate (the defaults come first) keeps its place unless beaten by more.
     auto best_of = [&](int upto) {
         TuneResult best = tune_ms_.front();
         for (int k = 0; k < upto; k++)
-            if (tune_ms_[k].ms < best.ms) best = tune_ms_[k];
+            if (tune_ms_[k].ms < 0.99f * best.ms) best = tune_ms_[k];
         return best;
     };
-    if (++tune_step_ >= kTuneSteps * (int)tune_cands_.size()) {
+    if (++tune_step_ - tune_stage_step_ >= kTuneSteps * ns) {
+        tune_score();
+        tune_stage_lo_ = (int)tune_cands_.size();
+        tune_stage_step_ = tune_step_;
         if ((int)tune_cands_.size() == tune_stage1_) {
-            // stage one done: averages, then stage two around its best
-            for (auto& r : tune_ms_) r.ms /= (float)(kTuneSteps - 1);
+            // stage one done: stage two around its best
             for (const Choice& ch : stage_two(best_of(tune_stage1_).c)) tune_cands_.push_back(ch);
+            tune_stage2_ = (int)tune_cands_.size();
+        } else if ((int)tune_cands_.size() == tune_stage2_ && graph_on_) {
+            // stage two done: the best so far, launched eagerly
+            Choice eager = best_of((int)tune_ms_.size()).c;
+            eager.graph = 0;
+            tune_cands_.push_back(eager);
         } else {
-            for (int k = tune_stage1_; k < (int)tune_ms_.size(); k++) tune_ms_[k].ms /= (float)(kTuneSteps - 1);
             apply(best_of((int)tune_ms_.size()).c);
             tune_done_ = true;
         }
@@ -222,7 +270,7 @@ void BatchRunner::run_inner(const MsplatDeviceState& st, const BatchView* views,
         sizes_[k] = {views[order[k]].cam.width, views[order[k]].cam.height};
         slot_of_view_[order[k]] = k;
     }
-    bool graphable = graph_on_ && !sequential_;
+    bool graphable = graph_on_ && graph_choice_ && !sequential_;
     if ((int)hwv_.size() < B) hwv_.resize(B, msk::Geom{});
     if (graphable) {
         for (int k = 0; k < B; k++) {

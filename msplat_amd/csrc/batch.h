@@ -44,16 +44,18 @@ public:
     // Autotuner of the launch choices the results do not depend on. Stage one: pixels per
     // lane (the raster sub-tile a lane covers: 1 pixel of a 256-lane workgroup per tile, or
     // 2x2 pixels of one wave per tile) x residency of the persistent raster kernels. Stage
-    // two, with the best of stage one: rank chunks of the tile-list count / bucket kernels
-    // x waves per bucket workgroup. The first steps cycle through the candidates, each timed
-    // per megapixel over three steps after the one that records its graph, and the fastest
-    // is kept for the remaining steps.
+    // two, with the best of stage one: rank chunks of the tile histogram x waves per
+    // block-list bucket workgroup. Stage three (when graphs are on): the best so far, run
+    // eagerly instead of replayed from a recorded graph. The first steps cycle through the
+    // candidates, each timed per megapixel over three steps after the one that records its
+    // graph, and the fastest is kept for the remaining steps.
     void set_autotune(bool on);
     bool tuned() const { return tune_done_; }
     struct Choice {
         int ppl = 1, grid = 0, chunks = 0, waves = 0;
+        int graph = 1;  // replay a recorded graph (when graphs are on), else launch eagerly
         bool operator==(const Choice& o) const {
-            return ppl == o.ppl && grid == o.grid && chunks == o.chunks && waves == o.waves;
+            return ppl == o.ppl && grid == o.grid && chunks == o.chunks && waves == o.waves && graph == o.graph;
         }
     };
     struct TuneResult {
@@ -61,7 +63,7 @@ public:
         float ms;
     };
     const std::vector<TuneResult>& tune_results() const { return tune_ms_; }
-    Choice choice() const { return {ppl_, persist_grid_, chunks_, bucket_waves_}; }
+    Choice choice() const { return {ppl_, persist_grid_, chunks_, bucket_waves_, graph_choice_ ? 1 : 0}; }
     void set_raster_ppl(int p);
     int raster_ppl() const { return ppl_; }
     // Rank chunks of the tile-list kernels and waves per bucket workgroup (0: defaults).
@@ -99,6 +101,7 @@ private:
     DBuf wq_, wmisc_;     // batch work queue
     ConstRing ring_;
     bool graph_on_ = false;
+    bool graph_choice_ = true;  // the autotuner's choice (replay or eager) when graphs are on
     bool sequential_ = false;
     int persist_grid_ = 0;
     msk::Geom hw_{};                                 // high-water launch geometry (graph mode)
@@ -107,7 +110,15 @@ private:
     int tune_step_ = 0;
     std::vector<Choice> tune_cands_;
     std::vector<TuneResult> tune_ms_;
+    struct TuneObs {
+        int c;
+        double ms, px;
+    };
+    std::vector<TuneObs> tune_obs_;  // every timed step: candidate, ms, pixels of its views
+    void tune_score();
     int tune_stage1_ = 0;  // candidates of stage one
+    int tune_stage2_ = 0;  // candidates of stages one and two
+    int tune_stage_lo_ = 0, tune_stage_step_ = 0;  // the running stage's first candidate and step
     int ppl_ = 1, chunks_ = 0, bucket_waves_ = 0;
     void apply(const Choice& c);
     hipEvent_t tev_[2] = {nullptr, nullptr};

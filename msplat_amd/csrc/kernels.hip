@@ -890,8 +890,12 @@ __global__ __launch_bounds__(COMPACT_THREADS) void k_tile_compact(const ViewCons
     __shared__ uint32_t wcount[2][COMPACT_PER][NW];  // two rounds' counts: one barrier per round
     const int tbx = vc->tbx, tby = vc->tby;
     const BlockGeom bgm = block_geom(tbx, tby);
-    const int per = (int)(gridDim.x / 8) / (BLK * BLK);  // blocks per XCD
+    // blocks per XCD, from the view's own block count (a recorded launch may be sized for a
+    // larger view: its spare workgroups exit, and the view's blocks still spread over all
+    // eight XCDs)
+    const int per = (bgm.NB + 7) / 8;
     const int j = (int)(blockIdx.x >> 3);
+    if (j >= per * BLK * BLK) return;
     const int b = (int)(blockIdx.x & 7) * per + j / (BLK * BLK), local = j % (BLK * BLK);
     if (b >= bgm.NB) return;
     const int tx = (b % bgm.nbx) * BLK + local % BLK, ty = (b / bgm.nbx) * BLK + local / BLK;
