@@ -1816,9 +1816,10 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
     __shared__ float red[4][BB][9];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const float ox = px - (float)tile_lx(), oy = py - (float)tile_ly();  // the tile's corner pixel
-    // the colour accumulated behind the current entry plus the background term T_final bg:
-    // the two only ever enter as their sum
-    float bb0 = fmaf(T_final, bg[0], buf0), bb1 = fmaf(T_final, bg[1], buf1), bb2 = fmaf(T_final, bg[2], buf2);
+    // the colour accumulated behind the current entry plus the background term T_final bg
+    // enters the alpha gradient only as its dot product with the pixel's colour gradient:
+    // S = sum_c (buf_c + T_final bg_c) vo_c, advanced by fac * (colour . vo) per entry taken
+    float S = fmaf(fmaf(T_final, bg[2], buf2), vo2, fmaf(fmaf(T_final, bg[1], buf1), vo1, fmaf(T_final, bg[0], buf0) * vo0));
     const int top = min(hi, tile_max + 1);  // entries past every pixel's last contributor are skipped
     // the last contributor of any pixel of this wave: entries behind it give the wave zeros
     int wmax = inside ? bin_final : INT_MIN;
@@ -1871,37 +1872,44 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
                 const float alpha = fminf(0.999f, a.z * ms_expf_core(in_range ? -sigma : 0.f));
                 const bool use = have && inside && e <= bin_final && in_range && alpha >= 1.f / 255.f &&
                                  alpha < 0.999f;
-                const float ra = recip_nr(1.f - (use ? alpha : 0.f));
-                const float Tn = use ? T * ra : T;
-                const float fac = alpha * Tn;
-                const float r0 = b.z, r1 = b.w;
-                const float q0 = fmaf(r0, Tn, -ra * bb0);
-                const float q1 = fmaf(r1, Tn, -ra * bb1);
-                const float q2 = fmaf(r2, Tn, -ra * bb2);
-                const float v_alpha = q0 * vo0 + q1 * vo1 + q2 * vo2;
-                bb0 = use ? fmaf(r0, fac, bb0) : bb0;
-                bb1 = use ? fmaf(r1, fac, bb1) : bb1;
-                bb2 = use ? fmaf(r2, fac, bb2) : bb2;
+                // one select: an entry not taken has alpha 0, so ra = 1 exactly, T and the
+                // colour sums stay as they are and every term below is a product with 0 (all
+                // factors are finite), which is what selecting each of them would give
+                const float au = use ? alpha : 0.f;
+                const float ra = recip_nr(1.f - au);
+                const float Tn = T * ra;
+                const float fac = au * Tn;
+                // d loss / d alpha = sum_c (colour_c Tn - ra (buf_c + T_final bg_c)) vo_c
+                const float rv = fmaf(r2, vo2, fmaf(b.w, vo1, b.z * vo0));
+                const float v_alpha = fmaf(Tn, rv, -ra * S);
+                S = fmaf(fac, rv, S);
                 T = Tn;
-                const float v_sigma = -alpha * v_alpha;
-                const float hs = 0.5f * v_sigma;
-                g[u][0] = use ? fac * vo0 : 0.f;
-                g[u][1] = use ? fac * vo1 : 0.f;
-                g[u][2] = use ? fac * vo2 : 0.f;
-                g[u][3] = use ? hs * (dx * dx) : 0.f;
-                g[u][4] = use ? hs * (dx * dy) : 0.f;
-                g[u][5] = use ? hs * (dy * dy) : 0.f;
-                g[u][6] = use ? v_sigma * fmaf(a.w, dx, b.x * dy) : 0.f;
-                g[u][7] = use ? v_sigma * fmaf(b.x, dx, b.y * dy) : 0.f;
-                g[u][8] = use ? -v_sigma * (1.f - a.z) : 0.f;
+                // the entry's own constants (conic, opacity) multiply the wave sums once per
+                // entry after the reduction: per pixel only v_sigma and its products with the
+                // offsets are summed
+                const float vs = -au * v_alpha;  // d loss / d sigma
+                const float vdx = vs * dx, vdy = vs * dy;
+                g[u][0] = fac * vo0;
+                g[u][1] = fac * vo1;
+                g[u][2] = fac * vo2;
+                g[u][3] = vdx * dx;
+                g[u][4] = vdx * dy;
+                g[u][5] = vdy * dy;
+                g[u][6] = vdx;
+                g[u][7] = vdy;
+                g[u][8] = vs;
             }
             // lane 15 of each row holds the sum of list entry t4 + tag
             const bool wr = (lane & 15) == 15 && t4 + tag < cnt;
             const int slot = wr ? sL[wid][t4 + tag] : 0;
+            // the nine reductions are independent: all of them first (their DPP and permlane
+            // steps interleave), then one masked store sequence
+            float v[9];
 #pragma unroll
-            for (int k = 0; k < 9; k++) {
-                const float v = wave_sum4_rows(g[0][k], g[1][k], g[2][k], g[3][k]);
-                if (wr) red[wid][slot][k] = v;
+            for (int k = 0; k < 9; k++) v[k] = wave_sum4_rows(g[0][k], g[1][k], g[2][k], g[3][k]);
+            if (wr) {
+#pragma unroll
+                for (int k = 0; k < 9; k++) red[wid][slot][k] = v[k];
             }
         }
         __syncthreads();
@@ -1919,17 +1927,22 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
 #pragma unroll
             for (int k = 0; k < 9; k++) any |= (s[k] != 0.f);
             if (any) {
+                // sums of v_sigma dx^2, dx dy, dy^2, dx, dy and v_sigma -> conic, mean and
+                // opacity-logit gradients (d sigma / d mean = conic (dx, dy); the logit's
+                // derivative of alpha is alpha (1 - opacity))
+                const float4 a = sA[tt];
+                const float4 b = sB[tt];
                 const uint32_t f = sF[tt];
                 const uint32_t gid = __builtin_bit_cast(uint32_t, sC[tt].y);
                 if (f & 1u) atomicAdd(&G.v_rgb[3 * (size_t)gid + 0], s[0]);
                 if (f & 2u) atomicAdd(&G.v_rgb[3 * (size_t)gid + 1], s[1]);
                 if (f & 4u) atomicAdd(&G.v_rgb[3 * (size_t)gid + 2], s[2]);
-                atomicAdd(&G.v_conic[3 * (size_t)gid + 0], s[3]);
-                atomicAdd(&G.v_conic[3 * (size_t)gid + 1], s[4]);
-                atomicAdd(&G.v_conic[3 * (size_t)gid + 2], s[5]);
-                atomicAdd(&G.v_xy[gid].x, s[6]);
-                atomicAdd(&G.v_xy[gid].y, s[7]);
-                atomicAdd(&G.v_opac[gid], s[8]);
+                atomicAdd(&G.v_conic[3 * (size_t)gid + 0], 0.5f * s[3]);
+                atomicAdd(&G.v_conic[3 * (size_t)gid + 1], 0.5f * s[4]);
+                atomicAdd(&G.v_conic[3 * (size_t)gid + 2], 0.5f * s[5]);
+                atomicAdd(&G.v_xy[gid].x, fmaf(a.w, s[6], b.x * s[7]));
+                atomicAdd(&G.v_xy[gid].y, fmaf(b.x, s[6], b.y * s[7]));
+                atomicAdd(&G.v_opac[gid], -(1.f - a.z) * s[8]);
             }
         }
     }
@@ -2167,24 +2180,21 @@ __global__ __launch_bounds__(RB4) void k_raster_bwd_persist4(const RasterView* _
         float px[4], py[4];
         int ix[4], iy[4];
         px4_coords(t, vc.tbx, px, py, ix, iy);
-        float T[4], buf[4][3], Tf[4], vo[4][3];
+        float T[4], S[4], vo[4][3];
         int bf[4];
         bool inside[4];
         int mx = INT_MIN;
         for (int q = 0; q < 4; q++) {
             inside[q] = ix[q] < W && iy[q] < H;
             const size_t pid = inside[q] ? (size_t)iy[q] * W + ix[q] : 0;
-            Tf[q] = inside[q] ? rv.final_T[pid] : 1.f;
+            T[q] = inside[q] ? rv.final_T[pid] : 1.f;
             bf[q] = inside[q] ? rv.final_idx[pid] : -1;
-            for (int c = 0; c < 3; c++) {
-                vo[q][c] = inside[q] ? rv.v_rend[3 * pid + c] : 0.f;
-                buf[q][c] = 0.f;
-            }
-            T[q] = Tf[q];
+            for (int c = 0; c < 3; c++) vo[q][c] = inside[q] ? rv.v_rend[3 * pid + c] : 0.f;
+            // sum_c (colour behind the entry + T_final bg_c) vo_c, as in backward_range
+            S[q] = fmaf(T[q] * vc.bg[2], vo[q][2], fmaf(T[q] * vc.bg[1], vo[q][1], T[q] * vc.bg[0] * vo[q][0]));
             if (inside[q]) mx = max(mx, bf[q]);
         }
         const int tmax = block_max_int(mx);
-        const float bgc[3] = {vc.bg[0], vc.bg[1], vc.bg[2]};
         const int top = min(r.y, tmax + 1);
         for (int bhi = top; bhi > r.x; bhi -= RB4) {
             const int blo = max(r.x, bhi - RB4);
@@ -2226,25 +2236,20 @@ __global__ __launch_bounds__(RB4) void k_raster_bwd_persist4(const RasterView* _
                     const float ra = recip_nr(1.f - alpha);
                     T[q] *= ra;
                     const float fac = alpha * T[q];
-                    const float tfb0 = Tf[q] * bgc[0], tfb1 = Tf[q] * bgc[1], tfb2 = Tf[q] * bgc[2];
-                    const float q0 = fmaf(r0, T[q], fmaf(-buf[q][0], ra, -ra * tfb0));
-                    const float q1 = fmaf(r1, T[q], fmaf(-buf[q][1], ra, -ra * tfb1));
-                    const float q2 = fmaf(r2, T[q], fmaf(-buf[q][2], ra, -ra * tfb2));
-                    const float v_alpha = q0 * vo[q][0] + q1 * vo[q][1] + q2 * vo[q][2];
-                    buf[q][0] = fmaf(r0, fac, buf[q][0]);
-                    buf[q][1] = fmaf(r1, fac, buf[q][1]);
-                    buf[q][2] = fmaf(r2, fac, buf[q][2]);
-                    const float v_sigma = -alpha * v_alpha;
+                    const float rv = fmaf(r2, vo[q][2], fmaf(r1, vo[q][1], r0 * vo[q][0]));
+                    const float v_alpha = fmaf(T[q], rv, -ra * S[q]);
+                    S[q] = fmaf(fac, rv, S[q]);
+                    const float vs = -alpha * v_alpha;
+                    const float vdx = vs * dx, vdy = vs * dy;
                     g[0] += fac * vo[q][0];
                     g[1] += fac * vo[q][1];
                     g[2] += fac * vo[q][2];
-                    const float hs = 0.5f * v_sigma;
-                    g[3] += hs * (dx * dx);
-                    g[4] += hs * (dx * dy);
-                    g[5] += hs * (dy * dy);
-                    g[6] += v_sigma * fmaf(a.w, dx, b.x * dy);
-                    g[7] += v_sigma * fmaf(b.x, dx, b.y * dy);
-                    g[8] += -v_sigma * (1.f - a.z);
+                    g[3] += vdx * dx;
+                    g[4] += vdx * dy;
+                    g[5] += vdy * dy;
+                    g[6] += vdx;
+                    g[7] += vdy;
+                    g[8] += vs;
                 }
                 if (__ballot(any) == 0ull) continue;
 #pragma unroll
@@ -2258,12 +2263,12 @@ __global__ __launch_bounds__(RB4) void k_raster_bwd_persist4(const RasterView* _
                         if (b.z + 0.5f >= 0.f) atomicAdd(&rv.G.v_rgb[3 * (size_t)gid + 0], g[0]);
                         if (b.w + 0.5f >= 0.f) atomicAdd(&rv.G.v_rgb[3 * (size_t)gid + 1], g[1]);
                         if (cz + 0.5f >= 0.f) atomicAdd(&rv.G.v_rgb[3 * (size_t)gid + 2], g[2]);
-                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 0], g[3]);
-                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 1], g[4]);
-                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 2], g[5]);
-                        atomicAdd(&rv.G.v_xy[gid].x, g[6]);
-                        atomicAdd(&rv.G.v_xy[gid].y, g[7]);
-                        atomicAdd(&rv.G.v_opac[gid], g[8]);
+                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 0], 0.5f * g[3]);
+                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 1], 0.5f * g[4]);
+                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 2], 0.5f * g[5]);
+                        atomicAdd(&rv.G.v_xy[gid].x, fmaf(a.w, g[6], b.x * g[7]));
+                        atomicAdd(&rv.G.v_xy[gid].y, fmaf(b.x, g[6], b.y * g[7]));
+                        atomicAdd(&rv.G.v_opac[gid], -(1.f - a.z) * g[8]);
                     }
                 }
             }
