@@ -1374,18 +1374,21 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
                 const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
                 const bool in_range = ms_finite(sigma) && sigma >= 0.f && sigma < 5.55f;
                 const float alpha = fminf(0.999f, a.z * ms_expf_core(in_range ? -sigma : 0.f));
-                const bool take = have && !done && in_range && alpha >= 1.f / 255.f;
+                // bitwise: no short-circuit branches around the arithmetic above
+                const bool take = have & !done & in_range & (alpha >= 1.f / 255.f);
                 const float nT = T * (1.f - alpha);
-                const bool sat = take && nT <= 1e-4f;
-                const bool acc = take && !sat;
-                const float vis = alpha * T;
-                C[0] = acc ? fmaf(b.z, vis, C[0]) : C[0];
-                C[1] = acc ? fmaf(b.w, vis, C[1]) : C[1];
-                C[2] = acc ? fmaf(cz, vis, C[2]) : C[2];
+                const bool sat = take & (nT <= 1e-4f);
+                const bool acc = take & !sat;
+                // an entry not taken adds colour x 0 (colours are finite): C stays exactly as
+                // it is, with no branch around the colour reads
+                const float vis = acc ? alpha * T : 0.f;
+                C[0] = fmaf(b.z, vis, C[0]);
+                C[1] = fmaf(b.w, vis, C[1]);
+                C[2] = fmaf(cz, vis, C[2]);
                 T = acc ? nT : T;
                 last = acc ? b0 + e : (sat ? b0 + e - 1 : last);
                 taken += acc ? 1 : 0;
-                done = done || sat;
+                done = done | sat;
             }
         }
     }
@@ -1870,8 +1873,10 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
                 const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
                 const bool in_range = sigma >= 0.f && sigma < 5.55f;
                 const float alpha = fminf(0.999f, a.z * ms_expf_core(in_range ? -sigma : 0.f));
-                const bool use = have && inside && e <= bin_final && in_range && alpha >= 1.f / 255.f &&
-                                 alpha < 0.999f;
+                // bitwise: no short-circuit branches around the arithmetic above (every lane
+                // evaluates every entry; the four entries' LDS reads are then issued together)
+                const bool use = have & inside & (e <= bin_final) & in_range & (alpha >= 1.f / 255.f) &
+                                 (alpha < 0.999f);
                 // one select: an entry not taken has alpha 0, so ra = 1 exactly, T and the
                 // colour sums stay as they are and every term below is a product with 0 (all
                 // factors are finite), which is what selecting each of them would give
