@@ -102,6 +102,14 @@ __device__ __forceinline__ float wave_sum4_rows(float a, float b, float c, float
     return v;
 }
 
+// Float atomic add to a device (global) address, issued as a global_atomic_add_f32 rather
+// than the flat atomic an unqualified pointer gets (the flat form also waits on the LDS
+// counter and checks the address aperture). Relaxed, device scope, no return value.
+typedef __attribute__((address_space(1))) float global_float;
+__device__ __forceinline__ void gadd(float* p, float v) {
+    __hip_atomic_fetch_add((global_float*)p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // 1 / d for d in [0.001, 1] (1 - alpha in the raster backward): v_rcp_f32 (within 1 ulp)
 // refined by one Newton step, instead of the correctly rounded division sequence (about ten
 // instructions). The backward's gradients are compared at 1e-4, not bit for bit.
@@ -1939,15 +1947,15 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
                 const float4 b = sB[tt];
                 const uint32_t f = sF[tt];
                 const uint32_t gid = __builtin_bit_cast(uint32_t, sC[tt].y);
-                if (f & 1u) atomicAdd(&G.v_rgb[3 * (size_t)gid + 0], s[0]);
-                if (f & 2u) atomicAdd(&G.v_rgb[3 * (size_t)gid + 1], s[1]);
-                if (f & 4u) atomicAdd(&G.v_rgb[3 * (size_t)gid + 2], s[2]);
-                atomicAdd(&G.v_conic[3 * (size_t)gid + 0], 0.5f * s[3]);
-                atomicAdd(&G.v_conic[3 * (size_t)gid + 1], 0.5f * s[4]);
-                atomicAdd(&G.v_conic[3 * (size_t)gid + 2], 0.5f * s[5]);
-                atomicAdd(&G.v_xy[gid].x, fmaf(a.w, s[6], b.x * s[7]));
-                atomicAdd(&G.v_xy[gid].y, fmaf(b.x, s[6], b.y * s[7]));
-                atomicAdd(&G.v_opac[gid], -(1.f - a.z) * s[8]);
+                if (f & 1u) gadd(&G.v_rgb[3 * (size_t)gid + 0], s[0]);
+                if (f & 2u) gadd(&G.v_rgb[3 * (size_t)gid + 1], s[1]);
+                if (f & 4u) gadd(&G.v_rgb[3 * (size_t)gid + 2], s[2]);
+                gadd(&G.v_conic[3 * (size_t)gid + 0], 0.5f * s[3]);
+                gadd(&G.v_conic[3 * (size_t)gid + 1], 0.5f * s[4]);
+                gadd(&G.v_conic[3 * (size_t)gid + 2], 0.5f * s[5]);
+                gadd(&G.v_xy[gid].x, fmaf(a.w, s[6], b.x * s[7]));
+                gadd(&G.v_xy[gid].y, fmaf(b.x, s[6], b.y * s[7]));
+                gadd(&G.v_opac[gid], -(1.f - a.z) * s[8]);
             }
         }
     }
@@ -2265,15 +2273,15 @@ __global__ __launch_bounds__(RB4) void k_raster_bwd_persist4(const RasterView* _
                     for (int k = 0; k < 9; k++) nz |= (g[k] != 0.f);
                     if (nz) {
                         const uint32_t gid = __builtin_bit_cast(uint32_t, sC[tt].y);
-                        if (b.z + 0.5f >= 0.f) atomicAdd(&rv.G.v_rgb[3 * (size_t)gid + 0], g[0]);
-                        if (b.w + 0.5f >= 0.f) atomicAdd(&rv.G.v_rgb[3 * (size_t)gid + 1], g[1]);
-                        if (cz + 0.5f >= 0.f) atomicAdd(&rv.G.v_rgb[3 * (size_t)gid + 2], g[2]);
-                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 0], 0.5f * g[3]);
-                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 1], 0.5f * g[4]);
-                        atomicAdd(&rv.G.v_conic[3 * (size_t)gid + 2], 0.5f * g[5]);
-                        atomicAdd(&rv.G.v_xy[gid].x, fmaf(a.w, g[6], b.x * g[7]));
-                        atomicAdd(&rv.G.v_xy[gid].y, fmaf(b.x, g[6], b.y * g[7]));
-                        atomicAdd(&rv.G.v_opac[gid], -(1.f - a.z) * g[8]);
+                        if (b.z + 0.5f >= 0.f) gadd(&rv.G.v_rgb[3 * (size_t)gid + 0], g[0]);
+                        if (b.w + 0.5f >= 0.f) gadd(&rv.G.v_rgb[3 * (size_t)gid + 1], g[1]);
+                        if (cz + 0.5f >= 0.f) gadd(&rv.G.v_rgb[3 * (size_t)gid + 2], g[2]);
+                        gadd(&rv.G.v_conic[3 * (size_t)gid + 0], 0.5f * g[3]);
+                        gadd(&rv.G.v_conic[3 * (size_t)gid + 1], 0.5f * g[4]);
+                        gadd(&rv.G.v_conic[3 * (size_t)gid + 2], 0.5f * g[5]);
+                        gadd(&rv.G.v_xy[gid].x, fmaf(a.w, g[6], b.x * g[7]));
+                        gadd(&rv.G.v_xy[gid].y, fmaf(b.x, g[6], b.y * g[7]));
+                        gadd(&rv.G.v_opac[gid], -(1.f - a.z) * g[8]);
                     }
                 }
             }

@@ -311,6 +311,10 @@ inline float atomicAdd(float* p, float v) {
         }
     }
 }
+// HIP's scoped atomic builtin (float add only; the scope is ignored on the CPU). The
+// address_space(1) qualifier of the device code is an attribute g++ ignores.
+#define __HIP_MEMORY_SCOPE_AGENT 3
+inline float __hip_atomic_fetch_add(float* p, float v, int, int) { return atomicAdd(p, v); }
 inline double atomicAdd(double* p, double v) {
     const double old = *p;
     *p = old + v;
