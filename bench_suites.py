@@ -183,6 +183,7 @@ def c3_bf16(a):
 
 def bf16_error(a):
     from msplat_amd import Dataset, GaussianTrainer, TrainingConfig, _lib, scene
+    from msplat_amd.device import G2D, G2D_CONIC, G2D_OPAC, G2D_RGB, G2D_XY
     O, threads = _oracle()
     L = _lib.lib()
     n, W, H = 20000, 128, 128
@@ -211,11 +212,13 @@ def bf16_error(a):
             for mode, tr in trs.items():
                 eng = tr.engine_handle
                 e = {"loss": abs(tr.last_loss - o64["loss"]) / abs(o64["loss"])}
-                for name, key, cnt in (("img", "img", W * H * 3), ("v_xy", "v_xy", 2 * n), ("v_conic", "v_conic", 3 * n),
-                                       ("v_rgb", "v_rgb", 3 * n), ("v_opac", "v_opac", n)):
-                    buf = np.zeros(cnt, np.float32)
-                    L.msplat_engine_read(eng, key.encode(), buf.ctypes.data, buf.nbytes)
-                    e[name] = _rel(buf, o64[name].reshape(-1))
+                img = np.zeros(W * H * 3, np.float32)
+                L.msplat_engine_read(eng, b"img", img.ctypes.data, img.nbytes)
+                e["img"] = _rel(img, o64["img"].reshape(-1))
+                g2d = np.zeros((n, G2D), np.float32)
+                L.msplat_engine_read(eng, b"grad2d", g2d.ctypes.data, g2d.nbytes)
+                for name, sl in (("v_xy", G2D_XY), ("v_conic", G2D_CONIC), ("v_rgb", G2D_RGB), ("v_opac", G2D_OPAC)):
+                    e[name] = _rel(np.ascontiguousarray(g2d[:, sl]).reshape(-1), o64[name].reshape(-1))
                 st = _state(tr)
                 for k in GROUPS:
                     e["param_" + k] = _rel(st[(k, 0)], ref.p[k][:n].reshape(-1))

@@ -111,8 +111,9 @@ int msplat_render_noalloc(const MsplatDeviceState* st, const MsplatCamera* cam, 
 
 /* Device pointer and size of an internal buffer of the last step/render: "radii",
  * "tiles_hit", "depths", "rect", "aabb", "recA", "recB", "recC", "tile_counts",
- * "tile_bins", "pkA", "pkB", "pkC", "img", "final_T", "final_idx", "v_rendered", "v_xy",
- * "v_conic", "v_rgb", "v_opac", "dbg_v_mean", "dbg_v_scale", "dbg_v_quat", "dbg_v_opac",
+ * "tile_bins", "pkA", "pkB", "pkC", "img", "final_T", "final_idx", "v_rendered", "grad2d"
+ * (float [N 9] per Gaussian: colour 0-2, conic 3-5, mean2d 6-7, opacity logit 8, the raster
+ * backward's screen-space gradients), "dbg_v_mean", "dbg_v_scale", "dbg_v_quat", "dbg_v_opac",
  * "nan_counts". Returns 0 on success, -1 for an unknown name. Replaces the reference's
  * diagnostics msplat_last_tile_counts / msplat_dump_step_buffers (msplat_metal.mm:633-736). */
 int msplat_engine_buffer(MsplatEngine* e, const char* name, void** dptr, size_t* bytes);

@@ -118,8 +118,8 @@ Engine::Engine(hipStream_t shared) {
 
 void Engine::trim() {
     if (stream_) (void)hipStreamSynchronize(stream_);
-    DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
-                   &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
+    DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &g2d_,
+                   &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &l1_, &l1m_, &blocks_, &hist_, &img_,
                    &finalT_, &finalIdx_, &ncontrib_, &vrend_, &dbuf_, &partials_, &chunkT_, &chunkC_, &chunkIdx_,
                    &prefixT_, &afterC_, &wq_, &wmisc_, &consts_, &srect_};
@@ -137,8 +137,8 @@ Engine::~Engine() {
     for (auto e : ev_pool_) hipEventDestroy(e);
     hipEventDestroy(mark_[0]);
     hipEventDestroy(mark_[1]);
-    DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &v_xy_, &v_conic_,
-                   &v_rgb_, &v_opac_, &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
+    DBuf* all[] = {&radii_, &tiles_, &depths_, &rect_, &aabb_, &recA_, &recB_, &recC_, &g2d_,
+                   &dbg_vm_, &dbg_vs_, &dbg_vq_, &dbg_vo_, &counts_, &exact_,
                    &bins_, &totals_, &dkeys_, &dkeys_out_, &dvals_, &sorted_ids_, &temp_, &ids_, &l1_, &l1m_, &blocks_, &hist_, &img_, &finalT_, &finalIdx_, &ncontrib_,
                    &vrend_, &dbuf_, &partials_, &loss_, &chunkT_, &chunkC_, &chunkIdx_, &prefixT_,
                    &afterC_, &nanc_, &wq_, &wmisc_, &consts_, &srect_};
@@ -198,7 +198,7 @@ void Engine::prepare(int N, const msk::Geom& launch, const msk::Geom& view, bool
         if (train) { ensure(prefixT_, (size_t)K_ * Pv * 4); ensure(afterC_, (size_t)K_ * Pv * 12); }
     }
     if (train) {
-        ensure(v_xy_, n * 8); ensure(v_conic_, n * 12); ensure(v_rgb_, n * 12); ensure(v_opac_, n * 4);
+        ensure(g2d_, n * msk::G2D * 4);
         ensure(vrend_, P * 12); ensure(dbuf_, P * 36);
         const size_t nblk = (size_t)((launch.W + 15) / 16) * ((launch.H + 15) / 16);
         ensure(partials_, nblk * 4);
@@ -237,8 +237,8 @@ msk::RasterView Engine::raster_view(bool train) const {
     r.final_T = finalT_.as<float>();
     r.final_idx = finalIdx_.as<int32_t>();
     r.v_rend = train ? vrend_.as<float>() : nullptr;
-    r.G = train ? msk::GradBufs{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()}
-                : msk::GradBufs{nullptr, nullptr, nullptr, nullptr};
+    r.G = train ? msk::GradBufs{g2d_.as<float>()}
+                : msk::GradBufs{nullptr};
     r.chunked = K_ > 1 ? 1 : 0;
     r.n_contrib = ncontrib_.as<int32_t>();
     return r;
@@ -304,7 +304,7 @@ void Engine::flush_events() {
 void Engine::front(const msk::Params& P, int degrees_to_use, bool train) {
     // accumulators of the raster backward (the reference blit-zeroes them, mm:1496-1512):
     // zeroed by the projection, row by row
-    const msk::GradBufs zero{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
+    const msk::GradBufs zero{g2d_.as<float>()};
     if (train) stage_event(ST_ZERO);
     msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
                     aabb_.as<float2>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
@@ -339,7 +339,7 @@ void Engine::loss_to(int slot) {
 void Engine::raster_bwd_own() {
     msk::RasterBufs R{img_.as<float>(), finalT_.as<float>(), finalIdx_.as<int32_t>(), chunkT_.as<float>(),
                       chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
-    msk::GradBufs G{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
+    msk::GradBufs G{g2d_.as<float>()};
     const msk::RasterWork w = own_work();
     msk::raster_backward(stream_, lists_, R, vcd_, launch_, vrend_.as<float>(), G, &w);
     stage_event(ST_RASTER_BWD);
@@ -411,7 +411,7 @@ void Engine::accumulate(const MsplatDeviceState& st, int degrees_to_use, msk::Ba
     const msk::Params P = params_of(st);
     msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
                     aabb_.as<float2>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
-    msk::GradBufs G{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
+    msk::GradBufs G{g2d_.as<float>()};
     msk::view_accumulate(stream_, P, vcd_, degrees_to_use, pj, G, acc, nanc_.as<uint32_t>());
     stage_event(ST_UPDATE);
 }
@@ -430,8 +430,8 @@ int Engine::train_step(const MsplatDeviceState& st, const MsplatCamera& cam, con
     if (rc != 0) return rc;
     const msk::Params P = params_of(st);
     const size_t n = (size_t)(st.N > 0 ? st.N : 1);
-    msk::GradBufs G{v_xy_.as<float2>(), v_conic_.as<float>(), v_rgb_.as<float>(), v_opac_.as<float>()};
-    msk::sh_rest_adam(stream_, P, cam, radii_.as<int32_t>(), v_rgb_.as<float>(), hp);
+    msk::GradBufs G{g2d_.as<float>()};
+    msk::sh_rest_adam(stream_, P, cam, radii_.as<int32_t>(), g2d_.as<float>(), hp);
     stage_event(ST_SH_ADAM);
     msk::ProjOut pj{radii_.as<int32_t>(), tiles_.as<int32_t>(), depths_.as<float>(), rect_.as<ushort4>(),
                     aabb_.as<float2>(), {recA_.as<float4>(), recB_.as<float4>(), recC_.as<float2>()}};
@@ -473,8 +473,7 @@ int Engine::buffer(const std::string& name, void** p, size_t* bytes) {
         {"aabb", &aabb_, n * 8}, {"recA", &recA_, n * 16}, {"recB", &recB_, n * 16}, {"recC", &recC_, n * 8},
         {"tile_counts", &counts_, T * 4}, {"tile_bins", &bins_, T * 8},
         {"ids", &ids_, (size_t)capacity_ * 4}, {"img", &img_, P * 12}, {"final_T", &finalT_, P * 4},
-        {"final_idx", &finalIdx_, P * 4}, {"n_contrib", &ncontrib_, P * 4}, {"v_rendered", &vrend_, P * 12}, {"v_xy", &v_xy_, n * 8},
-        {"v_conic", &v_conic_, n * 12}, {"v_rgb", &v_rgb_, n * 12}, {"v_opac", &v_opac_, n * 4},
+        {"final_idx", &finalIdx_, P * 4}, {"n_contrib", &ncontrib_, P * 4}, {"v_rendered", &vrend_, P * 12}, {"grad2d", &g2d_, n * msk::G2D * 4},
         {"dbg_v_mean", &dbg_vm_, n * 12}, {"dbg_v_scale", &dbg_vs_, n * 12}, {"dbg_v_quat", &dbg_vq_, n * 16},
         {"dbg_v_opac", &dbg_vo_, n * 4}, {"nan_counts", &nanc_, 64}, {"totals", &totals_, 32}, {"sorted_ids", &sorted_ids_, n * 4},
     };

@@ -100,7 +100,7 @@ public:
     // This view's projection outputs and raster gradients (valid after its backward).
     const int32_t* radii_dev() const { return radii_.as<int32_t>(); }
     const float* depths_dev() const { return depths_.as<float>(); }
-    const float2* v_xy_dev() const { return v_xy_.as<float2>(); }
+    const float* g2d_dev() const { return g2d_.as<float>(); }
     void set_persist_grid(int g) { persist_grid_ = g; }
     int persist_grid() const { return persist_grid_; }
     // Pixels per lane of the persistent raster kernels (1 or 4; the image is the same).
@@ -140,7 +140,7 @@ private:
     ConstRing ring_;
     // per-Gaussian
     DBuf radii_, tiles_, depths_, rect_, aabb_, recA_, recB_, recC_;
-    DBuf v_xy_, v_conic_, v_rgb_, v_opac_;
+    DBuf g2d_;  // [N G2D] screen-space gradient records
     DBuf dbg_vm_, dbg_vs_, dbg_vq_, dbg_vo_;
     // per-tile
     DBuf counts_, exact_, bins_, totals_;

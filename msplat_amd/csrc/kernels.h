@@ -88,11 +88,13 @@ struct RasterBufs {
     uint32_t K;
 };
 
+// Per-Gaussian screen-space gradients of the raster backward, interleaved: one record of G2D
+// floats per Gaussian (colour 3, conic 3, mean2d 2, opacity logit 1), so that the backward's
+// float atomics for one (tile, Gaussian) pair land in one or two cache lines, one lane per
+// field (scattered single-field atomics cost several times more).
+constexpr int G2D = 9, G2D_RGB = 0, G2D_CONIC = 3, G2D_XY = 6, G2D_OPAC = 8;
 struct GradBufs {
-    float2* v_xy;
-    float* v_conic;   // [N 3]
-    float* v_rgb;     // [N 3]
-    float* v_opac;    // [N]
+    float* g;  // [N G2D]
 };
 
 struct Params {
@@ -161,7 +163,8 @@ void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& g, const float* 
 void raster_backward(hipStream_t s, const ListBufs& L, const RasterBufs& R, const ViewConsts* vc, const Geom& g,
                      const float* v_rend, GradBufs& G, const RasterWork* work);
 void raster_backward_views(hipStream_t s, const RasterWork& w);
-void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const int32_t* radii, const float* v_rgb,
+// g2d: the [N G2D] gradient records (their colour fields are read)
+void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const int32_t* radii, const float* g2d,
                   const MsplatStepParams& hp);
 void gaussian_update(hipStream_t s, const Params& P, const MsplatCamera& cam, const ProjOut& pj, const GradBufs& G,
                      const MsplatStepParams& hp, uint32_t* nan_counts, float* dbg_v_mean, float* dbg_v_scale,

@@ -277,14 +277,9 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
         __syncthreads();
     }
     if (i >= N) return;
-    if (zero.v_xy) {  // a training step: the raster backward's accumulators start at zero
-        zero.v_xy[i] = make_float2(0.f, 0.f);
+    if (zero.g) {  // a training step: the raster backward's accumulators start at zero
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            zero.v_conic[3 * (size_t)i + k] = 0.f;
-            zero.v_rgb[3 * (size_t)i + k] = 0.f;
-        }
-        zero.v_opac[i] = 0.f;
+        for (int k = 0; k < G2D; k++) zero.g[G2D * (size_t)i + k] = 0.f;
     }
     const MsplatCamera cam = vc->cam;
     const int tbx = vc->tbx, tby = vc->tby;
@@ -435,7 +430,7 @@ __global__ __launch_bounds__(PROJ_BLOCK) void k_project(
 
 void project_forward(hipStream_t s, const Params& P, const ViewConsts* vc, int degrees_to_use, ProjOut& o,
                      uint32_t* nan_counts, const GradBufs* zero) {
-    const GradBufs z = zero ? *zero : GradBufs{nullptr, nullptr, nullptr, nullptr};
+    const GradBufs z = zero ? *zero : GradBufs{nullptr};
     if (P.N <= 0) {  // no Gaussians: only the transparent record (row 0)
         fill_zero4(s, o.rec.A, 16, o.rec.B, 16, o.rec.C, 8, nullptr, 0);
         return;
@@ -1926,6 +1921,10 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
             }
         }
         __syncthreads();
+        // per entry: the four waves' sums, then the entry's constants: sums of v_sigma dx^2,
+        // dx dy, dy^2, dx, dy and v_sigma -> conic, mean and opacity-logit gradients
+        // (d sigma / d mean = conic (dx, dy); the logit's derivative of alpha is
+        // alpha (1 - opacity)); the results overwrite the entry's row of wave 0's sums
         if ((int)threadIdx.x < n) {
             const int tt = threadIdx.x;
             float s[9];
@@ -1936,27 +1935,28 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
                 for (int w = 0; w < 4; w++) v += sIn[w][tt] ? red[w][tt][k] : 0.f;
                 s[k] = v;
             }
-            bool any = false;
-#pragma unroll
-            for (int k = 0; k < 9; k++) any |= (s[k] != 0.f);
-            if (any) {
-                // sums of v_sigma dx^2, dx dy, dy^2, dx, dy and v_sigma -> conic, mean and
-                // opacity-logit gradients (d sigma / d mean = conic (dx, dy); the logit's
-                // derivative of alpha is alpha (1 - opacity))
-                const float4 a = sA[tt];
-                const float4 b = sB[tt];
-                const uint32_t f = sF[tt];
-                const uint32_t gid = __builtin_bit_cast(uint32_t, sC[tt].y);
-                if (f & 1u) gadd(&G.v_rgb[3 * (size_t)gid + 0], s[0]);
-                if (f & 2u) gadd(&G.v_rgb[3 * (size_t)gid + 1], s[1]);
-                if (f & 4u) gadd(&G.v_rgb[3 * (size_t)gid + 2], s[2]);
-                gadd(&G.v_conic[3 * (size_t)gid + 0], 0.5f * s[3]);
-                gadd(&G.v_conic[3 * (size_t)gid + 1], 0.5f * s[4]);
-                gadd(&G.v_conic[3 * (size_t)gid + 2], 0.5f * s[5]);
-                gadd(&G.v_xy[gid].x, fmaf(a.w, s[6], b.x * s[7]));
-                gadd(&G.v_xy[gid].y, fmaf(b.x, s[6], b.y * s[7]));
-                gadd(&G.v_opac[gid], -(1.f - a.z) * s[8]);
-            }
+            const float4 a = sA[tt];
+            const float4 b = sB[tt];
+            const uint32_t f = sF[tt];
+            float* o = red[0][tt];
+            o[G2D_RGB + 0] = (f & 1u) ? s[0] : 0.f;  // colours clamped in the forward get none
+            o[G2D_RGB + 1] = (f & 2u) ? s[1] : 0.f;
+            o[G2D_RGB + 2] = (f & 4u) ? s[2] : 0.f;
+            o[G2D_CONIC + 0] = 0.5f * s[3];
+            o[G2D_CONIC + 1] = 0.5f * s[4];
+            o[G2D_CONIC + 2] = 0.5f * s[5];
+            o[G2D_XY + 0] = fmaf(a.w, s[6], b.x * s[7]);
+            o[G2D_XY + 1] = fmaf(b.x, s[6], b.y * s[7]);
+            o[G2D_OPAC] = -(1.f - a.z) * s[8];
+        }
+        __syncthreads();
+        // one lane per (entry, field): consecutive lanes add the fields of one record, so a
+        // wave's atomics touch about seven records' lines instead of 64 scattered ones per
+        // field; zero sums are skipped
+        for (int q = threadIdx.x; q < n * G2D; q += RB) {
+            const int tt = q / G2D, k = q - tt * G2D;
+            const float v = red[0][tt][k];
+            if (v != 0.f) gadd(&G.g[G2D * (size_t)__builtin_bit_cast(uint32_t, sC[tt].y) + k], v);
         }
     }
 }
@@ -2273,15 +2273,16 @@ __global__ __launch_bounds__(RB4) void k_raster_bwd_persist4(const RasterView* _
                     for (int k = 0; k < 9; k++) nz |= (g[k] != 0.f);
                     if (nz) {
                         const uint32_t gid = __builtin_bit_cast(uint32_t, sC[tt].y);
-                        if (b.z + 0.5f >= 0.f) gadd(&rv.G.v_rgb[3 * (size_t)gid + 0], g[0]);
-                        if (b.w + 0.5f >= 0.f) gadd(&rv.G.v_rgb[3 * (size_t)gid + 1], g[1]);
-                        if (cz + 0.5f >= 0.f) gadd(&rv.G.v_rgb[3 * (size_t)gid + 2], g[2]);
-                        gadd(&rv.G.v_conic[3 * (size_t)gid + 0], 0.5f * g[3]);
-                        gadd(&rv.G.v_conic[3 * (size_t)gid + 1], 0.5f * g[4]);
-                        gadd(&rv.G.v_conic[3 * (size_t)gid + 2], 0.5f * g[5]);
-                        gadd(&rv.G.v_xy[gid].x, fmaf(a.w, g[6], b.x * g[7]));
-                        gadd(&rv.G.v_xy[gid].y, fmaf(b.x, g[6], b.y * g[7]));
-                        gadd(&rv.G.v_opac[gid], -(1.f - a.z) * g[8]);
+                        float* o = rv.G.g + G2D * (size_t)gid;
+                        if (b.z + 0.5f >= 0.f) gadd(o + G2D_RGB + 0, g[0]);
+                        if (b.w + 0.5f >= 0.f) gadd(o + G2D_RGB + 1, g[1]);
+                        if (cz + 0.5f >= 0.f) gadd(o + G2D_RGB + 2, g[2]);
+                        gadd(o + G2D_CONIC + 0, 0.5f * g[3]);
+                        gadd(o + G2D_CONIC + 1, 0.5f * g[4]);
+                        gadd(o + G2D_CONIC + 2, 0.5f * g[5]);
+                        gadd(o + G2D_XY + 0, fmaf(a.w, g[6], b.x * g[7]));
+                        gadd(o + G2D_XY + 1, fmaf(b.x, g[6], b.y * g[7]));
+                        gadd(o + G2D_OPAC, -(1.f - a.z) * g[8]);
                     }
                 }
             }
@@ -2397,7 +2398,7 @@ template <> struct Mom<true> {
 template <int NB3, bool BF>
 __global__ __launch_bounds__(256) void k_sh_rest_adam(int N, int nuse3, const float* __restrict__ means, float cpx,
                                                       float cpy, float cpz, const int32_t* __restrict__ radii,
-                                                      const float* __restrict__ v_rgb, float* __restrict__ f_rest,
+                                                      const float* __restrict__ g2d, float* __restrict__ f_rest,
                                                       typename Mom<BF>::T* __restrict__ m4,
                                                       typename Mom<BF>::T* __restrict__ v4,
                                                       uint16_t* __restrict__ shadow, float step, float bc2s, float b1,
@@ -2412,7 +2413,8 @@ __global__ __launch_bounds__(256) void k_sh_rest_adam(int N, int nuse3, const fl
     const int ng = (int)(e_last / NB3) - g0 + 1;
     for (int k = threadIdx.x; k < ng; k += 256) {
         const int g = g0 + k;
-        const float vc0 = v_rgb[3 * (size_t)g], vc1 = v_rgb[3 * (size_t)g + 1], vc2 = v_rgb[3 * (size_t)g + 2];
+        const float* vr = g2d + G2D * (size_t)g + G2D_RGB;
+        const float vc0 = vr[0], vc1 = vr[1], vc2 = vr[2];
         // visible Gaussians with a finite colour gradient only (metal:1894, 1975-1978)
         const bool ok = radii[g] > 0 && ms_finite(vc0) && ms_finite(vc1) && ms_finite(vc2);
         s_b[k][15] = ok ? 1.f : 0.f;
@@ -2477,7 +2479,7 @@ __global__ __launch_bounds__(256) void k_sh_rest_adam(int N, int nuse3, const fl
     }
 }
 
-void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const int32_t* radii, const float* v_rgb,
+void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const int32_t* radii, const float* g2d,
                   const MsplatStepParams& hp) {
     const int du = hp.degrees_to_use < P.degree ? hp.degrees_to_use : P.degree;
     if (du < 1 || P.N <= 0) return;
@@ -2486,7 +2488,7 @@ void sh_rest_adam(hipStream_t s, const Params& P, const MsplatCamera& cam, const
     const size_t total = (size_t)P.N * nb3;
 #define LAUNCH_SH(NB, BF, MT)                                                                                      \
     hipLaunchKernelGGL((k_sh_rest_adam<NB, BF>), dim3(cdiv(total, SH_EPB)), dim3(256), 0, s, P.N, nuse3, P.means,  \
-                       cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2], radii, v_rgb, P.f_rest, (MT*)P.m[4],        \
+                       cam.cam_pos[0], cam.cam_pos[1], cam.cam_pos[2], radii, g2d, P.f_rest, (MT*)P.m[4],          \
                        (MT*)P.v[4], P.f_rest_h, hp.step_size[4], hp.bc2_sqrt[4], hp.beta1, hp.beta2, hp.eps)
     if (P.sh_bf16) {
         if (nb3 == 9) LAUNCH_SH(9, true, uint16_t);
@@ -2771,16 +2773,17 @@ __global__ __launch_bounds__(256) void k_gaussian_update(
     // (all are sized N), so their latency overlaps instead of following the radius load.
     const int rad = radii[i];
     const float4 a = rA[i], b = rB[i];
-    const float2 vxy = G.v_xy[i];
-    const float vcn[3] = {G.v_conic[3 * (size_t)i], G.v_conic[3 * (size_t)i + 1], G.v_conic[3 * (size_t)i + 2]};
-    const float vc[3] = {G.v_rgb[3 * (size_t)i], G.v_rgb[3 * (size_t)i + 1], G.v_rgb[3 * (size_t)i + 2]};
+    const float* gr = G.g + G2D * (size_t)i;
+    const float2 vxy = make_float2(gr[G2D_XY], gr[G2D_XY + 1]);
+    const float vcn[3] = {gr[G2D_CONIC], gr[G2D_CONIC + 1], gr[G2D_CONIC + 2]};
+    const float vc[3] = {gr[G2D_RGB], gr[G2D_RGB + 1], gr[G2D_RGB + 2]};
     float dc[3], dm[3], dv[3];
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         const size_t k = 3 * (size_t)i + c;
         dc[c] = f_dc[k]; dm[c] = Mom<BF>::ld(m3, k); dv[c] = Mom<BF>::ld(v3, k);
     }
-    float vo = G.v_opac[i];
+    float vo = gr[G2D_OPAC];
     float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
     const float b1 = hp.beta1, b2 = hp.beta2, eps = hp.eps;
     if (rad > 0) {
@@ -2876,8 +2879,9 @@ __global__ __launch_bounds__(256) void k_view_accum(int N, int nuse3, const floa
     const float q[4] = {q4.x, q4.y, q4.z, q4.w};
     const float4 a = rA[i], b = rB[i];
     const float conic[3] = {a.w, b.x, b.y};
-    const float2 vxy = G.v_xy[i];
-    const float vcn[3] = {G.v_conic[3 * (size_t)i], G.v_conic[3 * (size_t)i + 1], G.v_conic[3 * (size_t)i + 2]};
+    const float* gr = G.g + G2D * (size_t)i;
+    const float2 vxy = make_float2(gr[G2D_XY], gr[G2D_XY + 1]);
+    const float vcn[3] = {gr[G2D_CONIC], gr[G2D_CONIC + 1], gr[G2D_CONIC + 2]};
     float vm[3] = {0.f, 0.f, 0.f}, vs[3] = {0.f, 0.f, 0.f}, vq[4] = {0.f, 0.f, 0.f, 0.f};
     proj_vjp(p, ls, q, cam, conic, vxy.x, vxy.y, vcn, vm, vs, vq);
 #pragma unroll
@@ -2887,8 +2891,8 @@ __global__ __launch_bounds__(256) void k_view_accum(int N, int nuse3, const floa
     }
 #pragma unroll
     for (int k = 0; k < 4; k++) A.v_quat[4 * (size_t)i + k] += vq[k];
-    A.v_opac[i] += G.v_opac[i];
-    const float vcol[3] = {G.v_rgb[3 * (size_t)i], G.v_rgb[3 * (size_t)i + 1], G.v_rgb[3 * (size_t)i + 2]};
+    A.v_opac[i] += gr[G2D_OPAC];
+    const float vcol[3] = {gr[G2D_RGB], gr[G2D_RGB + 1], gr[G2D_RGB + 2]};
     if (!(ms_finite(vcol[0]) && ms_finite(vcol[1]) && ms_finite(vcol[2]))) {
         atomicAdd(&nanc[3], 1u);
         return;

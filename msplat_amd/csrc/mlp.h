@@ -34,7 +34,7 @@ struct ReloStats {
 };
 
 // One view's contribution to the statistics (after its raster backward).
-void relo_stats_accum(hipStream_t s, int N, const int32_t* radii, const float* depths, const float2* v_xy,
+void relo_stats_accum(hipStream_t s, int N, const int32_t* radii, const float* depths, const float* g2d,
                       ReloStats st);
 // Features [N 16] (bf16) of the current state and statistics over `views` views.
 void relo_features(hipStream_t s, const Params& P, const ReloStats& st, int views, uint16_t* X);

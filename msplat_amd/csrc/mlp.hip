@@ -12,6 +12,7 @@
 
 #include <algorithm>
 
+#include "kernels.h"
 #include "mlp.h"
 #include "ms_math.h"
 
@@ -49,12 +50,12 @@ __device__ __forceinline__ void wave_sync_lds() {
 // statistics and features
 
 __global__ void k_relo_stats(int N, const int32_t* __restrict__ radii, const float* __restrict__ depths,
-                             const float2* __restrict__ v_xy, ReloStats st) {
+                             const float* __restrict__ g2d, ReloStats st) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int r = radii[i];
     if (r <= 0) return;
-    const float2 g = v_xy[i];
+    const float2 g = make_float2(g2d[G2D * (size_t)i + G2D_XY], g2d[G2D * (size_t)i + G2D_XY + 1]);
     const float d = depths[i];
     st.vis[i] += 1.f;
     st.gsum[i] += sqrtf(g.x * g.x + g.y * g.y);
@@ -63,10 +64,10 @@ __global__ void k_relo_stats(int N, const int32_t* __restrict__ radii, const flo
     st.dsq[i] += d * d;
 }
 
-void relo_stats_accum(hipStream_t s, int N, const int32_t* radii, const float* depths, const float2* v_xy,
+void relo_stats_accum(hipStream_t s, int N, const int32_t* radii, const float* depths, const float* g2d,
                       ReloStats st) {
     if (N <= 0) return;
-    hipLaunchKernelGGL(k_relo_stats, dim3(cdiv_i(N, 256)), dim3(256), 0, s, N, radii, depths, v_xy, st);
+    hipLaunchKernelGGL(k_relo_stats, dim3(cdiv_i(N, 256)), dim3(256), 0, s, N, radii, depths, g2d, st);
 }
 
 __global__ void k_relo_features(int N, const float* __restrict__ scales, const float* __restrict__ quats,

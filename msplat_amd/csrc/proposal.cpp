@@ -143,7 +143,7 @@ int64_t Model::proposal_read(const std::string& name, void* host, int64_t bytes)
 }
 
 void Model::proposal_accumulate(const Engine& e) {
-    msk::relo_stats_accum(eng_.stream(), num_active_, e.radii_dev(), e.depths_dev(), e.v_xy_dev(), prop_.stats());
+    msk::relo_stats_accum(eng_.stream(), num_active_, e.radii_dev(), e.depths_dev(), e.g2d_dev(), prop_.stats());
     prop_.views++;
 }
 

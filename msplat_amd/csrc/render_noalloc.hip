@@ -133,7 +133,7 @@ int render(const MsplatDeviceState& st, const MsplatCamera& cam, int degrees_to_
     blk.r.final_T = final_T;
     blk.r.final_idx = final_idx;
     blk.r.v_rend = nullptr;
-    blk.r.G = msk::GradBufs{nullptr, nullptr, nullptr, nullptr};
+    blk.r.G = msk::GradBufs{nullptr};
     blk.r.chunked = L.K > 1 ? 1 : 0;
     hipLaunchKernelGGL(k_store_consts, dim3(1), dim3(64), 0, s, blk, dblk);
     const msk::ViewConsts* vcd = &dblk->v;

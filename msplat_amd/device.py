@@ -13,6 +13,10 @@ import numpy as np
 from . import _lib
 
 GROUPS = ("means", "scales", "quats", "f_dc", "f_rest", "opac")
+# layout of the engine's "grad2d" buffer: one record of G2D floats per Gaussian (the raster
+# backward's screen-space gradients: colour, conic, mean2d, opacity logit)
+G2D = 9
+G2D_RGB, G2D_CONIC, G2D_XY, G2D_OPAC = slice(0, 3), slice(3, 6), slice(6, 8), 8
 
 
 class DeviceArray:
@@ -199,11 +203,12 @@ class Engine:
         }
 
     def grads(self, N: int, W: int, H: int) -> dict:
+        g2d = self.read("grad2d", np.float32, (N, G2D))
         return {
             "v_rendered": self.read("v_rendered", np.float32, (H, W, 3)),
-            "v_xy": self.read("v_xy", np.float32, (N, 2)),
-            "v_conic": self.read("v_conic", np.float32, (N, 3)),
-            "v_rgb": self.read("v_rgb", np.float32, (N, 3)),
+            "v_xy": g2d[:, G2D_XY].copy(),
+            "v_conic": g2d[:, G2D_CONIC].copy(),
+            "v_rgb": g2d[:, G2D_RGB].copy(),
             "v_mean": self.read("dbg_v_mean", np.float32, (N, 3)),
             "v_scale": self.read("dbg_v_scale", np.float32, (N, 3)),
             "v_quat": self.read("dbg_v_quat", np.float32, (N, 4)),
