@@ -1638,11 +1638,12 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
                 for (int d = 0; d < 11; d++) {
                     const float w = c_gauss1d[d];
                     const float gv = sg[sy][sx + d], rv = sr[sy][sx + d];
-                    mx += w * gv;
-                    my += w * rv;
-                    sxq += w * gv * gv;
-                    syq += w * rv * rv;
-                    cxy += w * gv * rv;
+                    const float wg = w * gv, wr = w * rv;
+                    mx = fmaf(w, gv, mx);
+                    my = fmaf(w, rv, my);
+                    sxq = fmaf(wg, gv, sxq);
+                    syq = fmaf(wr, rv, syq);
+                    cxy = fmaf(wg, rv, cxy);
                 }
             }
             shp[sy][sx][0] = mx;
@@ -1659,7 +1660,7 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
             for (int kk = 0; kk < 11; kk++) {
                 const float w = c_gauss1d[kk];
 #pragma unroll
-                for (int q = 0; q < 5; q++) m[q] += w * shp[dy + kk][dx][q];
+                for (int q = 0; q < 5; q++) m[q] = fmaf(w, shp[dy + kk][dx][q], m[q]);
             }
             if (!(ms_finite(m[0]) && ms_finite(m[1]) && ms_finite(m[2]) && ms_finite(m[3]) && ms_finite(m[4]))) {
                 sf[0][dy][dx] = 0.f;
@@ -1696,9 +1697,9 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
 #pragma unroll
             for (int d = 0; d < 11; d++) {
                 const float w = c_gauss1d[10 - d];
-                h1 += w * sf[0][ly][lx + d];
-                h2 += w * sf[1][ly][lx + d];
-                h3 += w * sf[2][ly][lx + d];
+                h1 = fmaf(w, sf[0][ly][lx + d], h1);
+                h2 = fmaf(w, sf[1][ly][lx + d], h2);
+                h3 = fmaf(w, sf[2][ly][lx + d], h3);
             }
             const size_t pid = (size_t)py * W + px;
             dbuf[(c * 3 + 0) * P + pid] = h1;
@@ -1760,9 +1761,9 @@ __global__ __launch_bounds__(256) void k_ssim_vbwd(const ViewConsts* __restrict_
 #pragma unroll
             for (int d = 0; d < 11; d++) {
                 const float w = c_gauss1d[10 - d];
-                f1 += w * sh[c * 3 + 0][ly + d][lx];
-                f2 += w * sh[c * 3 + 1][ly + d][lx];
-                f3 += w * sh[c * 3 + 2][ly + d][lx];
+                f1 = fmaf(w, sh[c * 3 + 0][ly + d][lx], f1);
+                f2 = fmaf(w, sh[c * 3 + 1][ly + d][lx], f2);
+                f3 = fmaf(w, sh[c * 3 + 2][ly + d][lx], f3);
             }
             const float rv = img[3 * pid + c];
             const float gv = (float)gt[3 * pid + c] * (1.0f / 255.0f);

@@ -570,7 +570,11 @@ double ssim_loss(int W, int H, const T* rend, const uint8_t* gt, T ssim_w, T inv
                     T w = T(GAUSS_1D[k]);
                     T gv = 0, rv = 0;
                     if (gx >= 0 && gx < W) { gv = X(y, gx, c); rv = Y(y, gx, c); }
-                    mx += w * gv; my += w * rv; sx += w * gv * gv; sy += w * rv * rv; cxy += w * gv * rv;
+                    // fused multiply-adds, in the device kernel's order (its results are
+                    // compared bit for bit)
+                    const T wg = w * gv, wr = w * rv;
+                    mx = std::fma(w, gv, mx); my = std::fma(w, rv, my);
+                    sx = std::fma(wg, gv, sx); sy = std::fma(wr, rv, sy); cxy = std::fma(wg, rv, cxy);
                 }
                 size_t o = ((size_t)y * W + x) * 15 + c * 5;
                 hb[o] = mx; hb[o + 1] = my; hb[o + 2] = sx; hb[o + 3] = sy; hb[o + 4] = cxy;
@@ -592,7 +596,7 @@ double ssim_loss(int W, int H, const T* rend, const uint8_t* gt, T ssim_w, T inv
                         T w = T(GAUSS_1D[k]);
                         for (int q = 0; q < 5; q++) {
                             T hv = (gy >= 0 && gy < H) ? hb[((size_t)gy * W + xx) * 15 + c * 5 + q] : T(0);
-                            m[q] += w * hv;
+                            m[q] = std::fma(w, hv, m[q]);
                         }
                     }
                 }
@@ -640,7 +644,7 @@ double ssim_loss(int W, int H, const T* rend, const uint8_t* gt, T ssim_w, T inv
                     T h = 0;
                     for (int k = 0; k < 11; k++) {
                         int xx = x - 5 + k;
-                        h += T(GAUSS_1D[10 - k]) * f[(((size_t)y * WP + (xx + 5)) * 3 + c) * 3 + q];
+                        h = std::fma(T(GAUSS_1D[10 - k]), f[(((size_t)y * WP + (xx + 5)) * 3 + c) * 3 + q], h);
                     }
                     dh[(((size_t)y * W + x) * 3 + c) * 3 + q] = h;
                 }
@@ -655,7 +659,7 @@ double ssim_loss(int W, int H, const T* rend, const uint8_t* gt, T ssim_w, T inv
                     T w = T(GAUSS_1D[10 - k]);
                     for (int q = 0; q < 3; q++) {
                         T hv = (gy >= 0 && gy < H) ? dh[(((size_t)gy * W + x) * 3 + c) * 3 + q] : T(0);
-                        cf[q] += w * hv;
+                        cf[q] = std::fma(w, hv, cf[q]);
                     }
                 }
                 T rv = Y(y, x, c), gv = X(y, x, c);
