@@ -85,7 +85,7 @@ void BatchRunner::enqueue(const msk::Params& P, int B, int du, int Tmax, msk::Ba
     msk::fill_zero(s_, A.v_dc, n * 12);
     msk::fill_zero(s_, A.v_rest, n * (size_t)std::max(nb3, 1) * 4);
     msk::fill_zero(s_, A.vis, n * 4);
-    for (int v = 0; v < B; v++) slots_[v]->front(P, du, true);
+    for (int v = 0; v < B; v++) slots_[v]->front(P, du, true, false);  // the batch builds one queue for all
     msk::RasterWork w{};
     w.views = reinterpret_cast<const msk::RasterView*>(block_.as<uint8_t>() + align256(sizeof(MsplatStepParams)) +
                                                        align256(sizeof(msk::ViewConsts) * B));

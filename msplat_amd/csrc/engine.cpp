@@ -301,7 +301,7 @@ void Engine::flush_events() {
     ev_used_ = 0;
 }
 
-void Engine::front(const msk::Params& P, int degrees_to_use, bool train) {
+void Engine::front(const msk::Params& P, int degrees_to_use, bool train, bool own_queue) {
     // accumulators of the raster backward (the reference blit-zeroes them, mm:1496-1512):
     // zeroed by the projection, row by row
     const msk::GradBufs zero{g2d_.as<float>()};
@@ -315,7 +315,10 @@ void Engine::front(const msk::Params& P, int degrees_to_use, bool train) {
     msk::ListBufs& L = lists_;
     msk::lists_rank(stream_, P.N, pj, L);
     stage_event(ST_RANK);
-    msk::lists_count(stream_, P.N, pj, vcd_, launch_, L);
+    // an unchunked view's own raster work queue is written by the tile-offset scan
+    queue_built_ = own_queue && K_ <= 1;
+    const msk::RasterWork w = own_work();
+    msk::lists_count(stream_, P.N, pj, vcd_, launch_, L, queue_built_ ? &w : nullptr);
     stage_event(ST_COUNTS);
     msk::lists_bucket(stream_, P.N, vcd_, launch_, L);
     stage_event(ST_BUCKET);
@@ -325,7 +328,7 @@ void Engine::raster_fwd_own() {
     msk::RasterBufs R{img_.as<float>(), finalT_.as<float>(), finalIdx_.as<int32_t>(), chunkT_.as<float>(),
                       chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
     const msk::RasterWork w = own_work();
-    if (K_ <= 1) msk::raster_work_build(stream_, w);
+    if (K_ <= 1 && !queue_built_) msk::raster_work_build(stream_, w);
     msk::raster_forward(stream_, lists_, vcd_, launch_, R, &w);
     stage_event(ST_RASTER_FWD);
 }

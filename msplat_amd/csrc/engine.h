@@ -88,7 +88,9 @@ public:
     // raster_fwd_own / raster_bwd_own run this view alone.
     void own_view(const msk::ViewConsts& h) { upload_view(h, true); }
     // Gradient accumulators cleared (train), projection, tile lists.
-    void front(const msk::Params& P, int degrees_to_use, bool train);
+    // own_queue: also write this view's raster work queue (false when a batch builds one
+    // queue for all its views)
+    void front(const msk::Params& P, int degrees_to_use, bool train, bool own_queue = true);
     void loss_to(int slot);
     bool chunked() const { return K_ > 1; }
     int chunks() const { return (int)K_; }
@@ -140,6 +142,7 @@ private:
     ConstRing ring_;
     // per-Gaussian
     DBuf radii_, tiles_, depths_, rect_, aabb_, recA_, recB_, recC_;
+    bool queue_built_ = false;  // front() wrote the raster work queue of the current lists
     DBuf g2d_;  // [N G2D] screen-space gradient records
     DBuf dbg_vm_, dbg_vs_, dbg_vq_, dbg_vo_;
     // per-tile

@@ -141,7 +141,10 @@ size_t lists_hist_bytes(int N, int T, int chunk_target = 0);
 size_t lists_block_bytes(const Geom& g);
 bool lists_supported(int N, int T);
 void lists_rank(hipStream_t s, int N, const ProjOut& pj, ListBufs& L);
-void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L);
+// wq (a single unchunked view, B = 1): the raster work queue of view 0 is written with the
+// tile offsets, and raster_work_build is not needed for it
+void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L,
+                 const RasterWork* wq = nullptr);
 // Writes every tile hit's Gaussian id into its tile's bucket, in depth-rank order.
 void lists_bucket(hipStream_t s, int N, const ViewConsts* vc, const Geom& g, ListBufs& L);
 // Waves per bucket workgroup: 0 = 2 where the tile count's LDS allows, else 1; 1, 2 or 4 on

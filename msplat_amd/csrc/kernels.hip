@@ -545,9 +545,24 @@ __global__ __launch_bounds__(64 * COLSCAN_PARTS) void k_col_sum(const ViewConsts
     counts[t] = total;
 }
 
+// Work items of the persistent raster kernels: (view << 20) | tile, ordered by decreasing
+// tile list length so that the longest tiles start first and the launch ends on short ones.
+constexpr int WORK_TILE_BITS = 20;
+constexpr int WQ_THREADS = 1024, WQ_BINS = 256;
+// Queue bin of a tile list of `len` entries: 8-entry bins, longest first, all lists of 2040
+// entries or more in the first bin.
+__device__ __forceinline__ int wq_bin(uint32_t len) {
+    return WQ_BINS - 1 - (int)min(len >> 3, (uint32_t)(WQ_BINS - 1));
+}
+
+// queue != nullptr (a single unchunked view): the workgroup also writes the raster work
+// queue of view 0 from the bins it has just written (what k_work_queue does for a batch of
+// views), so the single-view step needs no launch of its own for it.
 __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ vc, const uint32_t* __restrict__ counts,
                                                  uint32_t* __restrict__ exact_off, int2* __restrict__ bins,
-                                                 uint32_t* __restrict__ totals, uint32_t cap) {
+                                                 uint32_t* __restrict__ totals, uint32_t cap,
+                                                 uint32_t* __restrict__ queue, uint32_t* __restrict__ n_work,
+                                                 uint32_t* __restrict__ ctr) {
     __shared__ uint32_t lw[17];
     const int T = vc->T;
     constexpr int PER = 8;  // consecutive tiles per thread: one block scan per 8192 tiles
@@ -587,6 +602,31 @@ __global__ __launch_bounds__(1024) void k_prefix(const ViewConsts* __restrict__ 
         totals[1] = spd;      // what the reference's padded buckets would occupy
         totals[2] = run_ex > cap ? 1u : 0u;
         totals[3] = smax;
+    }
+    if (queue == nullptr) return;
+    // the work queue: tiles binned by list length (the bins written above are visible to
+    // the workgroup after its barriers), bin starts by a scan, tiles scattered to them
+    __shared__ uint32_t qcnt[WQ_BINS];
+    if (threadIdx.x < WQ_BINS) qcnt[threadIdx.x] = 0u;
+    __syncthreads();
+    for (int t = threadIdx.x; t < T; t += 1024) {
+        const int2 r = bins[t];
+        atomicAdd(&qcnt[wq_bin((uint32_t)max(0, r.y - r.x))], 1u);
+    }
+    __syncthreads();
+    uint32_t qtot;
+    const uint32_t c = threadIdx.x < WQ_BINS ? qcnt[threadIdx.x] : 0u;
+    const uint32_t start = block_exclusive_scan(c, lw, &qtot);
+    if (threadIdx.x < WQ_BINS) qcnt[threadIdx.x] = start;
+    __syncthreads();
+    for (int t = threadIdx.x; t < T; t += 1024) {
+        const int2 r = bins[t];
+        queue[atomicAdd(&qcnt[wq_bin((uint32_t)max(0, r.y - r.x))], 1u)] = (uint32_t)t;  // view 0
+    }
+    if (threadIdx.x == 0) {
+        *n_work = (uint32_t)T;
+        ctr[0] = 0u;
+        ctr[1] = 0u;
     }
 }
 
@@ -1198,7 +1238,8 @@ static void allow_lds(const void* fn, size_t lds, const char* what) {
     if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
 }
 
-void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L) {
+void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, const Geom& g, ListBufs& L,
+                 const RasterWork* wq) {
     if (N > 0) {
         const int nblk = hist_blocks(N, L.chunk_target), G = (N + nblk - 1) / nblk;
         const size_t lds = (size_t)g.T * 4;
@@ -1211,7 +1252,8 @@ void lists_count(hipStream_t s, int N, const ProjOut& pj, const ViewConsts* vc, 
         fill_zero(s, L.tile_counts, (size_t)g.T * 4);
     }
     hipLaunchKernelGGL(k_prefix, dim3(1), dim3(1024), 0, s, vc, L.tile_counts, L.exact_off, L.bins, L.totals,
-                       (uint32_t)L.capacity);
+                       (uint32_t)L.capacity, wq ? wq->queue : nullptr, wq ? wq->n_work : nullptr,
+                       wq ? wq->ctr : nullptr);
 }
 
 // Level-1 bucket launch with WAVES waves (chunks) per workgroup (LDS by the launch's block
@@ -1397,15 +1439,11 @@ __device__ __forceinline__ void composite_range(int lo, int hi, float px, float 
     }
 }
 
-// Work items of the persistent raster kernels: (view << 20) | tile, ordered by decreasing
-// tile list length so that the longest tiles start first and the launch ends on short ones.
-constexpr int WORK_TILE_BITS = 20;
 
 // One workgroup: every valid (view, tile) item is binned by its list length (8-entry bins,
 // longest first, all lists of 2040 entries or more in the first bin) with LDS counters, and
 // placed bin by bin. The order inside a bin is arbitrary; it only balances the persistent
 // raster kernels, whose outputs do not depend on it.
-constexpr int WQ_THREADS = 1024, WQ_BINS = 256;
 __global__ __launch_bounds__(WQ_THREADS) void k_work_queue(const RasterView* __restrict__ views, int B, int Tmax,
                                                           uint32_t* __restrict__ queue, uint32_t* __restrict__ n_work,
                                                           uint32_t* __restrict__ ctr) {
@@ -1421,7 +1459,7 @@ __global__ __launch_bounds__(WQ_THREADS) void k_work_queue(const RasterView* __r
         const int2 r = rv.bins[t];
         const uint32_t len = (uint32_t)max(0, r.y - r.x);
         code = ((uint32_t)v << WORK_TILE_BITS) | (uint32_t)t;
-        return WQ_BINS - 1 - (int)min(len >> 3, (uint32_t)(WQ_BINS - 1));
+        return wq_bin(len);
     };
     constexpr int U = 8;  // items per thread in flight
     for (int k0 = 0; k0 < n; k0 += WQ_THREADS * U) {
@@ -1727,9 +1765,25 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
     if (threadIdx.x == 0) partials[blockIdx.y * gx + blockIdx.x] = red[0];
 }
 
+// Workgroup (0, 0) also sums the fused pass's per-workgroup loss partials (written by the
+// previous launch) into the loss, in index order, after its own pixels.
+__device__ __forceinline__ void loss_sum(int n, const float* __restrict__ partials, double* __restrict__ out) {
+    __shared__ double red[256];
+    double acc = 0.0;
+    for (int k = threadIdx.x; k < n; k += 256) acc += (double)partials[k];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
 __global__ __launch_bounds__(256) void k_ssim_vbwd(const ViewConsts* __restrict__ vc, const float* __restrict__ img,
                                                    const float* __restrict__ dbuf, float* __restrict__ v_rend,
-                                                   uint32_t* __restrict__ nanc) {
+                                                   uint32_t* __restrict__ nanc, const float* __restrict__ partials,
+                                                   double* __restrict__ loss_out) {
     constexpr int TH = SB + 10;
     __shared__ float sh[9][TH][FW];
     const int W = vc->W, H = vc->H;
@@ -1777,29 +1831,14 @@ __global__ __launch_bounds__(256) void k_ssim_vbwd(const ViewConsts* __restrict_
             v_rend[3 * pid + c] = v;
         }
     }
-}
-
-__global__ __launch_bounds__(256) void k_loss_final(const ViewConsts* __restrict__ vc, const float* __restrict__ partials,
-                                                    double* __restrict__ out) {
-    __shared__ double red[256];
-    const int n = ((vc->W + SB - 1) / SB) * ((vc->H + SB - 1) / SB);
-    double acc = 0.0;
-    for (int k = threadIdx.x; k < n; k += 256) acc += (double)partials[k];
-    red[threadIdx.x] = acc;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st) red[threadIdx.x] += red[threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) *out = red[0];
+    if (blockIdx.x == 0 && blockIdx.y == 0) loss_sum(((W + SB - 1) / SB) * ((H + SB - 1) / SB), partials, loss_out);
 }
 
 void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& gm, const float* img, float* dbuf,
                float* v_rend, float* loss_partials, double* loss_out, uint32_t* nan_counts) {
     dim3 g((gm.W + SB - 1) / SB, (gm.H + SB - 1) / SB), gw((gm.W + FW - 1) / FW, (gm.H + SB - 1) / SB);
     hipLaunchKernelGGL(k_ssim_fused, g, dim3(256), 0, s, vc, img, dbuf, loss_partials, nan_counts);
-    hipLaunchKernelGGL(k_ssim_vbwd, gw, dim3(256), 0, s, vc, img, dbuf, v_rend, nan_counts);
-    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(256), 0, s, vc, loss_partials, loss_out);
+    hipLaunchKernelGGL(k_ssim_vbwd, gw, dim3(256), 0, s, vc, img, dbuf, v_rend, nan_counts, loss_partials, loss_out);
 }
 
 // ------------------------------------------------------------------------------------
