@@ -813,7 +813,10 @@ __global__ __launch_bounds__(64 * WAVES) void k_block_bucket(int N, int G, int n
                                                              uint32_t* __restrict__ boff_out, uint32_t cap) {
     extern __shared__ uint32_t sm[];
     const int lane = (int)(threadIdx.x & 63), wid = (int)(threadIdx.x >> 6);
-    const int chunk = (int)blockIdx.x * WAVES + wid;
+    // consecutive chunks on one XCD (workgroups go to the 8 XCDs round robin)
+    const int per = (int)(gridDim.x >> 3);
+    const int wg = (int)(blockIdx.x & 7u) * per + (int)(blockIdx.x >> 3);
+    const int chunk = wg * WAVES + wid;
     if (chunk >= nblk) return;  // wave-uniform; no workgroup barrier follows
     const int tbx = vc->tbx, tby = vc->tby;
     const BlockGeom bgm = block_geom(tbx, tby);
@@ -1264,7 +1267,7 @@ static void launch_bucket(hipStream_t s, int nblk, int N, int G, int NBL, const 
     allow_lds((const void*)k_block_bucket<WAVES>, lds, "block bucket LDS");
     uint32_t* btot = L.blocks;
     uint32_t* boff = L.blocks + NBL;
-    hipLaunchKernelGGL(k_block_bucket<WAVES>, dim3(cdiv(nblk, WAVES)), dim3(64 * WAVES), lds, s, N, G, nblk, NBL,
+    hipLaunchKernelGGL(k_block_bucket<WAVES>, dim3(8 * cdiv(cdiv(nblk, WAVES), 8)), dim3(64 * WAVES), lds, s, N, G, nblk, NBL,
                        L.sorted_ids, L.srect, vc, L.hist, btot, L.exact_off, L.tile_counts, L.totals, L.l1_ids,
                        L.l1_mask, boff, (uint32_t)L.capacity);
     check_launch("block bucket launch");
