@@ -1314,6 +1314,43 @@ constexpr int RB = 256;
 __device__ __forceinline__ int tile_lx() { return (int)((threadIdx.x & 7) | ((threadIdx.x >> 3) & 8)); }
 __device__ __forceinline__ int tile_ly() { return (int)(((threadIdx.x >> 3) & 7) | ((threadIdx.x >> 4) & 8)); }
 
+// ms_expf_core (ms_math.h) of four arguments in lockstep, step by step, so that the four
+// chains interleave: alpha = min(0.999, opacity exp(-sigma)) for the sigma in range
+// [0, 5.55) (exp(0) for the others, whose `ok` is cleared). The same operations in the same
+// order as ms_expf_core: bit-identical results.
+__device__ __forceinline__ void ms_expf_core4(const float sg[4], bool ok[4], const float4 a[4], float al[4]) {
+    float x[4], n[4], r[4], p[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const bool in_range = sg[u] >= 0.f && sg[u] < 5.55f;
+        ok[u] = ok[u] & in_range;
+        x[u] = in_range ? -sg[u] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) n[u] = rintf(x[u] * 1.44269502162933349609375f);
+#pragma unroll
+    for (int u = 0; u < 4; u++) r[u] = fmaf(-n[u], 0.693147182464599609375f, x[u]);
+#pragma unroll
+    for (int u = 0; u < 4; u++) r[u] = fmaf(-n[u], -1.904654323148236e-09f, r[u]);
+#pragma unroll
+    for (int u = 0; u < 4; u++) p[u] = fmaf(1.98412701e-04f, r[u], 1.38888892e-03f);
+#pragma unroll
+    for (int u = 0; u < 4; u++) p[u] = fmaf(p[u], r[u], 8.33333377e-03f);
+#pragma unroll
+    for (int u = 0; u < 4; u++) p[u] = fmaf(p[u], r[u], 4.16666679e-02f);
+#pragma unroll
+    for (int u = 0; u < 4; u++) p[u] = fmaf(p[u], r[u], 1.66666672e-01f);
+#pragma unroll
+    for (int u = 0; u < 4; u++) p[u] = fmaf(p[u], r[u], 0.5f);
+#pragma unroll
+    for (int u = 0; u < 4; u++) p[u] = fmaf(p[u], r[u], 1.0f);
+#pragma unroll
+    for (int u = 0; u < 4; u++) p[u] = fmaf(p[u], r[u], 1.0f);
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+        al[u] = fminf(0.999f, a[u].z * (p[u] * ms_bits_to_float((uint32_t)((int)n[u] + 127) << 23)));
+}
+
 // Quadrants of the entry's 16x16 tile (bit q: the 8x8 quadrant of wave q, x half q & 1, y
 // half q >> 1) holding a pixel centre that can take the entry. A pixel takes it only where
 // sigma < 5.55 and o exp(-sigma) >= 1/255, i.e. sigma <= ln(255 o) (the early-outs of
@@ -1907,45 +1944,58 @@ __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bo
         for (int t4 = 0; t4 < cnt; t4 += 4) {
             float g[4][9];
             const uint32_t e4 = *reinterpret_cast<const uint32_t*>(&sL[wid][t4]);
+            // the four entries side by side, phase by phase, so that their independent chains
+            // (offsets, sigma, the exp polynomial, the reciprocal) interleave in the issue
+            float4 a[4], b[4];
+            float r2[4], dx[4], dy[4], sg[4], al[4];
+            bool ok[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
                 const bool have = t4 + u < cnt;  // the last step may hold fewer than four
                 const int tt = have ? (int)((e4 >> (8 * u)) & 255u) : 0;
-                const int e = bhi - 1 - tt;
-                const float4 a = sA[tt];
-                const float4 b = sB[tt];
-                const float r2 = sC[tt].x;
-                const float dx = a.x - px, dy = a.y - py;
-                const float sigma = fmaf(0.5f, fmaf(a.w, dx * dx, b.y * dy * dy), b.x * dx * dy);
-                const bool in_range = sigma >= 0.f && sigma < 5.55f;
-                const float alpha = fminf(0.999f, a.z * ms_expf_core(in_range ? -sigma : 0.f));
-                // bitwise: no short-circuit branches around the arithmetic above (every lane
-                // evaluates every entry; the four entries' LDS reads are then issued together)
-                const bool use = have & inside & (e <= bin_final) & in_range & (alpha >= 1.f / 255.f) &
-                                 (alpha < 0.999f);
+                a[u] = sA[tt];
+                b[u] = sB[tt];
+                r2[u] = sC[tt].x;
+                ok[u] = have & inside & (bhi - 1 - tt <= bin_final);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                dx[u] = a[u].x - px;
+                dy[u] = a[u].y - py;
+                sg[u] = fmaf(0.5f, fmaf(a[u].w, dx[u] * dx[u], b[u].y * dy[u] * dy[u]), b[u].x * dx[u] * dy[u]);
+            }
+            ms_expf_core4(sg, ok, a, al);
+            float au[4], ra[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                // bitwise: no short-circuit branches around the arithmetic above
+                const bool use = ok[u] & (al[u] >= 1.f / 255.f) & (al[u] < 0.999f);
                 // one select: an entry not taken has alpha 0, so ra = 1 exactly, T and the
                 // colour sums stay as they are and every term below is a product with 0 (all
                 // factors are finite), which is what selecting each of them would give
-                const float au = use ? alpha : 0.f;
-                const float ra = recip_nr(1.f - au);
-                const float Tn = T * ra;
-                const float fac = au * Tn;
+                au[u] = use ? al[u] : 0.f;
+                ra[u] = recip_nr(1.f - au[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const float Tn = T * ra[u];
+                const float fac = au[u] * Tn;
                 // d loss / d alpha = sum_c (colour_c Tn - ra (buf_c + T_final bg_c)) vo_c
-                const float rv = fmaf(r2, vo2, fmaf(b.w, vo1, b.z * vo0));
-                const float v_alpha = fmaf(Tn, rv, -ra * S);
+                const float rv = fmaf(r2[u], vo2, fmaf(b[u].w, vo1, b[u].z * vo0));
+                const float v_alpha = fmaf(Tn, rv, -ra[u] * S);
                 S = fmaf(fac, rv, S);
                 T = Tn;
                 // the entry's own constants (conic, opacity) multiply the wave sums once per
                 // entry after the reduction: per pixel only v_sigma and its products with the
                 // offsets are summed
-                const float vs = -au * v_alpha;  // d loss / d sigma
-                const float vdx = vs * dx, vdy = vs * dy;
+                const float vs = -au[u] * v_alpha;  // d loss / d sigma
+                const float vdx = vs * dx[u], vdy = vs * dy[u];
                 g[u][0] = fac * vo0;
                 g[u][1] = fac * vo1;
                 g[u][2] = fac * vo2;
-                g[u][3] = vdx * dx;
-                g[u][4] = vdx * dy;
-                g[u][5] = vdy * dy;
+                g[u][3] = vdx * dx[u];
+                g[u][4] = vdx * dy[u];
+                g[u][5] = vdy * dy[u];
                 g[u][6] = vdx;
                 g[u][7] = vdy;
                 g[u][8] = vs;
