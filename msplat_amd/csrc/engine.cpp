@@ -329,6 +329,7 @@ void Engine::raster_fwd_own() {
                       chunkC_.as<float>(), chunkIdx_.as<int32_t>(), prefixT_.as<float>(), afterC_.as<float>(), K_};
     const msk::RasterWork w = own_work();
     if (K_ <= 1 && !queue_built_) msk::raster_work_build(stream_, w);
+    queue_built_ = false;  // the forward consumes the queue's forward head
     msk::raster_forward(stream_, lists_, vcd_, launch_, R, &w);
     stage_event(ST_RASTER_FWD);
 }
