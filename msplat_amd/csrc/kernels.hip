@@ -1692,17 +1692,34 @@ __global__ __launch_bounds__(256) void k_ssim_fused(const ViewConsts* __restrict
     const size_t P = (size_t)W * H;
     s_ssim[ly][lx] = 0.f;
     s_l1[ly][lx] = 0.f;
+    // the target (x = gt / 255) and render over the halo, rows by .. by + 25, columns
+    // bx - 5 .. bx + 30 (0 outside), all three channels read once into registers (each
+    // thread's halo pixels k = tid + 256 q), so the channel passes below read no memory
+    constexpr int HALO_Q = (TD * TI + 255) / 256;
+    float hg[HALO_Q][3], hr[HALO_Q][3];
+#pragma unroll
+    for (int q = 0; q < HALO_Q; q++) {
+        const int k = (int)threadIdx.x + 256 * q;
+        const int sy = k / TI, sx = k % TI;
+        const int gy = by + sy, gxx = bx - 5 + sx;
+        const bool in = k < TD * TI && gxx >= 0 && gxx < W && gy >= 0 && gy < H;
+        const size_t pid = in ? (size_t)gy * W + gxx : 0;
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            hg[q][c] = in ? (float)gt[3 * pid + c] * (1.0f / 255.0f) : 0.f;
+            hr[q][c] = in ? img[3 * pid + c] : 0.f;
+        }
+    }
     for (int c = 0; c < 3; c++) {
-        // channel c of the target (x = gt / 255) and render, rows by .. by + 25, columns
-        // bx - 5 .. bx + 30 (0 outside); the previous channel's readers passed the barrier
-        // that ends its iteration
-        for (int k = threadIdx.x; k < TD * TI; k += 256) {
-            const int sy = k / TI, sx = k % TI;
-            const int gy = by + sy, gxx = bx - 5 + sx;
-            const bool in = gxx >= 0 && gxx < W && gy >= 0 && gy < H;
-            const size_t pid = in ? (size_t)gy * W + gxx : 0;
-            sg[sy][sx] = in ? (float)gt[3 * pid + c] * (1.0f / 255.0f) : 0.f;
-            sr[sy][sx] = in ? img[3 * pid + c] : 0.f;
+        // channel c into LDS; the previous channel's readers passed the barrier that ends
+        // its iteration
+#pragma unroll
+        for (int q = 0; q < HALO_Q; q++) {
+            const int k = (int)threadIdx.x + 256 * q;
+            if (k < TD * TI) {
+                sg[k / TI][k % TI] = hg[q][c];
+                sr[k / TI][k % TI] = hr[q][c];
+            }
         }
         __syncthreads();
         // horizontal 11-tap statistics at the halo's pixels; zero outside the image (the
