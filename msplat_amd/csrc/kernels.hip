@@ -1903,7 +1903,7 @@ void ssim_loss(hipStream_t s, const ViewConsts* vc, const Geom& gm, const float*
 // metal:3139-3366). Entries are walked back to front in LDS batches; each wave reduces
 // its 64 lanes' 9 gradient terms of four entries at a time (wave_sum4_rows), the 4 wave
 // sums meet in LDS and one thread per entry issues the global atomics.
-constexpr int BB = 128;  // backward batch
+constexpr int BB = 192;  // backward batch: most tiles' walks end within one (128 was about 1 % slower, 64 6 %)
 
 __device__ __forceinline__ void backward_range(int lo, int hi, int bin_final, bool inside, float px, float py,
                                                float T, float buf0, float buf1, float buf2, float T_final,
